@@ -1,0 +1,178 @@
+/*
+ * smili_hip.h -- C ABI of the MI355X (gfx950) implementation of SMILI's
+ * regularized maximum-likelihood imaging cost and its gradient.
+ *
+ * The unit this library replaces is the Fortran routine
+ *   fftim2d.calc_cost   (src/fortlib/fftim2d.f90:387-607)
+ * which SMILI's L-BFGS-B loop (src/fortlib/fftim2d.f90:286-351) calls at
+ * every 'FG' request, together with its callees fftlib.calc_chisq
+ * (src/fortlib/fftlib.f90:250-382), image.calc_cost_reg
+ * (src/fortlib/image.f90:12-200) and the FINUFFT type-1/type-2 transforms
+ * reached through src/wrapfinufft/wrapfinufft.f90:94-206.
+ *
+ * Conventions (identical to the reference's f2py interface,
+ * src/fortlib/fftim2d.f90:387-450 and smili/imaging/lbfgs.py:492-564):
+ *   - the image vector x has npix entries, pixel i sits at (xidx[i], yidx[i]),
+ *     1-based, x fastest;
+ *   - u, v are the unique uv points already scaled by 2*pi*dx_rad, 2*pi*dy_rad;
+ *   - uvidx* are signed 1-based indices into (u, v): a negative index means
+ *     the datum sits at (-u, -v) (smili/uvdata/uvtable/tools.py:12-132);
+ *   - closure index arrays hold 3 (closure phase) or 4 (log closure
+ *     amplitude) contiguous indices per closure (Fortran uvidxcp(3,Ncp),
+ *     uvidxca(4,Nca));
+ *   - Vfcv is already phase-shifted to the image centre and the data
+ *     weights are already normalised (the set-up fftim2d.imaging does at
+ *     src/fortlib/fftim2d.f90:178-241 before it calls calc_cost);
+ *   - a regularizer is enabled iff its lambda > 0 (image.f90:111-190);
+ *   - a data type is enabled iff its count > 0.
+ *
+ * All entry points return 0 on success and a negative status on error;
+ * smili_last_error() returns the message of the last error on the calling
+ * thread.  Calls on one handle must not overlap; distinct handles are
+ * independent.  No entry point holds a lock across the GPU work.
+ */
+#ifndef SMILI_HIP_H
+#define SMILI_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMILI_ABI_VERSION 1
+#define SMILI_NTERMS 15
+
+/* status codes */
+#define SMILI_OK 0
+#define SMILI_EINVAL (-1)  /* invalid argument or inconsistent descriptor */
+#define SMILI_EHIP (-2)    /* HIP runtime error */
+#define SMILI_ENOMEM (-3)  /* device allocation failed */
+#define SMILI_ENONFINITE (-4) /* cost or gradient is NaN/Inf (results still written) */
+
+/* Order of the 15 terms written by smili_cost_grad (the scalar outputs of
+ * calc_cost, src/fortlib/fftim2d.f90:435-453).  chisq_* are reported
+ * re-normalised as calc_chisq does (src/fortlib/fftlib.f90:342-366). */
+enum smili_term {
+  SMILI_T_CHISQ = 0, SMILI_T_CHISQ_FCV, SMILI_T_CHISQ_AMP, SMILI_T_CHISQ_CP,
+  SMILI_T_CHISQ_CA, SMILI_T_REG, SMILI_T_L1, SMILI_T_TV, SMILI_T_TSV,
+  SMILI_T_KL, SMILI_T_GS, SMILI_T_TFD, SMILI_T_CEN, SMILI_T_SM, SMILI_T_COST
+};
+
+/* Regularizer parameters: the l1_l .. sm_phi arguments of calc_cost
+ * (src/fortlib/fftim2d.f90:388-397).  Weight vectors have npix entries and
+ * may be NULL when the term is disabled. */
+typedef struct smili_reg {
+  double l1_l;  const double *l1_wgt;
+  double tv_l;  const double *tv_wgt;
+  double tsv_l; const double *tsv_wgt;
+  double kl_l;  const double *kl_wgt;
+  double gs_l;  const double *gs_wgt;
+  double tfd_l, tfd_tgtfd;
+  double cen_l, cen_alpha;
+  double sm_l, sm_maj, sm_min, sm_phi;
+} smili_reg;
+
+/* Data blocks: the isfcv .. wca arguments of calc_cost
+ * (src/fortlib/fftim2d.f90:398-401).  w* are the normalised weights. */
+typedef struct smili_data {
+  int32_t nfcv; const int32_t *uvidxfcv;
+  const double *vfcv_re, *vfcv_im, *varfcv; double wfcv;
+  int32_t namp; const int32_t *uvidxamp;
+  const double *vamp, *varamp; double wamp;
+  int32_t ncp;  const int32_t *uvidxcp;  /* 3*ncp */
+  const double *cp, *varcp; double wcp;
+  int32_t nca;  const int32_t *uvidxca;  /* 4*nca */
+  const double *ca, *varca; double wca;
+} smili_data;
+
+/* One imaging problem = the argument list of calc_cost minus x. */
+typedef struct smili_desc {
+  int32_t nx, ny;
+  double nxref, nyref;
+  int32_t npix;
+  const int32_t *xidx, *yidx;   /* npix, 1-based */
+  int32_t nuv;
+  const double *u, *v;          /* nuv, scaled */
+  smili_reg reg;
+  smili_data data;
+} smili_desc;
+
+typedef struct smili_problem smili_problem;
+
+/* Tunables of the spreading/interpolation kernels (0 = default). */
+typedef struct smili_tune {
+  int32_t bin;          /* bin edge in fine-grid cells (8..32, default 16) */
+  int32_t sub_points;   /* max points per subproblem (64..4096, default 256) */
+  int32_t col_group;    /* columns per column-FFT work-group (1..16, default 4) */
+  int32_t chunk;        /* data items per chi-square work-group (default 2048) */
+  int32_t use_graph;    /* >= 0: capture one evaluation in a hipGraph and replay it;
+                           < 0: launch the kernels one by one */
+} smili_tune;
+
+int smili_abi_version(void);
+const char *smili_last_error(void);
+
+/* Creates a problem on the current HIP device and uploads everything
+ * calc_cost reads except x (replaces the per-call argument marshalling of
+ * the f2py call, smili/imaging/lbfgs.py:492-564).  tune may be NULL. */
+int smili_problem_create(const smili_desc *desc, const smili_tune *tune,
+                         smili_problem **out);
+void smili_problem_destroy(smili_problem *p);
+
+/* cost + gradient + the 15 terms for x (host pointers): one call of
+ * calc_cost (src/fortlib/fftim2d.f90:387-607).  grad has npix entries;
+ * terms may be NULL; cost may be NULL. */
+int smili_cost_grad(smili_problem *p, const double *x, double *cost,
+                    double *grad, double *terms);
+
+/* The same on device memory, asynchronous on the problem's stream.
+ * x_dev/grad_dev are device pointers with npix entries, terms_dev a device
+ * pointer with 15 entries.  Use smili_synchronize() before reading. */
+int smili_cost_grad_device(smili_problem *p, const double *x_dev,
+                           double *grad_dev, double *terms_dev);
+int smili_synchronize(smili_problem *p);
+/* hipStream_t of the problem, as void* (plain pointer, no HIP types). */
+void *smili_problem_stream(smili_problem *p);
+
+/* Model visibilities at the nuv unique uv points for image x (host), i.e.
+ * the type-2 transform finufft2d2 (src/wrapfinufft/wrapfinufft.f90:94-128)
+ * that fftlib.model_fcv/amp/cp/ca evaluate first (src/fortlib/fftlib.f90:744-1254). */
+int smili_model_vis(smili_problem *p, const double *x, double *vre, double *vim);
+
+/* Re(A1 (vres_re - i vres_im)) at the npix window pixels, i.e.
+ * NUFFT_adj_resid (src/fortlib/fftlib.f90:152-174) for real residual
+ * vectors, followed by I1d_I2d_inv (src/fortlib/image.f90:1190-1207). */
+int smili_adjoint_resid(smili_problem *p, const double *vres_re,
+                        const double *vres_im, double *out);
+
+/* Kernel stages of one evaluation, in launch order. */
+enum smili_stage {
+  SMILI_S_SCATTER = 0,     /* window pixels -> 2-D image (I1d_I2d_fwd) */
+  SMILI_S_T2_ROWS,         /* deconvolve + zero-pad + FFT along x */
+  SMILI_S_T2_COLS,         /* FFT along y */
+  SMILI_S_INTERP,          /* ES interpolation to the uv points */
+  SMILI_S_MOMENTS,         /* image moments for tfd / cen / sm */
+  SMILI_S_MOMENTS_REDUCE,
+  SMILI_S_CHISQ,           /* the four chi-square terms + residual vectors */
+  SMILI_S_SPREAD,          /* ES spreading of the residuals */
+  SMILI_S_T1_COLS,         /* FFT along y */
+  SMILI_S_T1_ROWS,         /* FFT along x + deconvolve + crop */
+  SMILI_S_REG,             /* regularizers + gradient assembly */
+  SMILI_S_FINALIZE,        /* fixed-order sums -> the 15 terms */
+  SMILI_NSTAGES
+};
+
+/* Average device time (ms) of each stage over `reps` evaluations launched
+ * kernel by kernel with hipEvents on the problem's stream; ms has
+ * SMILI_NSTAGES entries (0 for stages that do not run). */
+int smili_problem_profile(smili_problem *p, int32_t reps, double *ms);
+
+/* Sizes: [0]=nf1 [1]=nf2 [2]=nuv [3]=subproblems [4]=bin [5]=chi2 chunks
+ * [6]=kernel launches per evaluation [7]=graph in use. */
+int smili_problem_info(smili_problem *p, int64_t *info8);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

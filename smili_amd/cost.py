@@ -1,0 +1,195 @@
+"""Host wrapper of one imaging cost on the GPU.
+
+``CostProblem`` takes the argument list of the reference's
+``fftim2d.calc_cost`` (src/fortlib/fftim2d.f90:387-407) -- image geometry,
+scaled uv list, regularizer lambdas and weights, and the four data blocks
+with signed 1-based uv indices -- uploads it once, and then evaluates
+``cost, grad, terms = problem.cost_grad(x)`` on the GPU through the C ABI of
+libsmili_hip.so.  Keyword names follow calc_cost's argument names.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import NTERMS, TERM_NAMES, check, dptr, iptr
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+
+
+class CostProblem:
+    """One RML imaging objective resident on the current GPU.
+
+    Parameters follow calc_cost (fftim2d.f90:387-407): ``xidx, yidx`` 1-based
+    pixel coordinates of the npix image vector; ``u, v`` scaled unique uv
+    points; ``uvidxcp`` of shape (3, Ncp) and ``uvidxca`` of shape (4, Nca)
+    as the Fortran arrays uvidxcp(3,Ncp), uvidxca(4,Nca); Vfcv complex and
+    already phase-shifted; w* normalised weights.  A regularizer is on iff
+    its lambda > 0; a data block is on iff it is given and non-empty.
+    """
+
+    def __init__(self, Nx, Ny, xidx, yidx, u, v, Nxref=0.0, Nyref=0.0,
+                 l1_l=-1.0, l1_wgt=None, tv_l=-1.0, tv_wgt=None,
+                 tsv_l=-1.0, tsv_wgt=None, kl_l=-1.0, kl_wgt=None,
+                 gs_l=-1.0, gs_wgt=None, tfd_l=-1.0, tfd_tgtfd=1.0,
+                 cen_l=-1.0, cen_alpha=1.0, sm_l=-1.0, sm_maj=0.0, sm_min=0.0, sm_phi=0.0,
+                 uvidxfcv=None, Vfcv=None, Varfcv=None, wfcv=0.0,
+                 uvidxamp=None, Vamp=None, Varamp=None, wamp=0.0,
+                 uvidxcp=None, CP=None, Varcp=None, wcp=0.0,
+                 uvidxca=None, CA=None, Varca=None, wca=0.0,
+                 tune=None, extra=None):
+        lib = _lib.load()
+        self._lib = lib
+        self._keep = []
+
+        def keep(a):
+            self._keep.append(a)
+            return a
+
+        self.Nx, self.Ny = int(Nx), int(Ny)
+        xidx = keep(_i32(xidx))
+        yidx = keep(_i32(yidx))
+        u = keep(_f64(u))
+        v = keep(_f64(v))
+        self.npix = int(xidx.size)
+        self.nuv = int(u.size)
+
+        reg = _lib.SmiliReg()
+        for name, lam, w in (("l1", l1_l, l1_wgt), ("tv", tv_l, tv_wgt), ("tsv", tsv_l, tsv_wgt),
+                             ("kl", kl_l, kl_wgt), ("gs", gs_l, gs_wgt)):
+            setattr(reg, name + "_l", float(lam))
+            if lam > 0:
+                arr = keep(_f64(np.broadcast_to(np.asarray(w, np.float64), (self.npix,))))
+                setattr(reg, name + "_wgt", dptr(arr))
+        reg.tfd_l, reg.tfd_tgtfd = float(tfd_l), float(tfd_tgtfd)
+        reg.cen_l, reg.cen_alpha = float(cen_l), float(cen_alpha)
+        reg.sm_l, reg.sm_maj, reg.sm_min, reg.sm_phi = map(float, (sm_l, sm_maj, sm_min, sm_phi))
+
+        data = _lib.SmiliData()
+        if uvidxfcv is not None and len(uvidxfcv) > 0:
+            Vfcv = np.asarray(Vfcv, np.complex128)
+            data.nfcv = len(uvidxfcv)
+            data.uvidxfcv = iptr(keep(_i32(uvidxfcv)))
+            data.vfcv_re = dptr(keep(_f64(Vfcv.real)))
+            data.vfcv_im = dptr(keep(_f64(Vfcv.imag)))
+            data.varfcv = dptr(keep(_f64(Varfcv)))
+            data.wfcv = float(wfcv)
+        if uvidxamp is not None and len(uvidxamp) > 0:
+            data.namp = len(uvidxamp)
+            data.uvidxamp = iptr(keep(_i32(uvidxamp)))
+            data.vamp = dptr(keep(_f64(Vamp)))
+            data.varamp = dptr(keep(_f64(Varamp)))
+            data.wamp = float(wamp)
+        if uvidxcp is not None and np.asarray(uvidxcp).shape[-1] > 0:
+            idx = np.asarray(uvidxcp).reshape(3, -1)
+            data.ncp = idx.shape[1]
+            data.uvidxcp = iptr(keep(_i32(idx.T)))      # 3 contiguous per closure
+            data.cp = dptr(keep(_f64(CP)))
+            data.varcp = dptr(keep(_f64(Varcp)))
+            data.wcp = float(wcp)
+        if uvidxca is not None and np.asarray(uvidxca).shape[-1] > 0:
+            idx = np.asarray(uvidxca).reshape(4, -1)
+            data.nca = idx.shape[1]
+            data.uvidxca = iptr(keep(_i32(idx.T)))      # 4 contiguous per closure
+            data.ca = dptr(keep(_f64(CA)))
+            data.varca = dptr(keep(_f64(Varca)))
+            data.wca = float(wca)
+
+        desc = _lib.SmiliDesc()
+        desc.nx, desc.ny = self.Nx, self.Ny
+        desc.nxref, desc.nyref = float(Nxref), float(Nyref)
+        desc.npix = self.npix
+        desc.xidx, desc.yidx = iptr(xidx), iptr(yidx)
+        desc.nuv = self.nuv
+        desc.u, desc.v = dptr(u), dptr(v)
+        desc.reg = reg
+        desc.data = data
+
+        t = None
+        if tune:
+            t = _lib.SmiliTune()
+            for k, val in tune.items():
+                setattr(t, k, int(val))
+        h = C.c_void_p()
+        check(lib.smili_problem_create(C.byref(desc), C.byref(t) if t is not None else None, C.byref(h)))
+        self._h = h
+        self._keep = []           # the library copied everything it needs
+        self._grad = np.empty(self.npix)
+        self._terms = np.empty(NTERMS)
+
+    # ------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.smili_problem_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ------------------------------------------------------------------
+    def cost_grad(self, x, allow_nonfinite=False):
+        """(cost, grad[npix], terms dict) for image vector x."""
+        x = _f64(x)
+        if x.size != self.npix:
+            raise ValueError(f"x has {x.size} entries, expected {self.npix}")
+        grad = np.empty(self.npix)
+        terms = np.empty(NTERMS)
+        cost = C.c_double()
+        check(self._lib.smili_cost_grad(self._h, dptr(x), C.byref(cost), dptr(grad), dptr(terms)),
+              allow_nonfinite)
+        return cost.value, grad, dict(zip(TERM_NAMES, terms.tolist()))
+
+    def cost_grad_device(self, x_ptr, grad_ptr, terms_ptr=None):
+        """Asynchronous evaluation on device pointers (ints)."""
+        check(self._lib.smili_cost_grad_device(self._h, C.c_void_p(x_ptr), C.c_void_p(grad_ptr),
+                                                C.c_void_p(terms_ptr) if terms_ptr else None))
+
+    def synchronize(self):
+        check(self._lib.smili_synchronize(self._h))
+
+    @property
+    def stream(self):
+        return self._lib.smili_problem_stream(self._h)
+
+    def model_vis(self, x):
+        """Complex model visibilities at the nuv uv points (type-2 transform)."""
+        x = _f64(x)
+        vre = np.empty(self.nuv)
+        vim = np.empty(self.nuv)
+        check(self._lib.smili_model_vis(self._h, dptr(x), dptr(vre), dptr(vim)))
+        return vre + 1j * vim
+
+    def adjoint_resid(self, vres_re, vres_im):
+        """Re(A1 (vres_re - i vres_im)) at the window pixels."""
+        out = np.empty(self.npix)
+        check(self._lib.smili_adjoint_resid(self._h, dptr(_f64(vres_re)), dptr(_f64(vres_im)), dptr(out)))
+        return out
+
+    def profile(self, reps=10):
+        """Average device ms per kernel stage (eager launches with hipEvents)."""
+        ms = np.zeros(len(_lib.STAGE_NAMES))
+        check(self._lib.smili_problem_profile(self._h, int(reps), dptr(ms)))
+        return dict(zip(_lib.STAGE_NAMES, ms.tolist()))
+
+    def info(self):
+        a = (C.c_int64 * 8)()
+        check(self._lib.smili_problem_info(self._h, a))
+        keys = ("nf1", "nf2", "nuv", "subproblems", "bin", "chunks", "launches", "graph")
+        return dict(zip(keys, list(a)))

@@ -1,0 +1,860 @@
+// Host side of libsmili_hip: builds the device plan of one or more imaging
+// problems (point binning, index remapping, FFT plans, work lists), owns the
+// device buffers, launches one objective+gradient evaluation (optionally as
+// a captured hipGraph) and exports the C ABI declared in include/smili_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/smili_hip.h"
+#include "smili_kernels.h"
+// the kernels are compiled in this translation unit
+#include "smili_kernels.hip"
+
+namespace smili {
+
+static thread_local std::string g_last_error;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIP_CHECK(expr)                                                                  \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess)                                                                \
+      throw Error(_e == hipErrorOutOfMemory ? SMILI_ENOMEM : SMILI_EHIP,                 \
+                  std::string(#expr) + ": " + hipGetErrorString(_e));                    \
+  } while (0)
+
+static void require(bool ok, const std::string& msg) {
+  if (!ok) throw Error(SMILI_EINVAL, msg);
+}
+
+// ---------------------------------------------------------------------------
+// numerics set-up (FP64, host)
+// ---------------------------------------------------------------------------
+static const double kBeta = 2.30 * kW;   // beta / w = 2.30 at upsampling 2
+
+static double es_phi(double z) {
+  if (std::fabs(z) > 1.0) return 0.0;
+  const double s = std::sqrt(1.0 - z * z);
+  return std::exp(-kBeta * z * z / (1.0 + s));
+}
+
+// Gauss-Legendre nodes on [-1, 1]
+static void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(n, 0.0);
+  w.assign(n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    double z = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = z;
+      for (int k = 2; k <= n; ++k) {
+        double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      double dp = n * (z * p1 - p0) / (z * z - 1.0);
+      double dz = p1 / dp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-16) break;
+    }
+    double p0 = 1.0, p1 = z;
+    for (int k = 2; k <= n; ++k) {
+      double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+      p0 = p1;
+      p1 = p2;
+    }
+    double dp = n * (z * p1 - p0) / (z * z - 1.0);
+    x[i] = z;
+    w[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+  }
+}
+
+// c(k) = 1 / ((w/2) int_{-1}^{1} phi(z) cos(pi k w z / nf) dz), k = ix - n/2
+static std::vector<double> es_correction(int n, int nf) {
+  static std::vector<double> gx, gw;
+  if (gx.empty()) gauss_legendre(200, gx, gw);
+  std::vector<double> c(n);
+  for (int ix = 0; ix < n; ++ix) {
+    const double k = ix - n / 2;
+    double s = 0.0;
+    for (size_t q = 0; q < gx.size(); ++q) s += gw[q] * es_phi(gx[q]) * std::cos(M_PI * k * kW * gx[q] / nf);
+    c[ix] = 1.0 / (0.5 * kW * s);
+  }
+  return c;
+}
+
+static int next_smooth(int n) {
+  for (;; ++n) {
+    int m = n;
+    for (int p : {2, 3, 5})
+      while (m % p == 0) m /= p;
+    if (m == 1) return n;
+  }
+}
+
+static int fine_size(int n) { return next_smooth(std::max(2 * n, 2 * kW)); }
+
+static std::vector<int> radices(int n) {
+  std::vector<int> r;
+  while (n % 8 == 0) { r.push_back(8); n /= 8; }
+  while (n % 4 == 0) { r.push_back(4); n /= 4; }
+  while (n % 2 == 0) { r.push_back(2); n /= 2; }
+  while (n % 3 == 0) { r.push_back(3); n /= 3; }
+  while (n % 5 == 0) { r.push_back(5); n /= 5; }
+  if (n != 1) throw Error(SMILI_EINVAL, "fine grid size has a prime factor other than 2, 3, 5");
+  return r;
+}
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    free();
+    n = count;
+    if (count) HIP_CHECK(hipMalloc(&p, count * sizeof(T)));
+  }
+  void upload(const std::vector<T>& h) {
+    alloc(h.size());
+    if (n) HIP_CHECK(hipMemcpy(p, h.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  }
+  void zero() {
+    if (n) HIP_CHECK(hipMemset(p, 0, n * sizeof(T)));
+  }
+  void free() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { free(); }
+};
+
+// ---------------------------------------------------------------------------
+// plan inputs
+// ---------------------------------------------------------------------------
+struct UnitSpec {
+  const smili_desc* d;   // geometry, window, regularizers; uv of this unit
+  int prob;
+  double reg_scale;
+  const double* u;
+  const double* v;
+  int m;
+};
+
+struct ProbSpec {
+  const smili_data* data;
+  int unit_begin, unit_end;
+  int nuv;               // length of the problem's uv list
+};
+
+struct Plan {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  smili_tune tune{};
+  int bin = 16, sub_points = 256, col_group = 4, chunk = 2048, rows_per_block = 4;
+
+  std::vector<UnitDev> units;
+  std::vector<ProbDev> probs;
+  std::vector<std::vector<int>> newpos;   // per problem: uv list index -> global point
+  std::vector<std::vector<int>> winpix;   // per unit: window pixel linear index
+  size_t npix_total = 0, npt = 0;
+  bool any_window = false, any_moments = false;
+
+  std::vector<RowItem> h_t2rows, h_cols, h_t1rows;
+  std::vector<SubProb> h_subs;
+  std::vector<Chunk> h_chunks;
+  std::vector<RegItem> h_regs;
+
+  DevBuf<UnitDev> d_units;
+  DevBuf<ProbDev> d_probs;
+  DevBuf<FftPlanDev> d_plans;
+  DevBuf<float2> d_tw;
+  DevBuf<float> d_corr;
+  DevBuf<int> d_pix;
+  DevBuf<double> d_wgt;
+  DevBuf<int> d_cell;
+  DevBuf<float> d_ex, d_ey;
+  DevBuf<float2> d_V;
+  DevBuf<double2> d_Vres;
+  DevBuf<float2> d_gridA, d_gridB;
+  DevBuf<double> d_I2d, d_grad2d;
+  DevBuf<double> d_x, d_grad, d_terms;
+  DevBuf<double> d_cpart, d_rpart, d_mpart;
+  DevBuf<UnitMoments> d_mom;
+  DevBuf<int> d_unit_reg_begin;
+  DevBuf<Chunk> d_chunks;
+  DevBuf<FcvItem> d_fcv;
+  DevBuf<AmpItem> d_amp;
+  DevBuf<CpItem> d_cp;
+  DevBuf<CaItem> d_ca;
+  DevBuf<SubProb> d_subs;
+  DevBuf<RowItem> d_t2rows, d_cols, d_t1rows;
+  DevBuf<RegItem> d_regs;
+
+  size_t lds_rows = 0, lds_cols = 0, lds_tile = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool graph_ready = false;
+  int launches = 0;
+  bool prof = false;                       // record events around each kernel
+  hipEvent_t prof_ev[2 * SMILI_NSTAGES] = {};
+  int prof_used[SMILI_NSTAGES] = {};
+
+  ~Plan() {
+    for (auto& e : prof_ev)
+      if (e) (void)hipEventDestroy(e);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& ps, const smili_tune* t);
+  void launch_forward();
+  void launch_adjoint();
+  void launch_eval();
+  void eval();
+};
+
+static void check_desc_geometry(const smili_desc* d) {
+  require(d != nullptr, "descriptor is NULL");
+  require(d->nx > 0 && d->ny > 0, "nx and ny must be positive");
+  require((long long)d->nx * d->ny < (1LL << 30), "image too large");
+  require(d->npix > 0 && d->xidx && d->yidx, "npix must be positive and xidx/yidx given");
+  for (int i = 0; i < d->npix; ++i) {
+    require(d->xidx[i] >= 1 && d->xidx[i] <= d->nx && d->yidx[i] >= 1 && d->yidx[i] <= d->ny,
+            "xidx/yidx out of range at pixel " + std::to_string(i));
+  }
+  const smili_reg& r = d->reg;
+  require(!(r.l1_l > 0) || r.l1_wgt, "l1_wgt missing");
+  require(!(r.tv_l > 0) || r.tv_wgt, "tv_wgt missing");
+  require(!(r.tsv_l > 0) || r.tsv_wgt, "tsv_wgt missing");
+  require(!(r.kl_l > 0) || r.kl_wgt, "kl_wgt missing");
+  require(!(r.gs_l > 0) || r.gs_wgt, "gs_wgt missing");
+}
+
+static void check_data(const smili_data& D, int nuv) {
+  auto chk_idx = [&](const int32_t* idx, long long n, const char* what) {
+    require(n == 0 || idx, std::string(what) + " indices missing");
+    for (long long i = 0; i < n; ++i)
+      require(idx[i] != 0 && std::abs(idx[i]) <= nuv,
+              std::string(what) + " uv index out of range at " + std::to_string(i));
+  };
+  auto chk_var = [&](const double* var, long long n, const char* what) {
+    require(n == 0 || var, std::string(what) + " variances missing");
+    for (long long i = 0; i < n; ++i)
+      require(var[i] > 0 && std::isfinite(var[i]),
+              std::string(what) + " variance must be positive and finite at " + std::to_string(i));
+  };
+  require(D.nfcv >= 0 && D.namp >= 0 && D.ncp >= 0 && D.nca >= 0, "negative data count");
+  require(D.nfcv + D.namp + D.ncp + D.nca > 0, "no data: all of nfcv, namp, ncp, nca are zero");
+  chk_idx(D.uvidxfcv, D.nfcv, "fcv");
+  chk_idx(D.uvidxamp, D.namp, "amp");
+  chk_idx(D.uvidxcp, 3LL * D.ncp, "closure phase");
+  chk_idx(D.uvidxca, 4LL * D.nca, "closure amplitude");
+  chk_var(D.varfcv, D.nfcv, "fcv");
+  chk_var(D.varamp, D.namp, "amp");
+  chk_var(D.varcp, D.ncp, "closure phase");
+  chk_var(D.varca, D.nca, "closure amplitude");
+  require(D.nfcv == 0 || (D.vfcv_re && D.vfcv_im), "fcv data missing");
+  require(D.namp == 0 || D.vamp, "amp data missing");
+  require(D.ncp == 0 || D.cp, "closure phase data missing");
+  require(D.nca == 0 || D.ca, "closure amplitude data missing");
+}
+
+void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& ps, const smili_tune* t) {
+  if (t) tune = *t;
+  bin = tune.bin > 0 ? tune.bin : 16;
+  sub_points = tune.sub_points > 0 ? tune.sub_points : 256;
+  col_group = tune.col_group > 0 ? tune.col_group : 4;
+  chunk = tune.chunk > 0 ? tune.chunk : 2048;
+  require(bin >= 8 && bin <= 64, "bin must be in [8, 64]");
+  require(sub_points >= 32 && sub_points <= 65536, "sub_points must be in [32, 65536]");
+  require(col_group >= 1 && col_group <= 16, "col_group must be in [1, 16]");
+  require(chunk >= 64, "chunk must be >= 64");
+  tune.use_graph = (tune.use_graph < 0) ? 0 : 1;
+
+  HIP_CHECK(hipGetDevice(&device));
+  HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+
+  // FFT plans and twiddles
+  std::map<int, int> plan_of;
+  std::vector<FftPlanDev> plans;
+  std::vector<float2> tw;
+  auto get_plan = [&](int n) {
+    auto it = plan_of.find(n);
+    if (it != plan_of.end()) return it->second;
+    FftPlanDev p{};
+    p.n = n;
+    auto r = radices(n);
+    require((int)r.size() <= kMaxRadixStages, "too many FFT stages");
+    p.nstages = (int)r.size();
+    for (size_t i = 0; i < r.size(); ++i) p.radix[i] = r[i];
+    p.tw_off = (int)tw.size();
+    for (int k = 0; k < n; ++k) {
+      const double a = 2.0 * M_PI * k / n;
+      tw.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
+    }
+    plans.push_back(p);
+    plan_of[n] = (int)plans.size() - 1;
+    return (int)plans.size() - 1;
+  };
+  std::map<std::pair<int, int>, int> corr_of;
+  std::vector<float> corr;
+  auto get_corr = [&](int n, int nf) {
+    auto key = std::make_pair(n, nf);
+    auto it = corr_of.find(key);
+    if (it != corr_of.end()) return it->second;
+    auto c = es_correction(n, nf);
+    int off = (int)corr.size();
+    for (double v : c) corr.push_back((float)v);
+    corr_of[key] = off;
+    return off;
+  };
+
+  // units: geometry, pixels, weights
+  std::vector<int> pix;
+  std::vector<double> wgt;
+  std::vector<size_t> wgt_off_l1(us.size()), wgt_off_tv(us.size()), wgt_off_tsv(us.size()),
+      wgt_off_kl(us.size()), wgt_off_gs(us.size());
+  long long grid_total = 0, img_total = 0;
+  int max_nf1 = 0, max_nf2 = 0;
+  units.resize(us.size());
+  winpix.resize(us.size());
+  npix_total = 0;
+  for (size_t ui = 0; ui < us.size(); ++ui) {
+    const smili_desc* d = us[ui].d;
+    UnitDev U{};
+    U.nx = d->nx;
+    U.ny = d->ny;
+    U.nf1 = fine_size(d->nx);
+    U.nf2 = fine_size(d->ny);
+    require(U.nf1 <= 4096 && U.nf2 <= 4096, "fine grid larger than 4096 is not supported");
+    max_nf1 = std::max(max_nf1, U.nf1);
+    max_nf2 = std::max(max_nf2, U.nf2);
+    U.npix = d->npix;
+    bool ident = (long long)d->npix == (long long)d->nx * d->ny;
+    for (int i = 0; ident && i < d->npix; ++i)
+      ident = d->xidx[i] == i % d->nx + 1 && d->yidx[i] == i / d->nx + 1;
+    U.identity = ident ? 1 : 0;
+    any_window = any_window || !ident;
+    U.m = us[ui].m;
+    U.x_off = (int)npix_total;
+    npix_total += d->npix;
+    U.fft1 = get_plan(U.nf1);
+    U.fft2 = get_plan(U.nf2);
+    U.grid_off = grid_total;
+    grid_total += (long long)U.nf1 * U.nf2;
+    U.img_off = img_total;
+    img_total += (long long)d->nx * d->ny;
+    U.cx_off = get_corr(d->nx, U.nf1);
+    U.cy_off = get_corr(d->ny, U.nf2);
+    U.pix_off = (int)pix.size();
+    winpix[ui].resize(d->npix);
+    for (int i = 0; i < d->npix; ++i) winpix[ui][i] = (d->yidx[i] - 1) * d->nx + (d->xidx[i] - 1);
+    pix.insert(pix.end(), winpix[ui].begin(), winpix[ui].end());
+    U.prob = us[ui].prob;
+    U.nxref = d->nxref;
+    U.nyref = d->nyref;
+    U.reg_scale = us[ui].reg_scale;
+    const smili_reg& r = d->reg;
+    U.l1_l = r.l1_l; U.tv_l = r.tv_l; U.tsv_l = r.tsv_l; U.kl_l = r.kl_l; U.gs_l = r.gs_l;
+    U.tfd_l = r.tfd_l; U.tfd_tgtfd = r.tfd_tgtfd; U.cen_l = r.cen_l; U.cen_alpha = r.cen_alpha;
+    U.sm_l = r.sm_l; U.sm_maj = r.sm_maj; U.sm_min = r.sm_min; U.sm_phi = r.sm_phi;
+    U.need_moments = (r.tfd_l > 0 || r.cen_l > 0 || r.sm_l > 0) ? 1 : 0;
+    any_moments = any_moments || U.need_moments;
+    auto put = [&](double lam, const double* w, size_t& off) {
+      off = wgt.size();
+      if (lam > 0) wgt.insert(wgt.end(), w, w + d->npix);
+    };
+    put(r.l1_l, r.l1_wgt, wgt_off_l1[ui]);
+    put(r.tv_l, r.tv_wgt, wgt_off_tv[ui]);
+    put(r.tsv_l, r.tsv_wgt, wgt_off_tsv[ui]);
+    put(r.kl_l, r.kl_wgt, wgt_off_kl[ui]);
+    put(r.gs_l, r.gs_wgt, wgt_off_gs[ui]);
+    if (r.kl_l > 0)
+      for (int i = 0; i < d->npix; ++i) require(r.kl_wgt[i] > 0, "kl weights (prior) must be positive");
+    if (r.gs_l > 0)
+      for (int i = 0; i < d->npix; ++i) require(r.gs_wgt[i] > 0, "gs weights (prior) must be positive");
+    units[ui] = U;
+  }
+
+  // points: grid coordinates, binning (counting sort), subproblems
+  std::vector<int> cell;
+  std::vector<float> ex, ey;
+  std::vector<std::vector<int>> unit_rank(us.size());   // local uv index -> local sorted position
+  npt = 0;
+  for (size_t ui = 0; ui < us.size(); ++ui) {
+    UnitDev& U = units[ui];
+    const int m = us[ui].m;
+    U.pt_off = (int)npt;
+    const int nbx = (U.nf1 + bin - 1) / bin, nby = (U.nf2 + bin - 1) / bin;
+    std::vector<int> l0x(m), l0y(m), bidx(m);
+    std::vector<float> exu(m), eyu(m);
+    for (int j = 0; j < m; ++j) {
+      const double uu = us[ui].u[j], vv = us[ui].v[j];
+      require(std::isfinite(uu) && std::isfinite(vv), "non-finite uv coordinate");
+      auto coord = [&](double x, int nf, int& l0, float& e) {
+        double tt = std::fmod(x * nf / (2.0 * M_PI), (double)nf);
+        if (tt < 0) tt += nf;
+        if (tt >= nf) tt -= nf;
+        double l = std::ceil(tt - 0.5 * kW);
+        double ee = tt - l - 0.5 * kW;                    // in (-1, 0]
+        int li = (int)l;
+        li %= nf;
+        if (li < 0) li += nf;
+        l0 = li;
+        e = (float)ee;
+      };
+      coord(uu, U.nf1, l0x[j], exu[j]);
+      coord(vv, U.nf2, l0y[j], eyu[j]);
+      bidx[j] = (l0y[j] / bin) * nbx + (l0x[j] / bin);
+    }
+    std::vector<int> count(nbx * nby + 1, 0);
+    for (int j = 0; j < m; ++j) count[bidx[j] + 1]++;
+    for (int b = 0; b < nbx * nby; ++b) count[b + 1] += count[b];
+    std::vector<int> order(m);
+    {
+      std::vector<int> fill(count.begin(), count.end() - 1);
+      for (int j = 0; j < m; ++j) order[fill[bidx[j]]++] = j;
+    }
+    unit_rank[ui].resize(m);
+    for (int s = 0; s < m; ++s) {
+      const int j = order[s];
+      unit_rank[ui][j] = s;
+      cell.push_back(l0x[j] | (l0y[j] << 16));
+      ex.push_back(exu[j]);
+      ey.push_back(eyu[j]);
+    }
+    for (int b = 0; b < nbx * nby; ++b) {
+      for (int s = count[b]; s < count[b + 1]; s += sub_points) {
+        SubProb sp{};
+        sp.unit = (int)ui;
+        sp.bin_x = b % nbx;
+        sp.bin_y = b / nbx;
+        sp.start = U.pt_off + s;
+        sp.count = std::min(sub_points, count[b + 1] - s);
+        h_subs.push_back(sp);
+      }
+    }
+    npt += m;
+  }
+  require(npt < (1u << 30), "too many uv points");
+
+  // problems: remapped data items and chunks
+  std::vector<FcvItem> fcv;
+  std::vector<AmpItem> amp;
+  std::vector<CpItem> cp;
+  std::vector<CaItem> ca;
+  probs.resize(ps.size());
+  newpos.resize(ps.size());
+  for (size_t pi = 0; pi < ps.size(); ++pi) {
+    const ProbSpec& P = ps[pi];
+    const smili_data& D = *P.data;
+    check_data(D, P.nuv);
+    auto& np = newpos[pi];
+    np.resize(P.nuv);
+    int o = 0;
+    for (int ui = P.unit_begin; ui < P.unit_end; ++ui) {
+      for (int j = 0; j < us[ui].m; ++j) np[o + j] = units[ui].pt_off + unit_rank[ui][j];
+      o += us[ui].m;
+    }
+    require(o == P.nuv, "uv list length does not match the units' point counts");
+    auto remap = [&](int idx) { return (idx > 0 ? 1 : -1) * (np[std::abs(idx) - 1] + 1); };
+    ProbDev& Q = probs[pi];
+    Q = ProbDev{};
+    Q.unit_begin = P.unit_begin;
+    Q.unit_end = P.unit_end;
+    Q.n[DT_FCV] = D.nfcv; Q.n[DT_AMP] = D.namp; Q.n[DT_CP] = D.ncp; Q.n[DT_CA] = D.nca;
+    Q.w[DT_FCV] = D.wfcv; Q.w[DT_AMP] = D.wamp; Q.w[DT_CP] = D.wcp; Q.w[DT_CA] = D.wca;
+    Q.terms_off = (int)(pi * SMILI_NTERMS);
+    auto add_chunks = [&](int type, long long base, long long n) {
+      Q.chunk_begin[type] = (int)h_chunks.size();
+      for (long long s = 0; s < n; s += chunk)
+        h_chunks.push_back(Chunk{type, (int)pi, (int)(base + s), (int)std::min<long long>(chunk, n - s)});
+      Q.chunk_end[type] = (int)h_chunks.size();
+    };
+    add_chunks(DT_FCV, (long long)fcv.size(), D.nfcv);
+    for (int i = 0; i < D.nfcv; ++i)
+      fcv.push_back(FcvItem{D.vfcv_re[i], D.vfcv_im[i], D.wfcv / D.varfcv[i], remap(D.uvidxfcv[i]), 0});
+    add_chunks(DT_AMP, (long long)amp.size(), D.namp);
+    for (int i = 0; i < D.namp; ++i)
+      amp.push_back(AmpItem{D.vamp[i], D.wamp / D.varamp[i], remap(D.uvidxamp[i]), 0});
+    add_chunks(DT_CP, (long long)cp.size(), D.ncp);
+    for (int i = 0; i < D.ncp; ++i) {
+      CpItem c{};
+      for (int k = 0; k < 3; ++k) c.idx[k] = remap(D.uvidxcp[3LL * i + k]);
+      c.idx[3] = 0;
+      c.cp = D.cp[i];
+      c.w = D.wcp / D.varcp[i];
+      cp.push_back(c);
+    }
+    add_chunks(DT_CA, (long long)ca.size(), D.nca);
+    for (int i = 0; i < D.nca; ++i) {
+      CaItem c{};
+      for (int k = 0; k < 4; ++k) c.idx[k] = remap(D.uvidxca[4LL * i + k]);
+      c.ca = D.ca[i];
+      c.w = D.wca / D.varca[i];
+      ca.push_back(c);
+    }
+  }
+
+  // regularizer items, row/column items
+  const int reg_chunk = 512;
+  std::vector<int> unit_reg_begin(units.size() + 1, 0);
+  rows_per_block = std::max(1, std::min(8, 2048 / std::max(max_nf1, 1)));
+  for (size_t ui = 0; ui < units.size(); ++ui) {
+    const UnitDev& U = units[ui];
+    unit_reg_begin[ui] = (int)h_regs.size();
+    for (int s = 0; s < U.npix; s += reg_chunk)
+      h_regs.push_back(RegItem{(int)ui, s, std::min(reg_chunk, U.npix - s), 0});
+    for (int s = 0; s < U.ny; s += rows_per_block) {
+      RowItem r{(int)ui, s, std::min(rows_per_block, U.ny - s), 0};
+      h_t2rows.push_back(r);
+      h_t1rows.push_back(r);
+    }
+    for (int s = 0; s < U.nf1; s += col_group)
+      h_cols.push_back(RowItem{(int)ui, s, std::min(col_group, U.nf1 - s), 0});
+  }
+  unit_reg_begin[units.size()] = (int)h_regs.size();
+  for (size_t pi = 0; pi < ps.size(); ++pi) {
+    probs[pi].reg_begin = unit_reg_begin[ps[pi].unit_begin];
+    probs[pi].reg_end = unit_reg_begin[ps[pi].unit_end];
+  }
+
+  // LDS budget
+  lds_rows = 2ull * rows_per_block * max_nf1 * sizeof(float2);
+  lds_cols = 2ull * col_group * (max_nf2 + 1) * sizeof(float2);
+  const int T = bin + kW - 1;
+  lds_tile = (size_t)T * T * sizeof(float2);
+  require(lds_rows <= 65536 && lds_cols <= 65536 && lds_tile <= 65536,
+          "LDS budget exceeded: lower col_group or bin");
+
+  // upload
+  // weights pointers (device) into the concatenated weight buffer
+  d_wgt.upload(wgt);
+  for (size_t ui = 0; ui < units.size(); ++ui) {
+    UnitDev& U = units[ui];
+    U.l1_w = U.l1_l > 0 ? d_wgt.p + wgt_off_l1[ui] : nullptr;
+    U.tv_w = U.tv_l > 0 ? d_wgt.p + wgt_off_tv[ui] : nullptr;
+    U.tsv_w = U.tsv_l > 0 ? d_wgt.p + wgt_off_tsv[ui] : nullptr;
+    U.kl_w = U.kl_l > 0 ? d_wgt.p + wgt_off_kl[ui] : nullptr;
+    U.gs_w = U.gs_l > 0 ? d_wgt.p + wgt_off_gs[ui] : nullptr;
+  }
+  d_units.upload(units);
+  d_probs.upload(probs);
+  d_plans.upload(plans);
+  d_tw.upload(tw);
+  d_corr.upload(corr);
+  d_pix.upload(pix);
+  d_cell.upload(cell);
+  d_ex.upload(ex);
+  d_ey.upload(ey);
+  d_V.alloc(std::max<size_t>(npt, 1));
+  d_Vres.alloc(std::max<size_t>(npt, 1));
+  d_Vres.zero();
+  d_gridA.alloc(grid_total);
+  d_gridA.zero();
+  d_gridB.alloc(grid_total);
+  d_gridB.zero();
+  d_I2d.alloc(img_total);
+  d_I2d.zero();
+  d_grad2d.alloc(img_total);
+  d_grad2d.zero();
+  d_x.alloc(npix_total);
+  d_x.zero();
+  d_grad.alloc(npix_total);
+  d_terms.alloc(SMILI_NTERMS * ps.size());
+  d_cpart.alloc(std::max<size_t>(h_chunks.size(), 1));
+  d_rpart.alloc(std::max<size_t>(h_regs.size(), 1) * kRegTerms);
+  d_mpart.alloc(std::max<size_t>(h_regs.size(), 1) * kMomTerms);
+  d_mpart.zero();
+  d_mom.alloc(units.size());
+  d_mom.zero();
+  d_unit_reg_begin.upload(unit_reg_begin);
+  d_chunks.upload(h_chunks);
+  d_fcv.upload(fcv);
+  d_amp.upload(amp);
+  d_cp.upload(cp);
+  d_ca.upload(ca);
+  d_subs.upload(h_subs);
+  d_t2rows.upload(h_t2rows);
+  d_cols.upload(h_cols);
+  d_t1rows.upload(h_t1rows);
+  d_regs.upload(h_regs);
+  HIP_CHECK(hipDeviceSynchronize());
+}
+
+#define LAUNCH(stage, kern, grid, block, lds, ...)                            \
+  do {                                                                        \
+    if ((grid) > 0) {                                                         \
+      if (prof) HIP_CHECK(hipEventRecord(prof_ev[2 * (stage)], stream));      \
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, stream, __VA_ARGS__); \
+      HIP_CHECK(hipGetLastError());                                           \
+      if (prof) {                                                             \
+        HIP_CHECK(hipEventRecord(prof_ev[2 * (stage) + 1], stream));          \
+        prof_used[stage] = 1;                                                 \
+      }                                                                       \
+      ++launches;                                                             \
+    }                                                                         \
+  } while (0)
+
+void Plan::launch_forward() {
+  const float beta = (float)kBeta;
+  if (any_window)
+    LAUNCH(SMILI_S_SCATTER, k_scatter, h_regs.size(), kThreads, 0, d_regs.p, d_units.p, d_pix.p, d_x.p, d_I2d.p);
+  if (npt == 0) return;
+  LAUNCH(SMILI_S_T2_ROWS, k_t2_rows, h_t2rows.size(), kThreads, lds_rows, d_t2rows.p, d_units.p, d_plans.p, d_tw.p,
+         d_corr.p, d_x.p, d_I2d.p, d_gridA.p);
+  LAUNCH(SMILI_S_T2_COLS, k_t2_cols, h_cols.size(), kThreads, lds_cols, d_cols.p, d_units.p, d_plans.p, d_tw.p, d_gridA.p);
+  LAUNCH(SMILI_S_INTERP, k_interp, h_subs.size(), kThreads, lds_tile, d_subs.p, d_units.p, d_gridA.p, d_cell.p, d_ex.p,
+         d_ey.p, d_V.p, d_Vres.p, bin, beta);
+}
+
+void Plan::launch_adjoint() {
+  if (npt == 0) return;
+  const float beta = (float)kBeta;
+  LAUNCH(SMILI_S_SPREAD, k_spread, h_subs.size(), kThreads, lds_tile, d_subs.p, d_units.p, d_cell.p, d_ex.p, d_ey.p,
+         d_Vres.p, d_gridB.p, bin, beta);
+  LAUNCH(SMILI_S_T1_COLS, k_t1_cols, h_cols.size(), kThreads, lds_cols, d_cols.p, d_units.p, d_plans.p, d_tw.p,
+         d_gridB.p, d_gridA.p);
+  LAUNCH(SMILI_S_T1_ROWS, k_t1_rows, h_t1rows.size(), kThreads, lds_rows, d_t1rows.p, d_units.p, d_plans.p, d_tw.p,
+         d_corr.p, d_gridA.p, d_grad2d.p);
+}
+
+void Plan::launch_eval() {
+  launches = 0;
+  launch_forward();
+  if (any_moments) {
+    LAUNCH(SMILI_S_MOMENTS, k_moments, h_regs.size(), kThreads, 0, d_regs.p, d_units.p, d_pix.p, d_x.p, d_mpart.p);
+    LAUNCH(SMILI_S_MOMENTS_REDUCE, k_moments_reduce, units.size(), 64, 0, d_units.p, d_unit_reg_begin.p, d_mpart.p, d_mom.p);
+  }
+  if (npt > 0) {
+    LAUNCH(SMILI_S_CHISQ, k_chisq, h_chunks.size(), kThreads, 0, d_chunks.p, d_fcv.p, d_amp.p, d_cp.p, d_ca.p, d_V.p,
+           d_Vres.p, d_cpart.p);
+    launch_adjoint();
+  }
+  LAUNCH(SMILI_S_REG, k_reg, h_regs.size(), kThreads, 0, d_regs.p, d_units.p, d_pix.p, d_x.p, d_I2d.p, d_grad2d.p,
+         d_mom.p, d_grad.p, d_rpart.p);
+  LAUNCH(SMILI_S_FINALIZE, k_finalize, probs.size(), kThreads, 0, d_probs.p, d_units.p, d_regs.p, d_cpart.p, d_rpart.p,
+         d_mom.p, d_terms.p);
+}
+
+void Plan::eval() {
+  if (tune.use_graph) {
+    if (!graph_ready) {
+      HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      try {
+        launch_eval();
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      HIP_CHECK(hipStreamEndCapture(stream, &graph));
+      HIP_CHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+      graph_ready = true;
+    }
+    HIP_CHECK(hipGraphLaunch(gexec, stream));
+  } else {
+    launch_eval();
+  }
+}
+
+}  // namespace smili
+
+using namespace smili;
+
+struct smili_problem {
+  Plan plan;
+  smili_desc desc;   // geometry copy (pointers not retained)
+};
+
+template <typename F>
+static int guarded(F&& f) {
+  try {
+    f();
+    return SMILI_OK;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return SMILI_ENOMEM;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return SMILI_EINVAL;
+  }
+}
+
+extern "C" {
+
+int smili_abi_version(void) { return SMILI_ABI_VERSION; }
+
+const char* smili_last_error(void) { return g_last_error.c_str(); }
+
+int smili_problem_create(const smili_desc* desc, const smili_tune* tune, smili_problem** out) {
+  return guarded([&] {
+    require(out != nullptr, "out is NULL");
+    *out = nullptr;
+    check_desc_geometry(desc);
+    require(desc->nuv >= 1 && desc->u && desc->v, "nuv must be >= 1 with u and v given");
+    check_data(desc->data, desc->nuv);     // all validation precedes the first HIP call
+    auto* p = new smili_problem();
+    try {
+      p->desc = *desc;
+      std::vector<UnitSpec> us{UnitSpec{desc, 0, 1.0, desc->u, desc->v, desc->nuv}};
+      std::vector<ProbSpec> ps{ProbSpec{&desc->data, 0, 1, desc->nuv}};
+      p->plan.build(us, ps, tune);
+    } catch (...) {
+      delete p;
+      throw;
+    }
+    *out = p;
+  });
+}
+
+void smili_problem_destroy(smili_problem* p) { delete p; }
+
+int smili_cost_grad(smili_problem* p, const double* x, double* cost, double* grad, double* terms) {
+  int nonfinite = 0;
+  int rc = guarded([&] {
+    require(p && x && grad, "NULL argument");
+    Plan& P = p->plan;
+    const size_t n = P.npix_total;
+    HIP_CHECK(hipMemcpyAsync(P.d_x.p, x, n * sizeof(double), hipMemcpyHostToDevice, P.stream));
+    P.eval();
+    double t[SMILI_NTERMS];
+    HIP_CHECK(hipMemcpyAsync(grad, P.d_grad.p, n * sizeof(double), hipMemcpyDeviceToHost, P.stream));
+    HIP_CHECK(hipMemcpyAsync(t, P.d_terms.p, sizeof(t), hipMemcpyDeviceToHost, P.stream));
+    HIP_CHECK(hipStreamSynchronize(P.stream));
+    if (cost) *cost = t[SMILI_T_COST];
+    if (terms) std::memcpy(terms, t, sizeof(t));
+    bool ok = std::isfinite(t[SMILI_T_COST]);
+    for (size_t i = 0; ok && i < n; ++i) ok = std::isfinite(grad[i]);
+    if (!ok) {
+      nonfinite = 1;
+      g_last_error = "cost or gradient is not finite";
+    }
+  });
+  if (rc == SMILI_OK && nonfinite) return SMILI_ENONFINITE;
+  return rc;
+}
+
+int smili_cost_grad_device(smili_problem* p, const double* x_dev, double* grad_dev, double* terms_dev) {
+  return guarded([&] {
+    require(p && x_dev && grad_dev, "NULL argument");
+    Plan& P = p->plan;
+    const size_t n = P.npix_total;
+    if (x_dev != P.d_x.p)
+      HIP_CHECK(hipMemcpyAsync(P.d_x.p, x_dev, n * sizeof(double), hipMemcpyDeviceToDevice, P.stream));
+    P.eval();
+    if (grad_dev != P.d_grad.p)
+      HIP_CHECK(hipMemcpyAsync(grad_dev, P.d_grad.p, n * sizeof(double), hipMemcpyDeviceToDevice, P.stream));
+    if (terms_dev && terms_dev != P.d_terms.p)
+      HIP_CHECK(hipMemcpyAsync(terms_dev, P.d_terms.p, SMILI_NTERMS * sizeof(double),
+                               hipMemcpyDeviceToDevice, P.stream));
+  });
+}
+
+int smili_synchronize(smili_problem* p) {
+  return guarded([&] {
+    require(p != nullptr, "NULL problem");
+    HIP_CHECK(hipStreamSynchronize(p->plan.stream));
+  });
+}
+
+void* smili_problem_stream(smili_problem* p) { return p ? (void*)p->plan.stream : nullptr; }
+
+int smili_model_vis(smili_problem* p, const double* x, double* vre, double* vim) {
+  return guarded([&] {
+    require(p && x && vre && vim, "NULL argument");
+    Plan& P = p->plan;
+    HIP_CHECK(hipMemcpyAsync(P.d_x.p, x, P.npix_total * sizeof(double), hipMemcpyHostToDevice, P.stream));
+    P.launch_forward();
+    std::vector<float2> V(P.npt);
+    HIP_CHECK(hipMemcpyAsync(V.data(), P.d_V.p, P.npt * sizeof(float2), hipMemcpyDeviceToHost, P.stream));
+    HIP_CHECK(hipStreamSynchronize(P.stream));
+    const auto& np = P.newpos[0];
+    for (size_t j = 0; j < np.size(); ++j) {
+      vre[j] = V[np[j]].x;
+      vim[j] = V[np[j]].y;
+    }
+  });
+}
+
+int smili_adjoint_resid(smili_problem* p, const double* vres_re, const double* vres_im, double* out) {
+  return guarded([&] {
+    require(p && vres_re && vres_im && out, "NULL argument");
+    Plan& P = p->plan;
+    const auto& np = P.newpos[0];
+    std::vector<double2> R(P.npt);
+    for (size_t j = 0; j < np.size(); ++j) R[np[j]] = make_double2(vres_re[j], vres_im[j]);
+    HIP_CHECK(hipMemcpyAsync(P.d_Vres.p, R.data(), P.npt * sizeof(double2), hipMemcpyHostToDevice, P.stream));
+    P.launch_adjoint();
+    const UnitDev& U = P.units[0];
+    std::vector<double> g2((size_t)U.nx * U.ny);
+    HIP_CHECK(hipMemcpyAsync(g2.data(), P.d_grad2d.p + U.img_off, g2.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, P.stream));
+    HIP_CHECK(hipStreamSynchronize(P.stream));
+    for (int i = 0; i < U.npix; ++i) out[i] = g2[P.winpix[0][i]];
+  });
+}
+
+int smili_problem_profile(smili_problem* p, int32_t reps, double* ms) {
+  return guarded([&] {
+    require(p && ms && reps > 0, "invalid argument");
+    Plan& P = p->plan;
+    for (auto& e : P.prof_ev)
+      if (!e) HIP_CHECK(hipEventCreate(&e));
+    double acc[SMILI_NSTAGES] = {};
+    P.prof = true;
+    try {
+      for (int r = 0; r < reps; ++r) {
+        std::fill(P.prof_used, P.prof_used + SMILI_NSTAGES, 0);
+        P.launch_eval();
+        HIP_CHECK(hipStreamSynchronize(P.stream));
+        for (int s = 0; s < SMILI_NSTAGES; ++s) {
+          if (!P.prof_used[s]) continue;
+          float t = 0.f;
+          HIP_CHECK(hipEventElapsedTime(&t, P.prof_ev[2 * s], P.prof_ev[2 * s + 1]));
+          acc[s] += t;
+        }
+      }
+    } catch (...) {
+      P.prof = false;
+      throw;
+    }
+    P.prof = false;
+    for (int s = 0; s < SMILI_NSTAGES; ++s) ms[s] = acc[s] / reps;
+  });
+}
+
+int smili_problem_info(smili_problem* p, int64_t* info) {
+  return guarded([&] {
+    require(p && info, "NULL argument");
+    Plan& P = p->plan;
+    info[0] = P.units[0].nf1;
+    info[1] = P.units[0].nf2;
+    info[2] = (int64_t)P.npt;
+    info[3] = (int64_t)P.h_subs.size();
+    info[4] = P.bin;
+    info[5] = (int64_t)P.h_chunks.size();
+    info[6] = P.launches;
+    info[7] = P.graph_ready ? 1 : 0;
+  });
+}
+
+}  // extern "C"

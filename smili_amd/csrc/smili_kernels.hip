@@ -1,0 +1,817 @@
+// gfx950 kernels of the RML imaging cost: ES-kernel NUFFT (type 2 forward,
+// type 1 adjoint) on a 2x oversampled grid with LDS-resident Stockham FFTs,
+// fused chi-square terms with FP64 accumulation, and the pixel regularizers.
+//
+// Reference algorithm: fftim2d.calc_cost (src/fortlib/fftim2d.f90:387-607),
+// fftlib.calc_chisq (src/fortlib/fftlib.f90:250-655), image.calc_cost_reg
+// (src/fortlib/image.f90:12-200, 245-754, 760-1158) and the FINUFFT calls of
+// src/wrapfinufft/wrapfinufft.f90:94-206 (both transforms use the +i sign).
+#include "smili_kernels.h"
+
+namespace smili {
+
+// ---------------------------------------------------------------------------
+// complex helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 mul_i(float2 a) { return make_float2(-a.y, a.x); }
+
+// ---------------------------------------------------------------------------
+// small DFTs with the +i sign: y_m = sum_q v_q exp(+2 pi i q m / r)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void dft2(float2* v) {
+  float2 a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+
+__device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+  float2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
+  float2 t2 = cadd(a1, a3), t3 = mul_i(csub(a1, a3));
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  float2 y1 = cadd(t1, t3), y3 = csub(t1, t3);
+  a1 = y1;
+  a3 = y3;
+}
+
+__device__ __forceinline__ void dft3(float2* v) {
+  const float s3 = 0.86602540378443864676f;
+  float2 t = cadd(v[1], v[2]);
+  float2 m = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
+  float2 d = mul_i(cscale(csub(v[1], v[2]), s3));
+  v[0] = cadd(v[0], t);
+  v[1] = cadd(m, d);
+  v[2] = csub(m, d);
+}
+
+__device__ __forceinline__ void dft5(float2* v) {
+  const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+  const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+  float2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+  float2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+  float2 a0 = v[0];
+  float2 b1 = make_float2(a0.x + c1 * t1.x + c2 * t2.x, a0.y + c1 * t1.y + c2 * t2.y);
+  float2 b2 = make_float2(a0.x + c2 * t1.x + c1 * t2.x, a0.y + c2 * t1.y + c1 * t2.y);
+  float2 d1 = mul_i(make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y));
+  float2 d2 = mul_i(make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y));
+  v[0] = cadd(a0, cadd(t1, t2));
+  v[1] = cadd(b1, d1);
+  v[4] = csub(b1, d1);
+  v[2] = cadd(b2, d2);
+  v[3] = csub(b2, d2);
+}
+
+__device__ __forceinline__ void dft8(float2* v) {
+  const float s = 0.70710678118654752440f;
+  float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  dft4(e0, e1, e2, e3);
+  dft4(o0, o1, o2, o3);
+  // twiddles W8^m = exp(+i pi m / 4)
+  float2 w1 = make_float2(s * (o1.x - o1.y), s * (o1.x + o1.y));
+  float2 w2 = mul_i(o2);
+  float2 w3 = make_float2(-s * (o3.x + o3.y), s * (o3.x - o3.y));
+  v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
+  v[1] = cadd(e1, w1); v[5] = csub(e1, w1);
+  v[2] = cadd(e2, w2); v[6] = csub(e2, w2);
+  v[3] = cadd(e3, w3); v[7] = csub(e3, w3);
+}
+
+// ---------------------------------------------------------------------------
+// LDS Stockham FFT (autosort, +i sign) of nb transforms of length n stored
+// at a + b*stride.  Uses scratch b of the same shape.  Returns the buffer
+// that holds the result.  All threads of the block must call it.
+// ---------------------------------------------------------------------------
+__device__ float2* lds_fft(float2* a, float2* s, int nb, int stride, const FftPlanDev& pl,
+                           const float2* __restrict__ tw) {
+  const int n = pl.n;
+  int Ns = 1;
+  for (int st = 0; st < pl.nstages; ++st) {
+    const int r = pl.radix[st];
+    const int nbf = n / r;
+    const int total = nbf * nb;
+    const int twstep = n / (Ns * r);
+    for (int t = threadIdx.x; t < total; t += blockDim.x) {
+      const int b = t / nbf;
+      const int i = t - b * nbf;
+      const int k = i % Ns;
+      const float2* in = a + b * stride;
+      float2* out = s + b * stride;
+      float2 v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (q < r) v[q] = in[i + q * nbf];
+      if (Ns > 1) {
+        const int step = k * twstep;
+#pragma unroll
+        for (int q = 1; q < 8; ++q)
+          if (q < r) v[q] = cmul(v[q], tw[pl.tw_off + q * step]);
+      }
+      switch (r) {
+        case 2: dft2(v); break;
+        case 3: dft3(v); break;
+        case 4: dft4(v[0], v[1], v[2], v[3]); break;
+        case 5: dft5(v); break;
+        default: dft8(v); break;
+      }
+      const int base = (i / Ns) * Ns * r + k;
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (q < r) out[base + q * Ns] = v[q];
+    }
+    __syncthreads();
+    float2* tmp = a; a = s; s = tmp;
+    Ns *= r;
+  }
+  return a;
+}
+
+// ---------------------------------------------------------------------------
+// ES kernel weights.  e = d - w/2 in (-1, 0] where d is the distance from the
+// point to its first tap in cells; z_q = (d - q) * 2 / w; phi(z) =
+// exp(-beta z^2 / (1 + sqrt(1 - z^2))) (cancellation-free form of
+// exp(beta (sqrt(1 - z^2) - 1))).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void es_weights(float e, float beta, float* wt) {
+  const float inv_half = 2.0f / kW;
+#pragma unroll
+  for (int q = 0; q < kW; ++q) {
+    float z = e * inv_half + (1.0f - q * inv_half);
+    float z2 = fminf(z * z, 1.0f);
+    float sq = sqrtf(1.0f - z2);
+    wt[q] = __expf(-beta * z2 / (1.0f + sq));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Type-2 step 1: image rows -> deconvolve, zero-pad, FFT along x.
+// One work item = `count` consecutive image rows of one unit.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads)
+k_t2_rows(const RowItem* __restrict__ items, const UnitDev* __restrict__ units,
+          const FftPlanDev* __restrict__ plans, const float2* __restrict__ tw,
+          const float* __restrict__ corr, const double* __restrict__ x,
+          const double* __restrict__ I2dbuf, float2* __restrict__ gridA) {
+  extern __shared__ float2 sm[];
+  const RowItem it = items[blockIdx.x];
+  const UnitDev U = units[it.unit];
+  const FftPlanDev pl = plans[U.fft1];
+  const int n = U.nf1, nb = it.count;
+  float2* A = sm;
+  float2* B = sm + nb * n;
+  const double* img = U.identity ? (x + U.x_off) : (I2dbuf + U.img_off);
+  const float* cx = corr + U.cx_off;
+  const float* cy = corr + U.cy_off;
+  const int hx = U.nx / 2;
+  for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
+    const int b = t / n, c = t - b * n;
+    // grid column c holds mode k1 = c (c < nf1/2) or c - nf1
+    const int k1 = (c < n - hx) ? c : c - n;
+    float2 val = make_float2(0.f, 0.f);
+    if (k1 >= -hx && k1 < U.nx - hx) {
+      const int ix = k1 + hx;
+      const int iy = it.first + b;
+      val.x = (float)(img[(long long)iy * U.nx + ix] * (double)cx[ix] * (double)cy[iy]);
+    }
+    A[t] = val;
+  }
+  __syncthreads();
+  float2* R = lds_fft(A, B, nb, n, pl, tw);
+  const int hy = U.ny / 2;
+  for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
+    const int b = t / n, c = t - b * n;
+    int row = it.first + b - hy;
+    if (row < 0) row += U.nf2;
+    gridA[U.grid_off + (long long)row * n + c] = R[t];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Type-2 step 2: FFT along y of `count` grid columns; only the ny rows that
+// hold image modes are read (the others are zero by construction).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads)
+k_t2_cols(const RowItem* __restrict__ items, const UnitDev* __restrict__ units,
+          const FftPlanDev* __restrict__ plans, const float2* __restrict__ tw,
+          float2* __restrict__ gridA) {
+  extern __shared__ float2 sm[];
+  const RowItem it = items[blockIdx.x];
+  const UnitDev U = units[it.unit];
+  const FftPlanDev pl = plans[U.fft2];
+  const int n = U.nf2, nb = it.count, stride = n + 1;
+  float2* A = sm;
+  float2* B = sm + nb * stride;
+  const int hy = U.ny / 2;
+  float2* g = gridA + U.grid_off;
+  for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
+    const int r = t / nb, b = t - r * nb;
+    const int k2 = (r < n - hy) ? r : r - n;
+    float2 val = make_float2(0.f, 0.f);
+    if (k2 >= -hy && k2 < U.ny - hy) val = g[(long long)r * U.nf1 + it.first + b];
+    A[b * stride + r] = val;
+  }
+  __syncthreads();
+  float2* R = lds_fft(A, B, nb, stride, pl, tw);
+  for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
+    const int r = t / nb, b = t - r * nb;
+    g[(long long)r * U.nf1 + it.first + b] = R[b * stride + r];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Type-2 step 3: interpolation from an LDS-staged tile of the grid.  One work
+// item = up to sub_points points of one bin.  Also clears the residual
+// accumulators of its points for the chi-square kernel.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads)
+k_interp(const SubProb* __restrict__ sps, const UnitDev* __restrict__ units,
+         const float2* __restrict__ gridA, const int* __restrict__ pt_cell,
+         const float* __restrict__ pt_ex, const float* __restrict__ pt_ey,
+         float2* __restrict__ V, double2* __restrict__ Vres, int bin, float beta) {
+  extern __shared__ float2 tile[];
+  const SubProb sp = sps[blockIdx.x];
+  const UnitDev U = units[sp.unit];
+  const int T = bin + kW - 1;
+  const float2* g = gridA + U.grid_off;
+  const int gx0 = sp.bin_x * bin, gy0 = sp.bin_y * bin;
+  for (int c = threadIdx.x; c < T * T; c += blockDim.x) {
+    const int ty = c / T, tx = c - ty * T;
+    const int gy = (gy0 + ty) % U.nf2;
+    const int gx = (gx0 + tx) % U.nf1;
+    tile[c] = g[(long long)gy * U.nf1 + gx];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < sp.count; i += blockDim.x) {
+    const int j = sp.start + i;
+    const int cell = pt_cell[j];
+    const int lx = (cell & 0xffff) - gx0;
+    const int ly = (cell >> 16) - gy0;
+    float wx[kW], wy[kW];
+    es_weights(pt_ex[j], beta, wx);
+    es_weights(pt_ey[j], beta, wy);
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int qy = 0; qy < kW; ++qy) {
+      const float2* row = tile + (ly + qy) * T + lx;
+      float2 ra = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int qx = 0; qx < kW; ++qx) {
+        const float2 gv = row[qx];
+        ra.x = fmaf(wx[qx], gv.x, ra.x);
+        ra.y = fmaf(wx[qx], gv.y, ra.y);
+      }
+      acc.x = fmaf(wy[qy], ra.x, acc.x);
+      acc.y = fmaf(wy[qy], ra.y, acc.y);
+    }
+    V[j] = acc;
+    Vres[j] = make_double2(0.0, 0.0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// block reduction of doubles (deterministic: fixed shuffle tree + fixed
+// wave order)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+template <int NV>
+__device__ __forceinline__ void block_sum(double* v, double* red /* NV * (kThreads/64) */) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = wave_sum(v[k]);
+    if (lane == 0) red[k * (kThreads / 64) + wid] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0.0;
+      for (int w = 0; w < kThreads / 64; ++w) s += red[k * (kThreads / 64) + w];
+      v[k] = s;
+    }
+  }
+}
+
+__device__ __forceinline__ void vres_add(double2* Vres, int j, double re, double im) {
+  unsafeAtomicAdd(&Vres[j].x, re);
+  unsafeAtomicAdd(&Vres[j].y, im);
+}
+
+// ---------------------------------------------------------------------------
+// chi-square terms and residual vectors (fftlib.f90:384-655).  One work item
+// = `count` data of one type of one problem.  The chi-square of the item is
+// written to partial[blockIdx.x] (FP64); residual vectors are accumulated
+// with FP64 atomics.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads)
+k_chisq(const Chunk* __restrict__ chunks, const FcvItem* __restrict__ fcv,
+        const AmpItem* __restrict__ amp, const CpItem* __restrict__ cp,
+        const CaItem* __restrict__ ca, const float2* __restrict__ V,
+        double2* __restrict__ Vres, double* __restrict__ partial) {
+  __shared__ double red[kThreads / 64];
+  const Chunk ch = chunks[blockIdx.x];
+  double acc = 0.0;
+  if (ch.type == DT_FCV) {
+    for (int i = threadIdx.x; i < ch.count; i += blockDim.x) {
+      const FcvItem d = fcv[ch.start + i];
+      const int j = abs(d.idx) - 1;
+      const double s = d.idx > 0 ? 1.0 : -1.0;
+      const float2 m = V[j];
+      const double rr = d.vr - (double)m.x;
+      const double ri = d.vi - s * (double)m.y;       // conj for negative index
+      acc += (rr * rr + ri * ri) * d.w;
+      const double f = -2.0 * d.w;
+      vres_add(Vres, j, f * rr, f * ri * s);
+    }
+  } else if (ch.type == DT_AMP) {
+    for (int i = threadIdx.x; i < ch.count; i += blockDim.x) {
+      const AmpItem d = amp[ch.start + i];
+      const int j = abs(d.idx) - 1;
+      const float2 m = V[j];
+      const double vr = m.x, vi = m.y;
+      const double mod = sqrt(vr * vr + vi * vi);
+      const double r = d.amp - mod;
+      acc += r * r * d.w;
+      const double f = -2.0 * r * d.w / mod;
+      vres_add(Vres, j, f * vr, f * vi);
+    }
+  } else if (ch.type == DT_CP) {
+    for (int i = threadIdx.x; i < ch.count; i += blockDim.x) {
+      const CpItem d = cp[ch.start + i];
+      double vr[3], vi[3], sg[3], model = 0.0;
+      int jj[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        jj[k] = abs(d.idx[k]) - 1;
+        sg[k] = d.idx[k] > 0 ? 1.0 : -1.0;
+        const float2 m = V[jj[k]];
+        vr[k] = m.x;
+        vi[k] = m.y;
+        model += atan2(vi[k], vr[k]) * sg[k];
+      }
+      const double r = d.cp - model;
+      double sr, cr;
+      sincos(r, &sr, &cr);
+      acc += 2.0 * (1.0 - cr) * d.w;
+      const double f = -2.0 * sr * d.w;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double a = f / (vr[k] * vr[k] + vi[k] * vi[k]) * sg[k];
+        vres_add(Vres, jj[k], -a * vi[k], a * vr[k]);
+      }
+    }
+  } else {
+    for (int i = threadIdx.x; i < ch.count; i += blockDim.x) {
+      const CaItem d = ca[ch.start + i];
+      double vr[4], vi[4], a2[4];
+      int jj[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        jj[k] = abs(d.idx[k]) - 1;
+        const float2 m = V[jj[k]];
+        vr[k] = m.x;
+        vi[k] = m.y;
+        a2[k] = vr[k] * vr[k] + vi[k] * vi[k];
+      }
+      // log|V| = 0.5 log|V|^2
+      const double model = 0.5 * (log(a2[0]) + log(a2[1]) - log(a2[2]) - log(a2[3]));
+      const double r = d.ca - model;
+      acc += r * r * d.w;
+      const double f = -2.0 * r * d.w;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double a = (k < 2 ? f : -f) / a2[k];
+        vres_add(Vres, jj[k], a * vr[k], a * vi[k]);
+      }
+    }
+  }
+  double v[1] = {acc};
+  block_sum<1>(v, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v[0];
+}
+
+// ---------------------------------------------------------------------------
+// Type-1 step 1: spread residuals c = Vres_re - i Vres_im onto the grid.
+// Subproblem tile accumulated in LDS with LDS atomics, then added to the
+// global grid with float atomics.  The grid is zero on entry (k_t1_cols
+// clears what it consumes).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads)
+k_spread(const SubProb* __restrict__ sps, const UnitDev* __restrict__ units,
+         const int* __restrict__ pt_cell, const float* __restrict__ pt_ex,
+         const float* __restrict__ pt_ey, const double2* __restrict__ Vres,
+         float2* __restrict__ gridB, int bin, float beta) {
+  extern __shared__ float2 tile[];
+  const SubProb sp = sps[blockIdx.x];
+  const UnitDev U = units[sp.unit];
+  const int T = bin + kW - 1;
+  for (int c = threadIdx.x; c < T * T; c += blockDim.x) tile[c] = make_float2(0.f, 0.f);
+  __syncthreads();
+  const int gx0 = sp.bin_x * bin, gy0 = sp.bin_y * bin;
+  for (int i = threadIdx.x; i < sp.count; i += blockDim.x) {
+    const int j = sp.start + i;
+    const int cell = pt_cell[j];
+    const int lx = (cell & 0xffff) - gx0;
+    const int ly = (cell >> 16) - gy0;
+    float wx[kW], wy[kW];
+    es_weights(pt_ex[j], beta, wx);
+    es_weights(pt_ey[j], beta, wy);
+    const double2 r = Vres[j];
+    const float cr = (float)r.x, ci = (float)(-r.y);
+#pragma unroll
+    for (int qy = 0; qy < kW; ++qy) {
+      const float ar = wy[qy] * cr, ai = wy[qy] * ci;
+      float2* row = tile + (ly + qy) * T + lx;
+#pragma unroll
+      for (int qx = 0; qx < kW; ++qx) {
+        atomicAdd(&row[qx].x, wx[qx] * ar);
+        atomicAdd(&row[qx].y, wx[qx] * ai);
+      }
+    }
+  }
+  __syncthreads();
+  float2* g = gridB + U.grid_off;
+  for (int c = threadIdx.x; c < T * T; c += blockDim.x) {
+    const float2 val = tile[c];
+    if (val.x != 0.f || val.y != 0.f) {
+      const int ty = c / T, tx = c - ty * T;
+      const int gy = (gy0 + ty) % U.nf2;
+      const int gx = (gx0 + tx) % U.nf1;
+      float2* dst = g + (long long)gy * U.nf1 + gx;
+      unsafeAtomicAdd(&dst->x, val.x);
+      unsafeAtomicAdd(&dst->y, val.y);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Type-1 step 2: FFT along y of `count` full grid columns; keeps the ny
+// output rows that are image modes (compact, into H = gridA) and clears the
+// consumed columns of gridB for the next evaluation.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads)
+k_t1_cols(const RowItem* __restrict__ items, const UnitDev* __restrict__ units,
+          const FftPlanDev* __restrict__ plans, const float2* __restrict__ tw,
+          float2* __restrict__ gridB, float2* __restrict__ H) {
+  extern __shared__ float2 sm[];
+  const RowItem it = items[blockIdx.x];
+  const UnitDev U = units[it.unit];
+  const FftPlanDev pl = plans[U.fft2];
+  const int n = U.nf2, nb = it.count, stride = n + 1;
+  float2* A = sm;
+  float2* B = sm + nb * stride;
+  float2* g = gridB + U.grid_off;
+  for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
+    const int r = t / nb, b = t - r * nb;
+    float2* src = g + (long long)r * U.nf1 + it.first + b;
+    A[b * stride + r] = *src;
+    *src = make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  float2* R = lds_fft(A, B, nb, stride, pl, tw);
+  const int hy = U.ny / 2;
+  float2* h = H + U.grid_off;
+  for (int t = threadIdx.x; t < nb * U.ny; t += blockDim.x) {
+    const int iy = t / nb, b = t - iy * nb;
+    int r = iy - hy;
+    if (r < 0) r += n;
+    h[(long long)iy * U.nf1 + it.first + b] = R[b * stride + r];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Type-1 step 3: FFT along x of `count` compact rows, keep the nx modes,
+// deconvolve, real part -> grad2d (FP64).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads)
+k_t1_rows(const RowItem* __restrict__ items, const UnitDev* __restrict__ units,
+          const FftPlanDev* __restrict__ plans, const float2* __restrict__ tw,
+          const float* __restrict__ corr, const float2* __restrict__ H,
+          double* __restrict__ grad2d) {
+  extern __shared__ float2 sm[];
+  const RowItem it = items[blockIdx.x];
+  const UnitDev U = units[it.unit];
+  const FftPlanDev pl = plans[U.fft1];
+  const int n = U.nf1, nb = it.count;
+  float2* A = sm;
+  float2* B = sm + nb * n;
+  const float2* h = H + U.grid_off + (long long)it.first * n;
+  for (int t = threadIdx.x; t < nb * n; t += blockDim.x) A[t] = h[t];
+  __syncthreads();
+  float2* R = lds_fft(A, B, nb, n, pl, tw);
+  const float* cx = corr + U.cx_off;
+  const float* cy = corr + U.cy_off;
+  const int hx = U.nx / 2;
+  for (int t = threadIdx.x; t < nb * U.nx; t += blockDim.x) {
+    const int b = t / U.nx, ix = t - b * U.nx;
+    const int iy = it.first + b;
+    int c = ix - hx;
+    if (c < 0) c += n;
+    grad2d[U.img_off + (long long)iy * U.nx + ix] =
+        (double)R[b * n + c].x * (double)cx[ix] * (double)cy[iy];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// window pixels -> 2-D image (I1d_I2d_fwd, image.f90:1167-1186).  Pixels
+// outside the window stay zero from allocation.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads)
+k_scatter(const RegItem* __restrict__ items, const UnitDev* __restrict__ units,
+          const int* __restrict__ pix, const double* __restrict__ x,
+          double* __restrict__ I2d) {
+  const RegItem it = items[blockIdx.x];
+  const UnitDev U = units[it.unit];
+  if (U.identity) return;
+  for (int i = threadIdx.x; i < it.count; i += blockDim.x) {
+    const int k = it.start + i;
+    I2d[U.img_off + pix[U.pix_off + k]] = x[U.x_off + k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pixel regularizers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double smabs(double v) { return sqrt(v * v + kZeroEps); }
+
+// moments for tfd / cen / sm (image.f90:774-1138)
+__global__ void __launch_bounds__(kThreads)
+k_moments(const RegItem* __restrict__ items, const UnitDev* __restrict__ units,
+          const int* __restrict__ pix, const double* __restrict__ x,
+          double* __restrict__ mpart) {
+  __shared__ double red[kMomTerms * (kThreads / 64)];
+  const RegItem it = items[blockIdx.x];
+  const UnitDev U = units[it.unit];
+  double v[kMomTerms];
+#pragma unroll
+  for (int k = 0; k < kMomTerms; ++k) v[k] = 0.0;
+  if (U.need_moments) {
+    const bool alpha1 = fabs(U.cen_alpha - 1.0) < kZeroEps;
+    for (int i = threadIdx.x; i < it.count; i += blockDim.x) {
+      const int k = it.start + i;
+      const int p = U.identity ? k : pix[U.pix_off + k];
+      const double xi = (double)(p % U.nx + 1), yi = (double)(p / U.nx + 1);
+      const double I = x[U.x_off + k];
+      v[0] += I;
+      v[1] += fabs(I);
+      v[2] += xi * I;
+      v[3] += yi * I;
+      v[4] += xi * xi * I;
+      v[5] += yi * yi * I;
+      v[6] += xi * yi * I;
+      const double Ip = alpha1 ? smabs(I) : pow(smabs(I), U.cen_alpha);
+      v[7] += Ip;
+      v[8] += Ip * (xi - U.nxref);
+      v[9] += Ip * (yi - U.nyref);
+    }
+  }
+  block_sum<kMomTerms>(v, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < kMomTerms; ++k) mpart[(long long)blockIdx.x * kMomTerms + k] = v[k];
+}
+
+// one block per unit: sum its items' moments
+__global__ void k_moments_reduce(const UnitDev* __restrict__ units, const int* __restrict__ unit_reg_begin,
+                                 const double* __restrict__ mpart, UnitMoments* __restrict__ mom) {
+  const int u = blockIdx.x;
+  const int b = unit_reg_begin[u], e = unit_reg_begin[u + 1];
+  if (threadIdx.x < kMomTerms) {
+    double s = 0.0;
+    for (int i = b; i < e; ++i) s += mpart[(long long)i * kMomTerms + threadIdx.x];
+    reinterpret_cast<double*>(&mom[u])[threadIdx.x] = s;
+  }
+}
+
+struct SmState {
+  double Isum, xc, yc, Sg[3], Sp[3], mom[2];
+};
+
+__device__ __forceinline__ SmState sm_state(const UnitDev& U, const UnitMoments& M) {
+  SmState s;
+  s.Isum = M.A;
+  s.xc = M.Sx / M.A;
+  s.yc = M.Sy / M.A;
+  s.Sg[0] = (M.Sxx - 2.0 * s.xc * M.Sx + s.xc * s.xc * M.S) / s.Isum;
+  s.Sg[1] = (M.Syy - 2.0 * s.yc * M.Sy + s.yc * s.yc * M.S) / s.Isum;
+  s.Sg[2] = (M.Sxy - s.xc * M.Sy - s.yc * M.Sx + s.xc * s.yc * M.S) / s.Isum;
+  s.mom[0] = M.Sx - s.xc * M.S;
+  s.mom[1] = M.Sy - s.yc * M.S;
+  double sp, cp;
+  sincos(U.sm_phi, &sp, &cp);
+  s.Sp[0] = U.sm_min * cp * cp + U.sm_maj * sp * sp;
+  s.Sp[1] = U.sm_min * sp * sp + U.sm_maj * cp * cp;
+  s.Sp[2] = (U.sm_maj - U.sm_min) * sp * cp;
+  return s;
+}
+
+__device__ __forceinline__ double img_at(const double* I2d, int nx, int ny, int ix, int iy) {
+  return (ix < 0 || iy < 0 || ix >= nx || iy >= ny) ? 0.0 : I2d[(long long)iy * nx + ix];
+}
+
+// tv_e (image.f90:477-508) at (ix, iy), 0-based
+__device__ __forceinline__ void fwd_diff(const double* I, int nx, int ny, int ix, int iy, double& dx, double& dy) {
+  const double c = I[(long long)iy * nx + ix];
+  dx = (ix + 1 < nx) ? I[(long long)iy * nx + ix + 1] - c : 0.0;
+  dy = (iy + 1 < ny) ? I[(long long)(iy + 1) * nx + ix] - c : 0.0;
+}
+
+// tv_grade with the reference's reset quirk (image.f90:512-612)
+__device__ __forceinline__ double tv_grade(const double* I, int nx, int ny, int ix, int iy) {
+  double g = 0.0, dx, dy;
+  fwd_diff(I, nx, ny, ix, iy, dx, dy);
+  double e = sqrt(dx * dx + dy * dy);
+  if (e > kZeroEps) g -= (dx + dy) / e;
+  if (ix > 0) {
+    dx = I[(long long)iy * nx + ix] - I[(long long)iy * nx + ix - 1];
+    dy = (iy + 1 < ny) ? I[(long long)(iy + 1) * nx + ix - 1] - I[(long long)iy * nx + ix - 1] : 0.0;
+    e = sqrt(dx * dx + dy * dy);
+    if (e > kZeroEps) g += dx / e; else g = 0.0;
+  }
+  if (iy > 0) {
+    dy = I[(long long)iy * nx + ix] - I[(long long)(iy - 1) * nx + ix];
+    dx = (ix + 1 < nx) ? I[(long long)(iy - 1) * nx + ix + 1] - I[(long long)(iy - 1) * nx + ix] : 0.0;
+    e = sqrt(dx * dx + dy * dy);
+    if (e > kZeroEps) g += dy / e; else g = 0.0;
+  }
+  return g;
+}
+
+// tsv_grade (image.f90:713-754)
+__device__ __forceinline__ double tsv_grade(const double* I, int nx, int ny, int ix, int iy) {
+  const double c = I[(long long)iy * nx + ix];
+  double g = 0.0;
+  if (ix + 1 < nx) g -= 2.0 * (I[(long long)iy * nx + ix + 1] - c);
+  if (iy + 1 < ny) g -= 2.0 * (I[(long long)(iy + 1) * nx + ix] - c);
+  if (ix > 0) g += 2.0 * (c - I[(long long)iy * nx + ix - 1]);
+  if (iy > 0) g += 2.0 * (c - I[(long long)(iy - 1) * nx + ix]);
+  return g;
+}
+
+// Per-pixel regularizers + final gradient assembly:
+// grad = gradchisq + reg_scale * gradreg (fftim2d.f90:553-554).
+__global__ void __launch_bounds__(kThreads)
+k_reg(const RegItem* __restrict__ items, const UnitDev* __restrict__ units,
+      const int* __restrict__ pix, const double* __restrict__ x,
+      const double* __restrict__ I2dbuf, const double* __restrict__ grad2d,
+      const UnitMoments* __restrict__ mom, double* __restrict__ grad,
+      double* __restrict__ rpart) {
+  __shared__ double red[kRegTerms * (kThreads / 64)];
+  const RegItem it = items[blockIdx.x];
+  const UnitDev U = units[it.unit];
+  const double* I2d = U.identity ? (x + U.x_off) : (I2dbuf + U.img_off);
+  double v[kRegTerms];
+#pragma unroll
+  for (int k = 0; k < kRegTerms; ++k) v[k] = 0.0;
+  // global quantities of tfd / cen / sm
+  double tfd_g = 0.0, cen_sx = 0.0, cen_sy = 0.0, cen_sIp = 1.0, cen_cost = 1.0;
+  SmState S;
+  if (U.need_moments) {
+    const UnitMoments M = mom[it.unit];
+    if (U.tfd_l > 0) tfd_g = U.tfd_l * 2.0 * (M.S - U.tfd_tgtfd);
+    if (U.cen_l > 0) {
+      cen_sIp = kZeroEps + M.P;
+      cen_sx = M.Px / cen_sIp;
+      cen_sy = M.Py / cen_sIp;
+      cen_cost = sqrt(cen_sx * cen_sx + cen_sy * cen_sy + kZeroEps);
+    }
+    if (U.sm_l > 0) S = sm_state(U, M);
+  }
+  const bool alpha1 = fabs(U.cen_alpha - 1.0) < kZeroEps;
+  for (int i = threadIdx.x; i < it.count; i += blockDim.x) {
+    const int k = it.start + i;
+    const int p = U.identity ? k : pix[U.pix_off + k];
+    const int ix = p % U.nx, iy = p / U.nx;
+    const double I = x[U.x_off + k];
+    double g = 0.0;
+    if (U.tfd_l > 0) g += tfd_g;
+    if (U.cen_l > 0) {
+      const double dix = (ix + 1) - U.nxref, diy = (iy + 1) - U.nyref;
+      const double sa = smabs(I);
+      const double gIp = alpha1 ? I / sa : I / sa * U.cen_alpha * pow(sa, U.cen_alpha - 1.0);
+      g += U.cen_l * (cen_sx * (dix - cen_sx) + cen_sy * (diy - cen_sy)) / cen_sIp / cen_cost * gIp;
+    }
+    if (U.l1_l > 0) {
+      const double w = U.l1_l * U.l1_w[k];
+      const double sa = smabs(I);
+      v[0] += w * sa;
+      g += w * (I / sa);
+    }
+    if (U.kl_l > 0) {
+      const double P = U.kl_w[k];
+      const double sa = smabs(I);
+      const double lg = log(sa / P);
+      v[3] += U.kl_l * (sa * lg + P / 2.71828182845904523536);
+      g += U.kl_l * (I / sa) * (lg + 1.0);
+    }
+    if (U.gs_l > 0) {
+      const double P = U.gs_w[k];
+      const double sa = smabs(I);
+      const double lg = log(sa / P);
+      v[4] += U.gs_l * (sa * (lg - 1.0) + P);
+      g += U.gs_l * (I / sa) * lg;
+    }
+    if (U.tv_l > 0) {
+      const double w = U.tv_l * U.tv_w[k];
+      double dx, dy;
+      fwd_diff(I2d, U.nx, U.ny, ix, iy, dx, dy);
+      v[1] += w * sqrt(dx * dx + dy * dy);
+      g += w * tv_grade(I2d, U.nx, U.ny, ix, iy);
+    }
+    if (U.tsv_l > 0) {
+      const double w = U.tsv_l * U.tsv_w[k];
+      double dx, dy;
+      fwd_diff(I2d, U.nx, U.ny, ix, iy, dx, dy);
+      v[2] += w * (dx * dx + dy * dy);
+      g += w * tsv_grade(I2d, U.nx, U.ny, ix, iy);
+    }
+    if (U.sm_l > 0) {
+      const double xi = ix + 1, yi = iy + 1;
+      const double xg = (xi - S.xc) / S.Isum, yg = (yi - S.yc) / S.Isum;
+      const double g1 = (((xi - S.xc) * (xi - S.xc) - 2.0 * xg * S.mom[0]) - S.Sg[0]) / S.Isum;
+      const double g2 = (((yi - S.yc) * (yi - S.yc) - 2.0 * yg * S.mom[1]) - S.Sg[1]) / S.Isum;
+      const double g3 = ((xi - S.xc) * (yi - S.yc) - xg * S.mom[1] - yg * S.mom[0] - S.Sg[2]) / S.Isum;
+      g += U.sm_l * (2.0 * (S.Sg[0] - S.Sp[0]) * g1 + 2.0 * (S.Sg[1] - S.Sp[1]) * g2 +
+                     4.0 * (S.Sg[2] - S.Sp[2]) * g3);
+    }
+    const double gc = (U.m > 0) ? grad2d[U.img_off + p] : 0.0;
+    grad[U.x_off + k] = gc + U.reg_scale * g;
+  }
+  block_sum<kRegTerms>(v, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < kRegTerms; ++k) rpart[(long long)blockIdx.x * kRegTerms + k] = v[k];
+}
+
+// ---------------------------------------------------------------------------
+// one block per problem: fixed-order sums of the partials -> 15 terms
+// ---------------------------------------------------------------------------
+__global__ void k_finalize(const ProbDev* __restrict__ probs, const UnitDev* __restrict__ units,
+                           const RegItem* __restrict__ reg_items, const double* __restrict__ cpart,
+                           const double* __restrict__ rpart, const UnitMoments* __restrict__ mom,
+                           double* __restrict__ terms) {
+  __shared__ double red[16 * (kThreads / 64)];
+  const ProbDev P = probs[blockIdx.x];
+  double v[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = 0.0;
+  for (int t = 0; t < 4; ++t)
+    for (int i = P.chunk_begin[t] + threadIdx.x; i < P.chunk_end[t]; i += blockDim.x) v[t] += cpart[i];
+  for (int i = P.reg_begin + threadIdx.x; i < P.reg_end; i += blockDim.x) {
+    const double sc = units[reg_items[i].unit].reg_scale;
+    for (int k = 0; k < kRegTerms; ++k) v[4 + k] += sc * rpart[(long long)i * kRegTerms + k];
+  }
+  // tfd / cen / sm costs per unit
+  for (int u = P.unit_begin + threadIdx.x; u < P.unit_end; u += blockDim.x) {
+    const UnitDev U = units[u];
+    if (!U.need_moments) continue;
+    const UnitMoments M = mom[u];
+    if (U.tfd_l > 0) v[9] += U.reg_scale * U.tfd_l * (M.S - U.tfd_tgtfd) * (M.S - U.tfd_tgtfd);
+    if (U.cen_l > 0) {
+      const double sIp = kZeroEps + M.P;
+      const double sx = M.Px / sIp, sy = M.Py / sIp;
+      v[10] += U.reg_scale * U.cen_l * sqrt(sx * sx + sy * sy + kZeroEps);
+    }
+    if (U.sm_l > 0) {
+      const SmState S = sm_state(U, M);
+      const double c = (S.Sg[0] - S.Sp[0]) * (S.Sg[0] - S.Sp[0]) + (S.Sg[1] - S.Sp[1]) * (S.Sg[1] - S.Sp[1]) +
+                       2.0 * (S.Sg[2] - S.Sp[2]) * (S.Sg[2] - S.Sp[2]);
+      v[11] += U.reg_scale * U.sm_l * c;
+    }
+  }
+  block_sum<12>(v, red);
+  if (threadIdx.x == 0) {
+    double* T = terms + P.terms_off;
+    const double cf = v[0], ca_ = v[1], cc = v[2], cq = v[3];
+    // order of accumulation of calc_chisq: fcv, amp, ca, cp (fftlib.f90:338-367)
+    double chisq = 0.0;
+    if (P.n[DT_FCV] > 0) chisq += cf;
+    if (P.n[DT_AMP] > 0) chisq += ca_;
+    if (P.n[DT_CA] > 0) chisq += cq;
+    if (P.n[DT_CP] > 0) chisq += cc;
+    T[0] = chisq;
+    T[1] = P.n[DT_FCV] > 0 ? cf / P.w[DT_FCV] / (2.0 * P.n[DT_FCV]) : 0.0;
+    T[2] = P.n[DT_AMP] > 0 ? ca_ / P.w[DT_AMP] / P.n[DT_AMP] : 0.0;
+    T[3] = P.n[DT_CP] > 0 ? cc / P.w[DT_CP] / P.n[DT_CP] : 0.0;
+    T[4] = P.n[DT_CA] > 0 ? cq / P.w[DT_CA] / P.n[DT_CA] : 0.0;
+    const double l1 = v[4], tv = v[5], tsv = v[6], kl = v[7], gs = v[8];
+    const double tfd = v[9], cen = v[10], smc = v[11];
+    // image.f90:199 order
+    const double reg = l1 + tv + tsv + kl + gs + tfd + cen + smc;
+    T[5] = reg;
+    T[6] = l1; T[7] = tv; T[8] = tsv; T[9] = kl; T[10] = gs;
+    T[11] = tfd; T[12] = cen; T[13] = smc;
+    T[14] = chisq + reg;
+  }
+}
+
+}  // namespace smili
