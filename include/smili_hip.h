@@ -103,8 +103,9 @@ typedef struct smili_problem smili_problem;
 /* Tunables of the spreading/interpolation kernels (0 = default). */
 typedef struct smili_tune {
   int32_t bin;          /* bin edge in fine-grid cells (8..32, default 16) */
-  int32_t sub_points;   /* max points per subproblem (64..4096, default 256) */
-  int32_t col_group;    /* columns per column-FFT work-group (1..16, default 4) */
+  int32_t sub_points;   /* max points per interpolation subproblem (32..65536,
+                           default 256) */
+  int32_t col_group;    /* columns per column-FFT work-group (1..4, default 2) */
   int32_t chunk;        /* data items per chi-square work-group (default 2048) */
   int32_t use_graph;    /* >= 0: capture one evaluation in a hipGraph and replay it;
                            < 0: launch the kernels one by one */
@@ -132,6 +133,12 @@ int smili_cost_grad(smili_problem *p, const double *x, double *cost,
 int smili_cost_grad_device(smili_problem *p, const double *x_dev,
                            double *grad_dev, double *terms_dev);
 int smili_synchronize(smili_problem *p);
+
+/* Zero-copy use: the problem's own device buffers -- x (npix doubles, the
+ * caller writes it), grad (npix) and terms (15) -- and one evaluation on
+ * them, asynchronous on the problem's stream (no copies). */
+int smili_problem_buffers(smili_problem *p, double **x, double **grad, double **terms);
+int smili_eval_async(smili_problem *p);
 /* hipStream_t of the problem, as void* (plain pointer, no HIP types). */
 void *smili_problem_stream(smili_problem *p);
 

@@ -71,6 +71,9 @@ SIGNATURES = {
     "smili_cost_grad": (C.c_int, [C.c_void_p, _pd, _pd, _pd, _pd]),
     "smili_cost_grad_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "smili_synchronize": (C.c_int, [C.c_void_p]),
+    "smili_problem_buffers": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                        C.POINTER(C.c_void_p)]),
+    "smili_eval_async": (C.c_int, [C.c_void_p]),
     "smili_problem_stream": (C.c_void_p, [C.c_void_p]),
     "smili_model_vis": (C.c_int, [C.c_void_p, _pd, _pd, _pd]),
     "smili_adjoint_resid": (C.c_int, [C.c_void_p, _pd, _pd, _pd]),

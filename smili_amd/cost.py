@@ -161,6 +161,16 @@ class CostProblem:
         check(self._lib.smili_cost_grad_device(self._h, C.c_void_p(x_ptr), C.c_void_p(grad_ptr),
                                                 C.c_void_p(terms_ptr) if terms_ptr else None))
 
+    def buffers(self):
+        """Device addresses (ints) of the problem's own x, grad and terms."""
+        x, g, t = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        check(self._lib.smili_problem_buffers(self._h, C.byref(x), C.byref(g), C.byref(t)))
+        return x.value, g.value, t.value
+
+    def eval_async(self):
+        """One evaluation on the problem's own buffers (no copies, async)."""
+        check(self._lib.smili_eval_async(self._h))
+
     def synchronize(self):
         check(self._lib.smili_synchronize(self._h))
 

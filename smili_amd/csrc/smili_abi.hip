@@ -163,7 +163,8 @@ struct Plan {
   int device = 0;
   hipStream_t stream = nullptr;
   smili_tune tune{};
-  int bin = 16, sub_points = 256, col_group = 4, chunk = 2048, rows_per_block = 4;
+  int bin = 16, sub_points = 256, col_group = 2, chunk = 2048, rows_per_block = 1;
+  int threads_rows = 64, threads_cols = 128;
 
   std::vector<UnitDev> units;
   std::vector<ProbDev> probs;
@@ -172,14 +173,16 @@ struct Plan {
   size_t npix_total = 0, npt = 0;
   bool any_window = false, any_moments = false;
 
-  std::vector<RowItem> h_t2rows, h_cols, h_t1rows;
+  std::vector<FftItem> h_t2rows, h_cols, h_t1rows;
   std::vector<SubProb> h_subs;
+  std::vector<CellBlock> h_cblocks;
+  std::vector<CellSpan> h_spans;
   std::vector<Chunk> h_chunks;
+  std::vector<int> h_slots;
   std::vector<RegItem> h_regs;
 
   DevBuf<UnitDev> d_units;
   DevBuf<ProbDev> d_probs;
-  DevBuf<FftPlanDev> d_plans;
   DevBuf<float2> d_tw;
   DevBuf<float> d_corr;
   DevBuf<int> d_pix;
@@ -195,12 +198,15 @@ struct Plan {
   DevBuf<UnitMoments> d_mom;
   DevBuf<int> d_unit_reg_begin;
   DevBuf<Chunk> d_chunks;
+  DevBuf<int> d_slots;
   DevBuf<FcvItem> d_fcv;
   DevBuf<AmpItem> d_amp;
   DevBuf<CpItem> d_cp;
   DevBuf<CaItem> d_ca;
   DevBuf<SubProb> d_subs;
-  DevBuf<RowItem> d_t2rows, d_cols, d_t1rows;
+  DevBuf<CellBlock> d_cblocks;
+  DevBuf<CellSpan> d_spans;
+  DevBuf<FftItem> d_t2rows, d_cols, d_t1rows;
   DevBuf<RegItem> d_regs;
 
   size_t lds_rows = 0, lds_cols = 0, lds_tile = 0;
@@ -277,7 +283,7 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   if (t) tune = *t;
   bin = tune.bin > 0 ? tune.bin : 16;
   sub_points = tune.sub_points > 0 ? tune.sub_points : 256;
-  col_group = tune.col_group > 0 ? tune.col_group : 4;
+  col_group = tune.col_group > 0 ? tune.col_group : 2;
   chunk = tune.chunk > 0 ? tune.chunk : 2048;
   require(bin >= 8 && bin <= 64, "bin must be in [8, 64]");
   require(sub_points >= 32 && sub_points <= 65536, "sub_points must be in [32, 65536]");
@@ -428,6 +434,12 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     {
       std::vector<int> fill(count.begin(), count.end() - 1);
       for (int j = 0; j < m; ++j) order[fill[bidx[j]]++] = j;
+      // inside a bin, order the points by their first-tap cell (the spreading
+      // kernel gathers per cell)
+      for (int b = 0; b < nbx * nby; ++b)
+        std::stable_sort(order.begin() + count[b], order.begin() + count[b + 1], [&](int a, int c) {
+          return (l0y[a] * 65536 + l0x[a]) < (l0y[c] * 65536 + l0x[c]);
+        });
     }
     unit_rank[ui].resize(m);
     for (int s = 0; s < m; ++s) {
@@ -437,10 +449,43 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
       ex.push_back(exu[j]);
       ey.push_back(eyu[j]);
     }
+    // spreading work items: 8x8 blocks of first-tap cells, occupied cells
+    // with their (contiguous, sorted) point ranges
+    {
+      std::map<long long, std::vector<CellSpan>> blocks;   // key: (by, bx) of the cell block
+      int s = 0;
+      while (s < m) {
+        const int j = order[s];
+        int e = s + 1;
+        while (e < m && l0x[order[e]] == l0x[j] && l0y[order[e]] == l0y[j]) ++e;
+        const int bxk = l0x[j] / kCellBlock, byk = l0y[j] / kCellBlock;
+        const int lc = (l0y[j] % kCellBlock) * kCellBlock + (l0x[j] % kCellBlock);
+        auto& spans = blocks[(long long)byk * 65536 + bxk];
+        for (int a = s; a < e; a += kSpanPoints)
+          spans.push_back(CellSpan{lc, (int)U.pt_off + a, (int)U.pt_off + std::min(e, a + kSpanPoints), 0});
+        s = e;
+      }
+      for (auto& kv : blocks) {
+        for (size_t a = 0; a < kv.second.size(); a += kSpansPerGroup) {
+          CellBlock cbk{};
+          cbk.nf1 = U.nf1;
+          cbk.nf2 = U.nf2;
+          cbk.grid_off = U.grid_off;
+          cbk.gx0 = (int)(kv.first % 65536) * kCellBlock;
+          cbk.gy0 = (int)(kv.first / 65536) * kCellBlock;
+          cbk.cell_begin = (int)h_spans.size();
+          cbk.cell_count = (int)std::min<size_t>(kSpansPerGroup, kv.second.size() - a);
+          h_spans.insert(h_spans.end(), kv.second.begin() + a, kv.second.begin() + a + cbk.cell_count);
+          h_cblocks.push_back(cbk);
+        }
+      }
+    }
     for (int b = 0; b < nbx * nby; ++b) {
       for (int s = count[b]; s < count[b + 1]; s += sub_points) {
         SubProb sp{};
-        sp.unit = (int)ui;
+        sp.nf1 = U.nf1;
+        sp.nf2 = U.nf2;
+        sp.grid_off = U.grid_off;
         sp.bin_x = b % nbx;
         sp.bin_y = b / nbx;
         sp.start = U.pt_off + s;
@@ -459,6 +504,7 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   std::vector<CaItem> ca;
   probs.resize(ps.size());
   newpos.resize(ps.size());
+  std::vector<int> slot_scratch(npt, -1);   // global point -> slot in the open chunk
   for (size_t pi = 0; pi < ps.size(); ++pi) {
     const ProbSpec& P = ps[pi];
     const smili_data& D = *P.data;
@@ -471,7 +517,6 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
       o += us[ui].m;
     }
     require(o == P.nuv, "uv list length does not match the units' point counts");
-    auto remap = [&](int idx) { return (idx > 0 ? 1 : -1) * (np[std::abs(idx) - 1] + 1); };
     ProbDev& Q = probs[pi];
     Q = ProbDev{};
     Q.unit_begin = P.unit_begin;
@@ -479,53 +524,110 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     Q.n[DT_FCV] = D.nfcv; Q.n[DT_AMP] = D.namp; Q.n[DT_CP] = D.ncp; Q.n[DT_CA] = D.nca;
     Q.w[DT_FCV] = D.wfcv; Q.w[DT_AMP] = D.wamp; Q.w[DT_CP] = D.wcp; Q.w[DT_CA] = D.wca;
     Q.terms_off = (int)(pi * SMILI_NTERMS);
-    auto add_chunks = [&](int type, long long base, long long n) {
+    // Greedy chunking: consecutive data of one type share a chunk while the
+    // chunk references at most kMaxSlots distinct uv points and holds at
+    // most `chunk` data.  Data are referenced by chunk-local slots.
+    std::vector<int>& slot_of = slot_scratch;
+    std::vector<int> cur_slots;
+    auto close_chunk = [&](int type, long long start, long long count) {
+      if (count <= 0) return;
+      h_chunks.push_back(Chunk{type, (int)pi, (int)start, (int)count, (int)h_slots.size(),
+                               (int)cur_slots.size(), 0, 0});
+      for (int g : cur_slots) {
+        h_slots.push_back(g);
+        slot_of[g] = -1;
+      }
+      cur_slots.clear();
+    };
+    // slot field of the uv index `idx` in the current chunk (creates it)
+    auto slot = [&](int idx) -> unsigned short {
+      const int g = np[std::abs(idx) - 1];
+      if (slot_of[g] < 0) {
+        slot_of[g] = (int)cur_slots.size();
+        cur_slots.push_back(g);
+      }
+      return (unsigned short)(slot_of[g] | (idx < 0 ? kSlotNeg : 0));
+    };
+    // number of slots the indices would add to the current chunk
+    auto new_slots = [&](const int32_t* idx, int k) {
+      int n = 0;
+      for (int a = 0; a < k; ++a) {
+        const int g = np[std::abs(idx[a]) - 1];
+        bool seen = slot_of[g] >= 0;
+        for (int b = 0; b < a && !seen; ++b) seen = np[std::abs(idx[b]) - 1] == g;
+        n += seen ? 0 : 1;
+      }
+      return n;
+    };
+    auto build_type = [&](int type, long long n, int k, const int32_t* idx, auto&& emit) {
       Q.chunk_begin[type] = (int)h_chunks.size();
-      for (long long s = 0; s < n; s += chunk)
-        h_chunks.push_back(Chunk{type, (int)pi, (int)(base + s), (int)std::min<long long>(chunk, n - s)});
+      long long start = 0;
+      for (long long i = 0; i < n; ++i) {
+        const int32_t* ix = idx + i * k;
+        if (i - start >= chunk || (int)cur_slots.size() + new_slots(ix, k) > kMaxSlots) {
+          close_chunk(type, start, i - start);
+          start = i;
+        }
+        unsigned short sl[4] = {0, 0, 0, 0};
+        for (int a = 0; a < k; ++a) sl[a] = slot(ix[a]);
+        emit(i, sl);
+      }
+      close_chunk(type, start, n - start);
       Q.chunk_end[type] = (int)h_chunks.size();
     };
-    add_chunks(DT_FCV, (long long)fcv.size(), D.nfcv);
-    for (int i = 0; i < D.nfcv; ++i)
-      fcv.push_back(FcvItem{D.vfcv_re[i], D.vfcv_im[i], D.wfcv / D.varfcv[i], remap(D.uvidxfcv[i]), 0});
-    add_chunks(DT_AMP, (long long)amp.size(), D.namp);
-    for (int i = 0; i < D.namp; ++i)
-      amp.push_back(AmpItem{D.vamp[i], D.wamp / D.varamp[i], remap(D.uvidxamp[i]), 0});
-    add_chunks(DT_CP, (long long)cp.size(), D.ncp);
-    for (int i = 0; i < D.ncp; ++i) {
-      CpItem c{};
-      for (int k = 0; k < 3; ++k) c.idx[k] = remap(D.uvidxcp[3LL * i + k]);
-      c.idx[3] = 0;
-      c.cp = D.cp[i];
-      c.w = D.wcp / D.varcp[i];
-      cp.push_back(c);
-    }
-    add_chunks(DT_CA, (long long)ca.size(), D.nca);
-    for (int i = 0; i < D.nca; ++i) {
-      CaItem c{};
-      for (int k = 0; k < 4; ++k) c.idx[k] = remap(D.uvidxca[4LL * i + k]);
-      c.ca = D.ca[i];
-      c.w = D.wca / D.varca[i];
-      ca.push_back(c);
-    }
+    const long long fcv0 = fcv.size(), amp0 = amp.size(), cp0 = cp.size(), ca0 = ca.size();
+    auto shift_chunks = [&](int type, long long base) {
+      for (int c = Q.chunk_begin[type]; c < Q.chunk_end[type]; ++c) h_chunks[c].start += (int)base;
+    };
+    build_type(DT_FCV, D.nfcv, 1, D.uvidxfcv, [&](long long i, const unsigned short* sl) {
+      fcv.push_back(FcvItem{D.vfcv_re[i], D.vfcv_im[i], D.wfcv / D.varfcv[i], sl[0], 0, 0});
+    });
+    shift_chunks(DT_FCV, fcv0);
+    build_type(DT_AMP, D.namp, 1, D.uvidxamp, [&](long long i, const unsigned short* sl) {
+      amp.push_back(AmpItem{D.vamp[i], D.wamp / D.varamp[i], sl[0], 0, 0});
+    });
+    shift_chunks(DT_AMP, amp0);
+    build_type(DT_CP, D.ncp, 3, D.uvidxcp, [&](long long i, const unsigned short* sl) {
+      cp.push_back(CpItem{D.cp[i], D.wcp / D.varcp[i], {sl[0], sl[1], sl[2]}, 0});
+    });
+    shift_chunks(DT_CP, cp0);
+    build_type(DT_CA, D.nca, 4, D.uvidxca, [&](long long i, const unsigned short* sl) {
+      ca.push_back(CaItem{D.ca[i], D.wca / D.varca[i], {sl[0], sl[1], sl[2], sl[3]}});
+    });
+    shift_chunks(DT_CA, ca0);
   }
 
   // regularizer items, row/column items
   const int reg_chunk = 512;
   std::vector<int> unit_reg_begin(units.size() + 1, 0);
-  rows_per_block = std::max(1, std::min(8, 2048 / std::max(max_nf1, 1)));
+  // FFT work-groups: about 512 points per row group (one 64-thread wave of
+  // radix-8 butterflies), 64 threads per column of a column group
+  rows_per_block = std::max(1, std::min(8, 512 / std::max(max_nf1, 1)));
+  threads_rows = 64;
+  threads_cols = std::min(kThreads, 64 * col_group);
   for (size_t ui = 0; ui < units.size(); ++ui) {
     const UnitDev& U = units[ui];
     unit_reg_begin[ui] = (int)h_regs.size();
     for (int s = 0; s < U.npix; s += reg_chunk)
       h_regs.push_back(RegItem{(int)ui, s, std::min(reg_chunk, U.npix - s), 0});
+    auto fft_item = [&](int first, int count) {
+      FftItem f{};
+      f.first = first;
+      f.count = count;
+      f.nx = U.nx; f.ny = U.ny; f.nf1 = U.nf1; f.nf2 = U.nf2;
+      f.tw1 = plans[U.fft1].tw_off;
+      f.tw2 = plans[U.fft2].tw_off;
+      f.cx_off = U.cx_off; f.cy_off = U.cy_off;
+      f.x_off = U.x_off; f.identity = U.identity;
+      f.grid_off = U.grid_off; f.img_off = U.img_off;
+      return f;
+    };
     for (int s = 0; s < U.ny; s += rows_per_block) {
-      RowItem r{(int)ui, s, std::min(rows_per_block, U.ny - s), 0};
+      const FftItem r = fft_item(s, std::min(rows_per_block, U.ny - s));
       h_t2rows.push_back(r);
       h_t1rows.push_back(r);
     }
-    for (int s = 0; s < U.nf1; s += col_group)
-      h_cols.push_back(RowItem{(int)ui, s, std::min(col_group, U.nf1 - s), 0});
+    for (int s = 0; s < U.nf1; s += col_group) h_cols.push_back(fft_item(s, std::min(col_group, U.nf1 - s)));
   }
   unit_reg_begin[units.size()] = (int)h_regs.size();
   for (size_t pi = 0; pi < ps.size(); ++pi) {
@@ -534,8 +636,8 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   }
 
   // LDS budget
-  lds_rows = 2ull * rows_per_block * max_nf1 * sizeof(float2);
-  lds_cols = 2ull * col_group * (max_nf2 + 1) * sizeof(float2);
+  lds_rows = (2ull * rows_per_block + 1) * max_nf1 * sizeof(float2);
+  lds_cols = (2ull * col_group * (max_nf2 + 1) + max_nf2) * sizeof(float2);
   const int T = bin + kW - 1;
   lds_tile = (size_t)T * T * sizeof(float2);
   require(lds_rows <= 65536 && lds_cols <= 65536 && lds_tile <= 65536,
@@ -554,7 +656,6 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   }
   d_units.upload(units);
   d_probs.upload(probs);
-  d_plans.upload(plans);
   d_tw.upload(tw);
   d_corr.upload(corr);
   d_pix.upload(pix);
@@ -584,11 +685,14 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   d_mom.zero();
   d_unit_reg_begin.upload(unit_reg_begin);
   d_chunks.upload(h_chunks);
+  d_slots.upload(h_slots);
   d_fcv.upload(fcv);
   d_amp.upload(amp);
   d_cp.upload(cp);
   d_ca.upload(ca);
   d_subs.upload(h_subs);
+  d_cblocks.upload(h_cblocks);
+  d_spans.upload(h_spans);
   d_t2rows.upload(h_t2rows);
   d_cols.upload(h_cols);
   d_t1rows.upload(h_t1rows);
@@ -615,22 +719,22 @@ void Plan::launch_forward() {
   if (any_window)
     LAUNCH(SMILI_S_SCATTER, k_scatter, h_regs.size(), kThreads, 0, d_regs.p, d_units.p, d_pix.p, d_x.p, d_I2d.p);
   if (npt == 0) return;
-  LAUNCH(SMILI_S_T2_ROWS, k_t2_rows, h_t2rows.size(), kThreads, lds_rows, d_t2rows.p, d_units.p, d_plans.p, d_tw.p,
-         d_corr.p, d_x.p, d_I2d.p, d_gridA.p);
-  LAUNCH(SMILI_S_T2_COLS, k_t2_cols, h_cols.size(), kThreads, lds_cols, d_cols.p, d_units.p, d_plans.p, d_tw.p, d_gridA.p);
-  LAUNCH(SMILI_S_INTERP, k_interp, h_subs.size(), kThreads, lds_tile, d_subs.p, d_units.p, d_gridA.p, d_cell.p, d_ex.p,
+  LAUNCH(SMILI_S_T2_ROWS, k_t2_rows, h_t2rows.size(), threads_rows, lds_rows, d_t2rows.p, d_tw.p, d_corr.p, d_x.p,
+         d_I2d.p, d_gridA.p);
+  LAUNCH(SMILI_S_T2_COLS, k_t2_cols, h_cols.size(), threads_cols, lds_cols, d_cols.p, d_tw.p, d_gridA.p);
+  LAUNCH(SMILI_S_INTERP, k_interp, h_subs.size(), kThreads, lds_tile, d_subs.p, d_gridA.p, d_cell.p, d_ex.p,
          d_ey.p, d_V.p, d_Vres.p, bin, beta);
 }
 
 void Plan::launch_adjoint() {
   if (npt == 0) return;
   const float beta = (float)kBeta;
-  LAUNCH(SMILI_S_SPREAD, k_spread, h_subs.size(), kThreads, lds_tile, d_subs.p, d_units.p, d_cell.p, d_ex.p, d_ey.p,
-         d_Vres.p, d_gridB.p, bin, beta);
-  LAUNCH(SMILI_S_T1_COLS, k_t1_cols, h_cols.size(), kThreads, lds_cols, d_cols.p, d_units.p, d_plans.p, d_tw.p,
-         d_gridB.p, d_gridA.p);
-  LAUNCH(SMILI_S_T1_ROWS, k_t1_rows, h_t1rows.size(), kThreads, lds_rows, d_t1rows.p, d_units.p, d_plans.p, d_tw.p,
-         d_corr.p, d_gridA.p, d_grad2d.p);
+  LAUNCH(SMILI_S_SPREAD, k_spread, h_cblocks.size(), kSpansPerGroup, 0, d_cblocks.p, d_spans.p, d_ex.p, d_ey.p, d_Vres.p,
+         d_gridB.p, beta);
+  LAUNCH(SMILI_S_T1_COLS, k_t1_cols, h_cols.size(), threads_cols, lds_cols, d_cols.p, d_tw.p, d_gridB.p,
+         d_gridA.p);
+  LAUNCH(SMILI_S_T1_ROWS, k_t1_rows, h_t1rows.size(), threads_rows, lds_rows, d_t1rows.p, d_tw.p, d_corr.p,
+         d_gridA.p, d_grad2d.p);
 }
 
 void Plan::launch_eval() {
@@ -641,14 +745,14 @@ void Plan::launch_eval() {
     LAUNCH(SMILI_S_MOMENTS_REDUCE, k_moments_reduce, units.size(), 64, 0, d_units.p, d_unit_reg_begin.p, d_mpart.p, d_mom.p);
   }
   if (npt > 0) {
-    LAUNCH(SMILI_S_CHISQ, k_chisq, h_chunks.size(), kThreads, 0, d_chunks.p, d_fcv.p, d_amp.p, d_cp.p, d_ca.p, d_V.p,
-           d_Vres.p, d_cpart.p);
+    LAUNCH(SMILI_S_CHISQ, k_chisq, h_chunks.size(), kThreads, 0, d_chunks.p, d_slots.p, d_fcv.p, d_amp.p, d_cp.p,
+           d_ca.p, d_V.p, d_Vres.p, d_cpart.p);
     launch_adjoint();
   }
   LAUNCH(SMILI_S_REG, k_reg, h_regs.size(), kThreads, 0, d_regs.p, d_units.p, d_pix.p, d_x.p, d_I2d.p, d_grad2d.p,
          d_mom.p, d_grad.p, d_rpart.p);
-  LAUNCH(SMILI_S_FINALIZE, k_finalize, probs.size(), kThreads, 0, d_probs.p, d_units.p, d_regs.p, d_cpart.p, d_rpart.p,
-         d_mom.p, d_terms.p);
+  LAUNCH(SMILI_S_FINALIZE, k_finalize, probs.size(), kThreads, 0, d_probs.p, d_units.p, d_regs.p, d_cpart.p,
+         d_rpart.p, d_mom.p, d_terms.p);
 }
 
 void Plan::eval() {
@@ -766,6 +870,22 @@ int smili_cost_grad_device(smili_problem* p, const double* x_dev, double* grad_d
     if (terms_dev && terms_dev != P.d_terms.p)
       HIP_CHECK(hipMemcpyAsync(terms_dev, P.d_terms.p, SMILI_NTERMS * sizeof(double),
                                hipMemcpyDeviceToDevice, P.stream));
+  });
+}
+
+int smili_problem_buffers(smili_problem* p, double** x, double** grad, double** terms) {
+  return guarded([&] {
+    require(p != nullptr, "NULL problem");
+    if (x) *x = p->plan.d_x.p;
+    if (grad) *grad = p->plan.d_grad.p;
+    if (terms) *terms = p->plan.d_terms.p;
+  });
+}
+
+int smili_eval_async(smili_problem* p) {
+  return guarded([&] {
+    require(p != nullptr, "NULL problem");
+    p->plan.eval();
   });
 }
 

@@ -16,6 +16,7 @@ constexpr int kRegTerms = 5;         // l1, tv, tsv, kl, gs partial sums per ite
 constexpr int kMomTerms = 10;        // moments for tfd / cen / sm
 constexpr double kZeroEps = 1e-10;   // image.f90:7
 
+// host-side FFT plan of one length (the kernels derive the radices)
 struct FftPlanDev {
   int n;
   int nstages;
@@ -43,19 +44,39 @@ struct UnitDev {
   int need_moments;
 };
 
-// chi-square data items; idx are signed 1-based indices into the global
-// (sorted) point arrays; w = fnorm / variance
-struct FcvItem { double vr, vi, w; int idx, pad; };
-struct AmpItem { double amp, w; int idx, pad; };
-struct CpItem  { int idx[4]; double cp, w; };
-struct CaItem  { int idx[4]; double ca, w; };
+// chi-square data items.  A work item ("chunk") of consecutive data stages
+// the model visibilities of the distinct uv points its data reference into
+// LDS; each datum refers to them by chunk-local slot numbers.  A slot field
+// holds the slot in bits 0..14 and the sign of the uv index (1 = the datum
+// sits at (-u, -v)) in bit 15.  w = fnorm / variance.
+constexpr int kMaxSlots = 1024;      // distinct uv points per chunk
+constexpr int kSlotNeg = 0x8000;
+struct FcvItem { double vr, vi, w; unsigned short slot, pad0; int pad1; };
+struct AmpItem { double amp, w; unsigned short slot, pad0; int pad1; };
+struct CpItem  { double cp, w; unsigned short slot[3], pad0; };
+struct CaItem  { double ca, w; unsigned short slot[4]; };
 
 enum { DT_FCV = 0, DT_AMP = 1, DT_CP = 2, DT_CA = 3 };
 
-struct Chunk { int type, prob, start, count; };
-struct RowItem { int unit, first, count, pad; };   // rows or columns
-struct SubProb { int unit, bin_x, bin_y, start, count, pad; };
+struct Chunk { int type, prob, start, count, slot_off, nslots, pad0, pad1; };
+// FFT work item: `count` rows or columns starting at `first` of one unit,
+// with the unit's geometry embedded so a work-group needs one descriptor load
+struct FftItem {
+  int first, count, nx, ny, nf1, nf2;
+  int tw1, tw2;                      // twiddle tables of length nf1, nf2
+  int cx_off, cy_off, x_off, identity;
+  long long grid_off, img_off;
+};
+struct SubProb { int bin_x, bin_y, start, count, nf1, nf2; long long grid_off; };
 struct RegItem { int unit, start, count, pad; };
+// spreading: a group of "spans" inside one 8x8 block of first-tap cells; a
+// span is up to kSpanPoints consecutive points of one cell (points are
+// sorted by cell), one thread per span
+constexpr int kCellBlock = 8;
+constexpr int kSpanPoints = 8;
+constexpr int kSpansPerGroup = 256;
+struct CellBlock { int gx0, gy0, cell_begin, cell_count, nf1, nf2; long long grid_off; };
+struct CellSpan { int lcell, pt_begin, pt_end, pad; };   // lcell = ly * 8 + lx
 
 struct ProbDev {
   int chunk_begin[4], chunk_end[4];  // chi-square chunk ranges per data type

@@ -85,15 +85,16 @@ __device__ __forceinline__ void dft8(float2* v) {
 
 // ---------------------------------------------------------------------------
 // LDS Stockham FFT (autosort, +i sign) of nb transforms of length n stored
-// at a + b*stride.  Uses scratch b of the same shape.  Returns the buffer
-// that holds the result.  All threads of the block must call it.
+// at a + b*stride.  Uses scratch s of the same shape and the twiddle table
+// tw[t] = exp(+2 pi i t / n) staged in LDS.  Returns the buffer that holds
+// the result.  All threads of the block must call it.
 // ---------------------------------------------------------------------------
-__device__ float2* lds_fft(float2* a, float2* s, int nb, int stride, const FftPlanDev& pl,
-                           const float2* __restrict__ tw) {
-  const int n = pl.n;
-  int Ns = 1;
-  for (int st = 0; st < pl.nstages; ++st) {
-    const int r = pl.radix[st];
+__device__ float2* lds_fft(float2* a, float2* s, int nb, int stride, int n, const float2* tw) {
+  int Ns = 1, rem = n;
+  while (rem > 1) {
+    // radices in the order of the host plan: all 8s, then 4, 2, 3, 5
+    const int r = (rem % 8 == 0) ? 8 : (rem % 4 == 0) ? 4 : (rem % 2 == 0) ? 2 : (rem % 3 == 0) ? 3 : 5;
+    rem /= r;
     const int nbf = n / r;
     const int total = nbf * nb;
     const int twstep = n / (Ns * r);
@@ -111,7 +112,7 @@ __device__ float2* lds_fft(float2* a, float2* s, int nb, int stride, const FftPl
         const int step = k * twstep;
 #pragma unroll
         for (int q = 1; q < 8; ++q)
-          if (q < r) v[q] = cmul(v[q], tw[pl.tw_off + q * step]);
+          if (q < r) v[q] = cmul(v[q], tw[q * step]);
       }
       switch (r) {
         case 2: dft2(v); break;
@@ -130,6 +131,10 @@ __device__ float2* lds_fft(float2* a, float2* s, int nb, int stride, const FftPl
     Ns *= r;
   }
   return a;
+}
+
+__device__ __forceinline__ void stage_twiddles(float2* dst, const float2* __restrict__ tw, int off, int n) {
+  for (int t = threadIdx.x; t < n; t += blockDim.x) dst[t] = tw[off + t];
 }
 
 // ---------------------------------------------------------------------------
@@ -154,41 +159,40 @@ __device__ __forceinline__ void es_weights(float e, float beta, float* wt) {
 // One work item = `count` consecutive image rows of one unit.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kThreads)
-k_t2_rows(const RowItem* __restrict__ items, const UnitDev* __restrict__ units,
-          const FftPlanDev* __restrict__ plans, const float2* __restrict__ tw,
+k_t2_rows(const FftItem* __restrict__ items, const float2* __restrict__ tw,
           const float* __restrict__ corr, const double* __restrict__ x,
           const double* __restrict__ I2dbuf, float2* __restrict__ gridA) {
   extern __shared__ float2 sm[];
-  const RowItem it = items[blockIdx.x];
-  const UnitDev U = units[it.unit];
-  const FftPlanDev pl = plans[U.fft1];
-  const int n = U.nf1, nb = it.count;
+  const FftItem it = items[blockIdx.x];
+  const int n = it.nf1, nb = it.count;
   float2* A = sm;
   float2* B = sm + nb * n;
-  const double* img = U.identity ? (x + U.x_off) : (I2dbuf + U.img_off);
-  const float* cx = corr + U.cx_off;
-  const float* cy = corr + U.cy_off;
-  const int hx = U.nx / 2;
+  float2* W = B + nb * n;
+  stage_twiddles(W, tw, it.tw1, n);
+  const double* img = it.identity ? (x + it.x_off) : (I2dbuf + it.img_off);
+  const float* cx = corr + it.cx_off;
+  const float* cy = corr + it.cy_off;
+  const int hx = it.nx / 2;
   for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
     const int b = t / n, c = t - b * n;
-    // grid column c holds mode k1 = c (c < nf1/2) or c - nf1
+    // grid column c holds mode k1 = c (c < nf1 - nx/2) or c - nf1
     const int k1 = (c < n - hx) ? c : c - n;
     float2 val = make_float2(0.f, 0.f);
-    if (k1 >= -hx && k1 < U.nx - hx) {
+    if (k1 >= -hx && k1 < it.nx - hx) {
       const int ix = k1 + hx;
       const int iy = it.first + b;
-      val.x = (float)(img[(long long)iy * U.nx + ix] * (double)cx[ix] * (double)cy[iy]);
+      val.x = (float)(img[(long long)iy * it.nx + ix] * (double)cx[ix] * (double)cy[iy]);
     }
     A[t] = val;
   }
   __syncthreads();
-  float2* R = lds_fft(A, B, nb, n, pl, tw);
-  const int hy = U.ny / 2;
+  float2* R = lds_fft(A, B, nb, n, n, W);
+  const int hy = it.ny / 2;
   for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
     const int b = t / n, c = t - b * n;
     int row = it.first + b - hy;
-    if (row < 0) row += U.nf2;
-    gridA[U.grid_off + (long long)row * n + c] = R[t];
+    if (row < 0) row += it.nf2;
+    gridA[it.grid_off + (long long)row * n + c] = R[t];
   }
 }
 
@@ -197,30 +201,28 @@ k_t2_rows(const RowItem* __restrict__ items, const UnitDev* __restrict__ units,
 // hold image modes are read (the others are zero by construction).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kThreads)
-k_t2_cols(const RowItem* __restrict__ items, const UnitDev* __restrict__ units,
-          const FftPlanDev* __restrict__ plans, const float2* __restrict__ tw,
-          float2* __restrict__ gridA) {
+k_t2_cols(const FftItem* __restrict__ items, const float2* __restrict__ tw, float2* __restrict__ gridA) {
   extern __shared__ float2 sm[];
-  const RowItem it = items[blockIdx.x];
-  const UnitDev U = units[it.unit];
-  const FftPlanDev pl = plans[U.fft2];
-  const int n = U.nf2, nb = it.count, stride = n + 1;
+  const FftItem it = items[blockIdx.x];
+  const int n = it.nf2, nb = it.count, stride = n + 1;
   float2* A = sm;
   float2* B = sm + nb * stride;
-  const int hy = U.ny / 2;
-  float2* g = gridA + U.grid_off;
+  float2* W = B + nb * stride;
+  stage_twiddles(W, tw, it.tw2, n);
+  const int hy = it.ny / 2;
+  float2* g = gridA + it.grid_off;
   for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
     const int r = t / nb, b = t - r * nb;
     const int k2 = (r < n - hy) ? r : r - n;
     float2 val = make_float2(0.f, 0.f);
-    if (k2 >= -hy && k2 < U.ny - hy) val = g[(long long)r * U.nf1 + it.first + b];
+    if (k2 >= -hy && k2 < it.ny - hy) val = g[(long long)r * it.nf1 + it.first + b];
     A[b * stride + r] = val;
   }
   __syncthreads();
-  float2* R = lds_fft(A, B, nb, stride, pl, tw);
+  float2* R = lds_fft(A, B, nb, stride, n, W);
   for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
     const int r = t / nb, b = t - r * nb;
-    g[(long long)r * U.nf1 + it.first + b] = R[b * stride + r];
+    g[(long long)r * it.nf1 + it.first + b] = R[b * stride + r];
   }
 }
 
@@ -230,21 +232,20 @@ k_t2_cols(const RowItem* __restrict__ items, const UnitDev* __restrict__ units,
 // accumulators of its points for the chi-square kernel.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kThreads)
-k_interp(const SubProb* __restrict__ sps, const UnitDev* __restrict__ units,
-         const float2* __restrict__ gridA, const int* __restrict__ pt_cell,
-         const float* __restrict__ pt_ex, const float* __restrict__ pt_ey,
-         float2* __restrict__ V, double2* __restrict__ Vres, int bin, float beta) {
+k_interp(const SubProb* __restrict__ sps, const float2* __restrict__ gridA,
+         const int* __restrict__ pt_cell, const float* __restrict__ pt_ex,
+         const float* __restrict__ pt_ey, float2* __restrict__ V, double2* __restrict__ Vres,
+         int bin, float beta) {
   extern __shared__ float2 tile[];
   const SubProb sp = sps[blockIdx.x];
-  const UnitDev U = units[sp.unit];
   const int T = bin + kW - 1;
-  const float2* g = gridA + U.grid_off;
+  const float2* g = gridA + sp.grid_off;
   const int gx0 = sp.bin_x * bin, gy0 = sp.bin_y * bin;
   for (int c = threadIdx.x; c < T * T; c += blockDim.x) {
     const int ty = c / T, tx = c - ty * T;
-    const int gy = (gy0 + ty) % U.nf2;
-    const int gx = (gx0 + tx) % U.nf1;
-    tile[c] = g[(long long)gy * U.nf1 + gx];
+    const int gy = (gy0 + ty) % sp.nf2;
+    const int gx = (gx0 + tx) % sp.nf1;
+    tile[c] = g[(long long)gy * sp.nf1 + gx];
   }
   __syncthreads();
   for (int i = threadIdx.x; i < sp.count; i += blockDim.x) {
@@ -303,62 +304,72 @@ __device__ __forceinline__ void block_sum(double* v, double* red /* NV * (kThrea
   }
 }
 
-__device__ __forceinline__ void vres_add(double2* Vres, int j, double re, double im) {
-  unsafeAtomicAdd(&Vres[j].x, re);
-  unsafeAtomicAdd(&Vres[j].y, im);
-}
-
 // ---------------------------------------------------------------------------
 // chi-square terms and residual vectors (fftlib.f90:384-655).  One work item
-// = `count` data of one type of one problem.  The chi-square of the item is
-// written to partial[blockIdx.x] (FP64); residual vectors are accumulated
-// with FP64 atomics.
+// = `count` consecutive data of one type of one problem that reference
+// `nslots` distinct uv points.  Their model visibilities are staged in LDS
+// (FP64), the residual vectors are accumulated in LDS with FP64 LDS atomics
+// and flushed with one global FP64 atomic per distinct point.  The
+// chi-square of the item is written to partial[blockIdx.x].
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void lds_add2(double2* R, int s, double re, double im) {
+  atomicAdd(&R[s].x, re);
+  atomicAdd(&R[s].y, im);
+}
+
 __global__ void __launch_bounds__(kThreads)
-k_chisq(const Chunk* __restrict__ chunks, const FcvItem* __restrict__ fcv,
-        const AmpItem* __restrict__ amp, const CpItem* __restrict__ cp,
-        const CaItem* __restrict__ ca, const float2* __restrict__ V,
-        double2* __restrict__ Vres, double* __restrict__ partial) {
+k_chisq(const Chunk* __restrict__ chunks, const int* __restrict__ slots,
+        const FcvItem* __restrict__ fcv, const AmpItem* __restrict__ amp,
+        const CpItem* __restrict__ cp, const CaItem* __restrict__ ca,
+        const float2* __restrict__ V, double2* __restrict__ Vres,
+        double* __restrict__ partial) {
+  __shared__ double2 Vl[kMaxSlots];
+  __shared__ double2 Rl[kMaxSlots];
   __shared__ double red[kThreads / 64];
   const Chunk ch = chunks[blockIdx.x];
+  const int* sl = slots + ch.slot_off;
+  for (int s = threadIdx.x; s < ch.nslots; s += blockDim.x) {
+    const float2 m = V[sl[s]];
+    Vl[s] = make_double2((double)m.x, (double)m.y);
+    Rl[s] = make_double2(0.0, 0.0);
+  }
+  __syncthreads();
   double acc = 0.0;
   if (ch.type == DT_FCV) {
     for (int i = threadIdx.x; i < ch.count; i += blockDim.x) {
       const FcvItem d = fcv[ch.start + i];
-      const int j = abs(d.idx) - 1;
-      const double s = d.idx > 0 ? 1.0 : -1.0;
-      const float2 m = V[j];
-      const double rr = d.vr - (double)m.x;
-      const double ri = d.vi - s * (double)m.y;       // conj for negative index
+      const int s = d.slot & 0x7fff;
+      const double sg = (d.slot & kSlotNeg) ? -1.0 : 1.0;
+      const double2 m = Vl[s];
+      const double rr = d.vr - m.x;
+      const double ri = d.vi - sg * m.y;               // conj for negative index
       acc += (rr * rr + ri * ri) * d.w;
       const double f = -2.0 * d.w;
-      vres_add(Vres, j, f * rr, f * ri * s);
+      lds_add2(Rl, s, f * rr, f * ri * sg);
     }
   } else if (ch.type == DT_AMP) {
     for (int i = threadIdx.x; i < ch.count; i += blockDim.x) {
       const AmpItem d = amp[ch.start + i];
-      const int j = abs(d.idx) - 1;
-      const float2 m = V[j];
-      const double vr = m.x, vi = m.y;
-      const double mod = sqrt(vr * vr + vi * vi);
+      const int s = d.slot & 0x7fff;
+      const double2 m = Vl[s];
+      const double mod = sqrt(m.x * m.x + m.y * m.y);
       const double r = d.amp - mod;
       acc += r * r * d.w;
       const double f = -2.0 * r * d.w / mod;
-      vres_add(Vres, j, f * vr, f * vi);
+      lds_add2(Rl, s, f * m.x, f * m.y);
     }
   } else if (ch.type == DT_CP) {
     for (int i = threadIdx.x; i < ch.count; i += blockDim.x) {
       const CpItem d = cp[ch.start + i];
-      double vr[3], vi[3], sg[3], model = 0.0;
-      int jj[3];
+      double2 m[3];
+      double sg[3], model = 0.0;
+      int s[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        jj[k] = abs(d.idx[k]) - 1;
-        sg[k] = d.idx[k] > 0 ? 1.0 : -1.0;
-        const float2 m = V[jj[k]];
-        vr[k] = m.x;
-        vi[k] = m.y;
-        model += atan2(vi[k], vr[k]) * sg[k];
+        s[k] = d.slot[k] & 0x7fff;
+        sg[k] = (d.slot[k] & kSlotNeg) ? -1.0 : 1.0;
+        m[k] = Vl[s[k]];
+        model += atan2(m[k].y, m[k].x) * sg[k];
       }
       const double r = d.cp - model;
       double sr, cr;
@@ -367,22 +378,21 @@ k_chisq(const Chunk* __restrict__ chunks, const FcvItem* __restrict__ fcv,
       const double f = -2.0 * sr * d.w;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const double a = f / (vr[k] * vr[k] + vi[k] * vi[k]) * sg[k];
-        vres_add(Vres, jj[k], -a * vi[k], a * vr[k]);
+        const double a = f / (m[k].x * m[k].x + m[k].y * m[k].y) * sg[k];
+        lds_add2(Rl, s[k], -a * m[k].y, a * m[k].x);
       }
     }
   } else {
     for (int i = threadIdx.x; i < ch.count; i += blockDim.x) {
       const CaItem d = ca[ch.start + i];
-      double vr[4], vi[4], a2[4];
-      int jj[4];
+      double2 m[4];
+      double a2[4];
+      int s[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        jj[k] = abs(d.idx[k]) - 1;
-        const float2 m = V[jj[k]];
-        vr[k] = m.x;
-        vi[k] = m.y;
-        a2[k] = vr[k] * vr[k] + vi[k] * vi[k];
+        s[k] = d.slot[k] & 0x7fff;       // signs are ignored (fftlib.f90:549-553)
+        m[k] = Vl[s[k]];
+        a2[k] = m[k].x * m[k].x + m[k].y * m[k].y;
       }
       // log|V| = 0.5 log|V|^2
       const double model = 0.5 * (log(a2[0]) + log(a2[1]) - log(a2[2]) - log(a2[3]));
@@ -392,9 +402,16 @@ k_chisq(const Chunk* __restrict__ chunks, const FcvItem* __restrict__ fcv,
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const double a = (k < 2 ? f : -f) / a2[k];
-        vres_add(Vres, jj[k], a * vr[k], a * vi[k]);
+        lds_add2(Rl, s[k], a * m[k].x, a * m[k].y);
       }
     }
+  }
+  __syncthreads();
+  for (int s = threadIdx.x; s < ch.nslots; s += blockDim.x) {
+    const double2 r = Rl[s];
+    const int j = sl[s];
+    unsafeAtomicAdd(&Vres[j].x, r.x);
+    unsafeAtomicAdd(&Vres[j].y, r.y);
   }
   double v[1] = {acc};
   block_sum<1>(v, red);
@@ -403,52 +420,70 @@ k_chisq(const Chunk* __restrict__ chunks, const FcvItem* __restrict__ fcv,
 
 // ---------------------------------------------------------------------------
 // Type-1 step 1: spread residuals c = Vres_re - i Vres_im onto the grid.
-// Subproblem tile accumulated in LDS with LDS atomics, then added to the
-// global grid with float atomics.  The grid is zero on entry (k_t1_cols
+// Work item = up to 256 spans of one 8x8 block of first-tap cells; a span is
+// up to 8 consecutive points of one cell (points are sorted by cell, so a
+// span's points share their 7x7 footprint).  One thread per span sums the
+// footprints of its points in registers, adds the sum to an LDS tile of the
+// block (spans of neighbouring cells overlap), and the tile is added to the
+// global grid with float atomics (work items of one block and of
+// neighbouring blocks overlap).  The grid is zero on entry (k_t1_cols
 // clears what it consumes).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kThreads)
-k_spread(const SubProb* __restrict__ sps, const UnitDev* __restrict__ units,
-         const int* __restrict__ pt_cell, const float* __restrict__ pt_ex,
+__global__ void __launch_bounds__(kSpansPerGroup)
+k_spread(const CellBlock* __restrict__ blocks, const CellSpan* __restrict__ spans,
+         const float* __restrict__ pt_ex,
          const float* __restrict__ pt_ey, const double2* __restrict__ Vres,
-         float2* __restrict__ gridB, int bin, float beta) {
-  extern __shared__ float2 tile[];
-  const SubProb sp = sps[blockIdx.x];
-  const UnitDev U = units[sp.unit];
-  const int T = bin + kW - 1;
+         float2* __restrict__ gridB, float beta) {
+  constexpr int T = kCellBlock + kW - 1;
+  __shared__ float2 tile[T * T];
+  const CellBlock cb = blocks[blockIdx.x];
   for (int c = threadIdx.x; c < T * T; c += blockDim.x) tile[c] = make_float2(0.f, 0.f);
   __syncthreads();
-  const int gx0 = sp.bin_x * bin, gy0 = sp.bin_y * bin;
-  for (int i = threadIdx.x; i < sp.count; i += blockDim.x) {
-    const int j = sp.start + i;
-    const int cell = pt_cell[j];
-    const int lx = (cell & 0xffff) - gx0;
-    const int ly = (cell >> 16) - gy0;
-    float wx[kW], wy[kW];
-    es_weights(pt_ex[j], beta, wx);
-    es_weights(pt_ey[j], beta, wy);
-    const double2 r = Vres[j];
-    const float cr = (float)r.x, ci = (float)(-r.y);
+  for (int k = threadIdx.x; k < cb.cell_count; k += blockDim.x) {
+    const CellSpan sp = spans[cb.cell_begin + k];
+    float ar[kW][kW], ai[kW][kW];
 #pragma unroll
-    for (int qy = 0; qy < kW; ++qy) {
-      const float ar = wy[qy] * cr, ai = wy[qy] * ci;
-      float2* row = tile + (ly + qy) * T + lx;
+    for (int qy = 0; qy < kW; ++qy)
 #pragma unroll
       for (int qx = 0; qx < kW; ++qx) {
-        atomicAdd(&row[qx].x, wx[qx] * ar);
-        atomicAdd(&row[qx].y, wx[qx] * ai);
+        ar[qy][qx] = 0.f;
+        ai[qy][qx] = 0.f;
+      }
+    for (int j = sp.pt_begin; j < sp.pt_end; ++j) {
+      float wx[kW], wy[kW];
+      es_weights(pt_ex[j], beta, wx);
+      es_weights(pt_ey[j], beta, wy);
+      const double2 r = Vres[j];
+      const float cr = (float)r.x, ci = (float)(-r.y);
+#pragma unroll
+      for (int qy = 0; qy < kW; ++qy) {
+        const float yr = wy[qy] * cr, yi = wy[qy] * ci;
+#pragma unroll
+        for (int qx = 0; qx < kW; ++qx) {
+          ar[qy][qx] = fmaf(wx[qx], yr, ar[qy][qx]);
+          ai[qy][qx] = fmaf(wx[qx], yi, ai[qy][qx]);
+        }
       }
     }
+    const int lx = sp.lcell % kCellBlock, ly = sp.lcell / kCellBlock;
+#pragma unroll
+    for (int qy = 0; qy < kW; ++qy)
+#pragma unroll
+      for (int qx = 0; qx < kW; ++qx) {
+        float2* dst = &tile[(ly + qy) * T + lx + qx];
+        atomicAdd(&dst->x, ar[qy][qx]);
+        atomicAdd(&dst->y, ai[qy][qx]);
+      }
   }
   __syncthreads();
-  float2* g = gridB + U.grid_off;
+  float2* g = gridB + cb.grid_off;
   for (int c = threadIdx.x; c < T * T; c += blockDim.x) {
     const float2 val = tile[c];
     if (val.x != 0.f || val.y != 0.f) {
       const int ty = c / T, tx = c - ty * T;
-      const int gy = (gy0 + ty) % U.nf2;
-      const int gx = (gx0 + tx) % U.nf1;
-      float2* dst = g + (long long)gy * U.nf1 + gx;
+      const int gy = (cb.gy0 + ty) % cb.nf2;
+      const int gx = (cb.gx0 + tx) % cb.nf1;
+      float2* dst = g + (long long)gy * cb.nf1 + gx;
       unsafeAtomicAdd(&dst->x, val.x);
       unsafeAtomicAdd(&dst->y, val.y);
     }
@@ -461,32 +496,31 @@ k_spread(const SubProb* __restrict__ sps, const UnitDev* __restrict__ units,
 // consumed columns of gridB for the next evaluation.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kThreads)
-k_t1_cols(const RowItem* __restrict__ items, const UnitDev* __restrict__ units,
-          const FftPlanDev* __restrict__ plans, const float2* __restrict__ tw,
+k_t1_cols(const FftItem* __restrict__ items, const float2* __restrict__ tw,
           float2* __restrict__ gridB, float2* __restrict__ H) {
   extern __shared__ float2 sm[];
-  const RowItem it = items[blockIdx.x];
-  const UnitDev U = units[it.unit];
-  const FftPlanDev pl = plans[U.fft2];
-  const int n = U.nf2, nb = it.count, stride = n + 1;
+  const FftItem it = items[blockIdx.x];
+  const int n = it.nf2, nb = it.count, stride = n + 1;
   float2* A = sm;
   float2* B = sm + nb * stride;
-  float2* g = gridB + U.grid_off;
+  float2* W = B + nb * stride;
+  stage_twiddles(W, tw, it.tw2, n);
+  float2* g = gridB + it.grid_off;
   for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
     const int r = t / nb, b = t - r * nb;
-    float2* src = g + (long long)r * U.nf1 + it.first + b;
+    float2* src = g + (long long)r * it.nf1 + it.first + b;
     A[b * stride + r] = *src;
     *src = make_float2(0.f, 0.f);
   }
   __syncthreads();
-  float2* R = lds_fft(A, B, nb, stride, pl, tw);
-  const int hy = U.ny / 2;
-  float2* h = H + U.grid_off;
-  for (int t = threadIdx.x; t < nb * U.ny; t += blockDim.x) {
+  float2* R = lds_fft(A, B, nb, stride, n, W);
+  const int hy = it.ny / 2;
+  float2* h = H + it.grid_off;
+  for (int t = threadIdx.x; t < nb * it.ny; t += blockDim.x) {
     const int iy = t / nb, b = t - iy * nb;
     int r = iy - hy;
     if (r < 0) r += n;
-    h[(long long)iy * U.nf1 + it.first + b] = R[b * stride + r];
+    h[(long long)iy * it.nf1 + it.first + b] = R[b * stride + r];
   }
 }
 
@@ -495,30 +529,29 @@ k_t1_cols(const RowItem* __restrict__ items, const UnitDev* __restrict__ units,
 // deconvolve, real part -> grad2d (FP64).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kThreads)
-k_t1_rows(const RowItem* __restrict__ items, const UnitDev* __restrict__ units,
-          const FftPlanDev* __restrict__ plans, const float2* __restrict__ tw,
+k_t1_rows(const FftItem* __restrict__ items, const float2* __restrict__ tw,
           const float* __restrict__ corr, const float2* __restrict__ H,
           double* __restrict__ grad2d) {
   extern __shared__ float2 sm[];
-  const RowItem it = items[blockIdx.x];
-  const UnitDev U = units[it.unit];
-  const FftPlanDev pl = plans[U.fft1];
-  const int n = U.nf1, nb = it.count;
+  const FftItem it = items[blockIdx.x];
+  const int n = it.nf1, nb = it.count;
   float2* A = sm;
   float2* B = sm + nb * n;
-  const float2* h = H + U.grid_off + (long long)it.first * n;
+  float2* W = B + nb * n;
+  stage_twiddles(W, tw, it.tw1, n);
+  const float2* h = H + it.grid_off + (long long)it.first * n;
   for (int t = threadIdx.x; t < nb * n; t += blockDim.x) A[t] = h[t];
   __syncthreads();
-  float2* R = lds_fft(A, B, nb, n, pl, tw);
-  const float* cx = corr + U.cx_off;
-  const float* cy = corr + U.cy_off;
-  const int hx = U.nx / 2;
-  for (int t = threadIdx.x; t < nb * U.nx; t += blockDim.x) {
-    const int b = t / U.nx, ix = t - b * U.nx;
+  float2* R = lds_fft(A, B, nb, n, n, W);
+  const float* cx = corr + it.cx_off;
+  const float* cy = corr + it.cy_off;
+  const int hx = it.nx / 2;
+  for (int t = threadIdx.x; t < nb * it.nx; t += blockDim.x) {
+    const int b = t / it.nx, ix = t - b * it.nx;
     const int iy = it.first + b;
     int c = ix - hx;
     if (c < 0) c += n;
-    grad2d[U.img_off + (long long)iy * U.nx + ix] =
+    grad2d[it.img_off + (long long)iy * it.nx + ix] =
         (double)R[b * n + c].x * (double)cx[ix] * (double)cy[iy];
   }
 }
@@ -753,14 +786,13 @@ k_reg(const RegItem* __restrict__ items, const UnitDev* __restrict__ units,
 }
 
 // ---------------------------------------------------------------------------
-// one block per problem: fixed-order sums of the partials -> 15 terms
+// fixed-order sums of one problem's partials -> its 15 terms
 // ---------------------------------------------------------------------------
-__global__ void k_finalize(const ProbDev* __restrict__ probs, const UnitDev* __restrict__ units,
-                           const RegItem* __restrict__ reg_items, const double* __restrict__ cpart,
-                           const double* __restrict__ rpart, const UnitMoments* __restrict__ mom,
-                           double* __restrict__ terms) {
-  __shared__ double red[16 * (kThreads / 64)];
-  const ProbDev P = probs[blockIdx.x];
+
+__device__ void finalize_problem(const ProbDev& P, const UnitDev* __restrict__ units,
+                                 const RegItem* __restrict__ reg_items, const double* cpart,
+                                 const double* rpart, const UnitMoments* __restrict__ mom,
+                                 double* __restrict__ terms, double* red) {
   double v[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) v[k] = 0.0;
@@ -812,6 +844,16 @@ __global__ void k_finalize(const ProbDev* __restrict__ probs, const UnitDev* __r
     T[11] = tfd; T[12] = cen; T[13] = smc;
     T[14] = chisq + reg;
   }
+}
+
+// one block per problem
+__global__ void __launch_bounds__(kThreads)
+k_finalize(const ProbDev* __restrict__ probs, const UnitDev* __restrict__ units,
+           const RegItem* __restrict__ reg_items, const double* __restrict__ cpart,
+           const double* __restrict__ rpart, const UnitMoments* __restrict__ mom,
+           double* __restrict__ terms) {
+  __shared__ double red[16 * (kThreads / 64)];
+  finalize_problem(probs[blockIdx.x], units, reg_items, cpart, rpart, mom, terms, red);
 }
 
 }  // namespace smili

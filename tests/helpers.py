@@ -43,6 +43,7 @@ def random_problem(rng, Nx, Ny, M=300, nfcv=200, namp=150, ncp=120, nca=100,
         j = np.abs(idx) - 1
         return np.where(idx > 0, Vt[j], np.conj(Vt[j]))
 
+    sd = noise if noise > 0 else 0.05          # variances stay positive without noise
     kw = dict(Nx=Nx, Ny=Ny, xidx=xidx, yidx=yidx, u=u, v=v,
               Nxref=Nx / 2.0 + 0.5, Nyref=Ny / 2.0 + 0.5)
     n_on = dict(fcv=nfcv, amp=namp, cp=ncp, ca=nca)
@@ -50,21 +51,21 @@ def random_problem(rng, Nx, Ny, M=300, nfcv=200, namp=150, ncp=120, nca=100,
     if nfcv:
         idx = sidx(nfcv)
         Vo = model(idx) + noise * scale * (rng.standard_normal(nfcv) + 1j * rng.standard_normal(nfcv))
-        kw.update(uvidxfcv=idx, Vfcv=Vo, Varfcv=(noise * scale) ** 2 * rng.uniform(0.5, 2, nfcv), wfcv=wt["fcv"])
+        kw.update(uvidxfcv=idx, Vfcv=Vo, Varfcv=(sd * scale) ** 2 * rng.uniform(0.5, 2, nfcv), wfcv=wt["fcv"])
     if namp:
         idx = sidx(namp)
         Ao = np.abs(model(idx)) * (1 + noise * rng.standard_normal(namp))
-        kw.update(uvidxamp=idx, Vamp=Ao, Varamp=(noise * scale) ** 2 * rng.uniform(0.5, 2, namp), wamp=wt["amp"])
+        kw.update(uvidxamp=idx, Vamp=Ao, Varamp=(sd * scale) ** 2 * rng.uniform(0.5, 2, namp), wamp=wt["amp"])
     if ncp:
         idx = sidx(ncp, 3)
         ph = sum(np.angle(model(idx[k])) for k in range(3))
         kw.update(uvidxcp=idx, CP=ph + noise * rng.standard_normal(ncp),
-                  Varcp=noise ** 2 * rng.uniform(0.5, 2, ncp), wcp=wt["cp"])
+                  Varcp=sd ** 2 * rng.uniform(0.5, 2, ncp), wcp=wt["cp"])
     if nca:
         idx = sidx(nca, 4)
         a = [np.log(np.abs(model(idx[k]))) for k in range(4)]
         kw.update(uvidxca=idx, CA=a[0] + a[1] - a[2] - a[3] + noise * rng.standard_normal(nca),
-                  Varca=noise ** 2 * rng.uniform(0.5, 2, nca), wca=wt["ca"])
+                  Varca=sd ** 2 * rng.uniform(0.5, 2, nca), wca=wt["ca"])
     for r in regs:
         lam = float(rng.uniform(0.1, 2.0))
         if r in ("kl", "gs"):
@@ -81,7 +82,9 @@ def random_problem(rng, Nx, Ny, M=300, nfcv=200, namp=150, ncp=120, nca=100,
         elif r == "sm":
             kw.update(sm_l=0.01, sm_maj=6.0, sm_min=3.0, sm_phi=0.3)
     x = np.abs(truth * rng.uniform(0.7, 1.3, npix))
-    return O.Problem(**kw), x
+    prob = O.Problem(**kw)
+    prob.extra["truth"] = truth
+    return prob, x
 
 
 def problem_kwargs(p: O.Problem):

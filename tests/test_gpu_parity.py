@@ -1,12 +1,21 @@
 """GPU parity of the HIP cost against the FP64 oracle, through the C ABI.
 
-Tolerances (FP32 spreading / interpolation / FFT with an ES kernel of width
-7 at eps 1e-6, FP64 chi-square and regularizer accumulation):
-  * type-2 model visibilities vs the exact DFT: rel-l2 <= 4e-6
-  * type-1 adjoint vs the exact DFT: rel-l2 <= 4e-6
-  * cost vs the exact-DFT cost: rel <= 2e-6; each chi-square term rel <= 2e-5
-  * gradient vs the exact-DFT gradient: rel-l2 <= 1e-5
-  * regularizer terms (FP64 end to end): rel <= 1e-12
+The GPU runs the ES-kernel NUFFT (width 7, eps 1e-6) with FP32 spreading,
+interpolation and FFT, and FP64 chi-square / regularizer arithmetic.  Two
+oracles check it:
+
+* the FP64 ES-kernel NUFFT oracle (same approximation, FP64 arithmetic):
+  isolates the FP32 error of the GPU path --
+  cost rel <= 1e-6, model visibilities rel-l2 <= 1e-6, gradient rel-l2 <=
+  5e-6 + 0.5 e_es (the closure terms' 1/|V|^2 factors amplify the FP32
+  rounding of V exactly as they amplify the kernel error e_es below);
+* the exact FP64 direct DFT (what the ES kernel approximates): the GPU must
+  be as close to it as the FP64 ES oracle is, up to the FP32 error --
+  visibilities / adjoint rel-l2 <= 4e-6, cost rel <= 2e-6 + 2 e_es,
+  gradient rel-l2 <= 5e-6 + 1.5 e_es, where e_es is the FP64 ES oracle's own
+  error against the DFT on the same case (1e-6 .. 2e-5 here: the 1/|V|^2
+  factors of the closure terms amplify the kernel's eps).
+Regularizer terms are FP64 end to end: rel <= 1e-12.
 """
 import math
 
@@ -19,9 +28,9 @@ from tests.helpers import problem_kwargs, random_problem, rel_l2
 pytestmark = pytest.mark.gpu
 
 TOL_V = 4e-6
-TOL_COST = 2e-6
-TOL_TERM = 2e-5
-TOL_GRAD = 1e-5
+TOL_V_ES = 1e-6
+TOL_COST_ES = 1e-6
+TOL_GRAD_ES = 5e-6
 TOL_REG = 1e-12
 
 
@@ -39,10 +48,13 @@ def test_type2_matches_dft(gpu_lib, shape):
     p, x = random_problem(rng, shape[0], shape[1], M=400)
     with _mk(p) as g:
         V = g.model_vis(x)
-    Vd = O.dft_fwd(O.image_1d_to_2d(x, p.xidx, p.yidx, p.Nx, p.Ny), p.u, p.v)
+    I2d = O.image_1d_to_2d(x, p.xidx, p.yidx, p.Nx, p.Ny)
+    Vd = O.dft_fwd(I2d, p.u, p.v)
+    Ve = O.es_nufft_type2(I2d, p.u, p.v)
     err = rel_l2(V, Vd)
-    print("type2", shape, err)
+    print("type2", shape, "vs DFT", err, "vs ES64", rel_l2(V, Ve))
     assert err <= TOL_V
+    assert rel_l2(V, Ve) <= TOL_V_ES
 
 
 @pytest.mark.parametrize("shape", SHAPES)
@@ -59,21 +71,27 @@ def test_type1_matches_dft(gpu_lib, shape):
     assert err <= TOL_V
 
 
-def _check_cost(p, x, g, tol_cost=TOL_COST, tol_grad=TOL_GRAD):
+def _check_cost(p, x, g):
     c_ref, g_ref, t_ref = O.calc_cost(x, p)
+    c_es, g_es, t_es = O.calc_cost(x, p, fwd=O.es_nufft_type2, adj=O.es_nufft_type1)
     c, gr, t = g.cost_grad(x)
-    errs = {k: abs(t[k] - t_ref[k]) / max(abs(t_ref[k]), 1e-300) for k in O.TERM_NAMES if t_ref[k] != 0}
-    print("cost", abs(c - c_ref) / abs(c_ref), "grad", rel_l2(gr, g_ref), errs)
-    assert abs(c - c_ref) <= tol_cost * abs(c_ref)
+    e_es_c = abs(c_es - c_ref) / abs(c_ref)
+    e_es_g = rel_l2(g_es, g_ref)
+    print("vs ES64: cost %.2e grad %.2e | vs DFT: cost %.2e grad %.2e | ES64 vs DFT: cost %.2e grad %.2e" % (
+        abs(c - c_es) / abs(c_es), rel_l2(gr, g_es), abs(c - c_ref) / abs(c_ref), rel_l2(gr, g_ref),
+        e_es_c, e_es_g))
+    assert abs(c - c_es) <= TOL_COST_ES * abs(c_es)
+    assert rel_l2(gr, g_es) <= TOL_GRAD_ES + 0.5 * e_es_g
+    assert abs(c - c_ref) <= (2e-6 + 2 * e_es_c) * abs(c_ref)
+    assert rel_l2(gr, g_ref) <= 5e-6 + 1.5 * e_es_g
     for k in ("chisq_fcv", "chisq_amp", "chisq_cp", "chisq_ca"):
-        if t_ref[k] != 0:
-            assert abs(t[k] - t_ref[k]) <= TOL_TERM * abs(t_ref[k]), k
+        if t_es[k] != 0:
+            assert abs(t[k] - t_es[k]) <= 1e-5 * abs(t_es[k]), k
     for k in ("l1", "tv", "tsv", "kl", "gs", "tfd", "cen", "sm"):
         if t_ref[k] != 0:
             assert abs(t[k] - t_ref[k]) <= TOL_REG * abs(t_ref[k]) + 1e-300, k
         else:
             assert t[k] == 0.0, k
-    assert rel_l2(gr, g_ref) <= tol_grad
 
 
 @pytest.mark.parametrize("shape,window", [((24, 20), False), ((32, 32), True), ((17, 13), False),
@@ -128,7 +146,7 @@ def test_graph_and_eager_agree_and_repeat(gpu_lib):
     assert rel_l2(gr1b, gr1) < 1e-6 and rel_l2(gr2, gr1) < 1e-6
 
 
-@pytest.mark.parametrize("tune", [dict(bin=8, sub_points=64), dict(bin=32, sub_points=1024, col_group=8),
+@pytest.mark.parametrize("tune", [dict(bin=8, sub_points=64), dict(bin=32, sub_points=512, col_group=4),
                                   dict(chunk=64)])
 def test_tunables_do_not_change_results(gpu_lib, tune):
     rng = np.random.default_rng(21)
@@ -156,8 +174,9 @@ def test_points_on_the_period_boundary(gpu_lib):
 
 
 def test_nonfinite_is_reported(gpu_lib):
-    """A zero image gives |V| = 0 and the amplitude term divides by it
-    (fftlib.f90:488): the reference returns NaN silently, the ABI flags it."""
+    """A zero image gives |V| = 0 and the amplitude gradient divides by it
+    (fftlib.f90:488): the reference returns a finite cost with a NaN
+    gradient silently; the ABI returns the same values and flags them."""
     from smili_amd._lib import SmiliError
     rng = np.random.default_rng(4)
     p, x = random_problem(rng, 16, 16, M=50, nfcv=0, ncp=0, nca=0, regs=())
@@ -166,4 +185,15 @@ def test_nonfinite_is_reported(gpu_lib):
             g.cost_grad(np.zeros_like(x))
         assert e.value.code == -4
         c, gr, t = g.cost_grad(np.zeros_like(x), allow_nonfinite=True)
-        assert not math.isfinite(c)
+    c_ref = O.calc_cost(np.zeros_like(x), p)[0]
+    assert math.isfinite(c) and abs(c - c_ref) <= 1e-12 * abs(c_ref)
+    assert not np.all(np.isfinite(gr))
+
+
+def test_chunks_split_at_the_slot_limit(gpu_lib):
+    """Random indices over 3000 uv points: every chi-square work item hits
+    the 1024 distinct-point limit and splits."""
+    rng = np.random.default_rng(41)
+    p, x = random_problem(rng, 32, 32, M=3000, nfcv=500, namp=800, ncp=1500, nca=2000)
+    with _mk(p) as g:
+        _check_cost(p, x, g)
