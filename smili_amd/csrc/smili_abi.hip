@@ -164,6 +164,7 @@ struct Plan {
   hipStream_t stream = nullptr;
   smili_tune tune{};
   int bin = 16, sub_points = 256, col_group = 2, chunk = 2048, rows_per_block = 1;
+  bool fp64 = false;                       // NUFFT arithmetic in FP64 instead of FP32
   int threads_rows = 64, threads_cols = 128;
 
   std::vector<UnitDev> units;
@@ -184,14 +185,17 @@ struct Plan {
   DevBuf<UnitDev> d_units;
   DevBuf<ProbDev> d_probs;
   DevBuf<float2> d_tw;
+  DevBuf<double2> d_twd;
   DevBuf<float> d_corr;
+  DevBuf<double> d_corrd;
   DevBuf<int> d_pix;
   DevBuf<double> d_wgt;
   DevBuf<int> d_cell;
   DevBuf<float> d_ex, d_ey;
-  DevBuf<float2> d_V;
+  DevBuf<double> d_exd, d_eyd;
+  DevBuf<double2> d_V;                     // float2 or double2 visibilities (sized for double2)
   DevBuf<double2> d_Vres;
-  DevBuf<float2> d_gridA, d_gridB;
+  DevBuf<double2> d_gridA, d_gridB;        // float2 or double2 grids (sized for double2)
   DevBuf<double> d_I2d, d_grad2d;
   DevBuf<double> d_x, d_grad, d_terms;
   DevBuf<double> d_cpart, d_rpart, d_mpart;
@@ -285,6 +289,8 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   sub_points = tune.sub_points > 0 ? tune.sub_points : 256;
   col_group = tune.col_group > 0 ? tune.col_group : 2;
   chunk = tune.chunk > 0 ? tune.chunk : 2048;
+  require(tune.precision == 0 || tune.precision == 1, "precision must be 0 (FP32) or 1 (FP64)");
+  fp64 = tune.precision == 1;
   require(bin >= 8 && bin <= 64, "bin must be in [8, 64]");
   require(sub_points >= 32 && sub_points <= 65536, "sub_points must be in [32, 65536]");
   require(col_group >= 1 && col_group <= 16, "col_group must be in [1, 16]");
@@ -298,6 +304,7 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   std::map<int, int> plan_of;
   std::vector<FftPlanDev> plans;
   std::vector<float2> tw;
+  std::vector<double2> twd;
   auto get_plan = [&](int n) {
     auto it = plan_of.find(n);
     if (it != plan_of.end()) return it->second;
@@ -311,6 +318,7 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     for (int k = 0; k < n; ++k) {
       const double a = 2.0 * M_PI * k / n;
       tw.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
+      twd.push_back(make_double2(std::cos(a), std::sin(a)));
     }
     plans.push_back(p);
     plan_of[n] = (int)plans.size() - 1;
@@ -318,13 +326,17 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   };
   std::map<std::pair<int, int>, int> corr_of;
   std::vector<float> corr;
+  std::vector<double> corrd;
   auto get_corr = [&](int n, int nf) {
     auto key = std::make_pair(n, nf);
     auto it = corr_of.find(key);
     if (it != corr_of.end()) return it->second;
     auto c = es_correction(n, nf);
     int off = (int)corr.size();
-    for (double v : c) corr.push_back((float)v);
+    for (double v : c) {
+      corr.push_back((float)v);
+      corrd.push_back(v);
+    }
     corr_of[key] = off;
     return off;
   };
@@ -399,6 +411,7 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   // points: grid coordinates, binning (counting sort), subproblems
   std::vector<int> cell;
   std::vector<float> ex, ey;
+  std::vector<double> exd, eyd;
   std::vector<std::vector<int>> unit_rank(us.size());   // local uv index -> local sorted position
   npt = 0;
   for (size_t ui = 0; ui < us.size(); ++ui) {
@@ -407,11 +420,11 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     U.pt_off = (int)npt;
     const int nbx = (U.nf1 + bin - 1) / bin, nby = (U.nf2 + bin - 1) / bin;
     std::vector<int> l0x(m), l0y(m), bidx(m);
-    std::vector<float> exu(m), eyu(m);
+    std::vector<double> exu(m), eyu(m);
     for (int j = 0; j < m; ++j) {
       const double uu = us[ui].u[j], vv = us[ui].v[j];
       require(std::isfinite(uu) && std::isfinite(vv), "non-finite uv coordinate");
-      auto coord = [&](double x, int nf, int& l0, float& e) {
+      auto coord = [&](double x, int nf, int& l0, double& e) {
         double tt = std::fmod(x * nf / (2.0 * M_PI), (double)nf);
         if (tt < 0) tt += nf;
         if (tt >= nf) tt -= nf;
@@ -421,7 +434,7 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
         li %= nf;
         if (li < 0) li += nf;
         l0 = li;
-        e = (float)ee;
+        e = ee;
       };
       coord(uu, U.nf1, l0x[j], exu[j]);
       coord(vv, U.nf2, l0y[j], eyu[j]);
@@ -446,8 +459,10 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
       const int j = order[s];
       unit_rank[ui][j] = s;
       cell.push_back(l0x[j] | (l0y[j] << 16));
-      ex.push_back(exu[j]);
-      ey.push_back(eyu[j]);
+      ex.push_back((float)exu[j]);
+      ey.push_back((float)eyu[j]);
+      exd.push_back(exu[j]);
+      eyd.push_back(eyu[j]);
     }
     // spreading work items: 8x8 blocks of first-tap cells, occupied cells
     // with their (contiguous, sorted) point ranges
@@ -462,10 +477,15 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
         const int lc = (l0y[j] % kCellBlock) * kCellBlock + (l0x[j] % kCellBlock);
         auto& spans = blocks[(long long)byk * 65536 + bxk];
         for (int a = s; a < e; a += kSpanPoints)
-          spans.push_back(CellSpan{lc, (int)U.pt_off + a, (int)U.pt_off + std::min(e, a + kSpanPoints), 0});
+          spans.push_back(CellSpan{lc, (int)U.pt_off + a, (int)U.pt_off + std::min(e, a + kSpanPoints),
+                                   (a - s) / kSpanPoints});
         s = e;
       }
       for (auto& kv : blocks) {
+        // interleave: first span of every cell, then second spans, ... so
+        // that neighbouring threads work on different cells
+        std::stable_sort(kv.second.begin(), kv.second.end(),
+                         [](const CellSpan& a, const CellSpan& b) { return a.rank < b.rank; });
         for (size_t a = 0; a < kv.second.size(); a += kSpansPerGroup) {
           CellBlock cbk{};
           cbk.nf1 = U.nf1;
@@ -636,10 +656,11 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   }
 
   // LDS budget
-  lds_rows = (2ull * rows_per_block + 1) * max_nf1 * sizeof(float2);
-  lds_cols = (2ull * col_group * (max_nf2 + 1) + max_nf2) * sizeof(float2);
+  const size_t vsz = fp64 ? sizeof(double2) : sizeof(float2);
+  lds_rows = (2ull * rows_per_block + 1) * max_nf1 * vsz;
+  lds_cols = (2ull * col_group * (max_nf2 + 1) + max_nf2) * vsz;
   const int T = bin + kW - 1;
-  lds_tile = (size_t)T * T * sizeof(float2);
+  lds_tile = (size_t)T * T * vsz;
   require(lds_rows <= 65536 && lds_cols <= 65536 && lds_tile <= 65536,
           "LDS budget exceeded: lower col_group or bin");
 
@@ -657,11 +678,15 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   d_units.upload(units);
   d_probs.upload(probs);
   d_tw.upload(tw);
+  d_twd.upload(twd);
   d_corr.upload(corr);
+  d_corrd.upload(corrd);
   d_pix.upload(pix);
   d_cell.upload(cell);
   d_ex.upload(ex);
   d_ey.upload(ey);
+  d_exd.upload(exd);
+  d_eyd.upload(eyd);
   d_V.alloc(std::max<size_t>(npt, 1));
   d_Vres.alloc(std::max<size_t>(npt, 1));
   d_Vres.zero();
@@ -714,27 +739,64 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     }                                                                         \
   } while (0)
 
+template <typename R>
+static void launch_forward_t(Plan& P) {
+  using V = typename Cx<R>::T;
+  hipStream_t stream = P.stream;
+  const bool prof = P.prof;
+  auto& prof_ev = P.prof_ev;
+  auto& prof_used = P.prof_used;
+  int& launches = P.launches;
+  const R beta = (R)kBeta;
+  const V* tw = reinterpret_cast<const V*>(P.fp64 ? (const void*)P.d_twd.p : (const void*)P.d_tw.p);
+  const R* corr = reinterpret_cast<const R*>(P.fp64 ? (const void*)P.d_corrd.p : (const void*)P.d_corr.p);
+  const R* ex = reinterpret_cast<const R*>(P.fp64 ? (const void*)P.d_exd.p : (const void*)P.d_ex.p);
+  const R* ey = reinterpret_cast<const R*>(P.fp64 ? (const void*)P.d_eyd.p : (const void*)P.d_ey.p);
+  V* gA = reinterpret_cast<V*>(P.d_gridA.p);
+  V* Vv = reinterpret_cast<V*>(P.d_V.p);
+  if (P.any_window)
+    LAUNCH(SMILI_S_SCATTER, k_scatter, P.h_regs.size(), kThreads, 0, P.d_regs.p, P.d_units.p, P.d_pix.p, P.d_x.p,
+           P.d_I2d.p);
+  if (P.npt == 0) return;
+  LAUNCH(SMILI_S_T2_ROWS, k_t2_rows<R>, P.h_t2rows.size(), P.threads_rows, P.lds_rows, P.d_t2rows.p, tw, corr,
+         P.d_x.p, P.d_I2d.p, gA);
+  LAUNCH(SMILI_S_T2_COLS, k_t2_cols<R>, P.h_cols.size(), P.threads_cols, P.lds_cols, P.d_cols.p, tw, gA);
+  LAUNCH(SMILI_S_INTERP, k_interp<R>, P.h_subs.size(), kThreads, P.lds_tile, P.d_subs.p, gA, P.d_cell.p, ex, ey,
+         Vv, P.d_Vres.p, P.bin, beta);
+}
+
+template <typename R>
+static void launch_adjoint_t(Plan& P, bool with_rows) {
+  using V = typename Cx<R>::T;
+  hipStream_t stream = P.stream;
+  const bool prof = P.prof;
+  auto& prof_ev = P.prof_ev;
+  auto& prof_used = P.prof_used;
+  int& launches = P.launches;
+  if (P.npt == 0) return;
+  const R beta = (R)kBeta;
+  const V* tw = reinterpret_cast<const V*>(P.fp64 ? (const void*)P.d_twd.p : (const void*)P.d_tw.p);
+  const R* corr = reinterpret_cast<const R*>(P.fp64 ? (const void*)P.d_corrd.p : (const void*)P.d_corr.p);
+  const R* ex = reinterpret_cast<const R*>(P.fp64 ? (const void*)P.d_exd.p : (const void*)P.d_ex.p);
+  const R* ey = reinterpret_cast<const R*>(P.fp64 ? (const void*)P.d_eyd.p : (const void*)P.d_ey.p);
+  V* gA = reinterpret_cast<V*>(P.d_gridA.p);
+  V* gB = reinterpret_cast<V*>(P.d_gridB.p);
+  LAUNCH(SMILI_S_SPREAD, k_spread<R>, P.h_cblocks.size(), kSpansPerGroup, 0, P.d_cblocks.p, P.d_spans.p, ex, ey,
+         P.d_Vres.p, gB, beta);
+  LAUNCH(SMILI_S_T1_COLS, k_t1_cols<R>, P.h_cols.size(), P.threads_cols, P.lds_cols, P.d_cols.p, tw, gB, gA);
+  if (with_rows)
+    LAUNCH(SMILI_S_T1_ROWS, k_t1_rows<R>, P.h_t1rows.size(), P.threads_rows, P.lds_rows, P.d_t1rows.p, tw, corr,
+           gA, P.d_grad2d.p);
+}
+
 void Plan::launch_forward() {
-  const float beta = (float)kBeta;
-  if (any_window)
-    LAUNCH(SMILI_S_SCATTER, k_scatter, h_regs.size(), kThreads, 0, d_regs.p, d_units.p, d_pix.p, d_x.p, d_I2d.p);
-  if (npt == 0) return;
-  LAUNCH(SMILI_S_T2_ROWS, k_t2_rows, h_t2rows.size(), threads_rows, lds_rows, d_t2rows.p, d_tw.p, d_corr.p, d_x.p,
-         d_I2d.p, d_gridA.p);
-  LAUNCH(SMILI_S_T2_COLS, k_t2_cols, h_cols.size(), threads_cols, lds_cols, d_cols.p, d_tw.p, d_gridA.p);
-  LAUNCH(SMILI_S_INTERP, k_interp, h_subs.size(), kThreads, lds_tile, d_subs.p, d_gridA.p, d_cell.p, d_ex.p,
-         d_ey.p, d_V.p, d_Vres.p, bin, beta);
+  if (fp64) launch_forward_t<double>(*this);
+  else launch_forward_t<float>(*this);
 }
 
 void Plan::launch_adjoint() {
-  if (npt == 0) return;
-  const float beta = (float)kBeta;
-  LAUNCH(SMILI_S_SPREAD, k_spread, h_cblocks.size(), kSpansPerGroup, 0, d_cblocks.p, d_spans.p, d_ex.p, d_ey.p, d_Vres.p,
-         d_gridB.p, beta);
-  LAUNCH(SMILI_S_T1_COLS, k_t1_cols, h_cols.size(), threads_cols, lds_cols, d_cols.p, d_tw.p, d_gridB.p,
-         d_gridA.p);
-  LAUNCH(SMILI_S_T1_ROWS, k_t1_rows, h_t1rows.size(), threads_rows, lds_rows, d_t1rows.p, d_tw.p, d_corr.p,
-         d_gridA.p, d_grad2d.p);
+  if (fp64) launch_adjoint_t<double>(*this, true);
+  else launch_adjoint_t<float>(*this, true);
 }
 
 void Plan::launch_eval() {
@@ -745,8 +807,12 @@ void Plan::launch_eval() {
     LAUNCH(SMILI_S_MOMENTS_REDUCE, k_moments_reduce, units.size(), 64, 0, d_units.p, d_unit_reg_begin.p, d_mpart.p, d_mom.p);
   }
   if (npt > 0) {
-    LAUNCH(SMILI_S_CHISQ, k_chisq, h_chunks.size(), kThreads, 0, d_chunks.p, d_slots.p, d_fcv.p, d_amp.p, d_cp.p,
-           d_ca.p, d_V.p, d_Vres.p, d_cpart.p);
+    if (fp64)
+      LAUNCH(SMILI_S_CHISQ, k_chisq<double2>, h_chunks.size(), kThreads, 0, d_chunks.p, d_slots.p, d_fcv.p, d_amp.p,
+             d_cp.p, d_ca.p, d_V.p, d_Vres.p, d_cpart.p);
+    else
+      LAUNCH(SMILI_S_CHISQ, k_chisq<float2>, h_chunks.size(), kThreads, 0, d_chunks.p, d_slots.p, d_fcv.p, d_amp.p,
+             d_cp.p, d_ca.p, reinterpret_cast<const float2*>(d_V.p), d_Vres.p, d_cpart.p);
     launch_adjoint();
   }
   LAUNCH(SMILI_S_REG, k_reg, h_regs.size(), kThreads, 0, d_regs.p, d_units.p, d_pix.p, d_x.p, d_I2d.p, d_grad2d.p,
@@ -904,8 +970,15 @@ int smili_model_vis(smili_problem* p, const double* x, double* vre, double* vim)
     Plan& P = p->plan;
     HIP_CHECK(hipMemcpyAsync(P.d_x.p, x, P.npix_total * sizeof(double), hipMemcpyHostToDevice, P.stream));
     P.launch_forward();
-    std::vector<float2> V(P.npt);
-    HIP_CHECK(hipMemcpyAsync(V.data(), P.d_V.p, P.npt * sizeof(float2), hipMemcpyDeviceToHost, P.stream));
+    std::vector<double2> V(P.npt);
+    if (P.fp64) {
+      HIP_CHECK(hipMemcpyAsync(V.data(), P.d_V.p, P.npt * sizeof(double2), hipMemcpyDeviceToHost, P.stream));
+    } else {
+      std::vector<float2> Vf(P.npt);
+      HIP_CHECK(hipMemcpyAsync(Vf.data(), P.d_V.p, P.npt * sizeof(float2), hipMemcpyDeviceToHost, P.stream));
+      HIP_CHECK(hipStreamSynchronize(P.stream));
+      for (size_t j = 0; j < P.npt; ++j) V[j] = make_double2(Vf[j].x, Vf[j].y);
+    }
     HIP_CHECK(hipStreamSynchronize(P.stream));
     const auto& np = P.newpos[0];
     for (size_t j = 0; j < np.size(); ++j) {

@@ -76,7 +76,7 @@ constexpr int kCellBlock = 8;
 constexpr int kSpanPoints = 8;
 constexpr int kSpansPerGroup = 256;
 struct CellBlock { int gx0, gy0, cell_begin, cell_count, nf1, nf2; long long grid_off; };
-struct CellSpan { int lcell, pt_begin, pt_end, pad; };   // lcell = ly * 8 + lx
+struct CellSpan { int lcell, pt_begin, pt_end, rank; };  // lcell = ly * 8 + lx; rank in its cell
 
 struct ProbDev {
   int chunk_begin[4], chunk_end[4];  // chi-square chunk ranges per data type

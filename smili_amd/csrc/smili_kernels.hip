@@ -1,279 +1,15 @@
-// gfx950 kernels of the RML imaging cost: ES-kernel NUFFT (type 2 forward,
-// type 1 adjoint) on a 2x oversampled grid with LDS-resident Stockham FFTs,
-// fused chi-square terms with FP64 accumulation, and the pixel regularizers.
+// gfx950 kernels of the RML imaging cost: the fused chi-square terms with
+// FP64 accumulation, the pixel regularizers and the final reductions (the
+// ES-kernel NUFFT kernels are in smili_nufft.hip).
 //
 // Reference algorithm: fftim2d.calc_cost (src/fortlib/fftim2d.f90:387-607),
 // fftlib.calc_chisq (src/fortlib/fftlib.f90:250-655), image.calc_cost_reg
 // (src/fortlib/image.f90:12-200, 245-754, 760-1158) and the FINUFFT calls of
 // src/wrapfinufft/wrapfinufft.f90:94-206 (both transforms use the +i sign).
 #include "smili_kernels.h"
+#include "smili_nufft.hip"
 
 namespace smili {
-
-// ---------------------------------------------------------------------------
-// complex helpers
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
-__device__ __forceinline__ float2 mul_i(float2 a) { return make_float2(-a.y, a.x); }
-
-// ---------------------------------------------------------------------------
-// small DFTs with the +i sign: y_m = sum_q v_q exp(+2 pi i q m / r)
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void dft2(float2* v) {
-  float2 a = v[0], b = v[1];
-  v[0] = cadd(a, b);
-  v[1] = csub(a, b);
-}
-
-__device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
-  float2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
-  float2 t2 = cadd(a1, a3), t3 = mul_i(csub(a1, a3));
-  a0 = cadd(t0, t2);
-  a2 = csub(t0, t2);
-  float2 y1 = cadd(t1, t3), y3 = csub(t1, t3);
-  a1 = y1;
-  a3 = y3;
-}
-
-__device__ __forceinline__ void dft3(float2* v) {
-  const float s3 = 0.86602540378443864676f;
-  float2 t = cadd(v[1], v[2]);
-  float2 m = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
-  float2 d = mul_i(cscale(csub(v[1], v[2]), s3));
-  v[0] = cadd(v[0], t);
-  v[1] = cadd(m, d);
-  v[2] = csub(m, d);
-}
-
-__device__ __forceinline__ void dft5(float2* v) {
-  const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
-  const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
-  float2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
-  float2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
-  float2 a0 = v[0];
-  float2 b1 = make_float2(a0.x + c1 * t1.x + c2 * t2.x, a0.y + c1 * t1.y + c2 * t2.y);
-  float2 b2 = make_float2(a0.x + c2 * t1.x + c1 * t2.x, a0.y + c2 * t1.y + c1 * t2.y);
-  float2 d1 = mul_i(make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y));
-  float2 d2 = mul_i(make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y));
-  v[0] = cadd(a0, cadd(t1, t2));
-  v[1] = cadd(b1, d1);
-  v[4] = csub(b1, d1);
-  v[2] = cadd(b2, d2);
-  v[3] = csub(b2, d2);
-}
-
-__device__ __forceinline__ void dft8(float2* v) {
-  const float s = 0.70710678118654752440f;
-  float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
-  float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
-  dft4(e0, e1, e2, e3);
-  dft4(o0, o1, o2, o3);
-  // twiddles W8^m = exp(+i pi m / 4)
-  float2 w1 = make_float2(s * (o1.x - o1.y), s * (o1.x + o1.y));
-  float2 w2 = mul_i(o2);
-  float2 w3 = make_float2(-s * (o3.x + o3.y), s * (o3.x - o3.y));
-  v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
-  v[1] = cadd(e1, w1); v[5] = csub(e1, w1);
-  v[2] = cadd(e2, w2); v[6] = csub(e2, w2);
-  v[3] = cadd(e3, w3); v[7] = csub(e3, w3);
-}
-
-// ---------------------------------------------------------------------------
-// LDS Stockham FFT (autosort, +i sign) of nb transforms of length n stored
-// at a + b*stride.  Uses scratch s of the same shape and the twiddle table
-// tw[t] = exp(+2 pi i t / n) staged in LDS.  Returns the buffer that holds
-// the result.  All threads of the block must call it.
-// ---------------------------------------------------------------------------
-__device__ float2* lds_fft(float2* a, float2* s, int nb, int stride, int n, const float2* tw) {
-  int Ns = 1, rem = n;
-  while (rem > 1) {
-    // radices in the order of the host plan: all 8s, then 4, 2, 3, 5
-    const int r = (rem % 8 == 0) ? 8 : (rem % 4 == 0) ? 4 : (rem % 2 == 0) ? 2 : (rem % 3 == 0) ? 3 : 5;
-    rem /= r;
-    const int nbf = n / r;
-    const int total = nbf * nb;
-    const int twstep = n / (Ns * r);
-    for (int t = threadIdx.x; t < total; t += blockDim.x) {
-      const int b = t / nbf;
-      const int i = t - b * nbf;
-      const int k = i % Ns;
-      const float2* in = a + b * stride;
-      float2* out = s + b * stride;
-      float2 v[8];
-#pragma unroll
-      for (int q = 0; q < 8; ++q)
-        if (q < r) v[q] = in[i + q * nbf];
-      if (Ns > 1) {
-        const int step = k * twstep;
-#pragma unroll
-        for (int q = 1; q < 8; ++q)
-          if (q < r) v[q] = cmul(v[q], tw[q * step]);
-      }
-      switch (r) {
-        case 2: dft2(v); break;
-        case 3: dft3(v); break;
-        case 4: dft4(v[0], v[1], v[2], v[3]); break;
-        case 5: dft5(v); break;
-        default: dft8(v); break;
-      }
-      const int base = (i / Ns) * Ns * r + k;
-#pragma unroll
-      for (int q = 0; q < 8; ++q)
-        if (q < r) out[base + q * Ns] = v[q];
-    }
-    __syncthreads();
-    float2* tmp = a; a = s; s = tmp;
-    Ns *= r;
-  }
-  return a;
-}
-
-__device__ __forceinline__ void stage_twiddles(float2* dst, const float2* __restrict__ tw, int off, int n) {
-  for (int t = threadIdx.x; t < n; t += blockDim.x) dst[t] = tw[off + t];
-}
-
-// ---------------------------------------------------------------------------
-// ES kernel weights.  e = d - w/2 in (-1, 0] where d is the distance from the
-// point to its first tap in cells; z_q = (d - q) * 2 / w; phi(z) =
-// exp(-beta z^2 / (1 + sqrt(1 - z^2))) (cancellation-free form of
-// exp(beta (sqrt(1 - z^2) - 1))).
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void es_weights(float e, float beta, float* wt) {
-  const float inv_half = 2.0f / kW;
-#pragma unroll
-  for (int q = 0; q < kW; ++q) {
-    float z = e * inv_half + (1.0f - q * inv_half);
-    float z2 = fminf(z * z, 1.0f);
-    float sq = sqrtf(1.0f - z2);
-    wt[q] = __expf(-beta * z2 / (1.0f + sq));
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Type-2 step 1: image rows -> deconvolve, zero-pad, FFT along x.
-// One work item = `count` consecutive image rows of one unit.
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kThreads)
-k_t2_rows(const FftItem* __restrict__ items, const float2* __restrict__ tw,
-          const float* __restrict__ corr, const double* __restrict__ x,
-          const double* __restrict__ I2dbuf, float2* __restrict__ gridA) {
-  extern __shared__ float2 sm[];
-  const FftItem it = items[blockIdx.x];
-  const int n = it.nf1, nb = it.count;
-  float2* A = sm;
-  float2* B = sm + nb * n;
-  float2* W = B + nb * n;
-  stage_twiddles(W, tw, it.tw1, n);
-  const double* img = it.identity ? (x + it.x_off) : (I2dbuf + it.img_off);
-  const float* cx = corr + it.cx_off;
-  const float* cy = corr + it.cy_off;
-  const int hx = it.nx / 2;
-  for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
-    const int b = t / n, c = t - b * n;
-    // grid column c holds mode k1 = c (c < nf1 - nx/2) or c - nf1
-    const int k1 = (c < n - hx) ? c : c - n;
-    float2 val = make_float2(0.f, 0.f);
-    if (k1 >= -hx && k1 < it.nx - hx) {
-      const int ix = k1 + hx;
-      const int iy = it.first + b;
-      val.x = (float)(img[(long long)iy * it.nx + ix] * (double)cx[ix] * (double)cy[iy]);
-    }
-    A[t] = val;
-  }
-  __syncthreads();
-  float2* R = lds_fft(A, B, nb, n, n, W);
-  const int hy = it.ny / 2;
-  for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
-    const int b = t / n, c = t - b * n;
-    int row = it.first + b - hy;
-    if (row < 0) row += it.nf2;
-    gridA[it.grid_off + (long long)row * n + c] = R[t];
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Type-2 step 2: FFT along y of `count` grid columns; only the ny rows that
-// hold image modes are read (the others are zero by construction).
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kThreads)
-k_t2_cols(const FftItem* __restrict__ items, const float2* __restrict__ tw, float2* __restrict__ gridA) {
-  extern __shared__ float2 sm[];
-  const FftItem it = items[blockIdx.x];
-  const int n = it.nf2, nb = it.count, stride = n + 1;
-  float2* A = sm;
-  float2* B = sm + nb * stride;
-  float2* W = B + nb * stride;
-  stage_twiddles(W, tw, it.tw2, n);
-  const int hy = it.ny / 2;
-  float2* g = gridA + it.grid_off;
-  for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
-    const int r = t / nb, b = t - r * nb;
-    const int k2 = (r < n - hy) ? r : r - n;
-    float2 val = make_float2(0.f, 0.f);
-    if (k2 >= -hy && k2 < it.ny - hy) val = g[(long long)r * it.nf1 + it.first + b];
-    A[b * stride + r] = val;
-  }
-  __syncthreads();
-  float2* R = lds_fft(A, B, nb, stride, n, W);
-  for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
-    const int r = t / nb, b = t - r * nb;
-    g[(long long)r * it.nf1 + it.first + b] = R[b * stride + r];
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Type-2 step 3: interpolation from an LDS-staged tile of the grid.  One work
-// item = up to sub_points points of one bin.  Also clears the residual
-// accumulators of its points for the chi-square kernel.
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kThreads)
-k_interp(const SubProb* __restrict__ sps, const float2* __restrict__ gridA,
-         const int* __restrict__ pt_cell, const float* __restrict__ pt_ex,
-         const float* __restrict__ pt_ey, float2* __restrict__ V, double2* __restrict__ Vres,
-         int bin, float beta) {
-  extern __shared__ float2 tile[];
-  const SubProb sp = sps[blockIdx.x];
-  const int T = bin + kW - 1;
-  const float2* g = gridA + sp.grid_off;
-  const int gx0 = sp.bin_x * bin, gy0 = sp.bin_y * bin;
-  for (int c = threadIdx.x; c < T * T; c += blockDim.x) {
-    const int ty = c / T, tx = c - ty * T;
-    const int gy = (gy0 + ty) % sp.nf2;
-    const int gx = (gx0 + tx) % sp.nf1;
-    tile[c] = g[(long long)gy * sp.nf1 + gx];
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < sp.count; i += blockDim.x) {
-    const int j = sp.start + i;
-    const int cell = pt_cell[j];
-    const int lx = (cell & 0xffff) - gx0;
-    const int ly = (cell >> 16) - gy0;
-    float wx[kW], wy[kW];
-    es_weights(pt_ex[j], beta, wx);
-    es_weights(pt_ey[j], beta, wy);
-    float2 acc = make_float2(0.f, 0.f);
-#pragma unroll
-    for (int qy = 0; qy < kW; ++qy) {
-      const float2* row = tile + (ly + qy) * T + lx;
-      float2 ra = make_float2(0.f, 0.f);
-#pragma unroll
-      for (int qx = 0; qx < kW; ++qx) {
-        const float2 gv = row[qx];
-        ra.x = fmaf(wx[qx], gv.x, ra.x);
-        ra.y = fmaf(wx[qx], gv.y, ra.y);
-      }
-      acc.x = fmaf(wy[qy], ra.x, acc.x);
-      acc.y = fmaf(wy[qy], ra.y, acc.y);
-    }
-    V[j] = acc;
-    Vres[j] = make_double2(0.0, 0.0);
-  }
-}
 
 // ---------------------------------------------------------------------------
 // block reduction of doubles (deterministic: fixed shuffle tree + fixed
@@ -285,9 +21,12 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// red must hold NV * (kThreads / 64) doubles; blockDim.x is a multiple of
+// 64 and at most kThreads.  The result is valid in thread 0.
 template <int NV>
-__device__ __forceinline__ void block_sum(double* v, double* red /* NV * (kThreads/64) */) {
+__device__ __forceinline__ void block_sum(double* v, double* red) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     double s = wave_sum(v[k]);
@@ -298,7 +37,7 @@ __device__ __forceinline__ void block_sum(double* v, double* red /* NV * (kThrea
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       double s = 0.0;
-      for (int w = 0; w < kThreads / 64; ++w) s += red[k * (kThreads / 64) + w];
+      for (int w = 0; w < nw; ++w) s += red[k * (kThreads / 64) + w];
       v[k] = s;
     }
   }
@@ -317,11 +56,12 @@ __device__ __forceinline__ void lds_add2(double2* R, int s, double re, double im
   atomicAdd(&R[s].y, im);
 }
 
+template <typename VT>
 __global__ void __launch_bounds__(kThreads)
 k_chisq(const Chunk* __restrict__ chunks, const int* __restrict__ slots,
         const FcvItem* __restrict__ fcv, const AmpItem* __restrict__ amp,
         const CpItem* __restrict__ cp, const CaItem* __restrict__ ca,
-        const float2* __restrict__ V, double2* __restrict__ Vres,
+        const VT* __restrict__ V, double2* __restrict__ Vres,
         double* __restrict__ partial) {
   __shared__ double2 Vl[kMaxSlots];
   __shared__ double2 Rl[kMaxSlots];
@@ -329,7 +69,7 @@ k_chisq(const Chunk* __restrict__ chunks, const int* __restrict__ slots,
   const Chunk ch = chunks[blockIdx.x];
   const int* sl = slots + ch.slot_off;
   for (int s = threadIdx.x; s < ch.nslots; s += blockDim.x) {
-    const float2 m = V[sl[s]];
+    const VT m = V[sl[s]];
     Vl[s] = make_double2((double)m.x, (double)m.y);
     Rl[s] = make_double2(0.0, 0.0);
   }
@@ -416,144 +156,6 @@ k_chisq(const Chunk* __restrict__ chunks, const int* __restrict__ slots,
   double v[1] = {acc};
   block_sum<1>(v, red);
   if (threadIdx.x == 0) partial[blockIdx.x] = v[0];
-}
-
-// ---------------------------------------------------------------------------
-// Type-1 step 1: spread residuals c = Vres_re - i Vres_im onto the grid.
-// Work item = up to 256 spans of one 8x8 block of first-tap cells; a span is
-// up to 8 consecutive points of one cell (points are sorted by cell, so a
-// span's points share their 7x7 footprint).  One thread per span sums the
-// footprints of its points in registers, adds the sum to an LDS tile of the
-// block (spans of neighbouring cells overlap), and the tile is added to the
-// global grid with float atomics (work items of one block and of
-// neighbouring blocks overlap).  The grid is zero on entry (k_t1_cols
-// clears what it consumes).
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kSpansPerGroup)
-k_spread(const CellBlock* __restrict__ blocks, const CellSpan* __restrict__ spans,
-         const float* __restrict__ pt_ex,
-         const float* __restrict__ pt_ey, const double2* __restrict__ Vres,
-         float2* __restrict__ gridB, float beta) {
-  constexpr int T = kCellBlock + kW - 1;
-  __shared__ float2 tile[T * T];
-  const CellBlock cb = blocks[blockIdx.x];
-  for (int c = threadIdx.x; c < T * T; c += blockDim.x) tile[c] = make_float2(0.f, 0.f);
-  __syncthreads();
-  for (int k = threadIdx.x; k < cb.cell_count; k += blockDim.x) {
-    const CellSpan sp = spans[cb.cell_begin + k];
-    float ar[kW][kW], ai[kW][kW];
-#pragma unroll
-    for (int qy = 0; qy < kW; ++qy)
-#pragma unroll
-      for (int qx = 0; qx < kW; ++qx) {
-        ar[qy][qx] = 0.f;
-        ai[qy][qx] = 0.f;
-      }
-    for (int j = sp.pt_begin; j < sp.pt_end; ++j) {
-      float wx[kW], wy[kW];
-      es_weights(pt_ex[j], beta, wx);
-      es_weights(pt_ey[j], beta, wy);
-      const double2 r = Vres[j];
-      const float cr = (float)r.x, ci = (float)(-r.y);
-#pragma unroll
-      for (int qy = 0; qy < kW; ++qy) {
-        const float yr = wy[qy] * cr, yi = wy[qy] * ci;
-#pragma unroll
-        for (int qx = 0; qx < kW; ++qx) {
-          ar[qy][qx] = fmaf(wx[qx], yr, ar[qy][qx]);
-          ai[qy][qx] = fmaf(wx[qx], yi, ai[qy][qx]);
-        }
-      }
-    }
-    const int lx = sp.lcell % kCellBlock, ly = sp.lcell / kCellBlock;
-#pragma unroll
-    for (int qy = 0; qy < kW; ++qy)
-#pragma unroll
-      for (int qx = 0; qx < kW; ++qx) {
-        float2* dst = &tile[(ly + qy) * T + lx + qx];
-        atomicAdd(&dst->x, ar[qy][qx]);
-        atomicAdd(&dst->y, ai[qy][qx]);
-      }
-  }
-  __syncthreads();
-  float2* g = gridB + cb.grid_off;
-  for (int c = threadIdx.x; c < T * T; c += blockDim.x) {
-    const float2 val = tile[c];
-    if (val.x != 0.f || val.y != 0.f) {
-      const int ty = c / T, tx = c - ty * T;
-      const int gy = (cb.gy0 + ty) % cb.nf2;
-      const int gx = (cb.gx0 + tx) % cb.nf1;
-      float2* dst = g + (long long)gy * cb.nf1 + gx;
-      unsafeAtomicAdd(&dst->x, val.x);
-      unsafeAtomicAdd(&dst->y, val.y);
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Type-1 step 2: FFT along y of `count` full grid columns; keeps the ny
-// output rows that are image modes (compact, into H = gridA) and clears the
-// consumed columns of gridB for the next evaluation.
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kThreads)
-k_t1_cols(const FftItem* __restrict__ items, const float2* __restrict__ tw,
-          float2* __restrict__ gridB, float2* __restrict__ H) {
-  extern __shared__ float2 sm[];
-  const FftItem it = items[blockIdx.x];
-  const int n = it.nf2, nb = it.count, stride = n + 1;
-  float2* A = sm;
-  float2* B = sm + nb * stride;
-  float2* W = B + nb * stride;
-  stage_twiddles(W, tw, it.tw2, n);
-  float2* g = gridB + it.grid_off;
-  for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
-    const int r = t / nb, b = t - r * nb;
-    float2* src = g + (long long)r * it.nf1 + it.first + b;
-    A[b * stride + r] = *src;
-    *src = make_float2(0.f, 0.f);
-  }
-  __syncthreads();
-  float2* R = lds_fft(A, B, nb, stride, n, W);
-  const int hy = it.ny / 2;
-  float2* h = H + it.grid_off;
-  for (int t = threadIdx.x; t < nb * it.ny; t += blockDim.x) {
-    const int iy = t / nb, b = t - iy * nb;
-    int r = iy - hy;
-    if (r < 0) r += n;
-    h[(long long)iy * it.nf1 + it.first + b] = R[b * stride + r];
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Type-1 step 3: FFT along x of `count` compact rows, keep the nx modes,
-// deconvolve, real part -> grad2d (FP64).
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kThreads)
-k_t1_rows(const FftItem* __restrict__ items, const float2* __restrict__ tw,
-          const float* __restrict__ corr, const float2* __restrict__ H,
-          double* __restrict__ grad2d) {
-  extern __shared__ float2 sm[];
-  const FftItem it = items[blockIdx.x];
-  const int n = it.nf1, nb = it.count;
-  float2* A = sm;
-  float2* B = sm + nb * n;
-  float2* W = B + nb * n;
-  stage_twiddles(W, tw, it.tw1, n);
-  const float2* h = H + it.grid_off + (long long)it.first * n;
-  for (int t = threadIdx.x; t < nb * n; t += blockDim.x) A[t] = h[t];
-  __syncthreads();
-  float2* R = lds_fft(A, B, nb, n, n, W);
-  const float* cx = corr + it.cx_off;
-  const float* cy = corr + it.cy_off;
-  const int hx = it.nx / 2;
-  for (int t = threadIdx.x; t < nb * it.nx; t += blockDim.x) {
-    const int b = t / it.nx, ix = t - b * it.nx;
-    const int iy = it.first + b;
-    int c = ix - hx;
-    if (c < 0) c += n;
-    grad2d[it.img_off + (long long)iy * it.nx + ix] =
-        (double)R[b * n + c].x * (double)cx[ix] * (double)cy[iy];
-  }
 }
 
 // ---------------------------------------------------------------------------

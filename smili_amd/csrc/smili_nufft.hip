@@ -1,0 +1,457 @@
+// ES-kernel NUFFT kernels for gfx950, templated on the real type R of the
+// spreading / interpolation / FFT arithmetic (float: the throughput path;
+// double: the reference-precision path).  Type 2 (image -> visibilities,
+// finufft2d2 with iflag=0, src/wrapfinufft/wrapfinufft.f90:94-128) and type 1
+// (residuals -> image, finufft2d1 with iflag=1, wrapfinufft.f90:172-206) both
+// use the +i sign.  Method: Barnett, Magland & af Klinteberg, SIAM J. Sci.
+// Comput. 41(5), 2019 (exponential-of-semicircle kernel, width 7, upsampling
+// 2, beta = 2.30 * 7).
+#pragma once
+#include "smili_kernels.h"
+
+namespace smili {
+
+template <typename R> struct Cx;
+template <> struct Cx<float> { using T = float2; };
+template <> struct Cx<double> { using T = double2; };
+
+template <typename V> __device__ __forceinline__ V cmk(decltype(V::x) a, decltype(V::x) b) {
+  V r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+template <typename V> __device__ __forceinline__ V cadd(V a, V b) { return cmk<V>(a.x + b.x, a.y + b.y); }
+template <typename V> __device__ __forceinline__ V csub(V a, V b) { return cmk<V>(a.x - b.x, a.y - b.y); }
+template <typename V> __device__ __forceinline__ V cmul(V a, V b) {
+  return cmk<V>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename V, typename S> __device__ __forceinline__ V cscale(V a, S s) { return cmk<V>(a.x * s, a.y * s); }
+template <typename V> __device__ __forceinline__ V mul_i(V a) { return cmk<V>(-a.y, a.x); }
+
+// ---------------------------------------------------------------------------
+// small DFTs with the +i sign: y_m = sum_q v_q exp(+2 pi i q m / r)
+// ---------------------------------------------------------------------------
+template <typename V> __device__ __forceinline__ void dft2(V* v) {
+  V a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+
+template <typename V> __device__ __forceinline__ void dft4(V& a0, V& a1, V& a2, V& a3) {
+  V t0 = cadd(a0, a2), t1 = csub(a0, a2);
+  V t2 = cadd(a1, a3), t3 = mul_i(csub(a1, a3));
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  V y1 = cadd(t1, t3), y3 = csub(t1, t3);
+  a1 = y1;
+  a3 = y3;
+}
+
+template <typename V> __device__ __forceinline__ void dft3(V* v) {
+  using R = decltype(V::x);
+  const R s3 = (R)0.86602540378443864676;
+  V t = cadd(v[1], v[2]);
+  V m = cmk<V>(v[0].x - (R)0.5 * t.x, v[0].y - (R)0.5 * t.y);
+  V d = mul_i(cscale(csub(v[1], v[2]), s3));
+  v[0] = cadd(v[0], t);
+  v[1] = cadd(m, d);
+  v[2] = csub(m, d);
+}
+
+template <typename V> __device__ __forceinline__ void dft5(V* v) {
+  using R = decltype(V::x);
+  const R c1 = (R)0.30901699437494742410, c2 = (R)-0.80901699437494742410;
+  const R s1 = (R)0.95105651629515357212, s2 = (R)0.58778525229247312917;
+  V t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+  V t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+  V a0 = v[0];
+  V b1 = cmk<V>(a0.x + c1 * t1.x + c2 * t2.x, a0.y + c1 * t1.y + c2 * t2.y);
+  V b2 = cmk<V>(a0.x + c2 * t1.x + c1 * t2.x, a0.y + c2 * t1.y + c1 * t2.y);
+  V d1 = mul_i(cmk<V>(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y));
+  V d2 = mul_i(cmk<V>(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y));
+  v[0] = cadd(a0, cadd(t1, t2));
+  v[1] = cadd(b1, d1);
+  v[4] = csub(b1, d1);
+  v[2] = cadd(b2, d2);
+  v[3] = csub(b2, d2);
+}
+
+template <typename V> __device__ __forceinline__ void dft8(V* v) {
+  using R = decltype(V::x);
+  const R s = (R)0.70710678118654752440;
+  V e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  V o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  dft4(e0, e1, e2, e3);
+  dft4(o0, o1, o2, o3);
+  // twiddles W8^m = exp(+i pi m / 4)
+  V w1 = cmk<V>(s * (o1.x - o1.y), s * (o1.x + o1.y));
+  V w2 = mul_i(o2);
+  V w3 = cmk<V>(-s * (o3.x + o3.y), s * (o3.x - o3.y));
+  v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
+  v[1] = cadd(e1, w1); v[5] = csub(e1, w1);
+  v[2] = cadd(e2, w2); v[6] = csub(e2, w2);
+  v[3] = cadd(e3, w3); v[7] = csub(e3, w3);
+}
+
+// ---------------------------------------------------------------------------
+// LDS Stockham FFT (autosort, +i sign) of nb transforms of length n stored
+// at a + b*stride.  Uses scratch s of the same shape and the twiddle table
+// tw[t] = exp(+2 pi i t / n) staged in LDS.  Radices in the order of the
+// host plan: all 8s, then 4, 2, 3, 5.  Returns the buffer that holds the
+// result.  All threads of the block must call it.
+// ---------------------------------------------------------------------------
+template <typename V>
+__device__ V* lds_fft(V* a, V* s, int nb, int stride, int n, const V* tw) {
+  int Ns = 1, rem = n;
+  while (rem > 1) {
+    const int r = (rem % 8 == 0) ? 8 : (rem % 4 == 0) ? 4 : (rem % 2 == 0) ? 2 : (rem % 3 == 0) ? 3 : 5;
+    rem /= r;
+    const int nbf = n / r;
+    const int total = nbf * nb;
+    const int twstep = n / (Ns * r);
+    for (int t = threadIdx.x; t < total; t += blockDim.x) {
+      const int b = t / nbf;
+      const int i = t - b * nbf;
+      const int k = i % Ns;
+      const V* in = a + b * stride;
+      V* out = s + b * stride;
+      V v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (q < r) v[q] = in[i + q * nbf];
+      if (Ns > 1) {
+        const int step = k * twstep;
+#pragma unroll
+        for (int q = 1; q < 8; ++q)
+          if (q < r) v[q] = cmul(v[q], tw[q * step]);
+      }
+      switch (r) {
+        case 2: dft2(v); break;
+        case 3: dft3(v); break;
+        case 4: dft4(v[0], v[1], v[2], v[3]); break;
+        case 5: dft5(v); break;
+        default: dft8(v); break;
+      }
+      const int base = (i / Ns) * Ns * r + k;
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (q < r) out[base + q * Ns] = v[q];
+    }
+    __syncthreads();
+    V* tmp = a; a = s; s = tmp;
+    Ns *= r;
+  }
+  return a;
+}
+
+template <typename V>
+__device__ __forceinline__ void stage_twiddles(V* dst, const V* __restrict__ tw, int off, int n) {
+  for (int t = threadIdx.x; t < n; t += blockDim.x) dst[t] = tw[off + t];
+}
+
+// ---------------------------------------------------------------------------
+// ES kernel weights.  e = d - w/2 in (-1, 0] where d is the distance from the
+// point to its first tap in cells; z_q = (d - q) * 2 / w; phi(z) =
+// exp(-beta z^2 / (1 + sqrt(1 - z^2))) (cancellation-free form of
+// exp(beta (sqrt(1 - z^2) - 1))).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float es_exp(float a) { return __expf(a); }
+__device__ __forceinline__ double es_exp(double a) { return exp(a); }
+
+template <typename R> __device__ __forceinline__ R es_tap(R e, R beta, int q) {
+  const R inv_half = (R)2 / (R)kW;
+  const R z = e * inv_half + ((R)1 - (R)q * inv_half);
+  const R z2 = z * z < (R)1 ? z * z : (R)1;
+  return es_exp(-beta * z2 / ((R)1 + sqrt((R)1 - z2)));
+}
+
+template <typename R> __device__ __forceinline__ void es_weights(R e, R beta, R* wt) {
+#pragma unroll
+  for (int q = 0; q < kW; ++q) wt[q] = es_tap(e, beta, q);
+}
+
+// ---------------------------------------------------------------------------
+// Type-2 step 1: image rows -> deconvolve, zero-pad, FFT along x.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(kThreads)
+k_t2_rows(const FftItem* __restrict__ items, const typename Cx<R>::T* __restrict__ tw,
+          const R* __restrict__ corr, const double* __restrict__ x,
+          const double* __restrict__ I2dbuf, typename Cx<R>::T* __restrict__ gridA) {
+  using V = typename Cx<R>::T;
+  extern __shared__ __align__(16) unsigned char smraw[];
+  V* sm = reinterpret_cast<V*>(smraw);
+  const FftItem it = items[blockIdx.x];
+  const int n = it.nf1, nb = it.count;
+  V* A = sm;
+  V* B = sm + nb * n;
+  V* W = B + nb * n;
+  stage_twiddles(W, tw, it.tw1, n);
+  const double* img = it.identity ? (x + it.x_off) : (I2dbuf + it.img_off);
+  const R* cx = corr + it.cx_off;
+  const R* cy = corr + it.cy_off;
+  const int hx = it.nx / 2;
+  for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
+    const int b = t / n, c = t - b * n;
+    // grid column c holds mode k1 = c (c < nf1 - nx/2) or c - nf1
+    const int k1 = (c < n - hx) ? c : c - n;
+    V val = cmk<V>(0, 0);
+    if (k1 >= -hx && k1 < it.nx - hx) {
+      const int ix = k1 + hx;
+      const int iy = it.first + b;
+      val.x = (R)(img[(long long)iy * it.nx + ix] * (double)cx[ix] * (double)cy[iy]);
+    }
+    A[t] = val;
+  }
+  __syncthreads();
+  V* Rv = lds_fft(A, B, nb, n, n, W);
+  const int hy = it.ny / 2;
+  for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
+    const int b = t / n, c = t - b * n;
+    int row = it.first + b - hy;
+    if (row < 0) row += it.nf2;
+    gridA[it.grid_off + (long long)row * n + c] = Rv[t];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Type-2 step 2: FFT along y of `count` grid columns; only the ny rows that
+// hold image modes are read (the others are zero by construction).
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(kThreads)
+k_t2_cols(const FftItem* __restrict__ items, const typename Cx<R>::T* __restrict__ tw,
+          typename Cx<R>::T* __restrict__ gridA) {
+  using V = typename Cx<R>::T;
+  extern __shared__ __align__(16) unsigned char smraw[];
+  V* sm = reinterpret_cast<V*>(smraw);
+  const FftItem it = items[blockIdx.x];
+  const int n = it.nf2, nb = it.count, stride = n + 1;
+  V* A = sm;
+  V* B = sm + nb * stride;
+  V* W = B + nb * stride;
+  stage_twiddles(W, tw, it.tw2, n);
+  const int hy = it.ny / 2;
+  V* g = gridA + it.grid_off;
+  for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
+    const int r = t / nb, b = t - r * nb;
+    const int k2 = (r < n - hy) ? r : r - n;
+    V val = cmk<V>(0, 0);
+    if (k2 >= -hy && k2 < it.ny - hy) val = g[(long long)r * it.nf1 + it.first + b];
+    A[b * stride + r] = val;
+  }
+  __syncthreads();
+  V* Rv = lds_fft(A, B, nb, stride, n, W);
+  for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
+    const int r = t / nb, b = t - r * nb;
+    g[(long long)r * it.nf1 + it.first + b] = Rv[b * stride + r];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Type-2 step 3: interpolation from an LDS-staged tile of the grid.  One work
+// item = up to sub_points points of one bin.  Also clears the residual
+// accumulators of its points for the chi-square kernel.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(kThreads)
+k_interp(const SubProb* __restrict__ sps, const typename Cx<R>::T* __restrict__ gridA,
+         const int* __restrict__ pt_cell, const R* __restrict__ pt_ex,
+         const R* __restrict__ pt_ey, typename Cx<R>::T* __restrict__ Vout,
+         double2* __restrict__ Vres, int bin, R beta) {
+  using V = typename Cx<R>::T;
+  extern __shared__ __align__(16) unsigned char smraw[];
+  V* tile = reinterpret_cast<V*>(smraw);
+  const SubProb sp = sps[blockIdx.x];
+  const int T = bin + kW - 1;
+  const V* g = gridA + sp.grid_off;
+  const int gx0 = sp.bin_x * bin, gy0 = sp.bin_y * bin;
+  for (int c = threadIdx.x; c < T * T; c += blockDim.x) {
+    const int ty = c / T, tx = c - ty * T;
+    const int gy = (gy0 + ty) % sp.nf2;
+    const int gx = (gx0 + tx) % sp.nf1;
+    tile[c] = g[(long long)gy * sp.nf1 + gx];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < sp.count; i += blockDim.x) {
+    const int j = sp.start + i;
+    const int cell = pt_cell[j];
+    const int lx = (cell & 0xffff) - gx0;
+    const int ly = (cell >> 16) - gy0;
+    R wx[kW], wy[kW];
+    es_weights(pt_ex[j], beta, wx);
+    es_weights(pt_ey[j], beta, wy);
+    R ar = 0, ai = 0;
+#pragma unroll
+    for (int qy = 0; qy < kW; ++qy) {
+      const V* row = tile + (ly + qy) * T + lx;
+      R rr = 0, ri = 0;
+#pragma unroll
+      for (int qx = 0; qx < kW; ++qx) {
+        const V gv = row[qx];
+        rr = fma(wx[qx], gv.x, rr);
+        ri = fma(wx[qx], gv.y, ri);
+      }
+      ar = fma(wy[qy], rr, ar);
+      ai = fma(wy[qy], ri, ai);
+    }
+    Vout[j] = cmk<V>(ar, ai);
+    Vres[j] = make_double2(0.0, 0.0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Type-1 step 1: spread residuals c = Vres_re - i Vres_im onto the grid.
+// Work item = up to 256 spans of one 8x8 block of first-tap cells; a span is
+// up to 8 consecutive points of one cell (points are sorted by cell, so a
+// span's points share their 7x7 footprint).  One thread per span sums the
+// footprints of its points in registers (QB rows of taps per pass: all 7
+// in FP32, 4 + 3 in FP64 to bound the register tile), adds the sum to an
+// LDS tile of the block (spans of neighbouring cells overlap), and the tile
+// is added to the global grid with atomics (work items of one block and of
+// neighbouring blocks overlap).  The grid is zero on entry (k_t1_cols
+// clears what it consumes).
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(kSpansPerGroup)
+k_spread(const CellBlock* __restrict__ blocks, const CellSpan* __restrict__ spans,
+         const R* __restrict__ pt_ex, const R* __restrict__ pt_ey,
+         const double2* __restrict__ Vres, typename Cx<R>::T* __restrict__ gridB, R beta) {
+  using V = typename Cx<R>::T;
+  constexpr int T = kCellBlock + kW - 1;
+  constexpr int QB = sizeof(R) == 4 ? kW : 4;
+  __shared__ V tile[T * T];
+  const CellBlock cb = blocks[blockIdx.x];
+  for (int c = threadIdx.x; c < T * T; c += blockDim.x) tile[c] = cmk<V>(0, 0);
+  __syncthreads();
+  for (int k = threadIdx.x; k < cb.cell_count; k += blockDim.x) {
+    const CellSpan sp = spans[cb.cell_begin + k];
+    const int lx = sp.lcell % kCellBlock, ly = sp.lcell / kCellBlock;
+    for (int q0 = 0; q0 < kW; q0 += QB) {
+      R ar[QB][kW], ai[QB][kW];
+#pragma unroll
+      for (int a = 0; a < QB; ++a)
+#pragma unroll
+        for (int qx = 0; qx < kW; ++qx) {
+          ar[a][qx] = 0;
+          ai[a][qx] = 0;
+        }
+      for (int j = sp.pt_begin; j < sp.pt_end; ++j) {
+        R wx[kW];
+        es_weights(pt_ex[j], beta, wx);
+        const R ey = pt_ey[j];
+        const double2 r = Vres[j];
+        const R cr = (R)r.x, ci = (R)(-r.y);
+#pragma unroll
+        for (int a = 0; a < QB; ++a) {
+          if (q0 + a >= kW) break;
+          const R wy = es_tap(ey, beta, q0 + a);
+          const R yr = wy * cr, yi = wy * ci;
+#pragma unroll
+          for (int qx = 0; qx < kW; ++qx) {
+            ar[a][qx] = fma(wx[qx], yr, ar[a][qx]);
+            ai[a][qx] = fma(wx[qx], yi, ai[a][qx]);
+          }
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < QB; ++a) {
+        if (q0 + a >= kW) break;
+#pragma unroll
+        for (int qx = 0; qx < kW; ++qx) {
+          V* dst = &tile[(ly + q0 + a) * T + lx + qx];
+          atomicAdd(&dst->x, ar[a][qx]);
+          atomicAdd(&dst->y, ai[a][qx]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  V* g = gridB + cb.grid_off;
+  for (int c = threadIdx.x; c < T * T; c += blockDim.x) {
+    const V val = tile[c];
+    if (val.x != 0 || val.y != 0) {
+      const int ty = c / T, tx = c - ty * T;
+      const int gy = (cb.gy0 + ty) % cb.nf2;
+      const int gx = (cb.gx0 + tx) % cb.nf1;
+      V* dst = g + (long long)gy * cb.nf1 + gx;
+      unsafeAtomicAdd(&dst->x, val.x);
+      unsafeAtomicAdd(&dst->y, val.y);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Type-1 step 2: FFT along y of `count` full grid columns; keeps the ny
+// output rows that are image modes (compact, into H = gridA) and clears the
+// consumed columns of gridB for the next evaluation.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(kThreads)
+k_t1_cols(const FftItem* __restrict__ items, const typename Cx<R>::T* __restrict__ tw,
+          typename Cx<R>::T* __restrict__ gridB, typename Cx<R>::T* __restrict__ H) {
+  using V = typename Cx<R>::T;
+  extern __shared__ __align__(16) unsigned char smraw[];
+  V* sm = reinterpret_cast<V*>(smraw);
+  const FftItem it = items[blockIdx.x];
+  const int n = it.nf2, nb = it.count, stride = n + 1;
+  V* A = sm;
+  V* B = sm + nb * stride;
+  V* W = B + nb * stride;
+  stage_twiddles(W, tw, it.tw2, n);
+  V* g = gridB + it.grid_off;
+  for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
+    const int r = t / nb, b = t - r * nb;
+    V* src = g + (long long)r * it.nf1 + it.first + b;
+    A[b * stride + r] = *src;
+    *src = cmk<V>(0, 0);
+  }
+  __syncthreads();
+  V* Rv = lds_fft(A, B, nb, stride, n, W);
+  const int hy = it.ny / 2;
+  V* h = H + it.grid_off;
+  for (int t = threadIdx.x; t < nb * it.ny; t += blockDim.x) {
+    const int iy = t / nb, b = t - iy * nb;
+    int r = iy - hy;
+    if (r < 0) r += n;
+    h[(long long)iy * it.nf1 + it.first + b] = Rv[b * stride + r];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Type-1 step 3: FFT along x of `count` compact rows, keep the nx modes,
+// deconvolve, real part -> grad2d (FP64).
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(kThreads)
+k_t1_rows(const FftItem* __restrict__ items, const typename Cx<R>::T* __restrict__ tw,
+          const R* __restrict__ corr, const typename Cx<R>::T* __restrict__ H,
+          double* __restrict__ grad2d) {
+  using V = typename Cx<R>::T;
+  extern __shared__ __align__(16) unsigned char smraw[];
+  V* sm = reinterpret_cast<V*>(smraw);
+  const FftItem it = items[blockIdx.x];
+  const int n = it.nf1, nb = it.count;
+  V* A = sm;
+  V* B = sm + nb * n;
+  V* W = B + nb * n;
+  stage_twiddles(W, tw, it.tw1, n);
+  const V* h = H + it.grid_off + (long long)it.first * n;
+  for (int t = threadIdx.x; t < nb * n; t += blockDim.x) A[t] = h[t];
+  __syncthreads();
+  V* Rv = lds_fft(A, B, nb, n, n, W);
+  const R* cx = corr + it.cx_off;
+  const R* cy = corr + it.cy_off;
+  const int hx = it.nx / 2;
+  for (int t = threadIdx.x; t < nb * it.nx; t += blockDim.x) {
+    const int b = t / it.nx, ix = t - b * it.nx;
+    const int iy = it.first + b;
+    int c = ix - hx;
+    if (c < 0) c += n;
+    grad2d[it.img_off + (long long)iy * it.nx + ix] =
+        (double)Rv[b * n + c].x * (double)cx[ix] * (double)cy[iy];
+  }
+}
+
+}  // namespace smili

@@ -1,0 +1,178 @@
+"""Static imager with SMILI's API: ``imaging(initimage, ..., vistable=...,
+l1_lambda=..., ...)`` (smili/imaging/lbfgs.py:44-576).
+
+The objective+gradient runs on the GPU (``CostProblem``, the C ABI of
+libsmili_hip.so) and the optimiser is L-BFGS-B (Byrd, Lu, Nocedal & Zhu,
+v3.0 of Zhu, Byrd, Lu & Nocedal / Morales & Nocedal 2011 -- the algorithm of
+the reference's src/lbfgsb/lbfgsb.f) in FP64 on the host, through SciPy's
+implementation of the same algorithm.  The reference's reverse-communication
+loop (src/fortlib/fftim2d.f90:286-351) stops once the iteration counter
+exceeds ``niter`` at a NEW_X, i.e. after ``niter + 1`` iterations; ``m``,
+``factr`` and ``pgtol`` come from the module dict ``lbfgsbprms``
+(lbfgs.py:34-38).
+"""
+from __future__ import annotations
+
+import copy
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+from scipy.optimize import minimize
+
+from ..cost import CostProblem
+from .prep import prepare
+
+lbfgsbprms = {
+    "m": 5,
+    "factr": 1e1,
+    "pgtol": 0.,
+}
+
+
+@dataclass
+class SolveInfo:
+    """Optimiser outcome (the reference prints it only, fftim2d.f90:377)."""
+    niter: int
+    nfev: int
+    cost: float
+    terms: dict
+    message: str
+    success: bool
+    wall_s: float
+    eval_s: float
+    history: list = field(default_factory=list)
+
+
+def _summary(terms, inormfactr, x):
+    """The summary calc_cost prints (fftim2d.f90:557-606)."""
+    lines = ["  Cost Function          : %13.6e" % terms["cost"],
+             "  Data term              : %13.6e" % terms["chisq"]]
+    for k, lab in (("chisq_fcv", "Complex Visibilities"), ("chisq_amp", "Amplitudes"),
+                   ("chisq_cp", "Closure Phase"), ("chisq_ca", "Closure Amplitudes")):
+        if terms[k] != 0:
+            lines.append("    %-20s : %13.6e" % (lab, terms[k]))
+    lines.append("  Reguarization term     : %13.6e" % terms["reg"])
+    for k, lab in (("l1", "l1-norm"), ("tv", "TV"), ("tsv", "TSV"), ("kl", "KL divergence"),
+                   ("gs", "GS entropy"), ("tfd", "Total Flux Density"), ("cen", "Centoroid Reg."),
+                   ("sm", "second momentum")):
+        if terms[k] != 0:
+            lines.append("    %-20s : %13.6e" % (lab, terms[k]))
+    lines.append("  Total flux             : %13.6e" % (np.sum(x) / inormfactr))
+    return "\n".join(lines)
+
+
+def solve(problem: CostProblem, x0, niter=1000, nonneg=True, m=None, factr=None, pgtol=None,
+          nprint=500, inormfactr=1.0, callback=None, verbose=True):
+    """L-BFGS-B over ``problem.cost_grad`` from x0 (scaled units).
+    Returns (x, SolveInfo)."""
+    m = lbfgsbprms["m"] if m is None else m
+    factr = lbfgsbprms["factr"] if factr is None else factr
+    pgtol = lbfgsbprms["pgtol"] if pgtol is None else pgtol
+    state = {"nfev": 0, "eval_s": 0.0, "it": 0, "last": None}
+    hist = []
+
+    def fg(x):
+        t = time.perf_counter()
+        c, g, terms = problem.cost_grad(x)
+        state["eval_s"] += time.perf_counter() - t
+        state["nfev"] += 1
+        state["last"] = terms
+        return c, g
+
+    def cb(xk):
+        state["it"] += 1
+        hist.append(state["last"]["cost"])
+        if verbose and nprint > 0 and state["it"] % nprint == 0:
+            print("Iteration :%5d/%5d" % (state["it"], niter))
+            print(_summary(state["last"], inormfactr, xk))
+        if callback is not None:
+            callback(state["it"], xk, state["last"])
+
+    bounds = [(0.0, None)] * np.asarray(x0).size if nonneg else None
+    t0 = time.perf_counter()
+    if verbose:
+        c, _, terms = problem.cost_grad(x0)
+        print("Summary of the initial image")
+        print(_summary(terms, inormfactr, x0))
+    res = minimize(fg, np.asarray(x0, np.float64), jac=True, method="L-BFGS-B", bounds=bounds,
+                   callback=cb,
+                   options=dict(maxcor=int(m), ftol=float(factr) * np.finfo(float).eps, gtol=float(pgtol),
+                                maxiter=int(niter) + 1, maxfun=10 ** 9, maxls=20))
+    wall = time.perf_counter() - t0
+    c, _, terms = problem.cost_grad(res.x)
+    if verbose:
+        print("Final Summary")
+        print("  Iteration ent at %5d/%5d" % (res.nit, niter))
+        print(_summary(terms, inormfactr, res.x))
+        print(res.message)
+    info = SolveInfo(niter=int(res.nit), nfev=int(state["nfev"]), cost=float(c), terms=terms,
+                     message=str(res.message), success=bool(res.success), wall_s=wall,
+                     eval_s=state["eval_s"], history=hist)
+    return res.x, info
+
+
+def _window_of(imregion, initimage, istokes, ifreq):
+    if imregion is None:
+        return None
+    if hasattr(imregion, "imagewin"):
+        return imregion.imagewin(initimage, istokes, ifreq)
+    if hasattr(imregion, "data"):
+        return np.asarray(imregion.data)[0, 0] > 0.5          # lbfgs.py:240-241
+    return np.asarray(imregion, bool)
+
+
+def imaging(initimage, imregion=None, vistable=None, amptable=None, bstable=None, catable=None,
+            w_vis=1, w_amp=1, w_bs=1, w_ca=1,
+            l1_lambda=-1., l1_prior=None, l1_noise=1e-2,
+            tv_lambda=-1, tv_prior=None, tsv_lambda=-1, tsv_prior=None,
+            kl_lambda=-1, kl_prior=None, gs_lambda=-1, gs_prior=None,
+            tfd_lambda=-1, tfd_tgterror=0.01, cen_lambda=-1, cen_alpha=3,
+            sm_lambda=-1, sm_maj=0., sm_min=0., sm_phi=0.,
+            pulse_fwhm=-1, niter=1000, nonneg=True, nprint=500, totalflux=None, inorm=1.,
+            istokes=0, ifreq=0, tune=None, verbose=True, return_info=False):
+    """smili.imaging.lbfgs.imaging on the GPU cost.
+
+    ``initimage`` is an image with ``header`` (nx, ny, dx, dy in degrees,
+    nxref, nyref) and ``data[istokes, ifreq, iy, ix]`` (an IMFITS or
+    ``smili_amd.imaging.image.Image``); tables are DataFrames or dicts with
+    the reference's column names.  Priors are images (their data[0, 0] is
+    used, lbfgs.py:279-283).  Returns a copy of initimage holding the
+    solution (lbfgs.py:565-576); with ``return_info`` also the SolveInfo.
+    """
+    if pulse_fwhm > 0:
+        raise NotImplementedError("pulse_fwhm (deburring, lbfgs.py:1114-1194) is not supported")
+    hdr = initimage.header
+    window = _window_of(imregion, initimage, istokes, ifreq)
+
+    def prior(p):
+        return None if p is None else np.asarray(p.data)[0, 0]
+
+    prep = prepare(np.asarray(initimage.data, np.float64)[istokes, ifreq],
+                   np.deg2rad(hdr["dx"]), np.deg2rad(hdr["dy"]), hdr["nxref"], hdr["nyref"], window,
+                   vistable, amptable, bstable, catable, w_vis, w_amp, w_bs, w_ca,
+                   l1_lambda, prior(l1_prior), l1_noise, tv_lambda, prior(tv_prior),
+                   tsv_lambda, prior(tsv_prior), kl_lambda, prior(kl_prior), gs_lambda, prior(gs_prior),
+                   tfd_lambda, tfd_tgterror, cen_lambda, cen_alpha, sm_lambda,
+                   np.deg2rad(sm_maj * _angconv(initimage)), np.deg2rad(sm_min * _angconv(initimage)), sm_phi,
+                   totalflux, inorm)
+    with CostProblem(**prep.kwargs, tune=tune) as prob:
+        x, info = solve(prob, prep.x0, niter=niter, nonneg=nonneg, nprint=nprint,
+                        inormfactr=prep.inormfactr, verbose=verbose)
+    Iout = x / prep.inormfactr
+    out = copy.deepcopy(initimage)
+    data = np.array(out.data, dtype=np.float64, copy=True)
+    data[istokes, ifreq] = 0.0
+    data[istokes, ifreq, prep.yidx - 1, prep.xidx - 1] = Iout
+    out.data = data
+    if hasattr(out, "update_fits"):
+        out.update_fits()
+    return (out, info) if return_info else out
+
+
+def _angconv(image):
+    """Degrees per unit of the image's angular unit (sm_maj / sm_min are given
+    in that unit, lbfgs.py:413-418)."""
+    unit = getattr(image, "angunit", "uas")
+    return {"deg": 1.0, "degree": 1.0, "arcmin": 1 / 60, "amin": 1 / 60, "arcsec": 1 / 3600,
+            "asec": 1 / 3600, "mas": 1 / 3.6e6, "uas": 1 / 3.6e9}[unit]

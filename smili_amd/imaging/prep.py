@@ -78,7 +78,8 @@ def init_tsvreg(xidx, yidx, Nx, Ny, isflat, prior):
 def init_klreg(kl_l_in, prior):
     """image.f90:278-291: (lambda / kl(P, P), P = smabs(prior))."""
     P = smabs(prior)
-    kl_pp = np.sum(P * np.log(P / P) + P / math.e)
+    a = smabs(P)                       # kl_e(I, P) smooths |I| (image.f90:325-331)
+    kl_pp = np.sum(a * np.log(a / P) + P / math.e)
     return kl_l_in / kl_pp, P
 
 

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_header_sizes():
     # pointer-size fields in declared order: 10 doubles + 5 pointers (reg)
     assert C.sizeof(_lib.SmiliReg) == 18 * 8
-    assert C.sizeof(_lib.SmiliTune) == 5 * 4
+    assert C.sizeof(_lib.SmiliTune) == 6 * 4
 
 
 def _desc(nx=4, ny=4, nuv=3, bad=None):

@@ -1,0 +1,69 @@
+"""GPU: golden calc_cost fixtures and the full imager (host FP64 L-BFGS-B
+driving the GPU cost) against the reference solve of the 64x64
+full-visibility workload."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from oracle import smili_oracle as O
+from tests.helpers import problem_kwargs, rel_l2
+from tests.test_golden import CASES, load_case
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.mark.parametrize("path", CASES, ids=[os.path.basename(c) for c in CASES])
+def test_gpu_cost_on_golden_cases(gpu_lib, path):
+    from smili_amd.cost import CostProblem
+    p, x, cost, grad, terms = load_case(path)
+    c_es, g_es, _ = O.calc_cost(x, p, fwd=O.es_nufft_type2, adj=O.es_nufft_type1)
+    e_c = abs(c_es - cost) / abs(cost)
+    e_g = rel_l2(g_es, grad)
+    with CostProblem(**problem_kwargs(p)) as g:
+        c, gr, t = g.cost_grad(x)
+    print(os.path.basename(path), "cost", abs(c - cost) / abs(cost), "grad", rel_l2(gr, grad), "es", e_c, e_g)
+    assert abs(c - cost) <= (2e-6 + 2 * e_c) * abs(cost)
+    assert rel_l2(gr, grad) <= 5e-6 + 1.5 * e_g
+    tv = np.array([t[k] for k in O.TERM_NAMES])
+    for i, k in enumerate(O.TERM_NAMES):
+        if k in ("l1", "tv", "tsv", "kl", "gs", "tfd", "cen", "sm"):
+            assert abs(tv[i] - terms[i]) <= 1e-12 * abs(terms[i]) + 1e-300, k
+
+
+def _solve_small_fcv(precision):
+    from smili_amd import synth, workloads
+    from smili_amd.imaging import lbfgs
+    from smili_amd.imaging.image import Image
+    prep, extra = workloads.small_fcv()
+    init = Image(64, 64, 2.0, angunit="uas")
+    init.data[0, 0] = synth.sky_image([synth.Gaussian(0.6, 20 * synth.UAS)], 64, 64, -2 * synth.UAS, 2 * synth.UAS)
+    old = dict(lbfgs.lbfgsbprms)
+    lbfgs.lbfgsbprms["m"] = 10
+    try:
+        return lbfgs.imaging(init, vistable=extra["vis"], l1_lambda=1.0, tsv_lambda=1e4, niter=1000,
+                             nonneg=True, nprint=0, verbose=False, return_info=True,
+                             tune=dict(precision=precision))
+    finally:
+        lbfgs.lbfgsbprms.clear()
+        lbfgs.lbfgsbprms.update(old)
+
+
+@pytest.mark.parametrize("precision,tol_img", [(1, 1e-4), (0, 1e-3)], ids=["fp64", "fp32"])
+def test_imaging_reproduces_reference_solve(gpu_lib, precision, tol_img):
+    """64x64 ring, ~1,900 complex visibilities, fcv chi-square + L1 (1) + TSV
+    (1e4), nonneg, L-BFGS-B m=10, 1000 iterations from a 20 uas Gaussian,
+    against the reference solve (exact-DFT oracle cost + L-BFGS-B): final
+    cost within 1e-6 relative; final image within NRMSE 1e-4 with the FP64
+    NUFFT, 1e-3 with the FP32 NUFFT (whose rounding noise in the cost ends
+    L-BFGS-B's factr test earlier)."""
+    from smili_amd.imaging.image import nrmse
+    gold = np.load(os.path.join(HERE, "solve_small_fcv.npz"))
+    out, info = _solve_small_fcv(precision)
+    e = nrmse(out.data[0, 0], gold["image"])
+    print("imaging precision", precision, ": nit", info.niter, "nfev", info.nfev, "cost", info.cost,
+          "ref", float(gold["cost"]), "NRMSE", e, "wall", info.wall_s, "eval", info.eval_s)
+    assert e <= tol_img
+    assert abs(info.cost - float(gold["cost"])) <= 1e-6 * abs(float(gold["cost"]))

@@ -197,3 +197,22 @@ def test_chunks_split_at_the_slot_limit(gpu_lib):
     p, x = random_problem(rng, 32, 32, M=3000, nfcv=500, namp=800, ncp=1500, nca=2000)
     with _mk(p) as g:
         _check_cost(p, x, g)
+
+
+@pytest.mark.parametrize("shape,window", [((24, 20), False), ((33, 17), True), ((64, 64), False)])
+def test_fp64_mode_matches_fp64_es_oracle(gpu_lib, shape, window):
+    """precision=1 runs the same ES-kernel NUFFT in FP64: it must agree with
+    the FP64 ES oracle to rounding (visibilities rel-l2 <= 1e-11, cost rel
+    <= 1e-12, gradient rel-l2 <= 1e-10)."""
+    rng = np.random.default_rng(300 + shape[0])
+    p, x = random_problem(rng, shape[0], shape[1], M=500, window=window)
+    I2d = O.image_1d_to_2d(x, p.xidx, p.yidx, p.Nx, p.Ny)
+    with _mk(p, tune=dict(precision=1)) as g:
+        V = g.model_vis(x)
+        c, gr, t = g.cost_grad(x)
+    Ve = O.es_nufft_type2(I2d, p.u, p.v)
+    c_es, g_es, _ = O.calc_cost(x, p, fwd=O.es_nufft_type2, adj=O.es_nufft_type1)
+    print("fp64", shape, "V", rel_l2(V, Ve), "cost", abs(c - c_es) / abs(c_es), "grad", rel_l2(gr, g_es))
+    assert rel_l2(V, Ve) <= 1e-11
+    assert abs(c - c_es) <= 1e-12 * abs(c_es)
+    assert rel_l2(gr, g_es) <= 1e-10
