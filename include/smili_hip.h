@@ -109,9 +109,9 @@ typedef struct smili_tune {
   int32_t chunk;        /* data items per chi-square work-group (default 2048) */
   int32_t use_graph;    /* >= 0: capture one evaluation in a hipGraph and replay it;
                            < 0: launch the kernels one by one */
-  int32_t precision;    /* NUFFT arithmetic (spreading, interpolation, FFT):
-                           0 = FP32 (default), 1 = FP64; chi-square and
-                           regularizers are FP64 in both */
+  int32_t precision;    /* bits of the NUFFT arithmetic (spreading,
+                           interpolation, FFT): 64 (default, also 0) or 32;
+                           chi-square and regularizers are FP64 in both */
 } smili_tune;
 
 int smili_abi_version(void);

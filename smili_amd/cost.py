@@ -25,6 +25,80 @@ def _i32(a):
     return None if a is None else np.ascontiguousarray(a, dtype=np.int32)
 
 
+def build_desc(Nx, Ny, xidx, yidx, u, v, Nxref=0.0, Nyref=0.0,
+               l1_l=-1.0, l1_wgt=None, tv_l=-1.0, tv_wgt=None,
+               tsv_l=-1.0, tsv_wgt=None, kl_l=-1.0, kl_wgt=None,
+               gs_l=-1.0, gs_wgt=None, tfd_l=-1.0, tfd_tgtfd=1.0,
+               cen_l=-1.0, cen_alpha=1.0, sm_l=-1.0, sm_maj=0.0, sm_min=0.0, sm_phi=0.0,
+               uvidxfcv=None, Vfcv=None, Varfcv=None, wfcv=0.0,
+               uvidxamp=None, Vamp=None, Varamp=None, wamp=0.0,
+               uvidxcp=None, CP=None, Varcp=None, wcp=0.0,
+               uvidxca=None, CA=None, Varca=None, wca=0.0, extra=None):
+    """smili_desc for calc_cost's argument list; returns (desc, keep) where
+    `keep` holds the arrays the descriptor points to (keep it alive until
+    the descriptor has been consumed)."""
+    keep = []
+
+    def k(a):
+        keep.append(a)
+        return a
+
+    xidx = k(_i32(xidx))
+    yidx = k(_i32(yidx))
+    u = k(_f64(u))
+    v = k(_f64(v))
+    npix = int(xidx.size)
+    reg = _lib.SmiliReg()
+    for name, lam, w in (("l1", l1_l, l1_wgt), ("tv", tv_l, tv_wgt), ("tsv", tsv_l, tsv_wgt),
+                         ("kl", kl_l, kl_wgt), ("gs", gs_l, gs_wgt)):
+        setattr(reg, name + "_l", float(lam))
+        if lam > 0:
+            arr = k(_f64(np.broadcast_to(np.asarray(w, np.float64), (npix,))))
+            setattr(reg, name + "_wgt", dptr(arr))
+    reg.tfd_l, reg.tfd_tgtfd = float(tfd_l), float(tfd_tgtfd)
+    reg.cen_l, reg.cen_alpha = float(cen_l), float(cen_alpha)
+    reg.sm_l, reg.sm_maj, reg.sm_min, reg.sm_phi = map(float, (sm_l, sm_maj, sm_min, sm_phi))
+    data = _lib.SmiliData()
+    if uvidxfcv is not None and len(uvidxfcv) > 0:
+        Vfcv = np.asarray(Vfcv, np.complex128)
+        data.nfcv = len(uvidxfcv)
+        data.uvidxfcv = iptr(k(_i32(uvidxfcv)))
+        data.vfcv_re = dptr(k(_f64(Vfcv.real)))
+        data.vfcv_im = dptr(k(_f64(Vfcv.imag)))
+        data.varfcv = dptr(k(_f64(Varfcv)))
+        data.wfcv = float(wfcv)
+    if uvidxamp is not None and len(uvidxamp) > 0:
+        data.namp = len(uvidxamp)
+        data.uvidxamp = iptr(k(_i32(uvidxamp)))
+        data.vamp = dptr(k(_f64(Vamp)))
+        data.varamp = dptr(k(_f64(Varamp)))
+        data.wamp = float(wamp)
+    if uvidxcp is not None and np.asarray(uvidxcp).shape[-1] > 0:
+        idx = np.asarray(uvidxcp).reshape(3, -1)
+        data.ncp = idx.shape[1]
+        data.uvidxcp = iptr(k(_i32(idx.T)))          # 3 contiguous per closure
+        data.cp = dptr(k(_f64(CP)))
+        data.varcp = dptr(k(_f64(Varcp)))
+        data.wcp = float(wcp)
+    if uvidxca is not None and np.asarray(uvidxca).shape[-1] > 0:
+        idx = np.asarray(uvidxca).reshape(4, -1)
+        data.nca = idx.shape[1]
+        data.uvidxca = iptr(k(_i32(idx.T)))          # 4 contiguous per closure
+        data.ca = dptr(k(_f64(CA)))
+        data.varca = dptr(k(_f64(Varca)))
+        data.wca = float(wca)
+    desc = _lib.SmiliDesc()
+    desc.nx, desc.ny = int(Nx), int(Ny)
+    desc.nxref, desc.nyref = float(Nxref), float(Nyref)
+    desc.npix = npix
+    desc.xidx, desc.yidx = iptr(xidx), iptr(yidx)
+    desc.nuv = int(u.size)
+    desc.u, desc.v = dptr(u), dptr(v)
+    desc.reg = reg
+    desc.data = data
+    return desc, keep
+
+
 class CostProblem:
     """One RML imaging objective resident on the current GPU.
 
@@ -34,85 +108,17 @@ class CostProblem:
     as the Fortran arrays uvidxcp(3,Ncp), uvidxca(4,Nca); Vfcv complex and
     already phase-shifted; w* normalised weights.  A regularizer is on iff
     its lambda > 0; a data block is on iff it is given and non-empty.
+    ``tune`` sets smili_tune fields (bin, sub_points, col_group, chunk,
+    use_graph, precision).
     """
 
-    def __init__(self, Nx, Ny, xidx, yidx, u, v, Nxref=0.0, Nyref=0.0,
-                 l1_l=-1.0, l1_wgt=None, tv_l=-1.0, tv_wgt=None,
-                 tsv_l=-1.0, tsv_wgt=None, kl_l=-1.0, kl_wgt=None,
-                 gs_l=-1.0, gs_wgt=None, tfd_l=-1.0, tfd_tgtfd=1.0,
-                 cen_l=-1.0, cen_alpha=1.0, sm_l=-1.0, sm_maj=0.0, sm_min=0.0, sm_phi=0.0,
-                 uvidxfcv=None, Vfcv=None, Varfcv=None, wfcv=0.0,
-                 uvidxamp=None, Vamp=None, Varamp=None, wamp=0.0,
-                 uvidxcp=None, CP=None, Varcp=None, wcp=0.0,
-                 uvidxca=None, CA=None, Varca=None, wca=0.0,
-                 tune=None, extra=None):
+    def __init__(self, tune=None, **kw):
         lib = _lib.load()
         self._lib = lib
-        self._keep = []
-
-        def keep(a):
-            self._keep.append(a)
-            return a
-
-        self.Nx, self.Ny = int(Nx), int(Ny)
-        xidx = keep(_i32(xidx))
-        yidx = keep(_i32(yidx))
-        u = keep(_f64(u))
-        v = keep(_f64(v))
-        self.npix = int(xidx.size)
-        self.nuv = int(u.size)
-
-        reg = _lib.SmiliReg()
-        for name, lam, w in (("l1", l1_l, l1_wgt), ("tv", tv_l, tv_wgt), ("tsv", tsv_l, tsv_wgt),
-                             ("kl", kl_l, kl_wgt), ("gs", gs_l, gs_wgt)):
-            setattr(reg, name + "_l", float(lam))
-            if lam > 0:
-                arr = keep(_f64(np.broadcast_to(np.asarray(w, np.float64), (self.npix,))))
-                setattr(reg, name + "_wgt", dptr(arr))
-        reg.tfd_l, reg.tfd_tgtfd = float(tfd_l), float(tfd_tgtfd)
-        reg.cen_l, reg.cen_alpha = float(cen_l), float(cen_alpha)
-        reg.sm_l, reg.sm_maj, reg.sm_min, reg.sm_phi = map(float, (sm_l, sm_maj, sm_min, sm_phi))
-
-        data = _lib.SmiliData()
-        if uvidxfcv is not None and len(uvidxfcv) > 0:
-            Vfcv = np.asarray(Vfcv, np.complex128)
-            data.nfcv = len(uvidxfcv)
-            data.uvidxfcv = iptr(keep(_i32(uvidxfcv)))
-            data.vfcv_re = dptr(keep(_f64(Vfcv.real)))
-            data.vfcv_im = dptr(keep(_f64(Vfcv.imag)))
-            data.varfcv = dptr(keep(_f64(Varfcv)))
-            data.wfcv = float(wfcv)
-        if uvidxamp is not None and len(uvidxamp) > 0:
-            data.namp = len(uvidxamp)
-            data.uvidxamp = iptr(keep(_i32(uvidxamp)))
-            data.vamp = dptr(keep(_f64(Vamp)))
-            data.varamp = dptr(keep(_f64(Varamp)))
-            data.wamp = float(wamp)
-        if uvidxcp is not None and np.asarray(uvidxcp).shape[-1] > 0:
-            idx = np.asarray(uvidxcp).reshape(3, -1)
-            data.ncp = idx.shape[1]
-            data.uvidxcp = iptr(keep(_i32(idx.T)))      # 3 contiguous per closure
-            data.cp = dptr(keep(_f64(CP)))
-            data.varcp = dptr(keep(_f64(Varcp)))
-            data.wcp = float(wcp)
-        if uvidxca is not None and np.asarray(uvidxca).shape[-1] > 0:
-            idx = np.asarray(uvidxca).reshape(4, -1)
-            data.nca = idx.shape[1]
-            data.uvidxca = iptr(keep(_i32(idx.T)))      # 4 contiguous per closure
-            data.ca = dptr(keep(_f64(CA)))
-            data.varca = dptr(keep(_f64(Varca)))
-            data.wca = float(wca)
-
-        desc = _lib.SmiliDesc()
-        desc.nx, desc.ny = self.Nx, self.Ny
-        desc.nxref, desc.nyref = float(Nxref), float(Nyref)
-        desc.npix = self.npix
-        desc.xidx, desc.yidx = iptr(xidx), iptr(yidx)
-        desc.nuv = self.nuv
-        desc.u, desc.v = dptr(u), dptr(v)
-        desc.reg = reg
-        desc.data = data
-
+        desc, keep = build_desc(**kw)
+        self.Nx, self.Ny = int(desc.nx), int(desc.ny)
+        self.npix = int(desc.npix)
+        self.nuv = int(desc.nuv)
         t = None
         if tune:
             t = _lib.SmiliTune()
@@ -120,10 +126,8 @@ class CostProblem:
                 setattr(t, k, int(val))
         h = C.c_void_p()
         check(lib.smili_problem_create(C.byref(desc), C.byref(t) if t is not None else None, C.byref(h)))
+        del keep                  # the library copied everything it needs
         self._h = h
-        self._keep = []           # the library copied everything it needs
-        self._grad = np.empty(self.npix)
-        self._terms = np.empty(NTERMS)
 
     # ------------------------------------------------------------------
     def close(self):

@@ -289,8 +289,9 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   sub_points = tune.sub_points > 0 ? tune.sub_points : 256;
   col_group = tune.col_group > 0 ? tune.col_group : 2;
   chunk = tune.chunk > 0 ? tune.chunk : 2048;
-  require(tune.precision == 0 || tune.precision == 1, "precision must be 0 (FP32) or 1 (FP64)");
-  fp64 = tune.precision == 1;
+  require(tune.precision == 0 || tune.precision == 32 || tune.precision == 64,
+          "precision must be 0 (default, FP64), 32 or 64");
+  fp64 = tune.precision != 32;
   require(bin >= 8 && bin <= 64, "bin must be in [8, 64]");
   require(sub_points >= 32 && sub_points <= 65536, "sub_points must be in [32, 65536]");
   require(col_group >= 1 && col_group <= 16, "col_group must be in [1, 16]");

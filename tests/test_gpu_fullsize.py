@@ -15,7 +15,7 @@ def big():
     from smili_amd import workloads
     from smili_amd.cost import CostProblem
     prep, _ = workloads.single_large(seed=0)
-    g = CostProblem(**prep.kwargs)
+    g = CostProblem(**prep.kwargs, tune=dict(precision=32))
     yield prep, g
     g.close()
 
@@ -57,3 +57,17 @@ def test_full_size_repeatable_cost(gpu_lib, big):
     c2, g2, _ = g.cost_grad(prep.x0)
     assert c1 == c2
     assert rel_l2(g2, g1) < 1e-6
+
+
+def test_full_size_fp64_matches_fp64_oracle(gpu_lib, big):
+    """Default (FP64) NUFFT at full size: same arithmetic as the FP64 ES
+    oracle, agreement to rounding."""
+    from smili_amd.cost import CostProblem
+    prep, _ = big
+    p = O.Problem(**prep.kwargs)
+    with CostProblem(**prep.kwargs) as g64:
+        c, gr, t = g64.cost_grad(prep.x0)
+    c_ref, g_ref, _ = O.calc_cost(prep.x0, p, fwd=O.es_nufft_type2, adj=O.es_nufft_type1)
+    print("full size fp64: cost", abs(c - c_ref) / abs(c_ref), "grad", rel_l2(gr, g_ref))
+    assert abs(c - c_ref) <= 1e-12 * abs(c_ref)
+    assert rel_l2(gr, g_ref) <= 1e-10

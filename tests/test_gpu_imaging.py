@@ -51,7 +51,7 @@ def _solve_small_fcv(precision):
         lbfgs.lbfgsbprms.update(old)
 
 
-@pytest.mark.parametrize("precision,tol_img", [(1, 1e-4), (0, 1e-3)], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("precision,tol_img", [(64, 1e-4), (32, 1e-3)], ids=["fp64", "fp32"])
 def test_imaging_reproduces_reference_solve(gpu_lib, precision, tol_img):
     """64x64 ring, ~1,900 complex visibilities, fcv chi-square + L1 (1) + TSV
     (1e4), nonneg, L-BFGS-B m=10, 1000 iterations from a 20 uas Gaussian,

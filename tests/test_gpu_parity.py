@@ -34,9 +34,13 @@ TOL_GRAD_ES = 5e-6
 TOL_REG = 1e-12
 
 
-def _mk(p, **kw):
+def _mk(p, tune=None):
+    """FP32 NUFFT unless the test asks otherwise (these tolerances are the
+    FP32 path's; the FP64 path is checked at rounding level below)."""
     from smili_amd.cost import CostProblem
-    return CostProblem(**problem_kwargs(p), **kw)
+    t = {"precision": 32}
+    t.update(tune or {})
+    return CostProblem(**problem_kwargs(p), tune=t)
 
 
 SHAPES = [(16, 16), (24, 20), (17, 13), (32, 48), (64, 64), (40, 30)]
@@ -201,13 +205,13 @@ def test_chunks_split_at_the_slot_limit(gpu_lib):
 
 @pytest.mark.parametrize("shape,window", [((24, 20), False), ((33, 17), True), ((64, 64), False)])
 def test_fp64_mode_matches_fp64_es_oracle(gpu_lib, shape, window):
-    """precision=1 runs the same ES-kernel NUFFT in FP64: it must agree with
+    """precision=64 runs the same ES-kernel NUFFT in FP64: it must agree with
     the FP64 ES oracle to rounding (visibilities rel-l2 <= 1e-11, cost rel
     <= 1e-12, gradient rel-l2 <= 1e-10)."""
     rng = np.random.default_rng(300 + shape[0])
     p, x = random_problem(rng, shape[0], shape[1], M=500, window=window)
     I2d = O.image_1d_to_2d(x, p.xidx, p.yidx, p.Nx, p.Ny)
-    with _mk(p, tune=dict(precision=1)) as g:
+    with _mk(p, tune=dict(precision=64)) as g:
         V = g.model_vis(x)
         c, gr, t = g.cost_grad(x)
     Ve = O.es_nufft_type2(I2d, p.u, p.v)
