@@ -100,6 +100,42 @@ typedef struct smili_desc {
 
 typedef struct smili_problem smili_problem;
 
+/* A movie (dynamic imaging, src/fortlib/fftim3d.f90:472-766): nz frames of
+ * one geometry.  `frame` holds the geometry, window, static regularizers and
+ * the data, whose uv indices refer to the concatenation of the frames' uv
+ * lists (frame-major, nuvs[z] points for frame z, nuvs[z] may be 0; tools.py
+ * get_uvlist_loop 255-336); frame.nuv = sum(nuvs).  The image vector is
+ * frame-major (npix * nz).
+ *
+ * Static regularizers are evaluated per frame and divided by nz
+ * (calc_cost_reg3d, image3d.f90:184-256).  Light-curve mode (lc_l != NULL and
+ * sum(lc_l) > 0, image3d.f90:185-219): frame z uses lc_l[z], lc_tgtfd[z] in
+ * place of tfd_l, tfd_tgtfd, and for lc_nidx 1 or 3 its l1/tv/kl/gs weights
+ * are scaled by tfd_tgtfd / lc_tgtfd[z] and its tsv weights by the square.
+ * Centroid regularization has no effect on a movie (image3d.f90:213, 231:
+ * frames get cen_l = -1; calc_cost_dynamical leaves cen_cost at 0).
+ * Temporal regularizers (calc_cost_dynamical, image3d.f90:280-392) on
+ * I_z = x_z / lc_tgtfd[z] (lc_tgtfd NULL = all ones):
+ *   Rt = rt_l sum_{z < nz-1} sum_k rt_wgt[k] (I_{z+1,k} - I_{z,k})^2
+ *   Ri = ri_l sum_z sum_k ri_wgt[k] (I_{z,k} - <I>_k)^2, <I> = mean_z I_z
+ * with the reference's gradients (the last frame receives no Rt gradient,
+ * image3d.f90:374; Ri treats <I> as a constant, image3d.f90:381-385). */
+typedef struct smili_movie_desc {
+  smili_desc frame;
+  int32_t nz;
+  const int32_t *nuvs;
+  double rt_l; const double *rt_wgt;
+  double ri_l; const double *ri_wgt;
+  const double *lc_l;
+  const double *lc_tgtfd;
+  int32_t lc_nidx;
+} smili_movie_desc;
+
+/* Terms of a movie: the 15 of smili_term (reg includes the temporal and
+ * light-curve terms; tfd is 0 in light-curve mode) followed by
+ * [15] = Rt, [16] = Ri, [17] = light curve. */
+#define SMILI_NTERMS_MOVIE 18
+
 /* Tunables of the spreading/interpolation kernels (0 = default). */
 typedef struct smili_tune {
   int32_t bin;          /* bin edge in fine-grid cells (8..32, default 16) */
@@ -124,15 +160,19 @@ int smili_problem_create(const smili_desc *desc, const smili_tune *tune,
                          smili_problem **out);
 void smili_problem_destroy(smili_problem *p);
 
-/* cost + gradient + the 15 terms for x (host pointers): one call of
- * calc_cost (src/fortlib/fftim2d.f90:387-607).  grad has npix entries;
- * terms may be NULL; cost may be NULL. */
+/* cost + gradient + the terms for x (host pointers): one call of
+ * calc_cost (src/fortlib/fftim2d.f90:387-607).  grad has npix entries
+ * (npix * nz for a movie); terms (15, or 18 for a movie) may be NULL; cost
+ * may be NULL.  Handles created by smili_batch_create with more than one
+ * problem use smili_batch_cost_grad. */
 int smili_cost_grad(smili_problem *p, const double *x, double *cost,
                     double *grad, double *terms);
 
 /* The same on device memory, asynchronous on the problem's stream.
- * x_dev/grad_dev are device pointers with npix entries, terms_dev a device
- * pointer with 15 entries.  Use smili_synchronize() before reading. */
+ * x_dev/grad_dev are device pointers with the handle's image-vector length
+ * (smili_problem_sizes), terms_dev a device pointer with nprob * nterms
+ * entries (15 per problem, 18 for a movie).  Use smili_synchronize() before
+ * reading. */
 int smili_cost_grad_device(smili_problem *p, const double *x_dev,
                            double *grad_dev, double *terms_dev);
 int smili_synchronize(smili_problem *p);
@@ -145,14 +185,34 @@ int smili_eval_async(smili_problem *p);
 /* hipStream_t of the problem, as void* (plain pointer, no HIP types). */
 void *smili_problem_stream(smili_problem *p);
 
+/* Batched survey: nprob independent problems (any mix of sizes and data)
+ * evaluated together, one launch per kernel stage for all of them (captured
+ * in one hipGraph).  x / grad are the problems' image vectors concatenated
+ * in order; costs has nprob entries, terms 15 * nprob (may be NULL). */
+int smili_batch_create(const smili_desc *descs, int32_t nprob, const smili_tune *tune,
+                       smili_problem **out);
+int smili_batch_cost_grad(smili_problem *p, const double *x, double *costs, double *grad,
+                          double *terms);
+
+/* Movie: one problem over nz frames (x, grad: npix * nz; terms: 18).
+ * smili_cost_grad / smili_eval_async evaluate it like a single problem. */
+int smili_movie_create(const smili_movie_desc *desc, const smili_tune *tune, smili_problem **out);
+
+/* Counts of a handle: problems, image-vector length, terms per problem. */
+int smili_problem_sizes(smili_problem *p, int32_t *nprob, int64_t *nx_total, int32_t *nterms);
+
 /* Model visibilities at the nuv unique uv points for image x (host), i.e.
  * the type-2 transform finufft2d2 (src/wrapfinufft/wrapfinufft.f90:94-128)
- * that fftlib.model_fcv/amp/cp/ca evaluate first (src/fortlib/fftlib.f90:744-1254). */
+ * that fftlib.model_fcv/amp/cp/ca evaluate first (src/fortlib/fftlib.f90:744-1254).
+ * For a movie or a batch the uv lists of all frames / problems follow each
+ * other in order (NUFFT_fwd per frame, src/fortlib/fftlib3d.f90:76-90). */
 int smili_model_vis(smili_problem *p, const double *x, double *vre, double *vim);
 
 /* Re(A1 (vres_re - i vres_im)) at the npix window pixels, i.e.
  * NUFFT_adj_resid (src/fortlib/fftlib.f90:152-174) for real residual
- * vectors, followed by I1d_I2d_inv (src/fortlib/image.f90:1190-1207). */
+ * vectors, followed by I1d_I2d_inv (src/fortlib/image.f90:1190-1207).
+ * Movies and batches: residuals in uv-list order as above, out has the
+ * handle's image-vector length (fftlib3d.f90:126-142). */
 int smili_adjoint_resid(smili_problem *p, const double *vres_re,
                         const double *vres_im, double *out);
 
@@ -170,6 +230,7 @@ enum smili_stage {
   SMILI_S_T1_ROWS,         /* FFT along x + deconvolve + crop */
   SMILI_S_REG,             /* regularizers + gradient assembly */
   SMILI_S_FINALIZE,        /* fixed-order sums -> the 15 terms */
+  SMILI_S_FRAME_MEAN,      /* movies: mean normalised frame for Ri */
   SMILI_NSTAGES
 };
 

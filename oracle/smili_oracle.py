@@ -19,6 +19,10 @@ It restates, in plain numpy FP64, the reference algorithm of
 * problem set-up     -- src/fortlib/fftim2d.f90:178-241 and
                         smili/imaging/lbfgs.py:202-489
 * uv list            -- smili/uvdata/uvtable/tools.py:12-132
+* dynamic imaging    -- fftim3d.calc_cost (src/fortlib/fftim3d.f90:472-640):
+                        calc_chisq3d (src/fortlib/fftlib3d.f90:14-201),
+                        calc_cost_reg3d and calc_cost_dynamical
+                        (src/fortlib/image3d.f90:15-440)
 
 A second forward/adjoint pair, ``es_nufft_type2`` / ``es_nufft_type1``,
 implements the exponential-of-semicircle (ES) spreading NUFFT of Barnett,
@@ -33,10 +37,15 @@ against the adjoint identity and against closed-form Gaussian visibilities;
 each chi-square gradient against central finite differences and against zero
 residual at noise-free data; the regularizers against brute-force per-pixel
 loops written from image.f90 and against finite differences; the TV reset
-quirk (image.f90:577-578, 603-604) against a hand-built flat patch.
+quirk (image.f90:577-578, 603-604) against a hand-built flat patch.  The
+3-D cost is pinned by reductions to the 2-D one (one frame; identical frames
+sharing one uv list), by finite differences of Rt / Ri (and the missing Rt
+gradient of the last frame, image3d.f90:374), by closed forms on constant
+and linear-in-time movies, and by the light-curve weight rescaling.
 """
 from __future__ import annotations
 
+import dataclasses
 import math
 from dataclasses import dataclass, field
 
@@ -576,7 +585,16 @@ def calc_chisq(x, p: Problem, fwd=None, adj=None):
     fwd = fwd or dft_fwd
     I2d = image_1d_to_2d(x, p.xidx, p.yidx, p.Nx, p.Ny)
     V = fwd(I2d, p.u, p.v)
-    M = p.u.size
+    chisq, t, Vresre, Vresim = _chisq_blocks(V, p)
+    g2d = nufft_adj_resid(Vresre, Vresim, p.u, p.v, p.Nx, p.Ny, adj)
+    grad = g2d[np.asarray(p.yidx) - 1, np.asarray(p.xidx) - 1]
+    return chisq, grad, t, V, Vresre, Vresim
+
+
+def _chisq_blocks(V, p: Problem):
+    """The four data blocks of calc_chisq on model visibilities V
+    (fftlib.f90:338-367; the same sequence in fftlib3d.f90:96-121)."""
+    M = V.size
     Vresre = np.zeros(M)
     Vresim = np.zeros(M)
     chisq = 0.0
@@ -597,10 +615,8 @@ def calc_chisq(x, p: Problem, fwd=None, adj=None):
         c = chisq_cp(V, p.uvidxcp, p.CP, p.Varcp, p.wcp, Vresre, Vresim)
         chisq += c
         t["chisq_cp"] = c / p.wcp / p.uvidxcp.shape[1]
-    g2d = nufft_adj_resid(Vresre, Vresim, p.u, p.v, p.Nx, p.Ny, adj)
-    grad = g2d[np.asarray(p.yidx) - 1, np.asarray(p.xidx) - 1]
     t["chisq"] = chisq
-    return chisq, grad, t, V, Vresre, Vresim
+    return chisq, t, Vresre, Vresim
 
 
 def calc_cost_reg(x, p: Problem):
@@ -659,6 +675,166 @@ def calc_cost(x, p: Problem, fwd=None, adj=None):
 
 def terms_vector(terms):
     return np.array([terms[k] for k in TERM_NAMES], dtype=np.float64)
+
+
+# ----------------------------------------------------------------------------
+# dynamic imaging: fftim3d.calc_cost (fftim3d.f90:472-640)
+# ----------------------------------------------------------------------------
+TERM_NAMES_MOVIE = TERM_NAMES + ("rt", "ri", "lc")
+
+
+@dataclass
+class Movie:
+    """Arguments of fftim3d.calc_cost (fftim3d.f90:472-498): ``frame`` holds
+    the geometry, the static regularizers and the data over the
+    concatenation of the frames' uv lists (frame z owns Nuvs[z] consecutive
+    points, tools.py get_uvlist_loop 255-336).  Image vector: Nz * Npix,
+    frame-major.  lc_l None = light-curve mode off (lbfgs3d.py:528-530);
+    lc_tgtfd None = ones."""
+    frame: Problem
+    Nuvs: np.ndarray
+    rt_l: float = -1.0
+    rt_wgt: np.ndarray | None = None
+    ri_l: float = -1.0
+    ri_wgt: np.ndarray | None = None
+    lc_l: np.ndarray | None = None
+    lc_tgtfd: np.ndarray | None = None
+    lc_nidx: int = -1
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def Nz(self):
+        return int(np.asarray(self.Nuvs).size)
+
+    @property
+    def lc(self):
+        return np.ones(self.Nz) if self.lc_tgtfd is None else np.asarray(self.lc_tgtfd, np.float64)
+
+    @property
+    def lc_mode(self):
+        return self.lc_l is not None and float(np.sum(self.lc_l)) > 0
+
+
+def calc_chisq3d(x, mv: Movie, fwd=None, adj=None):
+    """fftlib3d.f90:14-201: per-frame type-2 transforms into one
+    visibility vector, the shared chi-square blocks, per-frame adjoints of
+    each frame's residual slice.  Frames without uv points are skipped
+    (fftlib3d.f90:77, 127).  Returns (chisq, grad[Nz*Npix], terms, V)."""
+    p = mv.frame
+    fwd = fwd or dft_fwd
+    Nz, npix = mv.Nz, p.npix
+    nuvs = np.asarray(mv.Nuvs, np.int64)
+    start = np.concatenate([[0], np.cumsum(nuvs)])
+    V = np.zeros(p.u.size, np.complex128)
+    for z in range(Nz):
+        if nuvs[z] == 0:
+            continue
+        s, e = start[z], start[z + 1]
+        I2d = image_1d_to_2d(x[z * npix:(z + 1) * npix], p.xidx, p.yidx, p.Nx, p.Ny)
+        V[s:e] = fwd(I2d, p.u[s:e], p.v[s:e])
+    chisq, t, Vresre, Vresim = _chisq_blocks(V, p)
+    grad = np.zeros(Nz * npix)
+    sel = (np.asarray(p.yidx) - 1, np.asarray(p.xidx) - 1)
+    for z in range(Nz):
+        if nuvs[z] == 0:
+            continue
+        s, e = start[z], start[z + 1]
+        g2d = nufft_adj_resid(Vresre[s:e], Vresim[s:e], p.u[s:e], p.v[s:e], p.Nx, p.Ny, adj)
+        grad[z * npix:(z + 1) * npix] = g2d[sel]
+    return chisq, grad, t, V
+
+
+def frame_problem(mv: Movie, z):
+    """The 2-D regularizer arguments frame z sees in calc_cost_reg3d
+    (image3d.f90:184-236): centroid off; in light-curve mode the frame's
+    lc_l / lc_tgtfd replace tfd_l / tfd_tgtfd and, for lc_nidx 1 or 3, the
+    weights are rescaled by tfd_tgtfd / lc_tgtfd (squared for TSV)."""
+    p = mv.frame
+    rep = dict(cen_l=-1.0, cen_alpha=1.0)
+    if mv.lc_mode:
+        lc = mv.lc
+        rep.update(tfd_l=float(mv.lc_l[z]), tfd_tgtfd=float(lc[z]))
+        if mv.lc_nidx in (1, 3):
+            s = p.tfd_tgtfd / lc[z]
+            for name in ("l1", "tv", "kl", "gs"):
+                w = getattr(p, name + "_wgt")
+                if w is not None:
+                    rep[name + "_wgt"] = np.asarray(w) * p.tfd_tgtfd / lc[z]
+            if p.tsv_wgt is not None:
+                rep["tsv_wgt"] = np.asarray(p.tsv_wgt) * s ** 2
+    return dataclasses.replace(p, **rep)
+
+
+def calc_cost_dynamical(x, mv: Movie):
+    """image3d.f90:280-392 (Rt and Ri; Rs, Rf and the centroid term stay 0
+    there).  Returns (rt_cost, ri_cost, grad[Nz*Npix])."""
+    Nz, npix = mv.Nz, mv.frame.npix
+    lc = mv.lc
+    Iin = np.asarray(x, np.float64).reshape(Nz, npix)
+    Iavg = np.zeros(npix)
+    for z in range(Nz):
+        Iavg = Iavg + (Iin[z] / lc[z]) / Nz
+    rt_cost = 0.0
+    ri_cost = 0.0
+    g = np.zeros((Nz, npix))
+    for z in range(Nz):
+        II = Iin[z] / lc[z]
+        Iu = Iin[z + 1] / lc[z + 1] if z < Nz - 1 else np.zeros(npix)
+        Il = Iin[z - 1] / lc[z - 1] if z > 0 else np.zeros(npix)
+        if mv.rt_l > 0 and z < Nz - 1:
+            rt_cost += np.sum(mv.rt_l * mv.rt_wgt * (Iu - II) ** 2)
+            # rt_grade (image3d.f90:409-421); its iz == Nz branch is never
+            # reached because of the z < Nz - 1 guard above
+            gr = 2.0 * (II - Iu) if z == 0 else 2.0 * (2.0 * II - Il - Iu)
+            g[z] += mv.rt_l * mv.rt_wgt * gr / lc[z]
+        if mv.ri_l > 0:
+            ri_cost += np.sum(mv.ri_l * mv.ri_wgt * (II - Iavg) ** 2)
+            g[z] += mv.ri_l * mv.ri_wgt * (2.0 * (II - Iavg)) / lc[z]
+    return rt_cost, ri_cost, g.reshape(-1)
+
+
+def calc_cost_reg3d(x, mv: Movie):
+    """image3d.f90:15-274.  Returns (reg, gradreg[Nz*Npix], terms)."""
+    Nz, npix = mv.Nz, mv.frame.npix
+    x = np.asarray(x, np.float64)
+    g = np.zeros(Nz * npix)
+    keys = ("l1", "tv", "tsv", "kl", "gs", "tfd", "cen", "sm")
+    t = {k: 0.0 for k in keys}
+    t.update(rt=0.0, ri=0.0, lc=0.0)
+    reg = 0.0
+    for z in range(Nz):
+        pz = frame_problem(mv, z)
+        rz, gz, tz = calc_cost_reg(x[z * npix:(z + 1) * npix], pz)
+        for k in keys:
+            if k == "tfd" and mv.lc_mode:
+                t["lc"] += tz[k] / Nz
+            else:
+                t[k] += tz[k] / Nz
+        reg += rz / Nz
+        g[z * npix:(z + 1) * npix] = gz / Nz
+    if mv.frame.cen_l > 0 or mv.rt_l > 0 or mv.ri_l > 0:
+        rt, ri, gd = calc_cost_dynamical(x, mv)
+        t["rt"], t["ri"] = rt, ri
+        reg += rt + ri
+        g = g + gd
+    return reg, g, t
+
+
+def calc_cost3d(x, mv: Movie, fwd=None, adj=None):
+    """fftim3d.f90:472-640: cost = chisq3d + reg3d.  Returns (cost, grad,
+    terms) with terms keyed by TERM_NAMES_MOVIE."""
+    x = np.asarray(x, np.float64)
+    chisq, gchi, tc, _ = calc_chisq3d(x, mv, fwd, adj)
+    reg, greg, tr = calc_cost_reg3d(x, mv)
+    terms = dict(tc)
+    terms.update(tr)
+    terms["reg"] = reg
+    terms["cost"] = chisq + reg
+    return chisq + reg, gchi + greg, terms
+
+
+def terms_vector_movie(terms):
+    return np.array([terms[k] for k in TERM_NAMES_MOVIE], dtype=np.float64)
 
 
 # ----------------------------------------------------------------------------

@@ -17,6 +17,8 @@ LIB_PATH = os.path.join(_HERE, "libsmili_hip.so")
 NTERMS = 15
 TERM_NAMES = ("chisq", "chisq_fcv", "chisq_amp", "chisq_cp", "chisq_ca",
               "reg", "l1", "tv", "tsv", "kl", "gs", "tfd", "cen", "sm", "cost")
+NTERMS_MOVIE = 18
+TERM_NAMES_MOVIE = TERM_NAMES + ("rt", "ri", "lc")
 
 SMILI_OK = 0
 SMILI_ENONFINITE = -4
@@ -56,6 +58,13 @@ class SmiliDesc(C.Structure):
                 ("reg", SmiliReg), ("data", SmiliData)]
 
 
+class SmiliMovieDesc(C.Structure):
+    _fields_ = [("frame", SmiliDesc), ("nz", C.c_int32), ("nuvs", _pi),
+                ("rt_l", C.c_double), ("rt_wgt", _pd),
+                ("ri_l", C.c_double), ("ri_wgt", _pd),
+                ("lc_l", _pd), ("lc_tgtfd", _pd), ("lc_nidx", C.c_int32)]
+
+
 class SmiliTune(C.Structure):
     _fields_ = [("bin", C.c_int32), ("sub_points", C.c_int32),
                 ("col_group", C.c_int32), ("chunk", C.c_int32),
@@ -79,13 +88,19 @@ SIGNATURES = {
     "smili_adjoint_resid": (C.c_int, [C.c_void_p, _pd, _pd, _pd]),
     "smili_problem_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "smili_problem_profile": (C.c_int, [C.c_void_p, C.c_int32, _pd]),
+    "smili_batch_create": (C.c_int, [C.POINTER(SmiliDesc), C.c_int32, C.POINTER(SmiliTune),
+                                     C.POINTER(C.c_void_p)]),
+    "smili_batch_cost_grad": (C.c_int, [C.c_void_p, _pd, _pd, _pd, _pd]),
+    "smili_movie_create": (C.c_int, [C.POINTER(SmiliMovieDesc), C.POINTER(SmiliTune), C.POINTER(C.c_void_p)]),
+    "smili_problem_sizes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                                      C.POINTER(C.c_int32)]),
 }
 
 STAGE_NAMES = ("scatter", "t2_rows", "t2_cols", "interp", "moments", "moments_reduce",
-               "chisq", "spread", "t1_cols", "t1_rows", "reg", "finalize")
+               "chisq", "spread", "t1_cols", "t1_rows", "reg", "finalize", "frame_mean")
 KERNEL_OF_STAGE = dict(zip(STAGE_NAMES, ("k_scatter", "k_t2_rows", "k_t2_cols", "k_interp", "k_moments",
                                          "k_moments_reduce", "k_chisq", "k_spread", "k_t1_cols",
-                                         "k_t1_rows", "k_reg", "k_finalize")))
+                                         "k_t1_rows", "k_reg", "k_finalize", "k_frame_mean")))
 
 _lib = None
 

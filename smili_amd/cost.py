@@ -1,4 +1,4 @@
-"""Host wrapper of one imaging cost on the GPU.
+"""Host wrappers of the imaging costs on the GPU.
 
 ``CostProblem`` takes the argument list of the reference's
 ``fftim2d.calc_cost`` (src/fortlib/fftim2d.f90:387-407) -- image geometry,
@@ -6,6 +6,10 @@ scaled uv list, regularizer lambdas and weights, and the four data blocks
 with signed 1-based uv indices -- uploads it once, and then evaluates
 ``cost, grad, terms = problem.cost_grad(x)`` on the GPU through the C ABI of
 libsmili_hip.so.  Keyword names follow calc_cost's argument names.
+
+``MovieProblem`` does the same for the dynamic-imaging cost
+``fftim3d.calc_cost`` (src/fortlib/fftim3d.f90:472-640) and
+``BatchProblem`` for many independent 2-D problems evaluated together.
 """
 from __future__ import annotations
 
@@ -14,7 +18,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import NTERMS, TERM_NAMES, check, dptr, iptr
+from ._lib import NTERMS, NTERMS_MOVIE, TERM_NAMES, TERM_NAMES_MOVIE, check, dptr, iptr
 
 
 def _f64(a):
@@ -99,35 +103,27 @@ def build_desc(Nx, Ny, xidx, yidx, u, v, Nxref=0.0, Nyref=0.0,
     return desc, keep
 
 
-class CostProblem:
-    """One RML imaging objective resident on the current GPU.
+def _tune(tune):
+    if not tune:
+        return None
+    t = _lib.SmiliTune()
+    for k, val in tune.items():
+        setattr(t, k, int(val))
+    return t
 
-    Parameters follow calc_cost (fftim2d.f90:387-407): ``xidx, yidx`` 1-based
-    pixel coordinates of the npix image vector; ``u, v`` scaled unique uv
-    points; ``uvidxcp`` of shape (3, Ncp) and ``uvidxca`` of shape (4, Nca)
-    as the Fortran arrays uvidxcp(3,Ncp), uvidxca(4,Nca); Vfcv complex and
-    already phase-shifted; w* normalised weights.  A regularizer is on iff
-    its lambda > 0; a data block is on iff it is given and non-empty.
-    ``tune`` sets smili_tune fields (bin, sub_points, col_group, chunk,
-    use_graph, precision).
-    """
 
-    def __init__(self, tune=None, **kw):
-        lib = _lib.load()
+class _Handle:
+    """Common part of the handle wrappers: lifetime, device buffers, the
+    transforms and profiling."""
+
+    _term_names = TERM_NAMES
+
+    def _adopt(self, lib, h):
         self._lib = lib
-        desc, keep = build_desc(**kw)
-        self.Nx, self.Ny = int(desc.nx), int(desc.ny)
-        self.npix = int(desc.npix)
-        self.nuv = int(desc.nuv)
-        t = None
-        if tune:
-            t = _lib.SmiliTune()
-            for k, val in tune.items():
-                setattr(t, k, int(val))
-        h = C.c_void_p()
-        check(lib.smili_problem_create(C.byref(desc), C.byref(t) if t is not None else None, C.byref(h)))
-        del keep                  # the library copied everything it needs
         self._h = h
+        nprob, nxt, nterms = C.c_int32(), C.c_int64(), C.c_int32()
+        check(lib.smili_problem_sizes(h, C.byref(nprob), C.byref(nxt), C.byref(nterms)))
+        self.nprob, self.nx_total, self.nterms = int(nprob.value), int(nxt.value), int(nterms.value)
 
     # ------------------------------------------------------------------
     def close(self):
@@ -148,31 +144,19 @@ class CostProblem:
         self.close()
 
     # ------------------------------------------------------------------
-    def cost_grad(self, x, allow_nonfinite=False):
-        """(cost, grad[npix], terms dict) for image vector x."""
-        x = _f64(x)
-        if x.size != self.npix:
-            raise ValueError(f"x has {x.size} entries, expected {self.npix}")
-        grad = np.empty(self.npix)
-        terms = np.empty(NTERMS)
-        cost = C.c_double()
-        check(self._lib.smili_cost_grad(self._h, dptr(x), C.byref(cost), dptr(grad), dptr(terms)),
-              allow_nonfinite)
-        return cost.value, grad, dict(zip(TERM_NAMES, terms.tolist()))
-
     def cost_grad_device(self, x_ptr, grad_ptr, terms_ptr=None):
         """Asynchronous evaluation on device pointers (ints)."""
         check(self._lib.smili_cost_grad_device(self._h, C.c_void_p(x_ptr), C.c_void_p(grad_ptr),
                                                 C.c_void_p(terms_ptr) if terms_ptr else None))
 
     def buffers(self):
-        """Device addresses (ints) of the problem's own x, grad and terms."""
+        """Device addresses (ints) of the handle's own x, grad and terms."""
         x, g, t = C.c_void_p(), C.c_void_p(), C.c_void_p()
         check(self._lib.smili_problem_buffers(self._h, C.byref(x), C.byref(g), C.byref(t)))
         return x.value, g.value, t.value
 
     def eval_async(self):
-        """One evaluation on the problem's own buffers (no copies, async)."""
+        """One evaluation on the handle's own buffers (no copies, async)."""
         check(self._lib.smili_eval_async(self._h))
 
     def synchronize(self):
@@ -183,16 +167,20 @@ class CostProblem:
         return self._lib.smili_problem_stream(self._h)
 
     def model_vis(self, x):
-        """Complex model visibilities at the nuv uv points (type-2 transform)."""
+        """Complex model visibilities at the uv points (type-2 transforms),
+        all frames / problems in order."""
         x = _f64(x)
+        if x.size != self.nx_total:
+            raise ValueError(f"x has {x.size} entries, expected {self.nx_total}")
         vre = np.empty(self.nuv)
         vim = np.empty(self.nuv)
         check(self._lib.smili_model_vis(self._h, dptr(x), dptr(vre), dptr(vim)))
         return vre + 1j * vim
 
     def adjoint_resid(self, vres_re, vres_im):
-        """Re(A1 (vres_re - i vres_im)) at the window pixels."""
-        out = np.empty(self.npix)
+        """Re(A1 (vres_re - i vres_im)) at the window pixels of every frame /
+        problem."""
+        out = np.empty(self.nx_total)
         check(self._lib.smili_adjoint_resid(self._h, dptr(_f64(vres_re)), dptr(_f64(vres_im)), dptr(out)))
         return out
 
@@ -207,3 +195,140 @@ class CostProblem:
         check(self._lib.smili_problem_info(self._h, a))
         keys = ("nf1", "nf2", "nuv", "subproblems", "bin", "chunks", "launches", "graph")
         return dict(zip(keys, list(a)))
+
+    def _cost_grad(self, x, allow_nonfinite):
+        x = _f64(x)
+        if x.size != self.nx_total:
+            raise ValueError(f"x has {x.size} entries, expected {self.nx_total}")
+        grad = np.empty(self.nx_total)
+        terms = np.empty(self.nterms)
+        cost = C.c_double()
+        check(self._lib.smili_cost_grad(self._h, dptr(x), C.byref(cost), dptr(grad), dptr(terms)),
+              allow_nonfinite)
+        return cost.value, grad, dict(zip(self._term_names, terms.tolist()))
+
+
+class CostProblem(_Handle):
+    """One RML imaging objective resident on the current GPU.
+
+    Parameters follow calc_cost (fftim2d.f90:387-407): ``xidx, yidx`` 1-based
+    pixel coordinates of the npix image vector; ``u, v`` scaled unique uv
+    points; ``uvidxcp`` of shape (3, Ncp) and ``uvidxca`` of shape (4, Nca)
+    as the Fortran arrays uvidxcp(3,Ncp), uvidxca(4,Nca); Vfcv complex and
+    already phase-shifted; w* normalised weights.  A regularizer is on iff
+    its lambda > 0; a data block is on iff it is given and non-empty.
+    ``tune`` sets smili_tune fields (bin, sub_points, col_group, chunk,
+    use_graph, precision).
+    """
+
+    def __init__(self, tune=None, **kw):
+        lib = _lib.load()
+        desc, keep = build_desc(**kw)
+        self.Nx, self.Ny = int(desc.nx), int(desc.ny)
+        self.npix = int(desc.npix)
+        self.nuv = int(desc.nuv)
+        t = _tune(tune)
+        h = C.c_void_p()
+        check(lib.smili_problem_create(C.byref(desc), C.byref(t) if t is not None else None, C.byref(h)))
+        del keep                  # the library copied everything it needs
+        self._adopt(lib, h)
+
+    def cost_grad(self, x, allow_nonfinite=False):
+        """(cost, grad[npix], terms dict) for image vector x."""
+        return self._cost_grad(x, allow_nonfinite)
+
+
+class MovieProblem(_Handle):
+    """The dynamic-imaging objective of fftim3d.calc_cost
+    (src/fortlib/fftim3d.f90:472-640) resident on the current GPU.
+
+    ``Nz`` frames share the geometry and the static regularizers of the 2-D
+    keywords (as CostProblem); ``u, v`` is the concatenation of the frames'
+    uv lists with ``Nuvs[z]`` points for frame z, and the data's uv indices
+    refer to that concatenation (tools.py get_uvlist_loop, 255-336).
+    Temporal terms: ``rt_l, rt_wgt`` (Rt), ``ri_l, ri_wgt`` (Ri); light curve
+    ``lc_l, lc_tgtfd, lc_nidx`` as calc_cost_reg3d (image3d.f90:15-274).
+    The image vector is frame-major (Nz * npix).
+    """
+
+    _term_names = TERM_NAMES_MOVIE
+
+    def __init__(self, Nz, Nuvs, rt_l=-1.0, rt_wgt=None, ri_l=-1.0, ri_wgt=None,
+                 lc_l=None, lc_tgtfd=None, lc_nidx=-1, tune=None, **kw):
+        lib = _lib.load()
+        desc, keep = build_desc(**kw)
+        self.Nx, self.Ny = int(desc.nx), int(desc.ny)
+        self.npix = int(desc.npix)
+        self.nuv = int(desc.nuv)
+        self.Nz = int(Nz)
+        md = _lib.SmiliMovieDesc()
+        md.frame = desc
+        md.nz = self.Nz
+        nuvs = _i32(Nuvs)
+        if nuvs.size != self.Nz:
+            raise ValueError("Nuvs must have Nz entries")
+        keep.append(nuvs)
+        md.nuvs = iptr(nuvs)
+        md.rt_l, md.ri_l = float(rt_l), float(ri_l)
+        for name, lam, w in (("rt_wgt", rt_l, rt_wgt), ("ri_wgt", ri_l, ri_wgt)):
+            if lam > 0:
+                arr = _f64(np.broadcast_to(np.asarray(w, np.float64), (self.npix,)))
+                keep.append(arr)
+                setattr(md, name, dptr(arr))
+        for name, arr in (("lc_l", lc_l), ("lc_tgtfd", lc_tgtfd)):
+            if arr is not None:
+                arr = _f64(np.broadcast_to(np.asarray(arr, np.float64), (self.Nz,)))
+                keep.append(arr)
+                setattr(md, name, dptr(arr))
+        md.lc_nidx = int(lc_nidx)
+        t = _tune(tune)
+        h = C.c_void_p()
+        check(lib.smili_movie_create(C.byref(md), C.byref(t) if t is not None else None, C.byref(h)))
+        del keep
+        self._adopt(lib, h)
+
+    def cost_grad(self, x, allow_nonfinite=False):
+        """(cost, grad[Nz * npix], terms dict incl. rt, ri, lc)."""
+        return self._cost_grad(x, allow_nonfinite)
+
+
+class BatchProblem(_Handle):
+    """Many independent 2-D objectives evaluated together (one launch per
+    kernel stage for all of them).  ``problems`` is a list of keyword dicts
+    as CostProblem takes them."""
+
+    def __init__(self, problems, tune=None):
+        lib = _lib.load()
+        descs = (_lib.SmiliDesc * len(problems))()
+        keep = []
+        self.npixs, self.nuvs = [], []
+        for i, kw in enumerate(problems):
+            d, k = build_desc(**kw)
+            keep.append(k)
+            descs[i] = d
+            self.npixs.append(int(d.npix))
+            self.nuvs.append(int(d.nuv))
+        self.nuv = int(sum(self.nuvs))
+        self.offsets = np.concatenate([[0], np.cumsum(self.npixs)]).astype(np.int64)
+        t = _tune(tune)
+        h = C.c_void_p()
+        check(lib.smili_batch_create(descs, len(problems), C.byref(t) if t is not None else None, C.byref(h)))
+        del keep
+        self._adopt(lib, h)
+
+    def split(self, vec):
+        """Per-problem views of a concatenated image vector."""
+        return [vec[a:b] for a, b in zip(self.offsets[:-1], self.offsets[1:])]
+
+    def cost_grad(self, x, allow_nonfinite=False):
+        """(costs[nprob], grad (concatenated), terms[nprob, 15]) for the
+        concatenated image vectors x."""
+        x = _f64(x)
+        if x.size != self.nx_total:
+            raise ValueError(f"x has {x.size} entries, expected {self.nx_total}")
+        grad = np.empty(self.nx_total)
+        terms = np.empty((self.nprob, NTERMS))
+        costs = np.empty(self.nprob)
+        check(self._lib.smili_batch_cost_grad(self._h, dptr(x), dptr(costs), dptr(grad), dptr(terms)),
+              allow_nonfinite)
+        return costs, grad, terms

@@ -151,12 +151,25 @@ struct UnitSpec {
   const double* u;
   const double* v;
   int m;
+  int movie = -1, frame = 0;
+  bool tfd_override = false;             // light-curve mode: per-frame tfd
+  double tfd_l = 0.0, tfd_tgtfd = 0.0;
+  double wscale[5] = {1.0, 1.0, 1.0, 1.0, 1.0};   // l1, tv, tsv, kl, gs weight factors
+  bool no_cen = false;
 };
 
 struct ProbSpec {
   const smili_data* data;
   int unit_begin, unit_end;
   int nuv;               // length of the problem's uv list
+  int nterms = SMILI_NTERMS;
+  int lc_mode = 0;
+};
+
+struct MovieSpec {
+  int unit_begin, nz, npix;
+  double rt_l, ri_l;
+  std::vector<double> rt_w, ri_w, lc;
 };
 
 struct Plan {
@@ -169,6 +182,9 @@ struct Plan {
 
   std::vector<UnitDev> units;
   std::vector<ProbDev> probs;
+  std::vector<MovieDev> movies;
+  size_t nterms_total = 0;
+  bool any_ri = false;
   std::vector<std::vector<int>> newpos;   // per problem: uv list index -> global point
   std::vector<std::vector<int>> winpix;   // per unit: window pixel linear index
   size_t npix_total = 0, npt = 0;
@@ -212,6 +228,8 @@ struct Plan {
   DevBuf<CellSpan> d_spans;
   DevBuf<FftItem> d_t2rows, d_cols, d_t1rows;
   DevBuf<RegItem> d_regs;
+  DevBuf<MovieDev> d_movies;
+  DevBuf<double> d_movw, d_iavg;
 
   size_t lds_rows = 0, lds_cols = 0, lds_tile = 0;
   hipGraph_t graph = nullptr;
@@ -230,7 +248,8 @@ struct Plan {
     if (stream) (void)hipStreamDestroy(stream);
   }
 
-  void build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& ps, const smili_tune* t);
+  void build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& ps, const smili_tune* t,
+             const std::vector<MovieSpec>& ms = {});
   void launch_forward();
   void launch_adjoint();
   void launch_eval();
@@ -283,7 +302,8 @@ static void check_data(const smili_data& D, int nuv) {
   require(D.nca == 0 || D.ca, "closure amplitude data missing");
 }
 
-void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& ps, const smili_tune* t) {
+void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& ps, const smili_tune* t,
+                 const std::vector<MovieSpec>& ms) {
   if (t) tune = *t;
   bin = tune.bin > 0 ? tune.bin : 16;
   sub_points = tune.sub_points > 0 ? tune.sub_points : 256;
@@ -384,6 +404,8 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     for (int i = 0; i < d->npix; ++i) winpix[ui][i] = (d->yidx[i] - 1) * d->nx + (d->xidx[i] - 1);
     pix.insert(pix.end(), winpix[ui].begin(), winpix[ui].end());
     U.prob = us[ui].prob;
+    U.movie = us[ui].movie;
+    U.frame = us[ui].frame;
     U.nxref = d->nxref;
     U.nyref = d->nyref;
     U.reg_scale = us[ui].reg_scale;
@@ -391,21 +413,34 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     U.l1_l = r.l1_l; U.tv_l = r.tv_l; U.tsv_l = r.tsv_l; U.kl_l = r.kl_l; U.gs_l = r.gs_l;
     U.tfd_l = r.tfd_l; U.tfd_tgtfd = r.tfd_tgtfd; U.cen_l = r.cen_l; U.cen_alpha = r.cen_alpha;
     U.sm_l = r.sm_l; U.sm_maj = r.sm_maj; U.sm_min = r.sm_min; U.sm_phi = r.sm_phi;
-    U.need_moments = (r.tfd_l > 0 || r.cen_l > 0 || r.sm_l > 0) ? 1 : 0;
+    if (us[ui].tfd_override) {
+      U.tfd_l = us[ui].tfd_l;
+      U.tfd_tgtfd = us[ui].tfd_tgtfd;
+    }
+    if (us[ui].no_cen) U.cen_l = -1.0;
+    U.need_moments = (U.tfd_l > 0 || U.cen_l > 0 || U.sm_l > 0) ? 1 : 0;
     any_moments = any_moments || U.need_moments;
-    auto put = [&](double lam, const double* w, size_t& off) {
+    auto put = [&](double lam, const double* w, double s, size_t& off) {
       off = wgt.size();
-      if (lam > 0) wgt.insert(wgt.end(), w, w + d->npix);
+      if (!(lam > 0)) return;
+      if (s == 1.0) {
+        wgt.insert(wgt.end(), w, w + d->npix);
+      } else {
+        for (int i = 0; i < d->npix; ++i) wgt.push_back(w[i] * s);
+      }
     };
-    put(r.l1_l, r.l1_wgt, wgt_off_l1[ui]);
-    put(r.tv_l, r.tv_wgt, wgt_off_tv[ui]);
-    put(r.tsv_l, r.tsv_wgt, wgt_off_tsv[ui]);
-    put(r.kl_l, r.kl_wgt, wgt_off_kl[ui]);
-    put(r.gs_l, r.gs_wgt, wgt_off_gs[ui]);
+    const double* ws = us[ui].wscale;
+    put(r.l1_l, r.l1_wgt, ws[0], wgt_off_l1[ui]);
+    put(r.tv_l, r.tv_wgt, ws[1], wgt_off_tv[ui]);
+    put(r.tsv_l, r.tsv_wgt, ws[2], wgt_off_tsv[ui]);
+    put(r.kl_l, r.kl_wgt, ws[3], wgt_off_kl[ui]);
+    put(r.gs_l, r.gs_wgt, ws[4], wgt_off_gs[ui]);
     if (r.kl_l > 0)
-      for (int i = 0; i < d->npix; ++i) require(r.kl_wgt[i] > 0, "kl weights (prior) must be positive");
+      for (int i = 0; i < d->npix; ++i)
+        require(wgt[wgt_off_kl[ui] + i] > 0, "kl weights (prior) must be positive");
     if (r.gs_l > 0)
-      for (int i = 0; i < d->npix; ++i) require(r.gs_wgt[i] > 0, "gs weights (prior) must be positive");
+      for (int i = 0; i < d->npix; ++i)
+        require(wgt[wgt_off_gs[ui] + i] > 0, "gs weights (prior) must be positive");
     units[ui] = U;
   }
 
@@ -526,6 +561,7 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   probs.resize(ps.size());
   newpos.resize(ps.size());
   std::vector<int> slot_scratch(npt, -1);   // global point -> slot in the open chunk
+  nterms_total = 0;
   for (size_t pi = 0; pi < ps.size(); ++pi) {
     const ProbSpec& P = ps[pi];
     const smili_data& D = *P.data;
@@ -544,7 +580,10 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     Q.unit_end = P.unit_end;
     Q.n[DT_FCV] = D.nfcv; Q.n[DT_AMP] = D.namp; Q.n[DT_CP] = D.ncp; Q.n[DT_CA] = D.nca;
     Q.w[DT_FCV] = D.wfcv; Q.w[DT_AMP] = D.wamp; Q.w[DT_CP] = D.wcp; Q.w[DT_CA] = D.wca;
-    Q.terms_off = (int)(pi * SMILI_NTERMS);
+    Q.terms_off = (int)nterms_total;
+    Q.nterms = P.nterms;
+    Q.lc_mode = P.lc_mode;
+    nterms_total += P.nterms;
     // Greedy chunking: consecutive data of one type share a chunk while the
     // chunk references at most kMaxSlots distinct uv points and holds at
     // most `chunk` data.  Data are referenced by chunk-local slots.
@@ -643,6 +682,7 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
       f.grid_off = U.grid_off; f.img_off = U.img_off;
       return f;
     };
+    if (U.m == 0) continue;     // a frame without data: no transforms, zero chi-square gradient
     for (int s = 0; s < U.ny; s += rows_per_block) {
       const FftItem r = fft_item(s, std::min(rows_per_block, U.ny - s));
       h_t2rows.push_back(r);
@@ -676,6 +716,41 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     U.kl_w = U.kl_l > 0 ? d_wgt.p + wgt_off_kl[ui] : nullptr;
     U.gs_w = U.gs_l > 0 ? d_wgt.p + wgt_off_gs[ui] : nullptr;
   }
+  // movies: temporal weights and light curves, mean-frame buffers
+  {
+    std::vector<double> mw;
+    std::vector<size_t> rt_off(ms.size()), ri_off(ms.size()), lc_off(ms.size()), avg_off(ms.size());
+    size_t navg = 0;
+    for (size_t mi = 0; mi < ms.size(); ++mi) {
+      rt_off[mi] = mw.size();
+      mw.insert(mw.end(), ms[mi].rt_w.begin(), ms[mi].rt_w.end());
+      ri_off[mi] = mw.size();
+      mw.insert(mw.end(), ms[mi].ri_w.begin(), ms[mi].ri_w.end());
+      lc_off[mi] = mw.size();
+      mw.insert(mw.end(), ms[mi].lc.begin(), ms[mi].lc.end());
+      avg_off[mi] = navg;
+      navg += ms[mi].npix;
+    }
+    d_movw.upload(mw);
+    d_iavg.alloc(std::max<size_t>(navg, 1));
+    d_iavg.zero();
+    movies.resize(ms.size());
+    for (size_t mi = 0; mi < ms.size(); ++mi) {
+      MovieDev M{};
+      M.nz = ms[mi].nz;
+      M.npix = ms[mi].npix;
+      M.x_off = units[ms[mi].unit_begin].x_off;
+      M.rt_l = ms[mi].rt_l;
+      M.ri_l = ms[mi].ri_l;
+      M.rt_w = ms[mi].rt_w.empty() ? nullptr : d_movw.p + rt_off[mi];
+      M.ri_w = ms[mi].ri_w.empty() ? nullptr : d_movw.p + ri_off[mi];
+      M.lc = d_movw.p + lc_off[mi];
+      M.iavg = d_iavg.p + avg_off[mi];
+      any_ri = any_ri || M.ri_l > 0;
+      movies[mi] = M;
+    }
+    d_movies.upload(movies);
+  }
   d_units.upload(units);
   d_probs.upload(probs);
   d_tw.upload(tw);
@@ -702,7 +777,7 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   d_x.alloc(npix_total);
   d_x.zero();
   d_grad.alloc(npix_total);
-  d_terms.alloc(SMILI_NTERMS * ps.size());
+  d_terms.alloc(nterms_total);
   d_cpart.alloc(std::max<size_t>(h_chunks.size(), 1));
   d_rpart.alloc(std::max<size_t>(h_regs.size(), 1) * kRegTerms);
   d_mpart.alloc(std::max<size_t>(h_regs.size(), 1) * kMomTerms);
@@ -816,8 +891,12 @@ void Plan::launch_eval() {
              d_cp.p, d_ca.p, reinterpret_cast<const float2*>(d_V.p), d_Vres.p, d_cpart.p);
     launch_adjoint();
   }
+  if (any_ri)
+    for (size_t mi = 0; mi < movies.size(); ++mi)
+      LAUNCH(SMILI_S_FRAME_MEAN, k_frame_mean, (movies[mi].npix + kThreads - 1) / kThreads, kThreads, 0,
+             d_movies.p + mi, d_x.p);
   LAUNCH(SMILI_S_REG, k_reg, h_regs.size(), kThreads, 0, d_regs.p, d_units.p, d_pix.p, d_x.p, d_I2d.p, d_grad2d.p,
-         d_mom.p, d_grad.p, d_rpart.p);
+         d_mom.p, d_movies.p, d_grad.p, d_rpart.p);
   LAUNCH(SMILI_S_FINALIZE, k_finalize, probs.size(), kThreads, 0, d_probs.p, d_units.p, d_regs.p, d_cpart.p,
          d_rpart.p, d_mom.p, d_terms.p);
 }
@@ -850,7 +929,6 @@ using namespace smili;
 
 struct smili_problem {
   Plan plan;
-  smili_desc desc;   // geometry copy (pointers not retained)
 };
 
 template <typename F>
@@ -877,18 +955,114 @@ int smili_abi_version(void) { return SMILI_ABI_VERSION; }
 const char* smili_last_error(void) { return g_last_error.c_str(); }
 
 int smili_problem_create(const smili_desc* desc, const smili_tune* tune, smili_problem** out) {
+  return smili_batch_create(desc, 1, tune, out);
+}
+
+int smili_batch_create(const smili_desc* descs, int32_t nprob, const smili_tune* tune, smili_problem** out) {
   return guarded([&] {
     require(out != nullptr, "out is NULL");
     *out = nullptr;
-    check_desc_geometry(desc);
-    require(desc->nuv >= 1 && desc->u && desc->v, "nuv must be >= 1 with u and v given");
-    check_data(desc->data, desc->nuv);     // all validation precedes the first HIP call
+    require(descs != nullptr, "descriptor is NULL");
+    require(nprob >= 1, "nprob must be >= 1");
+    // all validation precedes the first HIP call
+    for (int i = 0; i < nprob; ++i) {
+      const smili_desc* d = descs + i;
+      try {
+        check_desc_geometry(d);
+        require(d->nuv >= 1 && d->u && d->v, "nuv must be >= 1 with u and v given");
+        check_data(d->data, d->nuv);
+      } catch (const Error& e) {
+        if (nprob == 1) throw;
+        throw Error(e.code, "problem " + std::to_string(i) + ": " + e.what());
+      }
+    }
     auto* p = new smili_problem();
     try {
-      p->desc = *desc;
-      std::vector<UnitSpec> us{UnitSpec{desc, 0, 1.0, desc->u, desc->v, desc->nuv}};
-      std::vector<ProbSpec> ps{ProbSpec{&desc->data, 0, 1, desc->nuv}};
+      std::vector<UnitSpec> us;
+      std::vector<ProbSpec> ps;
+      for (int i = 0; i < nprob; ++i) {
+        const smili_desc* d = descs + i;
+        us.push_back(UnitSpec{d, i, 1.0, d->u, d->v, d->nuv});
+        ps.push_back(ProbSpec{&d->data, i, i + 1, d->nuv});
+      }
       p->plan.build(us, ps, tune);
+    } catch (...) {
+      delete p;
+      throw;
+    }
+    *out = p;
+  });
+}
+
+int smili_movie_create(const smili_movie_desc* md, const smili_tune* tune, smili_problem** out) {
+  return guarded([&] {
+    require(out != nullptr, "out is NULL");
+    *out = nullptr;
+    require(md != nullptr, "descriptor is NULL");
+    const smili_desc* d = &md->frame;
+    check_desc_geometry(d);
+    const int nz = md->nz;
+    require(nz >= 1, "nz must be >= 1");
+    require(md->nuvs != nullptr, "nuvs missing");
+    long long sum = 0;
+    for (int z = 0; z < nz; ++z) {
+      require(md->nuvs[z] >= 0, "nuvs must be >= 0 (frame " + std::to_string(z) + ")");
+      sum += md->nuvs[z];
+    }
+    require(sum == d->nuv, "frame.nuv must equal the sum of nuvs");
+    require(d->nuv >= 1 && d->u && d->v, "nuv must be >= 1 with u and v given");
+    check_data(d->data, d->nuv);
+    require(!(md->rt_l > 0) || md->rt_wgt, "rt_wgt missing");
+    require(!(md->ri_l > 0) || md->ri_wgt, "ri_wgt missing");
+    std::vector<double> lc(nz, 1.0);
+    if (md->lc_tgtfd)
+      for (int z = 0; z < nz; ++z) {
+        require(std::isfinite(md->lc_tgtfd[z]) && md->lc_tgtfd[z] != 0.0,
+                "lc_tgtfd must be finite and nonzero (frame " + std::to_string(z) + ")");
+        lc[z] = md->lc_tgtfd[z];
+      }
+    // light-curve mode (image3d.f90:185): sum(lc_l) > 0
+    double lc_sum = 0.0;
+    if (md->lc_l)
+      for (int z = 0; z < nz; ++z) lc_sum += md->lc_l[z];
+    const bool lc_mode = md->lc_l && lc_sum > 0;
+    require(!lc_mode || md->lc_tgtfd, "lc_tgtfd missing in light-curve mode");
+    const bool lc_scale = lc_mode && (md->lc_nidx == 1 || md->lc_nidx == 3);
+    auto* p = new smili_problem();
+    try {
+      std::vector<UnitSpec> us;
+      long long off = 0;
+      for (int z = 0; z < nz; ++z) {
+        UnitSpec U{d, 0, 1.0 / nz, d->u + off, d->v + off, md->nuvs[z]};
+        U.movie = 0;
+        U.frame = z;
+        U.no_cen = true;
+        if (lc_mode) {
+          U.tfd_override = true;
+          U.tfd_l = md->lc_l[z];
+          U.tfd_tgtfd = lc[z];
+        }
+        if (lc_scale) {
+          const double s = d->reg.tfd_tgtfd / lc[z];
+          U.wscale[0] = U.wscale[1] = U.wscale[3] = U.wscale[4] = s;
+          U.wscale[2] = s * s;
+        }
+        us.push_back(U);
+        off += md->nuvs[z];
+      }
+      ProbSpec P{&d->data, 0, nz, d->nuv};
+      P.nterms = SMILI_NTERMS_MOVIE;
+      P.lc_mode = lc_mode ? 1 : 0;
+      MovieSpec M;
+      M.unit_begin = 0;
+      M.nz = nz;
+      M.npix = d->npix;
+      M.rt_l = md->rt_l;
+      M.ri_l = md->ri_l;
+      if (md->rt_l > 0) M.rt_w.assign(md->rt_wgt, md->rt_wgt + d->npix);
+      if (md->ri_l > 0) M.ri_w.assign(md->ri_wgt, md->ri_wgt + d->npix);
+      M.lc = lc;
+      p->plan.build(us, std::vector<ProbSpec>{P}, tune, std::vector<MovieSpec>{M});
     } catch (...) {
       delete p;
       throw;
@@ -899,28 +1073,50 @@ int smili_problem_create(const smili_desc* desc, const smili_tune* tune, smili_p
 
 void smili_problem_destroy(smili_problem* p) { delete p; }
 
+// copy x in, evaluate, copy grad and all terms out; returns whether every
+// cost and gradient entry is finite
+static bool eval_host(Plan& P, const double* x, double* grad, std::vector<double>& t) {
+  const size_t n = P.npix_total;
+  HIP_CHECK(hipMemcpyAsync(P.d_x.p, x, n * sizeof(double), hipMemcpyHostToDevice, P.stream));
+  P.eval();
+  t.resize(P.nterms_total);
+  HIP_CHECK(hipMemcpyAsync(grad, P.d_grad.p, n * sizeof(double), hipMemcpyDeviceToHost, P.stream));
+  HIP_CHECK(hipMemcpyAsync(t.data(), P.d_terms.p, t.size() * sizeof(double), hipMemcpyDeviceToHost, P.stream));
+  HIP_CHECK(hipStreamSynchronize(P.stream));
+  bool ok = true;
+  for (const ProbDev& Q : P.probs) ok = ok && std::isfinite(t[Q.terms_off + SMILI_T_COST]);
+  for (size_t i = 0; ok && i < n; ++i) ok = std::isfinite(grad[i]);
+  if (!ok) g_last_error = "cost or gradient is not finite";
+  return ok;
+}
+
 int smili_cost_grad(smili_problem* p, const double* x, double* cost, double* grad, double* terms) {
-  int nonfinite = 0;
+  bool ok = true;
   int rc = guarded([&] {
     require(p && x && grad, "NULL argument");
     Plan& P = p->plan;
-    const size_t n = P.npix_total;
-    HIP_CHECK(hipMemcpyAsync(P.d_x.p, x, n * sizeof(double), hipMemcpyHostToDevice, P.stream));
-    P.eval();
-    double t[SMILI_NTERMS];
-    HIP_CHECK(hipMemcpyAsync(grad, P.d_grad.p, n * sizeof(double), hipMemcpyDeviceToHost, P.stream));
-    HIP_CHECK(hipMemcpyAsync(t, P.d_terms.p, sizeof(t), hipMemcpyDeviceToHost, P.stream));
-    HIP_CHECK(hipStreamSynchronize(P.stream));
+    require(P.probs.size() == 1, "handle holds several problems: use smili_batch_cost_grad");
+    std::vector<double> t;
+    ok = eval_host(P, x, grad, t);
     if (cost) *cost = t[SMILI_T_COST];
-    if (terms) std::memcpy(terms, t, sizeof(t));
-    bool ok = std::isfinite(t[SMILI_T_COST]);
-    for (size_t i = 0; ok && i < n; ++i) ok = std::isfinite(grad[i]);
-    if (!ok) {
-      nonfinite = 1;
-      g_last_error = "cost or gradient is not finite";
-    }
+    if (terms) std::memcpy(terms, t.data(), t.size() * sizeof(double));
   });
-  if (rc == SMILI_OK && nonfinite) return SMILI_ENONFINITE;
+  if (rc == SMILI_OK && !ok) return SMILI_ENONFINITE;
+  return rc;
+}
+
+int smili_batch_cost_grad(smili_problem* p, const double* x, double* costs, double* grad, double* terms) {
+  bool ok = true;
+  int rc = guarded([&] {
+    require(p && x && grad, "NULL argument");
+    Plan& P = p->plan;
+    std::vector<double> t;
+    ok = eval_host(P, x, grad, t);
+    if (costs)
+      for (size_t i = 0; i < P.probs.size(); ++i) costs[i] = t[P.probs[i].terms_off + SMILI_T_COST];
+    if (terms) std::memcpy(terms, t.data(), t.size() * sizeof(double));
+  });
+  if (rc == SMILI_OK && !ok) return SMILI_ENONFINITE;
   return rc;
 }
 
@@ -935,8 +1131,18 @@ int smili_cost_grad_device(smili_problem* p, const double* x_dev, double* grad_d
     if (grad_dev != P.d_grad.p)
       HIP_CHECK(hipMemcpyAsync(grad_dev, P.d_grad.p, n * sizeof(double), hipMemcpyDeviceToDevice, P.stream));
     if (terms_dev && terms_dev != P.d_terms.p)
-      HIP_CHECK(hipMemcpyAsync(terms_dev, P.d_terms.p, SMILI_NTERMS * sizeof(double),
+      HIP_CHECK(hipMemcpyAsync(terms_dev, P.d_terms.p, P.nterms_total * sizeof(double),
                                hipMemcpyDeviceToDevice, P.stream));
+  });
+}
+
+int smili_problem_sizes(smili_problem* p, int32_t* nprob, int64_t* nx_total, int32_t* nterms) {
+  return guarded([&] {
+    require(p != nullptr, "NULL problem");
+    const Plan& P = p->plan;
+    if (nprob) *nprob = (int32_t)P.probs.size();
+    if (nx_total) *nx_total = (int64_t)P.npix_total;
+    if (nterms) *nterms = P.probs.empty() ? SMILI_NTERMS : P.probs[0].nterms;
   });
 }
 
@@ -981,11 +1187,12 @@ int smili_model_vis(smili_problem* p, const double* x, double* vre, double* vim)
       for (size_t j = 0; j < P.npt; ++j) V[j] = make_double2(Vf[j].x, Vf[j].y);
     }
     HIP_CHECK(hipStreamSynchronize(P.stream));
-    const auto& np = P.newpos[0];
-    for (size_t j = 0; j < np.size(); ++j) {
-      vre[j] = V[np[j]].x;
-      vim[j] = V[np[j]].y;
-    }
+    size_t o = 0;
+    for (const auto& np : P.newpos)
+      for (size_t j = 0; j < np.size(); ++j, ++o) {
+        vre[o] = V[np[j]].x;
+        vim[o] = V[np[j]].y;
+      }
   });
 }
 
@@ -993,17 +1200,20 @@ int smili_adjoint_resid(smili_problem* p, const double* vres_re, const double* v
   return guarded([&] {
     require(p && vres_re && vres_im && out, "NULL argument");
     Plan& P = p->plan;
-    const auto& np = P.newpos[0];
-    std::vector<double2> R(P.npt);
-    for (size_t j = 0; j < np.size(); ++j) R[np[j]] = make_double2(vres_re[j], vres_im[j]);
+    std::vector<double2> R(std::max<size_t>(P.npt, 1));
+    size_t o = 0;
+    for (const auto& np : P.newpos)
+      for (size_t j = 0; j < np.size(); ++j, ++o) R[np[j]] = make_double2(vres_re[o], vres_im[o]);
     HIP_CHECK(hipMemcpyAsync(P.d_Vres.p, R.data(), P.npt * sizeof(double2), hipMemcpyHostToDevice, P.stream));
     P.launch_adjoint();
-    const UnitDev& U = P.units[0];
-    std::vector<double> g2((size_t)U.nx * U.ny);
-    HIP_CHECK(hipMemcpyAsync(g2.data(), P.d_grad2d.p + U.img_off, g2.size() * sizeof(double),
-                             hipMemcpyDeviceToHost, P.stream));
+    std::vector<double> g2(P.d_grad2d.n);
+    HIP_CHECK(hipMemcpyAsync(g2.data(), P.d_grad2d.p, g2.size() * sizeof(double), hipMemcpyDeviceToHost,
+                             P.stream));
     HIP_CHECK(hipStreamSynchronize(P.stream));
-    for (int i = 0; i < U.npix; ++i) out[i] = g2[P.winpix[0][i]];
+    for (size_t u = 0; u < P.units.size(); ++u) {
+      const UnitDev& U = P.units[u];
+      for (int i = 0; i < U.npix; ++i) out[U.x_off + i] = U.m > 0 ? g2[U.img_off + P.winpix[u][i]] : 0.0;
+    }
   });
 }
 

@@ -12,7 +12,7 @@ namespace smili {
 constexpr int kThreads = 256;        // work-group size of every kernel
 constexpr int kW = 7;                // ES kernel width (eps 1e-6)
 constexpr int kMaxRadixStages = 12;
-constexpr int kRegTerms = 5;         // l1, tv, tsv, kl, gs partial sums per item
+constexpr int kRegTerms = 7;         // l1, tv, tsv, kl, gs, rt, ri partial sums per item
 constexpr int kMomTerms = 10;        // moments for tfd / cen / sm
 constexpr double kZeroEps = 1e-10;   // image.f90:7
 
@@ -35,6 +35,7 @@ struct UnitDev {
   int cx_off, cy_off;                // deconvolution factors (float)
   int pix_off;                       // window pixel list offset (pix = iy*nx+ix)
   int prob;                          // problem (cost) this unit belongs to
+  int movie, frame;                  // movie index (-1: none) and frame number
   double nxref, nyref;
   double reg_scale;                  // multiplies this unit's reg cost and gradient
   // regularizers (lambda <= 0 disables)
@@ -78,6 +79,18 @@ constexpr int kSpansPerGroup = 256;
 struct CellBlock { int gx0, gy0, cell_begin, cell_count, nf1, nf2; long long grid_off; };
 struct CellSpan { int lcell, pt_begin, pt_end, rank; };  // lcell = ly * 8 + lx; rank in its cell
 
+// temporal regularizers of one movie (frames = consecutive units with
+// consecutive x ranges of npix pixels each)
+struct MovieDev {
+  int nz, npix;
+  int x_off;                         // offset of frame 0 in x / grad
+  int pad;
+  double rt_l, ri_l;
+  const double *rt_w, *ri_w;         // per window pixel
+  const double *lc;                  // light-curve normalisation per frame
+  double *iavg;                      // mean normalised frame (npix)
+};
+
 struct ProbDev {
   int chunk_begin[4], chunk_end[4];  // chi-square chunk ranges per data type
   int reg_begin, reg_end;            // reg item range
@@ -85,6 +98,8 @@ struct ProbDev {
   int n[4];                          // data counts
   double w[4];                       // normalised weights
   int terms_off;
+  int nterms;                        // 15, or 18 for a movie
+  int lc_mode;                       // movie in light-curve mode: tfd terms are the light curve
 };
 
 struct UnitMoments {

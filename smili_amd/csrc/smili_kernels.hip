@@ -293,14 +293,29 @@ __device__ __forceinline__ double tsv_grade(const double* I, int nx, int ny, int
   return g;
 }
 
+// Mean normalised frame of a movie, <I>_k = sum_z (x_zk / lc_z) / nz in
+// frame order (image3d.f90:344-348); one thread per window pixel,
+// blockIdx.y = movie.
+__global__ void __launch_bounds__(kThreads)
+k_frame_mean(const MovieDev* __restrict__ movies, const double* __restrict__ x) {
+  const MovieDev Mv = movies[blockIdx.y];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= Mv.npix || !(Mv.ri_l > 0)) return;
+  double a = 0.0;
+  for (int z = 0; z < Mv.nz; ++z) a += x[Mv.x_off + (long long)z * Mv.npix + k] / Mv.lc[z] / (double)Mv.nz;
+  Mv.iavg[k] = a;
+}
+
 // Per-pixel regularizers + final gradient assembly:
-// grad = gradchisq + reg_scale * gradreg (fftim2d.f90:553-554).
+// grad = gradchisq + reg_scale * gradreg (fftim2d.f90:553-554); for a movie
+// frame also the Rt / Ri terms of calc_cost_dynamical (image3d.f90:358-388):
+// grad = gradchisq + (gradreg / nz + graddyn).
 __global__ void __launch_bounds__(kThreads)
 k_reg(const RegItem* __restrict__ items, const UnitDev* __restrict__ units,
       const int* __restrict__ pix, const double* __restrict__ x,
       const double* __restrict__ I2dbuf, const double* __restrict__ grad2d,
-      const UnitMoments* __restrict__ mom, double* __restrict__ grad,
-      double* __restrict__ rpart) {
+      const UnitMoments* __restrict__ mom, const MovieDev* __restrict__ movies,
+      double* __restrict__ grad, double* __restrict__ rpart) {
   __shared__ double red[kRegTerms * (kThreads / 64)];
   const RegItem it = items[blockIdx.x];
   const UnitDev U = units[it.unit];
@@ -323,6 +338,17 @@ k_reg(const RegItem* __restrict__ items, const UnitDev* __restrict__ units,
     if (U.sm_l > 0) S = sm_state(U, M);
   }
   const bool alpha1 = fabs(U.cen_alpha - 1.0) < kZeroEps;
+  MovieDev Mv{};
+  bool rt_on = false, ri_on = false;
+  double lz = 1.0, lu = 1.0, ll = 1.0;
+  if (U.movie >= 0) {
+    Mv = movies[U.movie];
+    lz = Mv.lc[U.frame];
+    rt_on = Mv.rt_l > 0 && U.frame < Mv.nz - 1;
+    ri_on = Mv.ri_l > 0;
+    if (rt_on) lu = Mv.lc[U.frame + 1];
+    if (rt_on && U.frame > 0) ll = Mv.lc[U.frame - 1];
+  }
   for (int i = threadIdx.x; i < it.count; i += blockDim.x) {
     const int k = it.start + i;
     const int p = U.identity ? k : pix[U.pix_off + k];
@@ -379,8 +405,31 @@ k_reg(const RegItem* __restrict__ items, const UnitDev* __restrict__ units,
       g += U.sm_l * (2.0 * (S.Sg[0] - S.Sp[0]) * g1 + 2.0 * (S.Sg[1] - S.Sp[1]) * g2 +
                      4.0 * (S.Sg[2] - S.Sp[2]) * g3);
     }
+    double gt = 0.0;
+    if (rt_on || ri_on) {
+      const double II = I / lz;
+      if (rt_on) {
+        const double w = Mv.rt_l * Mv.rt_w[k];
+        const double Iu = x[U.x_off + Mv.npix + k] / lu;
+        v[5] += w * ((Iu - II) * (Iu - II));
+        double gr;
+        if (U.frame == 0) {
+          gr = 2.0 * (II - Iu);
+        } else {
+          const double Il = x[U.x_off - Mv.npix + k] / ll;
+          gr = 2.0 * (2.0 * II - Il - Iu);
+        }
+        gt += w * gr / lz;
+      }
+      if (ri_on) {
+        const double w = Mv.ri_l * Mv.ri_w[k];
+        const double d = II - Mv.iavg[k];
+        v[6] += w * (d * d);
+        gt += w * (2.0 * d) / lz;
+      }
+    }
     const double gc = (U.m > 0) ? grad2d[U.img_off + p] : 0.0;
-    grad[U.x_off + k] = gc + U.reg_scale * g;
+    grad[U.x_off + k] = gc + (U.reg_scale * g + gt);
   }
   block_sum<kRegTerms>(v, red);
   if (threadIdx.x == 0)
@@ -402,7 +451,9 @@ __device__ void finalize_problem(const ProbDev& P, const UnitDev* __restrict__ u
     for (int i = P.chunk_begin[t] + threadIdx.x; i < P.chunk_end[t]; i += blockDim.x) v[t] += cpart[i];
   for (int i = P.reg_begin + threadIdx.x; i < P.reg_end; i += blockDim.x) {
     const double sc = units[reg_items[i].unit].reg_scale;
-    for (int k = 0; k < kRegTerms; ++k) v[4 + k] += sc * rpart[(long long)i * kRegTerms + k];
+    for (int k = 0; k < 5; ++k) v[4 + k] += sc * rpart[(long long)i * kRegTerms + k];
+    v[12] += rpart[(long long)i * kRegTerms + 5];
+    v[13] += rpart[(long long)i * kRegTerms + 6];
   }
   // tfd / cen / sm costs per unit
   for (int u = P.unit_begin + threadIdx.x; u < P.unit_end; u += blockDim.x) {
@@ -422,7 +473,7 @@ __device__ void finalize_problem(const ProbDev& P, const UnitDev* __restrict__ u
       v[11] += U.reg_scale * U.sm_l * c;
     }
   }
-  block_sum<12>(v, red);
+  block_sum<14>(v, red);
   if (threadIdx.x == 0) {
     double* T = terms + P.terms_off;
     const double cf = v[0], ca_ = v[1], cc = v[2], cq = v[3];
@@ -438,13 +489,18 @@ __device__ void finalize_problem(const ProbDev& P, const UnitDev* __restrict__ u
     T[3] = P.n[DT_CP] > 0 ? cc / P.w[DT_CP] / P.n[DT_CP] : 0.0;
     T[4] = P.n[DT_CA] > 0 ? cq / P.w[DT_CA] / P.n[DT_CA] : 0.0;
     const double l1 = v[4], tv = v[5], tsv = v[6], kl = v[7], gs = v[8];
-    const double tfd = v[9], cen = v[10], smc = v[11];
-    // image.f90:199 order
-    const double reg = l1 + tv + tsv + kl + gs + tfd + cen + smc;
+    const double tfd = v[9], cen = v[10], smc = v[11], rt = v[12], ri = v[13];
+    // image.f90:199 order, then calc_cost_dynamical (image3d.f90:390)
+    const double reg = (l1 + tv + tsv + kl + gs + tfd + cen + smc) + (ri + rt);
     T[5] = reg;
     T[6] = l1; T[7] = tv; T[8] = tsv; T[9] = kl; T[10] = gs;
-    T[11] = tfd; T[12] = cen; T[13] = smc;
+    T[11] = P.lc_mode ? 0.0 : tfd; T[12] = cen; T[13] = smc;
     T[14] = chisq + reg;
+    if (P.nterms > 15) {                  // movie: Rt, Ri, light curve
+      T[15] = rt;
+      T[16] = ri;
+      T[17] = P.lc_mode ? tfd : 0.0;
+    }
   }
 }
 

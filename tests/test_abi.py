@@ -5,6 +5,7 @@ Python -1 returns, lbfgs.py:186-201)."""
 import ctypes as C
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -34,6 +35,28 @@ def test_struct_layouts_match_header_sizes():
     # pointer-size fields in declared order: 10 doubles + 5 pointers (reg)
     assert C.sizeof(_lib.SmiliReg) == 18 * 8
     assert C.sizeof(_lib.SmiliTune) == 6 * 4
+
+
+def test_struct_layouts_match_the_c_compiler(tmp_path):
+    """sizeof / offsetof of every ABI struct as gcc lays out the header."""
+    structs = {"smili_reg": _lib.SmiliReg, "smili_data": _lib.SmiliData, "smili_desc": _lib.SmiliDesc,
+               "smili_movie_desc": _lib.SmiliMovieDesc, "smili_tune": _lib.SmiliTune}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "smili_hip.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("  return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    for line in filter(None, out):
+        cname, what, val = line.split()
+        py = structs[cname]
+        got = C.sizeof(py) if what == "size" else getattr(py, what).offset
+        assert got == int(val), (cname, what)
 
 
 def _desc(nx=4, ny=4, nuv=3, bad=None):
@@ -96,3 +119,47 @@ def test_null_arguments():
 def test_host_wrapper_fails_loudly_without_library(tmp_path):
     with pytest.raises(OSError):
         _lib.load(str(tmp_path / "missing.so"))
+
+
+def _movie(bad=None):
+    d, keep = _desc(nuv=3)
+    md = _lib.SmiliMovieDesc()
+    md.frame = d
+    md.nz = 2
+    nuvs = np.array([2, 1], np.int32)
+    if bad == "nuvs":
+        nuvs[1] = 2
+    keep.append(nuvs)
+    md.nuvs = nuvs.ctypes.data_as(C.POINTER(C.c_int32))
+    if bad == "nz":
+        md.nz = 0
+    if bad == "rt":
+        md.rt_l = 1.0
+    if bad == "lc":
+        lc = np.array([1.0, 0.0])
+        keep.append(lc)
+        md.lc_tgtfd = lc.ctypes.data_as(C.POINTER(C.c_double))
+    return md, keep
+
+
+@pytest.mark.parametrize("bad,msg", [("nuvs", "sum of nuvs"), ("nz", "nz must be"), ("rt", "rt_wgt missing"),
+                                     ("lc", "lc_tgtfd must be finite and nonzero")])
+def test_invalid_movie_rejected(bad, msg):
+    lib = _lib.load()
+    md, keep = _movie(bad)
+    h = C.c_void_p()
+    assert lib.smili_movie_create(C.byref(md), None, C.byref(h)) == -1
+    assert msg in lib.smili_last_error().decode()
+    assert not h.value
+
+
+def test_invalid_batch_member_named():
+    lib = _lib.load()
+    good, k1 = _desc()
+    bad, k2 = _desc(bad="var")
+    arr = (_lib.SmiliDesc * 2)(good, bad)
+    h = C.c_void_p()
+    assert lib.smili_batch_create(arr, 2, None, C.byref(h)) == -1
+    assert "problem 1: " in lib.smili_last_error().decode()
+    assert lib.smili_batch_create(arr, 0, None, C.byref(h)) == -1
+    assert not h.value
