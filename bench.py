@@ -1,10 +1,16 @@
 """Objective+gradient throughput of the GPU RML imaging cost.
 
-Workload (one GPU): 256x256 image, fine grid 512x512, ES width 7; amplitudes
-on 1e5 uv points in a 10 Glambda disk, closure phases (220 triangles) and log
-closure amplitudes (495 quadrangles x 2) of 12 stations x 500 timestamps;
-L1 + TV + TSV + KL.  One step = one objective+gradient evaluation
-(calc_cost) with x already in HBM.
+Default workload (one GPU): 256x256 image, fine grid 512x512, ES width 7;
+amplitudes on 1e5 uv points in a 10 Glambda disk, closure phases (220
+triangles) and log closure amplitudes (495 quadrangles x 2) of 12 stations x
+500 timestamps; L1 + TV + TSV + KL.  One step = one objective+gradient
+evaluation (calc_cost) with x already in HBM.
+
+``--workload movie``: 64 frames of 128x128, 3e5 visibilities, per-frame
+chi-square + L1 + TSV + Rt as one joint cost (fftim3d.calc_cost); one step =
+one evaluation of the whole movie.  ``--workload survey``: 512 independent
+problems of 64^2..256^2 evaluated together; one step = one evaluation of all
+512 (the value counts problem evaluations per second).
 
 Prints one JSON line.  Besides the throughput it reports:
   roofline      -- the dominant kernel's algorithmic bytes per launch over its
@@ -28,29 +34,98 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 
 
-def stage_bytes(kw, nf1, nf2):
-    """Algorithmic HBM bytes per launch of each kernel stage."""
+def fine_size(n):
+    """Fine-grid length of an n-pixel axis: the smallest 2,3,5-smooth
+    integer >= max(2 n, 14) (upsampling 2, ES width 7)."""
+    m = max(2 * n, 14)
+    while True:
+        k = m
+        for p in (2, 3, 5):
+            while k % p == 0:
+                k //= p
+        if k == 1:
+            return m
+        m += 1
+
+
+def stage_bytes(kws, fp64=True):
+    """Algorithmic HBM bytes per launch of each kernel stage, summed over
+    the problems (2-D cost keyword dicts; a movie's dict carries one entry
+    per frame in ``units``).  fp64: complex grid / visibilities of 16 bytes
+    and FP64 point offsets, else 8 bytes and FP32 offsets."""
     import numpy as np
-    Nx, Ny = kw["Nx"], kw["Ny"]
-    N = np.asarray(kw["xidx"]).size
-    M = np.asarray(kw["u"]).size
-    G = nf1 * nf2
-    nfcv = 0 if kw.get("uvidxfcv") is None else len(kw["uvidxfcv"])
-    namp = 0 if kw.get("uvidxamp") is None else len(kw["uvidxamp"])
-    ncp = 0 if kw.get("uvidxcp") is None else kw["uvidxcp"].shape[1]
-    nca = 0 if kw.get("uvidxca") is None else kw["uvidxca"].shape[1]
-    nreg = sum(1 for r in ("l1", "tv", "tsv", "kl", "gs") if kw.get(r + "_l", -1) > 0)
-    half = 8 * nf1 * Ny
-    return {
-        "t2_rows": 8 * Nx * Ny + half,                       # x in (f64), half grid out
-        "t2_cols": half + 8 * G,                             # half grid in, grid out
-        "interp": 8 * G + 12 * M + 8 * M + 16 * M,           # grid, point coords, V out, Vres cleared
-        "chisq": 32 * nfcv + 24 * namp + 32 * ncp + 32 * nca + 8 * M + 32 * M,  # data, V, Vres r+w
-        "spread": 12 * M + 16 * M + 8 * G,                   # coords, Vres, grid out
-        "t1_cols": 8 * G + 8 * G + half,                     # grid in, grid cleared, half out
-        "t1_rows": half + 8 * Nx * Ny,                       # half in, grad2d out (f64)
-        "reg": 8 * N * (3 + nreg) + 8 * N,                   # x, grad2d, weights in, grad out
-    }
+    cb = 16 if fp64 else 8                 # complex grid element / model visibility
+    pb = 4 + (16 if fp64 else 8)           # packed cell + two offsets per point
+    tot = {}
+    for kw in kws:
+        for u in kw.get("units", [kw]):
+            Nx, Ny = u["Nx"], u["Ny"]
+            nf1, nf2 = fine_size(Nx), fine_size(Ny)
+            N = np.asarray(kw["xidx"]).size
+            M = u["M"] if "M" in u else np.asarray(kw["u"]).size
+            G = nf1 * nf2
+            half = cb * nf1 * Ny
+            nreg = sum(1 for r in ("l1", "tv", "tsv", "kl", "gs") if kw.get(r + "_l", -1) > 0)
+            nreg += sum(1 for r in ("rt", "ri") if kw.get(r + "_l", -1) > 0)
+            live = M > 0
+            part = {
+                "t2_rows": (8 * Nx * Ny + half) * live,          # x in (f64), half grid out
+                "t2_cols": (half + cb * G) * live,               # half grid in, grid out
+                "interp": cb * G * live + pb * M + cb * M + 16 * M,  # grid, points, V out, Vres cleared
+                "spread": pb * M + 16 * M + cb * G * live,       # points, Vres, grid out
+                "t1_cols": (2 * cb * G + half) * live,           # grid in, grid cleared, half out
+                "t1_rows": (half + 8 * Nx * Ny) * live,          # half in, grad2d out (f64)
+                "reg": 8 * N * (3 + nreg) + 8 * N,               # x, grad2d, weights in, grad out
+            }
+            for k, v in part.items():
+                tot[k] = tot.get(k, 0) + int(v)
+        M = np.asarray(kw["u"]).size
+        nfcv = 0 if kw.get("uvidxfcv") is None else len(kw["uvidxfcv"])
+        namp = 0 if kw.get("uvidxamp") is None else len(kw["uvidxamp"])
+        ncp = 0 if kw.get("uvidxcp") is None else kw["uvidxcp"].shape[1]
+        nca = 0 if kw.get("uvidxca") is None else kw["uvidxca"].shape[1]
+        tot["chisq"] = tot.get("chisq", 0) + int(32 * nfcv + 24 * namp + 32 * ncp + 32 * nca +
+                                                 cb * M + 32 * M)  # data, V, Vres r+w
+    return tot
+
+
+def _movie_units(kw):
+    return [dict(Nx=kw["Nx"], Ny=kw["Ny"], M=int(m)) for m in kw["Nuvs"]]
+
+
+def _timed_loop(g, steps, warmup):
+    """Average wall time of `steps` evaluations on resident buffers."""
+    import torch
+    for _ in range(warmup):
+        g.eval_async()
+    g.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.eval_async()
+    g.synchronize()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def _cpu_time_per_eval(kw, budget_s, max_evals=2000):
+    """Seconds per evaluation of the oracle's C + OpenMP FP64 port of the
+    2-D cost on one problem, and the thread count."""
+    from oracle.cpu import CpuCost
+    import numpy as np
+    cc = CpuCost(**{k: v for k, v in kw.items() if k not in ("x0",)})
+    x = kw["x0"]
+    cc.cost_grad(x)                       # warm-up (page-in, thread pool)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        cc.cost_grad(x)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or n >= max_evals:
+            break
+    thr = cc.threads
+    cc.close()
+    return el / n, n, el, thr
 
 
 def cpu_baseline(prep, budget_s=15.0):
@@ -58,23 +133,86 @@ def cpu_baseline(prep, budget_s=15.0):
     and two type-1 ES-kernel NUFFTs as the reference's calc_chisq does, the
     four chi-square loops, the regularizers) timed on a bounded number of
     evaluations of the same workload on this host's cores."""
-    from oracle.cpu import CpuCost
-    cc = CpuCost(**prep.kwargs)
-    x = prep.x0
-    cc.cost_grad(x)                       # warm-up (page-in, thread pool)
-    n, t0 = 0, time.perf_counter()
-    while True:
-        cc.cost_grad(x)
-        n += 1
-        el = time.perf_counter() - t0
-        if el > budget_s or n >= 2000:
-            break
-    thr = cc.threads
-    cc.close()
-    return {"value": round(n / el, 3), "unit": "evals/s", "cores": thr, "kind": "port",
+    per, n, el, thr = _cpu_time_per_eval(dict(prep.kwargs, x0=prep.x0), budget_s)
+    return {"value": round(1.0 / per, 3), "unit": "evals/s", "cores": thr, "kind": "port",
             "sample": f"{n} full evaluations of the same workload, {el:.1f} s, C + OpenMP FP64 "
                       f"restatement (ES-kernel NUFFT, 2 type-1 transforms as in the reference) "
                       f"on {thr} threads"}
+
+
+def _frame_kwargs(kw, x0, z):
+    """Frame z of a movie as a 2-D cost (its uv slice and the full complex
+    visibilities on it, re-indexed; static regularizers at 1/Nz)."""
+    import numpy as np
+    nuvs = np.asarray(kw["Nuvs"])
+    s = int(nuvs[:z].sum())
+    e = s + int(nuvs[z])
+    npix = np.asarray(kw["xidx"]).size
+    idx = np.asarray(kw["uvidxfcv"])
+    sel = (np.abs(idx) > s) & (np.abs(idx) <= e)
+    f = {k: kw[k] for k in ("Nx", "Ny", "xidx", "yidx", "Nxref", "Nyref")}
+    f.update(u=kw["u"][s:e], v=kw["v"][s:e], uvidxfcv=np.sign(idx[sel]) * (np.abs(idx[sel]) - s),
+             Vfcv=kw["Vfcv"][sel], Varfcv=kw["Varfcv"][sel], wfcv=kw["wfcv"])
+    for r in ("l1", "tsv", "tv", "kl", "gs"):
+        if kw.get(r + "_l", -1) > 0:
+            f[r + "_l"] = kw[r + "_l"] / len(nuvs)
+            f[r + "_wgt"] = kw[r + "_wgt"]
+    f["x0"] = x0[z * npix:(z + 1) * npix]
+    return f
+
+
+def cpu_baseline_movie(prep, budget_s=15.0, n_sample=8):
+    """CPU port on a sample of frames: frame z's chi-square and static
+    regularizers are the 2-D cost of its own data (the movie's data here are
+    per-frame visibilities; Rt adds one pass over the frames), so one movie
+    evaluation costs the sum over frames; n_sample evenly spaced frames are
+    timed and the mean scaled to all frames."""
+    import numpy as np
+    kw = prep.kwargs
+    Nz = len(kw["Nuvs"])
+    zs = np.linspace(0, Nz - 1, n_sample).round().astype(int)
+    per = []
+    thr = 1
+    for z in zs:
+        t, _, _, thr = _cpu_time_per_eval(_frame_kwargs(kw, prep.x0, z), budget_s / n_sample)
+        per.append(t)
+    tot = float(np.mean(per)) * Nz
+    return {"value": round(1.0 / tot, 4), "unit": "evals/s", "cores": thr, "kind": "port",
+            "sample": f"C + OpenMP FP64 2-D cost of {n_sample} of the {Nz} frames (each frame's "
+                      f"visibilities, L1 + TSV), {budget_s:.0f} s in all, mean per frame "
+                      f"{np.mean(per) * 1e3:.2f} ms scaled by {Nz} frames; Rt (one pass) not timed"}
+
+
+def cpu_baseline_survey(preps, budget_s=15.0, n_sample=16, seed=0):
+    """CPU port on a random sample of the survey's problems, scaled to all."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    pick = rng.choice(len(preps), n_sample, replace=False)
+    per = []
+    thr = 1
+    for i in pick:
+        t, _, _, thr = _cpu_time_per_eval(dict(preps[i].kwargs, x0=preps[i].x0), budget_s / n_sample)
+        per.append(t)
+    tot = float(np.mean(per)) * len(preps)
+    return {"value": round(len(preps) / tot, 3), "unit": "problem-evals/s", "cores": thr, "kind": "port",
+            "sample": f"C + OpenMP FP64 cost of {n_sample} random problems of the {len(preps)}, "
+                      f"{budget_s:.0f} s in all, mean {np.mean(per) * 1e3:.2f} ms per problem "
+                      f"scaled to the whole survey"}
+
+
+def _roofline(g, kws, fp64, ms, steps):
+    prof = g.profile(reps=max(10, min(steps, 100)))
+    sb = stage_bytes(kws, fp64)
+    dom = max((k for k in prof if k in sb), key=lambda k: prof[k])
+    achieved = sb[dom] / (prof[dom] * 1e-3) / 1e9
+    eval_bytes = int(sum(sb.values()))
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "kernel": "k_" + dom, "kernel_ms": round(prof[dom], 5),
+            "kernel_bytes": sb[dom],
+            "eval_bytes": eval_bytes,
+            "eval_achieved_GBs": round(eval_bytes / (ms * 1e-3) / 1e9, 2),
+            "stage_ms": {k: round(v, 5) for k, v in prof.items() if v > 0}}
 
 
 def main():
@@ -88,6 +226,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp64",
                     help="NUFFT arithmetic (chi-square and regularizers are FP64 in both)")
+    ap.add_argument("--workload", choices=("single_large", "movie", "survey"), default="single_large")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the objective+gradient path has no multi-GPU "
@@ -100,62 +239,81 @@ def main():
 
     torch.cuda.set_device(0)
     from smili_amd import workloads
-    from smili_amd.cost import CostProblem
+    from smili_amd.cost import BatchProblem, CostProblem, MovieProblem
 
+    fp64 = args.precision == "fp64"
+    tune = dict(use_graph=-1 if args.no_graph else 0, precision=64 if fp64 else 32)
     t_setup = time.perf_counter()
-    prep, _ = workloads.single_large(seed=args.seed)
-    kw = prep.kwargs
-    tune = dict(use_graph=-1 if args.no_graph else 0, precision=64 if args.precision == "fp64" else 32)
-    g = CostProblem(**kw, tune=tune)
+    if args.workload == "single_large":
+        prep, _ = workloads.single_large(seed=args.seed)
+        kw = prep.kwargs
+        g = CostProblem(**kw, tune=tune)
+        x0 = prep.x0
+        kws = [kw]
+        per_step = 1
+        metric, unit = "obj+grad evals/s (256^2, 1e5 vis)", "evals/s"
+        desc = ("single_large: 256x256 (fine grid 512x512, ES w=7, eps 1e-6), 1e5 amplitudes in a "
+                "10 Glambda disk + 110,000 closure phases + 495,000 log closure amplitudes "
+                "(12 stations x 500 timestamps), L1+TV+TSV+KL")
+        extra = {"npix": int(np.asarray(kw["xidx"]).size)}
+    elif args.workload == "movie":
+        prep, meta = workloads.dynamic_movie(seed=5 + args.seed)
+        kw = dict(prep.kwargs)
+        g = MovieProblem(**kw, tune=tune)
+        x0 = prep.x0
+        kws = [dict(kw, units=_movie_units(kw))]
+        per_step = 1
+        metric, unit = "movie obj+grad evals/s (64 x 128^2, 3e5 vis)", "evals/s"
+        desc = ("dynamic_movie: 64 frames of 128x128 (fine grids 256x256), 3e5 complex visibilities "
+                "(2e3..8e3 per frame, 10 stations, 12 h), rotating-asymmetry ring, per-frame chi-square "
+                "+ L1 + TSV + Rt as one joint cost")
+        extra = {"frames": int(prep.Nt), "npix_total": int(x0.size),
+                 "vis_per_frame": [int(meta["counts"].min()), int(meta["counts"].max())]}
+    else:
+        preps, meta = workloads.survey(seed=3 + args.seed)
+        kws = [p.kwargs for p in preps]
+        g = BatchProblem(kws, tune=tune)
+        x0 = np.concatenate([p.x0 for p in preps])
+        per_step = len(preps)
+        metric, unit = "survey problem-evals/s (512 problems, 64^2..256^2)", "problem-evals/s"
+        desc = ("survey: 512 independent problems, sizes uniform over {64,96,128,192,256}^2, "
+                "2e3..5e4 complex visibilities (log-uniform) from 6..12 stations, L1 x TSV log grid "
+                "8 x 8, TV and MEM on/off; one step evaluates all 512")
+        extra = {"problems": len(preps), "nvis_total": int(sum(m["nvis"] for m in meta)),
+                 "npix_total": int(x0.size)}
     info = g.info()
     t_setup = time.perf_counter() - t_setup
-    # x goes to the problem's own device buffer once; every timed step is one
+    # x goes to the handle's own device buffer once; every timed step is one
     # evaluation on resident data (the captured graph of all kernels)
-    cost_host, _, _ = g.cost_grad(prep.x0)
-    torch.cuda.synchronize()
-    for _ in range(args.warmup):
-        g.eval_async()
-    g.synchronize()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        g.eval_async()
-    g.synchronize()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    first = g.cost_grad(x0)
+    dt = _timed_loop(g, args.steps, args.warmup)
     ms = dt / args.steps * 1e3
-    value = args.steps / dt
-    cost = g.cost_grad(prep.x0)[0]
-    assert cost == cost_host
+    value = per_step * args.steps / dt
+    again = g.cost_grad(x0)
+    assert np.array_equal(np.asarray(first[0]), np.asarray(again[0]))
     info = g.info()
     # host-pointer API (x host->device, grad device->host each call): PCIe-inclusive
-    nh = max(20, args.steps // 10)
+    nh = max(5, min(args.steps // 10, 50))
     th = time.perf_counter()
     for _ in range(nh):
-        g.cost_grad(prep.x0)
-    host_rate = nh / (time.perf_counter() - th)
-
-    prof = g.profile(reps=max(10, min(args.steps, 100)))
-    sb = stage_bytes(kw, info["nf1"], info["nf2"])
-    dom = max((k for k in prof if k in sb), key=lambda k: prof[k])
-    achieved = sb[dom] / (prof[dom] * 1e-3) / 1e9
-    eval_bytes = workloads.algorithmic_bytes(prep)
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "k_" + dom, "kernel_ms": round(prof[dom], 5),
-                "kernel_bytes": sb[dom],
-                "eval_bytes": eval_bytes,
-                "eval_achieved_GBs": round(eval_bytes / (ms * 1e-3) / 1e9, 2),
-                "stage_ms": {k: round(v, 5) for k, v in prof.items() if v > 0}}
+        g.cost_grad(x0)
+    host_rate = per_step * nh / (time.perf_counter() - th)
+    roofline = _roofline(g, kws, fp64, ms, args.steps)
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(prep, args.cpu_budget)
+        if args.workload == "single_large":
+            cpu = cpu_baseline(prep, args.cpu_budget)
+        elif args.workload == "movie":
+            cpu = cpu_baseline_movie(prep, args.cpu_budget)
+        else:
+            cpu = cpu_baseline_survey(preps, args.cpu_budget)
 
+    cost = first[0] if per_step == 1 else float(np.sum(first[0]))
     line = {
-        "metric": "obj+grad evals/s (256^2, 1e5 vis)",
+        "metric": metric,
         "value": round(value, 2),
-        "unit": "evals/s",
+        "unit": unit,
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -163,16 +321,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("f32" if args.precision == "fp32" else "f64") + " NUFFT + f64 chi-square/regularizers",
+        "dtype": ("f64" if fp64 else "f32") + " NUFFT + f64 chi-square/regularizers",
         "data": "synthetic",
-        "config": {"workload": "single_large: 256x256 (fine grid 512x512, ES w=7, eps 1e-6), "
-                               "1e5 amplitudes in a 10 Glambda disk + 110,000 closure phases + "
-                               "495,000 log closure amplitudes (12 stations x 500 timestamps), "
-                               "L1+TV+TSV+KL",
-                   "npix": int(np.asarray(kw["xidx"]).size), "nuv": info["nuv"],
-                   "graph": bool(info["graph"]), "setup_s": round(t_setup, 2),
-                   "precision": args.precision, "cost": cost,
-                   "host_api_evals_s": round(host_rate, 1)},
+        "config": dict({"workload": desc, "nuv": info["nuv"], "graph": bool(info["graph"]),
+                        "setup_s": round(t_setup, 2), "precision": args.precision, "cost": cost,
+                        "host_api_" + unit.replace("/", "_"): round(host_rate, 1)}, **extra),
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -61,3 +61,28 @@ def nrmse(image, ref, istokes=0, ifreq=0):
     if b.ndim == 4:
         b = b[istokes, ifreq]
     return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))
+
+
+class Movie:
+    """Frames of one geometry with the attributes imaging3d reads from
+    SMILI's MOVIE (smili/imdata/movie.py:46-90: ``Nt``, ``images[i]`` IMFITS-like
+    with ``header`` / ``data``)."""
+
+    def __init__(self, images):
+        self.images = list(images)
+
+    @classmethod
+    def from_frames(cls, frames, dx, dy=None, angunit="uas", nxref=None, nyref=None):
+        frames = np.asarray(frames, np.float64)
+        Nt, ny, nx = frames.shape
+        return cls([Image(nx, ny, dx, dy, angunit, nxref, nyref, frames[i]) for i in range(Nt)])
+
+    @property
+    def Nt(self):
+        return len(self.images)
+
+    def frames(self, istokes=0, ifreq=0):
+        return np.stack([np.asarray(im.data)[istokes, ifreq] for im in self.images])
+
+    def copy(self):
+        return copy.deepcopy(self)

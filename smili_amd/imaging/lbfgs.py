@@ -44,8 +44,9 @@ class SolveInfo:
     history: list = field(default_factory=list)
 
 
-def _summary(terms, inormfactr, x):
-    """The summary calc_cost prints (fftim2d.f90:557-606)."""
+def _summary(terms, inormfactr, x, nz=1):
+    """The summary calc_cost prints (fftim2d.f90:557-606; for movies
+    fftim3d.f90:649-713)."""
     lines = ["  Cost Function          : %13.6e" % terms["cost"],
              "  Data term              : %13.6e" % terms["chisq"]]
     for k, lab in (("chisq_fcv", "Complex Visibilities"), ("chisq_amp", "Amplitudes"),
@@ -58,6 +59,14 @@ def _summary(terms, inormfactr, x):
                    ("sm", "second momentum")):
         if terms[k] != 0:
             lines.append("    %-20s : %13.6e" % (lab, terms[k]))
+    if "lc" in terms and terms["lc"] != 0:
+        lines.append("    %-20s : %13.6e" % ("Light curve", terms["lc"]))
+    if "rt" in terms:
+        lines.append("  Total flux (avg)       : %13.6e" % (np.sum(x) / nz / inormfactr))
+        for k, lab in (("rt", "Dynamical  Rt Reg."), ("ri", "Dynamical  Ri Reg.")):
+            if terms[k] != 0:
+                lines.append("  %-22s : %13.6e" % (lab, terms[k]))
+        return "\n".join(lines)
     lines.append("  Total flux             : %13.6e" % (np.sum(x) / inormfactr))
     return "\n".join(lines)
 
@@ -71,6 +80,7 @@ def solve(problem: CostProblem, x0, niter=1000, nonneg=True, m=None, factr=None,
     pgtol = lbfgsbprms["pgtol"] if pgtol is None else pgtol
     state = {"nfev": 0, "eval_s": 0.0, "it": 0, "last": None}
     hist = []
+    nz = getattr(problem, "Nz", 1)
 
     def fg(x):
         t = time.perf_counter()
@@ -85,7 +95,7 @@ def solve(problem: CostProblem, x0, niter=1000, nonneg=True, m=None, factr=None,
         hist.append(state["last"]["cost"])
         if verbose and nprint > 0 and state["it"] % nprint == 0:
             print("Iteration :%5d/%5d" % (state["it"], niter))
-            print(_summary(state["last"], inormfactr, xk))
+            print(_summary(state["last"], inormfactr, xk, nz))
         if callback is not None:
             callback(state["it"], xk, state["last"])
 
@@ -94,7 +104,7 @@ def solve(problem: CostProblem, x0, niter=1000, nonneg=True, m=None, factr=None,
     if verbose:
         c, _, terms = problem.cost_grad(x0)
         print("Summary of the initial image")
-        print(_summary(terms, inormfactr, x0))
+        print(_summary(terms, inormfactr, x0, nz))
     res = minimize(fg, np.asarray(x0, np.float64), jac=True, method="L-BFGS-B", bounds=bounds,
                    callback=cb,
                    options=dict(maxcor=int(m), ftol=float(factr) * np.finfo(float).eps, gtol=float(pgtol),
@@ -104,7 +114,7 @@ def solve(problem: CostProblem, x0, niter=1000, nonneg=True, m=None, factr=None,
     if verbose:
         print("Final Summary")
         print("  Iteration ent at %5d/%5d" % (res.nit, niter))
-        print(_summary(terms, inormfactr, res.x))
+        print(_summary(terms, inormfactr, res.x, nz))
         print(res.message)
     info = SolveInfo(niter=int(res.nit), nfev=int(state["nfev"]), cost=float(c), terms=terms,
                      message=str(res.message), success=bool(res.success), wall_s=wall,

@@ -213,14 +213,44 @@ def prepare(image, dx_rad, dy_rad, Nxref=None, Nyref=None, window=None,
 
     kw = dict(Nx=Nx, Ny=Ny, xidx=xidx, yidx=yidx, Nxref=Nxref, Nyref=Nyref)
     # regularizers (lbfgs.py:263-423)
+    kw.update(_static_regs(xidx, yidx, Nx, Ny, winidx, Nyx, tf, dx_rad,
+                           l1_lambda, l1_prior, l1_noise, 1, 0.0, tv_lambda, tv_prior, tsv_lambda, tsv_prior,
+                           kl_lambda, kl_prior, gs_lambda, gs_prior, cen_lambda, cen_alpha,
+                           sm_lambda, sm_maj, sm_min, -sm_phi))
+    if tfd_lambda > 0:
+        kw.update(tfd_l=init_tfdreg(tfd_lambda, tf, tfd_tgterror), tfd_tgtfd=tf)
+
+    # data (lbfgs.py:429-489), weights and phase shift (fftim2d.f90:178-241)
+    u, v, ifcv, iamp, icp, ica = get_uvlist(vistable, amptable, bstable, catable)
+    kw.update(_data_kwargs(u, v, ifcv, iamp, icp, ica, vistable, amptable, bstable, catable,
+                           inormfactr, dx_rad, dy_rad, w_vis, w_amp, w_bs, w_ca, Nx, Ny, Nxref, Nyref))
+    return Prepared(kw, Iin * inormfactr, inormfactr, xidx, yidx, Nx, Ny)
+
+
+def _static_regs(xidx, yidx, Nx, Ny, winidx, Nyx, tf, dx_rad,
+                 l1_lambda, l1_prior, l1_noise, l1_type, l1_floor, tv_lambda, tv_prior, tsv_lambda, tsv_prior,
+                 kl_lambda, kl_prior, gs_lambda, gs_prior, cen_lambda, cen_alpha,
+                 sm_lambda, sm_maj, sm_min, sm_phi_deg):
+    """Weights of the image-domain regularizers (lbfgs.py:263-423,
+    lbfgs3d.py:349-512); ``tf`` is the scaled total flux, priors are
+    [Ny, Nx] arrays or None (flat); sm_maj / sm_min are FWHMs in radians and
+    sm_phi_deg the angle handed to the cost (sign as the caller's
+    reference)."""
+    Npix = xidx.size
+    kw = {}
     if l1_lambda > 0:
         pr = _prior_vec(l1_prior, winidx, Nyx)
         if pr is None:
             pr = np.full(Npix, tf / Npix)
+            pr *= tf / pr.sum()
+        elif l1_type == 2:                                    # lbfgs3d.py:372-384
+            pr *= tf / np.sum(pr)
+            amax = np.abs(pr).max()
+            pr[np.abs(pr) < amax * l1_noise] = amax * l1_floor
         else:
             amax = np.abs(pr).max()
             pr[np.abs(pr) < amax * l1_noise] = amax * l1_noise
-        pr *= tf / pr.sum()
+            pr *= tf / pr.sum()
         kw.update(l1_l=l1_lambda, l1_wgt=init_l1reg(pr))
     if tv_lambda > 0:
         pr = _prior_vec(tv_prior, winidx, Nyx)
@@ -245,19 +275,24 @@ def prepare(image, dx_rad, dy_rad, Nxref=None, Nyref=None, window=None,
         pr = np.full(Npix, tf / Npix) if pr is None else pr
         pr *= tf / pr.sum()
         kw.update(gs_l=gs_lambda, gs_wgt=init_gsreg(pr))
-    if tfd_lambda > 0:
-        kw.update(tfd_l=init_tfdreg(tfd_lambda, tf, tfd_tgterror), tfd_tgtfd=tf)
     if cen_lambda > 0:
         kw.update(cen_l=cen_lambda, cen_alpha=cen_alpha)
     if sm_lambda > 0:
-        dtheta = abs(dx_rad)      # sm_maj / sm_min are FWHMs in radians here
+        dtheta = abs(dx_rad)
         kw.update(sm_l=sm_lambda,
                   sm_maj=(sm_maj / dtheta) ** 2 / (8. * math.log(2.)),
                   sm_min=(sm_min / dtheta) ** 2 / (8. * math.log(2.)),
-                  sm_phi=-sm_phi / 180. * math.pi)
+                  sm_phi=sm_phi_deg / 180. * math.pi)
+    return kw
 
-    # data (lbfgs.py:429-480)
-    u, v, ifcv, iamp, icp, ica = get_uvlist(vistable, amptable, bstable, catable)
+
+def _data_kwargs(u, v, ifcv, iamp, icp, ica, vistable, amptable, bstable, catable,
+                 inormfactr, dx_rad, dy_rad, w_vis, w_amp, w_bs, w_ca, Nx, Ny, Nxref, Nyref):
+    """Data arrays (lbfgs.py:429-480), uv scaling (lbfgs.py:487-489), data
+    weights (fftim2d.f90:178-213, fftim3d.f90:225-259) and the phase shift of
+    the full complex visibilities to the image centre (fftim2d.f90:224-241,
+    fftim3d.f90:270-289)."""
+    kw = {}
     nf = na = nc = nq = 0
     if vistable is not None:
         ph = np.deg2rad(_col(vistable, "phase"))
@@ -276,19 +311,16 @@ def prepare(image, dx_rad, dy_rad, Nxref=None, Nyref=None, window=None,
     if catable is not None:
         kw.update(uvidxca=ica, CA=_col(catable, "logamp"), Varca=np.square(_col(catable, "logsigma")))
         nq = ica.shape[1]
-    # uv scaling (lbfgs.py:487-489)
     u = u * (2 * np.pi * dx_rad)
     v = v * (2 * np.pi * dy_rad)
     kw.update(u=u, v=v)
-    # data weights (fftim2d.f90:178-213)
     wtotal = 2 * nf * w_vis + na * w_amp + nc * w_bs + nq * w_ca
     kw.update(wfcv=w_vis / wtotal if nf else 0.0, wamp=w_amp / wtotal if na else 0.0,
               wcp=w_bs / wtotal if nc else 0.0, wca=w_ca / wtotal if nq else 0.0)
-    # phase shift of the full complex visibilities (fftim2d.f90:224-241)
     if nf:
         s = np.sign(ifcv)
         j = np.abs(ifcv) - 1
         dix = Nxref - Nx / 2.0 - 1.0
         diy = Nyref - Ny / 2.0 - 1.0
         kw["Vfcv"] = kw["Vfcv"] * np.exp(1j * (u[j] * s * dix + v[j] * s * diy))
-    return Prepared(kw, Iin * inormfactr, inormfactr, xidx, yidx, Nx, Ny)
+    return kw

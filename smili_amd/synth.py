@@ -95,6 +95,18 @@ def sky_image(sky, Nx, Ny, dx, dy, Nxref=None, Nyref=None, oversample=4):
     return img.reshape(Ny, os_, Nx, os_).sum(axis=(1, 3))
 
 
+def asymmetric_ring(flux, diameter, width_fwhm, angle, asym=0.7, n_beads=48):
+    """A ring of diameter d as n_beads Gaussian beads of FWHM width_fwhm
+    whose brightness follows 1 + asym cos(theta - angle) (theta measured
+    from +y towards +x): a crescent whose bright side points at `angle`."""
+    th = 2 * math.pi * np.arange(n_beads) / n_beads
+    w = 1.0 + asym * np.cos(th - angle)
+    w *= flux / w.sum()
+    r = diameter / 2
+    return [Gaussian(float(w[k]), width_fwhm, float(r * math.sin(th[k])), float(r * math.cos(th[k])))
+            for k in range(n_beads)]
+
+
 def random_gaussians(rng, n_min=3, n_max=6, total_flux=1.0, fov=200 * UAS,
                      fwhm_range=(10 * UAS, 40 * UAS)):
     n = int(rng.integers(n_min, n_max + 1))

@@ -11,6 +11,16 @@
   stations over 12 h (100 scans), sigma = 1% of the zero-baseline flux;
   full-visibility chi-square + L1 (1) + TSV (1e4), non-negative, from a flat
   20 uas Gaussian.
+* ``dynamic_movie`` -- 64 frames of 128x128 at 2 uas/pixel; 3e5 complex
+  visibilities in total, 2e3..8e3 per frame, from 10 seeded stations over
+  12 h; a ring (diameter 40 uas) whose brightness asymmetry angle rotates by
+  90 degrees across the frames; per-frame visibility chi-square + L1 + TSV +
+  Rt, one joint problem.
+* ``survey`` -- 512 independent problems: image sizes uniform over
+  {64, 96, 128, 192, 256}^2, visibility counts log-uniform in [2e3, 5e4]
+  from 6..12 seeded stations (Earth rotation, 12 h), L1 log-spaced over
+  [1e-2, 1e2] and TSV over [1e2, 1e6] (8 x 8), TV and MEM (KL) each on/off,
+  two seeded skies per setting.
 """
 from __future__ import annotations
 
@@ -20,6 +30,7 @@ import numpy as np
 
 from . import synth
 from .imaging.prep import prepare
+from .imaging.prep3d import prepare3d
 
 UAS = synth.UAS
 
@@ -58,23 +69,81 @@ def small_fcv(seed=1, n=64, n_st=8, n_scans=100, elev_cut=10.0):
     return prep, dict(sky=sky, image=synth.sky_image(sky, n, n, dx, dy), dx=dx, dy=dy, vis=vis)
 
 
-def algorithmic_bytes(prep, Nx=None, Ny=None):
-    """HBM bytes one objective+gradient evaluation must move at minimum:
-    16 Npix (x in, grad out, FP64) + 32 G (fine grid complex64, two passes per
-    transform, two transforms) + 16 M (uv' read by both transforms) + 16 M
-    (model V write + read) + 32 M (FP64 residual write + read) + per-datum
-    index/data/variance reads (fcv 28, amp 20, cp 28, ca 32 bytes)."""
-    kw = prep.kwargs
-    Nx = Nx or kw["Nx"]
-    Ny = Ny or kw["Ny"]
-    nf1 = 2 * Nx
-    nf2 = 2 * Ny
-    npix = np.asarray(kw["xidx"]).size
-    M = np.asarray(kw["u"]).size
-    nfcv = 0 if kw.get("uvidxfcv") is None else len(kw["uvidxfcv"])
-    namp = 0 if kw.get("uvidxamp") is None else len(kw["uvidxamp"])
-    ncp = 0 if kw.get("uvidxcp") is None else kw["uvidxcp"].shape[1]
-    nca = 0 if kw.get("uvidxca") is None else kw["uvidxca"].shape[1]
-    G = nf1 * nf2
-    return int(16 * npix + 32 * G + 16 * M + 16 * M + 32 * M +
-               28 * nfcv + 20 * namp + 28 * ncp + 32 * nca)
+def _frame_counts(rng, n_frames, total, lo, hi):
+    """Per-frame counts in [lo, hi] summing to total (uniform draws scaled,
+    clipped, and the remainder spread over frames with room)."""
+    c = rng.uniform(lo, hi, n_frames)
+    c = np.clip(np.round(c * total / c.sum()), lo, hi).astype(np.int64)
+    while c.sum() != total:
+        d = total - int(c.sum())
+        room = np.nonzero(c < hi)[0] if d > 0 else np.nonzero(c > lo)[0]
+        k = room[: min(abs(d), room.size)]
+        c[k] += 1 if d > 0 else -1
+    return c
+
+
+def dynamic_movie(seed=5, n_frames=64, n=128, n_vis=300_000, n_st=10, hours=12.0,
+                  per_frame=(2000, 8000), rt_lambda=1.0, l1_lambda=1.0, tsv_lambda=1e4):
+    rng = np.random.default_rng(seed)
+    dx, dy = -2.0 * UAS, 2.0 * UAS
+    nbl = n_st * (n_st - 1) // 2
+    scans_per_frame = -(-per_frame[1] // nbl) + 1
+    scans = synth.earth_rotation_scans(rng, n_st, 12.0, 230e9, n_frames * scans_per_frame, hours)
+    counts = _frame_counts(rng, n_frames, n_vis, *per_frame)
+    phi0 = float(rng.uniform(0, 2 * math.pi))
+    skies, cols = [], {k: [] for k in ("u", "v", "amp", "phase", "sigma", "frmidx", "utc")}
+    flux = 0.6
+    for z in range(n_frames):
+        sky = synth.asymmetric_ring(flux, 40 * UAS, 10 * UAS, phi0 + 0.5 * math.pi * z / max(n_frames - 1, 1))
+        skies.append(sky)
+        fs = scans[z * scans_per_frame:(z + 1) * scans_per_frame]
+        u = np.concatenate([s.u for s in fs])
+        v = np.concatenate([s.v for s in fs])
+        t = np.concatenate([np.full(s.u.size, s.gst) for s in fs])
+        pick = np.sort(rng.choice(u.size, int(counts[z]), replace=False))
+        tab = synth.vis_table(rng, sky, u[pick], v[pick], 0.01 * flux)
+        for k in ("u", "v", "amp", "phase", "sigma"):
+            cols[k].append(tab[k])
+        cols["frmidx"].append(np.full(pick.size, z))
+        cols["utc"].append(t[pick])
+    vis = {k: np.concatenate(a) for k, a in cols.items()}
+    init = synth.sky_image([synth.Gaussian(flux, 20 * UAS)], n, n, dx, dy)
+    frames0 = np.repeat(init[None], n_frames, axis=0)
+    prep = prepare3d(frames0, dx, dy, vistable=vis, l1_lambda=l1_lambda, tsv_lambda=tsv_lambda,
+                     rt_lambda=rt_lambda)
+    truth = np.stack([synth.sky_image(s, n, n, dx, dy) for s in skies])
+    return prep, dict(skies=skies, frames=truth, dx=dx, dy=dy, vis=vis, counts=counts)
+
+
+SURVEY_SIZES = (64, 96, 128, 192, 256)
+
+
+def survey(seed=3, n_problems=512, n_vis=(2_000, 50_000), stations=(6, 12)):
+    """Returns (list of Prepared, meta).  Settings: the 8 x 8 grid of
+    (L1, TSV) x TV on/off x KL on/off = 256, each with two skies."""
+    rng = np.random.default_rng(seed)
+    l1s = np.logspace(-2, 2, 8)
+    tsvs = np.logspace(2, 6, 8)
+    settings = [(a, b, tv, kl) for a in l1s for b in tsvs for tv in (False, True) for kl in (False, True)]
+    preps, meta = [], []
+    for i in range(n_problems):
+        l1, tsv, tv, kl = settings[i % len(settings)]
+        n = int(rng.choice(SURVEY_SIZES))
+        nv = int(round(math.exp(rng.uniform(math.log(n_vis[0]), math.log(n_vis[1])))))
+        n_st = int(rng.integers(stations[0], stations[1] + 1))
+        nbl = n_st * (n_st - 1) // 2
+        scans = synth.earth_rotation_scans(rng, n_st, float(rng.uniform(-30, 60)), 230e9,
+                                           -(-nv // nbl) + 1, 12.0)
+        u = np.concatenate([s.u for s in scans])
+        v = np.concatenate([s.v for s in scans])
+        pick = np.sort(rng.choice(u.size, nv, replace=False))
+        dx, dy = -2.0 * UAS, 2.0 * UAS
+        sky = synth.random_gaussians(rng, 3, 6, total_flux=float(rng.uniform(0.3, 2.0)),
+                                     fov=min(n * 2.0 * UAS * 0.5, 150 * UAS))
+        flux = sum(c.flux for c in sky)
+        vis = synth.vis_table(rng, sky, u[pick], v[pick], 0.01 * flux)
+        init = synth.sky_image([synth.Gaussian(flux, 30 * UAS)], n, n, dx, dy)
+        preps.append(prepare(init, dx, dy, vistable=vis, l1_lambda=float(l1), tsv_lambda=float(tsv),
+                             tv_lambda=1.0 if tv else -1, kl_lambda=1.0 if kl else -1))
+        meta.append(dict(n=n, nvis=nv, stations=n_st, l1=float(l1), tsv=float(tsv), tv=tv, kl=kl))
+    return preps, meta

@@ -177,3 +177,52 @@ def test_batch_survey_of_many_small_problems(gpu_lib):
                 c1, g1, _ = s.cost_grad(x)
             assert abs(costs[i] - c1) <= 1e-13 * abs(c1)
             assert rel_l2(b.split(grad)[i], g1) <= 1e-12
+
+
+# ---------------------------------------------------------------------------
+# the dynamic imager end to end
+# ---------------------------------------------------------------------------
+def _oracle_movie(kw):
+    kw = dict(kw)
+    m = {k: kw.pop(k) for k in ("Nz", "Nuvs", "rt_l", "rt_wgt", "ri_l", "ri_wgt", "lc_l", "lc_tgtfd", "lc_nidx")}
+    lc_l = None if np.sum(m["lc_l"]) <= 0 else m["lc_l"]
+    return O.Movie(frame=O.Problem(**kw), Nuvs=m["Nuvs"], rt_l=m["rt_l"], rt_wgt=m["rt_wgt"], ri_l=m["ri_l"],
+                   ri_wgt=m["ri_wgt"], lc_l=lc_l, lc_tgtfd=m["lc_tgtfd"], lc_nidx=m["lc_nidx"])
+
+
+def test_imaging3d_matches_oracle_solve(gpu_lib):
+    """imaging3d (GPU cost, host L-BFGS-B) against the same L-BFGS-B driven
+    by the FP64 ES-kernel oracle of fftim3d.calc_cost: frames agree to
+    NRMSE <= 1e-4 after 30 iterations, final costs to 1e-6 relative."""
+    from scipy.optimize import minimize
+
+    from smili_amd import workloads
+    from smili_amd.imaging import lbfgs3d
+    from smili_amd.imaging.image import Movie
+    from smili_amd.imaging.prep3d import prepare3d
+
+    Nt, n = 4, 32
+    _, meta = workloads.dynamic_movie(seed=21, n_frames=Nt, n=n, n_vis=1200, n_st=6, per_frame=(200, 400))
+    vis = meta["vis"]
+    init = synth_init = np.full((Nt, n, n), 0.6 / (n * n))
+    regs = dict(l1_lambda=1.0, tsv_lambda=1e3, rt_lambda=1.0, ri_lambda=0.3)
+    movie = Movie.from_frames(init, dx=2.0, angunit="uas")
+    out, info = lbfgs3d.imaging3d(movie, vistable=vis, niter=30, verbose=False, return_info=True, **regs)
+    got = out.frames()
+
+    pm = prepare3d(synth_init, -2.0 * np.pi / 180 / 3600e6, 2.0 * np.pi / 180 / 3600e6, vistable=vis, **regs)
+    mv = _oracle_movie(pm.kwargs)
+
+    def fg(x):
+        c, g, _ = O.calc_cost3d(x, mv, **ES)
+        return c, g
+
+    res = minimize(fg, pm.x0, jac=True, method="L-BFGS-B", bounds=[(0.0, None)] * pm.x0.size,
+                   options=dict(maxcor=5, ftol=1e1 * np.finfo(float).eps, gtol=0.0, maxiter=31, maxfun=10 ** 9,
+                                maxls=20))
+    ref = (res.x / pm.inormfactr).reshape(Nt, n, n)
+    err = float(np.sqrt(np.sum((got - ref) ** 2) / np.sum(ref ** 2)))
+    print("imaging3d nit", info.niter, "ref nit", res.nit, "NRMSE", err, "cost", info.cost, res.fun)
+    assert info.niter == res.nit
+    assert err <= 1e-4
+    assert abs(info.cost - res.fun) <= 1e-6 * abs(res.fun)
