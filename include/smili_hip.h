@@ -198,6 +198,16 @@ int smili_batch_cost_grad(smili_problem *p, const double *x, double *costs, doub
  * smili_cost_grad / smili_eval_async evaluate it like a single problem. */
 int smili_movie_create(const smili_movie_desc *desc, const smili_tune *tune, smili_problem **out);
 
+/* Bare transforms on an nx x ny image (all pixels, x fastest) and nuv
+ * points (u, v already scaled to radians per pixel, any sign): a handle
+ * without data or regularizers for smili_model_vis (finufft2d2 with +i,
+ * src/wrapfinufft/wrapfinufft.f90:94-128 and the real-input variant used by
+ * nufft_fwd_real, 130-166) and smili_adjoint_resid (finufft2d1 with +i,
+ * wrapfinufft.f90:172-245: nufft_adj_real1d(u, v, a, b) = the real part of
+ * A1 (a + i b) = smili_adjoint_resid(p, a, -b)). */
+int smili_transform_create(int32_t nx, int32_t ny, int32_t nuv, const double *u, const double *v,
+                           const smili_tune *tune, smili_problem **out);
+
 /* Counts of a handle: problems, image-vector length, terms per problem. */
 int smili_problem_sizes(smili_problem *p, int32_t *nprob, int64_t *nx_total, int32_t *nterms);
 

@@ -94,6 +94,8 @@ SIGNATURES = {
     "smili_movie_create": (C.c_int, [C.POINTER(SmiliMovieDesc), C.POINTER(SmiliTune), C.POINTER(C.c_void_p)]),
     "smili_problem_sizes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
                                       C.POINTER(C.c_int32)]),
+    "smili_transform_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, _pd, _pd, C.POINTER(SmiliTune),
+                                         C.POINTER(C.c_void_p)]),
 }
 
 STAGE_NAMES = ("scatter", "t2_rows", "t2_cols", "interp", "moments", "moments_reduce",

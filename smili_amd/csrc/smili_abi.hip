@@ -164,6 +164,7 @@ struct ProbSpec {
   int nuv;               // length of the problem's uv list
   int nterms = SMILI_NTERMS;
   int lc_mode = 0;
+  bool transform_only = false;           // no data: a plan for the bare transforms
 };
 
 struct MovieSpec {
@@ -565,7 +566,7 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   for (size_t pi = 0; pi < ps.size(); ++pi) {
     const ProbSpec& P = ps[pi];
     const smili_data& D = *P.data;
-    check_data(D, P.nuv);
+    if (!P.transform_only) check_data(D, P.nuv);
     auto& np = newpos[pi];
     np.resize(P.nuv);
     int o = 0;
@@ -986,6 +987,46 @@ int smili_batch_create(const smili_desc* descs, int32_t nprob, const smili_tune*
         ps.push_back(ProbSpec{&d->data, i, i + 1, d->nuv});
       }
       p->plan.build(us, ps, tune);
+    } catch (...) {
+      delete p;
+      throw;
+    }
+    *out = p;
+  });
+}
+
+int smili_transform_create(int32_t nx, int32_t ny, int32_t nuv, const double* u, const double* v,
+                           const smili_tune* tune, smili_problem** out) {
+  return guarded([&] {
+    require(out != nullptr, "out is NULL");
+    *out = nullptr;
+    require(nx > 0 && ny > 0 && (long long)nx * ny < (1LL << 30), "nx and ny must be positive");
+    require(nuv >= 1 && u && v, "nuv must be >= 1 with u and v given");
+    for (int j = 0; j < nuv; ++j)
+      require(std::isfinite(u[j]) && std::isfinite(v[j]), "non-finite uv coordinate at " + std::to_string(j));
+    std::vector<int32_t> xi((size_t)nx * ny), yi((size_t)nx * ny);
+    for (int iy = 0; iy < ny; ++iy)
+      for (int ix = 0; ix < nx; ++ix) {
+        xi[(size_t)iy * nx + ix] = ix + 1;
+        yi[(size_t)iy * nx + ix] = iy + 1;
+      }
+    smili_desc d{};
+    d.nx = nx;
+    d.ny = ny;
+    d.nxref = (nx + 1) / 2.0;
+    d.nyref = (ny + 1) / 2.0;
+    d.npix = nx * ny;
+    d.xidx = xi.data();
+    d.yidx = yi.data();
+    d.nuv = nuv;
+    d.u = u;
+    d.v = v;
+    auto* p = new smili_problem();
+    try {
+      std::vector<UnitSpec> us{UnitSpec{&d, 0, 1.0, u, v, nuv}};
+      ProbSpec P{&d.data, 0, 1, nuv};
+      P.transform_only = true;
+      p->plan.build(us, std::vector<ProbSpec>{P}, tune);
     } catch (...) {
       delete p;
       throw;
