@@ -200,14 +200,84 @@ def cpu_baseline_survey(preps, budget_s=15.0, n_sample=16, seed=0):
                       f"scaled to the whole survey"}
 
 
-def _roofline(g, kws, fp64, ms, steps):
+COPY_CAL_BYTES = 512 * 2 ** 20        # calibration copy: 512 MiB read, 512 MiB written
+
+
+def traffic_file(workload, precision):
+    return os.path.join(ROOT, "profiles", f"traffic_{workload}_{precision}.json")
+
+
+def _pmc_rows(path):
+    """(kernel name, counter value) per dispatch from a rocprofv3
+    counter_collection.csv."""
+    import csv
+    out = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            out.append((r["Kernel_Name"], r.get("Counter_Name", ""), float(r["Counter_Value"])))
+    return out
+
+
+def ingest_pmc(fetch_csv, write_csv, out_json, workload, precision):
+    """Per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) of the library's kernels,
+    each scaled by what the same counter reads for the calibration copy of
+    COPY_CAL_BYTES (an elementwise x * 1 over FP64, one read and one write
+    per byte), -> HBM bytes per launch of every kernel."""
+    import numpy as np
+    res = {}
+    for label, path in (("fetch", fetch_csv), ("write", write_csv)):
+        per = {}
+        for name, _, val in _pmc_rows(path):
+            per.setdefault(name, []).append(val)
+        res[label] = per
+    cal = {}
+    for label in ("fetch", "write"):
+        cands = {k: v for k, v in res[label].items() if "smili" not in k and "Mul" in k}
+        if not cands:
+            raise SystemExit(f"no calibration copy kernel in the {label} pass")
+        name = max(cands, key=lambda k: np.median(cands[k]))
+        cal[label] = {"kernel": name[:120], "kib": float(np.median(cands[name])),
+                      "factor": COPY_CAL_BYTES / 1024.0 / float(np.median(cands[name]))}
+    kernels = {}
+    for name in res["fetch"]:
+        if "smili" not in name:
+            continue
+        short = name.split("(")[0].replace("void ", "").replace("smili::", "").split("<")[0].strip()
+        f = float(np.mean(res["fetch"][name])) * cal["fetch"]["factor"] * 1024
+        w = float(np.mean(res["write"].get(name, [0.0]))) * cal["write"]["factor"] * 1024
+        k = kernels.setdefault(short, {"fetch_bytes": 0.0, "write_bytes": 0.0, "dispatches": 0})
+        k["fetch_bytes"] += f
+        k["write_bytes"] += w
+        k["dispatches"] = len(res["fetch"][name])
+    for k in kernels.values():
+        k["traffic_bytes"] = int(k["fetch_bytes"] + k["write_bytes"])
+        k["fetch_bytes"] = int(k["fetch_bytes"])
+        k["write_bytes"] = int(k["write_bytes"])
+    doc = {"workload": workload, "precision": precision, "calibration": cal, "kernels": kernels}
+    with open(out_json, "w") as f:
+        json.dump(doc, f, indent=1)
+    print(json.dumps(doc, indent=1))
+
+
+def _traffic(workload, precision, kernel):
+    path = traffic_file(workload, precision)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        doc = json.load(f)
+    k = doc.get("kernels", {}).get(kernel)
+    return None if k is None else k["traffic_bytes"]
+
+
+def _roofline(g, kws, fp64, ms, steps, workload="single_large"):
     prof = g.profile(reps=max(10, min(steps, 100)))
     sb = stage_bytes(kws, fp64)
     dom = max((k for k in prof if k in sb), key=lambda k: prof[k])
     achieved = sb[dom] / (prof[dom] * 1e-3) / 1e9
     eval_bytes = int(sum(sb.values()))
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": _traffic(workload, "fp64" if fp64 else "fp32", "k_" + dom),
             "kernel": "k_" + dom, "kernel_ms": round(prof[dom], 5),
             "kernel_bytes": sb[dom],
             "eval_bytes": eval_bytes,
@@ -227,7 +297,14 @@ def main():
     ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp64",
                     help="NUFFT arithmetic (chi-square and regularizers are FP64 in both)")
     ap.add_argument("--workload", choices=("single_large", "movie", "survey"), default="single_large")
+    ap.add_argument("--pmc-run", action="store_true",
+                    help="only evaluations plus the calibration copy (run under rocprofv3 --pmc)")
+    ap.add_argument("--pmc-ingest", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="turn two rocprofv3 --pmc counter files into profiles/traffic_*.json")
     args = ap.parse_args()
+    if args.pmc_ingest:
+        ingest_pmc(*args.pmc_ingest, traffic_file(args.workload, args.precision), args.workload, args.precision)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the objective+gradient path has no multi-GPU "
               "component in this build (see DESIGN.md for how it would shard); run with --gpus 1",
@@ -283,6 +360,17 @@ def main():
                  "npix_total": int(x0.size)}
     info = g.info()
     t_setup = time.perf_counter() - t_setup
+    if args.pmc_run:
+        g.cost_grad(x0)
+        dt = _timed_loop(g, args.steps, args.warmup)
+        src = torch.ones(COPY_CAL_BYTES // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        print(json.dumps({"pmc_run": args.workload, "steps": args.steps, "ms_per_step": dt / args.steps * 1e3}),
+              flush=True)
+        return
     # x goes to the handle's own device buffer once; every timed step is one
     # evaluation on resident data (the captured graph of all kernels)
     first = g.cost_grad(x0)
@@ -298,7 +386,7 @@ def main():
     for _ in range(nh):
         g.cost_grad(x0)
     host_rate = per_step * nh / (time.perf_counter() - th)
-    roofline = _roofline(g, kws, fp64, ms, args.steps)
+    roofline = _roofline(g, kws, fp64, ms, args.steps, args.workload)
 
     cpu = None
     if not args.no_cpu:

@@ -802,6 +802,9 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   HIP_CHECK(hipDeviceSynchronize());
 }
 
+// grid of an XCD-mapped launch (see xcd_item): a multiple of the XCD count
+static size_t xcd_grid(size_t nitems) { return (nitems + kXcds - 1) / kXcds * kXcds; }
+
 #define LAUNCH(stage, kern, grid, block, lds, ...)                            \
   do {                                                                        \
     if ((grid) > 0) {                                                         \
@@ -837,7 +840,8 @@ static void launch_forward_t(Plan& P) {
   if (P.npt == 0) return;
   LAUNCH(SMILI_S_T2_ROWS, k_t2_rows<R>, P.h_t2rows.size(), P.threads_rows, P.lds_rows, P.d_t2rows.p, tw, corr,
          P.d_x.p, P.d_I2d.p, gA);
-  LAUNCH(SMILI_S_T2_COLS, k_t2_cols<R>, P.h_cols.size(), P.threads_cols, P.lds_cols, P.d_cols.p, tw, gA);
+  LAUNCH(SMILI_S_T2_COLS, k_t2_cols<R>, xcd_grid(P.h_cols.size()), P.threads_cols, P.lds_cols, P.d_cols.p,
+         (int)P.h_cols.size(), tw, gA);
   LAUNCH(SMILI_S_INTERP, k_interp<R>, P.h_subs.size(), kThreads, P.lds_tile, P.d_subs.p, gA, P.d_cell.p, ex, ey,
          Vv, P.d_Vres.p, P.bin, beta);
 }
@@ -860,7 +864,8 @@ static void launch_adjoint_t(Plan& P, bool with_rows) {
   V* gB = reinterpret_cast<V*>(P.d_gridB.p);
   LAUNCH(SMILI_S_SPREAD, k_spread<R>, P.h_cblocks.size(), kSpansPerGroup, 0, P.d_cblocks.p, P.d_spans.p, ex, ey,
          P.d_Vres.p, gB, beta);
-  LAUNCH(SMILI_S_T1_COLS, k_t1_cols<R>, P.h_cols.size(), P.threads_cols, P.lds_cols, P.d_cols.p, tw, gB, gA);
+  LAUNCH(SMILI_S_T1_COLS, k_t1_cols<R>, xcd_grid(P.h_cols.size()), P.threads_cols, P.lds_cols, P.d_cols.p,
+         (int)P.h_cols.size(), tw, gB, gA);
   if (with_rows)
     LAUNCH(SMILI_S_T1_ROWS, k_t1_rows<R>, P.h_t1rows.size(), P.threads_rows, P.lds_rows, P.d_t1rows.p, tw, corr,
            gA, P.d_grad2d.p);

@@ -174,6 +174,20 @@ template <typename R> __device__ __forceinline__ void es_weights(R e, R beta, R*
 // ---------------------------------------------------------------------------
 // Type-2 step 1: image rows -> deconvolve, zero-pad, FFT along x.
 // ---------------------------------------------------------------------------
+// Work item of this work-group for kernels whose neighbouring work items
+// share cache lines (the column passes: a 128-byte line holds 4-8 columns).
+// Work-groups are dispatched round-robin over the 8 XCDs, each with its own
+// L2, so work-group b runs on XCD b % 8; XCD x is given the contiguous item
+// range [x * per, (x + 1) * per) so that line-sharing items meet in one L2.
+// The grid is 8 * per work-groups (per = ceil(nitems / 8)); -1 past the end.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_item(int nitems) {
+  const int per = gridDim.x / kXcds;
+  const int b = blockIdx.x;
+  const int item = (b % kXcds) * per + b / kXcds;
+  return item < nitems ? item : -1;
+}
+
 template <typename R>
 __global__ void __launch_bounds__(kThreads)
 k_t2_rows(const FftItem* __restrict__ items, const typename Cx<R>::T* __restrict__ tw,
@@ -221,12 +235,14 @@ k_t2_rows(const FftItem* __restrict__ items, const typename Cx<R>::T* __restrict
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ void __launch_bounds__(kThreads)
-k_t2_cols(const FftItem* __restrict__ items, const typename Cx<R>::T* __restrict__ tw,
+k_t2_cols(const FftItem* __restrict__ items, int nitems, const typename Cx<R>::T* __restrict__ tw,
           typename Cx<R>::T* __restrict__ gridA) {
   using V = typename Cx<R>::T;
   extern __shared__ __align__(16) unsigned char smraw[];
   V* sm = reinterpret_cast<V*>(smraw);
-  const FftItem it = items[blockIdx.x];
+  const int item = xcd_item(nitems);
+  if (item < 0) return;
+  const FftItem it = items[item];
   const int n = it.nf2, nb = it.count, stride = n + 1;
   V* A = sm;
   V* B = sm + nb * stride;
@@ -389,12 +405,14 @@ k_spread(const CellBlock* __restrict__ blocks, const CellSpan* __restrict__ span
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ void __launch_bounds__(kThreads)
-k_t1_cols(const FftItem* __restrict__ items, const typename Cx<R>::T* __restrict__ tw,
+k_t1_cols(const FftItem* __restrict__ items, int nitems, const typename Cx<R>::T* __restrict__ tw,
           typename Cx<R>::T* __restrict__ gridB, typename Cx<R>::T* __restrict__ H) {
   using V = typename Cx<R>::T;
   extern __shared__ __align__(16) unsigned char smraw[];
   V* sm = reinterpret_cast<V*>(smraw);
-  const FftItem it = items[blockIdx.x];
+  const int item = xcd_item(nitems);
+  if (item < 0) return;
+  const FftItem it = items[item];
   const int n = it.nf2, nb = it.count, stride = n + 1;
   V* A = sm;
   V* B = sm + nb * stride;
