@@ -193,8 +193,7 @@ struct Plan {
 
   std::vector<FftItem> h_t2rows, h_cols, h_t1rows;
   std::vector<SubProb> h_subs;
-  std::vector<CellBlock> h_cblocks;
-  std::vector<CellSpan> h_spans;
+  std::vector<SpreadItem> h_spread;
   std::vector<Chunk> h_chunks;
   std::vector<int> h_slots;
   std::vector<RegItem> h_regs;
@@ -225,8 +224,7 @@ struct Plan {
   DevBuf<CpItem> d_cp;
   DevBuf<CaItem> d_ca;
   DevBuf<SubProb> d_subs;
-  DevBuf<CellBlock> d_cblocks;
-  DevBuf<CellSpan> d_spans;
+  DevBuf<SpreadItem> d_spread;
   DevBuf<FftItem> d_t2rows, d_cols, d_t1rows;
   DevBuf<RegItem> d_regs;
   DevBuf<MovieDev> d_movies;
@@ -313,7 +311,7 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   require(tune.precision == 0 || tune.precision == 32 || tune.precision == 64,
           "precision must be 0 (default, FP64), 32 or 64");
   fp64 = tune.precision != 32;
-  require(bin >= 8 && bin <= 64, "bin must be in [8, 64]");
+  require(bin >= 8 && bin <= 64 && bin % kCellBlock == 0, "bin must be a multiple of 8 in [8, 64]");
   require(sub_points >= 32 && sub_points <= 65536, "sub_points must be in [32, 65536]");
   require(col_group >= 1 && col_group <= 16, "col_group must be in [1, 16]");
   require(chunk >= 64, "chunk must be >= 64");
@@ -484,12 +482,16 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     {
       std::vector<int> fill(count.begin(), count.end() - 1);
       for (int j = 0; j < m; ++j) order[fill[bidx[j]]++] = j;
-      // inside a bin, order the points by their first-tap cell (the spreading
-      // kernel gathers per cell)
+      // inside a bin, order the points by 8x8 cell block, then by first-tap
+      // cell (bins are whole cell blocks, so a cell block's points are
+      // contiguous for the spreading work items, sorted by cell inside)
+      auto key = [&](int a) {
+        return ((long long)(l0y[a] / kCellBlock) << 48) | ((long long)(l0x[a] / kCellBlock) << 32) |
+               ((long long)l0y[a] << 16) | (long long)l0x[a];
+      };
       for (int b = 0; b < nbx * nby; ++b)
-        std::stable_sort(order.begin() + count[b], order.begin() + count[b + 1], [&](int a, int c) {
-          return (l0y[a] * 65536 + l0x[a]) < (l0y[c] * 65536 + l0x[c]);
-        });
+        std::stable_sort(order.begin() + count[b], order.begin() + count[b + 1],
+                         [&](int a, int c) { return key(a) < key(c); });
     }
     unit_rank[ui].resize(m);
     for (int s = 0; s < m; ++s) {
@@ -501,40 +503,27 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
       exd.push_back(exu[j]);
       eyd.push_back(eyu[j]);
     }
-    // spreading work items: 8x8 blocks of first-tap cells, occupied cells
-    // with their (contiguous, sorted) point ranges
+    // spreading work items: runs of at most kSpreadPoints consecutive
+    // points of one 8x8 block of first-tap cells
     {
-      std::map<long long, std::vector<CellSpan>> blocks;   // key: (by, bx) of the cell block
       int s = 0;
       while (s < m) {
-        const int j = order[s];
+        const int j0 = order[s];
+        const int bxk = l0x[j0] / kCellBlock, byk = l0y[j0] / kCellBlock;
         int e = s + 1;
-        while (e < m && l0x[order[e]] == l0x[j] && l0y[order[e]] == l0y[j]) ++e;
-        const int bxk = l0x[j] / kCellBlock, byk = l0y[j] / kCellBlock;
-        const int lc = (l0y[j] % kCellBlock) * kCellBlock + (l0x[j] % kCellBlock);
-        auto& spans = blocks[(long long)byk * 65536 + bxk];
-        for (int a = s; a < e; a += kSpanPoints)
-          spans.push_back(CellSpan{lc, (int)U.pt_off + a, (int)U.pt_off + std::min(e, a + kSpanPoints),
-                                   (a - s) / kSpanPoints});
+        while (e < m && e - s < kSpreadPoints && l0x[order[e]] / kCellBlock == bxk &&
+               l0y[order[e]] / kCellBlock == byk)
+          ++e;
+        SpreadItem si{};
+        si.gx0 = bxk * kCellBlock;
+        si.gy0 = byk * kCellBlock;
+        si.pt_begin = (int)U.pt_off + s;
+        si.pt_end = (int)U.pt_off + e;
+        si.nf1 = U.nf1;
+        si.nf2 = U.nf2;
+        si.grid_off = U.grid_off;
+        h_spread.push_back(si);
         s = e;
-      }
-      for (auto& kv : blocks) {
-        // interleave: first span of every cell, then second spans, ... so
-        // that neighbouring threads work on different cells
-        std::stable_sort(kv.second.begin(), kv.second.end(),
-                         [](const CellSpan& a, const CellSpan& b) { return a.rank < b.rank; });
-        for (size_t a = 0; a < kv.second.size(); a += kSpansPerGroup) {
-          CellBlock cbk{};
-          cbk.nf1 = U.nf1;
-          cbk.nf2 = U.nf2;
-          cbk.grid_off = U.grid_off;
-          cbk.gx0 = (int)(kv.first % 65536) * kCellBlock;
-          cbk.gy0 = (int)(kv.first / 65536) * kCellBlock;
-          cbk.cell_begin = (int)h_spans.size();
-          cbk.cell_count = (int)std::min<size_t>(kSpansPerGroup, kv.second.size() - a);
-          h_spans.insert(h_spans.end(), kv.second.begin() + a, kv.second.begin() + a + cbk.cell_count);
-          h_cblocks.push_back(cbk);
-        }
       }
     }
     for (int b = 0; b < nbx * nby; ++b) {
@@ -793,8 +782,7 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   d_cp.upload(cp);
   d_ca.upload(ca);
   d_subs.upload(h_subs);
-  d_cblocks.upload(h_cblocks);
-  d_spans.upload(h_spans);
+  d_spread.upload(h_spread);
   d_t2rows.upload(h_t2rows);
   d_cols.upload(h_cols);
   d_t1rows.upload(h_t1rows);
@@ -862,7 +850,7 @@ static void launch_adjoint_t(Plan& P, bool with_rows) {
   const R* ey = reinterpret_cast<const R*>(P.fp64 ? (const void*)P.d_eyd.p : (const void*)P.d_ey.p);
   V* gA = reinterpret_cast<V*>(P.d_gridA.p);
   V* gB = reinterpret_cast<V*>(P.d_gridB.p);
-  LAUNCH(SMILI_S_SPREAD, k_spread<R>, P.h_cblocks.size(), kSpansPerGroup, 0, P.d_cblocks.p, P.d_spans.p, ex, ey,
+  LAUNCH(SMILI_S_SPREAD, k_spread<R>, P.h_spread.size(), kThreads, 0, P.d_spread.p, P.d_cell.p, ex, ey,
          P.d_Vres.p, gB, beta);
   LAUNCH(SMILI_S_T1_COLS, k_t1_cols<R>, xcd_grid(P.h_cols.size()), P.threads_cols, P.lds_cols, P.d_cols.p,
          (int)P.h_cols.size(), tw, gB, gA);

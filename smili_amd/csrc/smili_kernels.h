@@ -70,14 +70,12 @@ struct FftItem {
 };
 struct SubProb { int bin_x, bin_y, start, count, nf1, nf2; long long grid_off; };
 struct RegItem { int unit, start, count, pad; };
-// spreading: a group of "spans" inside one 8x8 block of first-tap cells; a
-// span is up to kSpanPoints consecutive points of one cell (points are
-// sorted by cell), one thread per span
+// spreading: up to kSpreadPoints consecutive points (sorted by cell) of one
+// 8x8 block of first-tap cells; the block's 14x14 grid footprint is
+// accumulated in LDS and added to the grid once per work item
 constexpr int kCellBlock = 8;
-constexpr int kSpanPoints = 8;
-constexpr int kSpansPerGroup = 256;
-struct CellBlock { int gx0, gy0, cell_begin, cell_count, nf1, nf2; long long grid_off; };
-struct CellSpan { int lcell, pt_begin, pt_end, rank; };  // lcell = ly * 8 + lx; rank in its cell
+constexpr int kSpreadPoints = 256;   // = work-group size: one staging round per wave
+struct SpreadItem { int gx0, gy0, pt_begin, pt_end, nf1, nf2; long long grid_off; };
 
 // temporal regularizers of one movie (frames = consecutive units with
 // consecutive x ranges of npix pixels each)

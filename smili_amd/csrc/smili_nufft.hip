@@ -319,79 +319,80 @@ k_interp(const SubProb* __restrict__ sps, const typename Cx<R>::T* __restrict__ 
 
 // ---------------------------------------------------------------------------
 // Type-1 step 1: spread residuals c = Vres_re - i Vres_im onto the grid.
-// Work item = up to 256 spans of one 8x8 block of first-tap cells; a span is
-// up to 8 consecutive points of one cell (points are sorted by cell, so a
-// span's points share their 7x7 footprint).  One thread per span sums the
-// footprints of its points in registers (QB rows of taps per pass: all 7
-// in FP32, 4 + 3 in FP64 to bound the register tile), adds the sum to an
-// LDS tile of the block (spans of neighbouring cells overlap), and the tile
-// is added to the global grid with atomics (work items of one block and of
-// neighbouring blocks overlap).  The grid is zero on entry (k_t1_cols
-// clears what it consumes).
+// One work item = up to 256 points (sorted by first-tap cell) of one 8x8
+// block of cells.  Each wave stages the ES weights of its 64 points in LDS
+// (lane = point: 7 + 7 kernel values and the residual), then switches roles
+// (lane = one of the 49 taps) and walks the points in order, accumulating
+// each run of points of one cell in registers and adding the run to the
+// block's 14x14 LDS footprint when the cell changes.  The footprint is
+// added to the grid with one atomic per touched cell (work items of one
+// block and of neighbouring blocks overlap).  The grid is zero on entry
+// (k_t1_cols clears what it consumes).
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ void __launch_bounds__(kSpansPerGroup)
-k_spread(const CellBlock* __restrict__ blocks, const CellSpan* __restrict__ spans,
+__global__ void __launch_bounds__(kThreads)
+k_spread(const SpreadItem* __restrict__ items, const int* __restrict__ pt_cell,
          const R* __restrict__ pt_ex, const R* __restrict__ pt_ey,
          const double2* __restrict__ Vres, typename Cx<R>::T* __restrict__ gridB, R beta) {
   using V = typename Cx<R>::T;
   constexpr int T = kCellBlock + kW - 1;
-  constexpr int QB = sizeof(R) == 4 ? kW : 4;
+  constexpr int NW = kThreads / 64;
   __shared__ V tile[T * T];
-  const CellBlock cb = blocks[blockIdx.x];
+  __shared__ R wxs[NW][kW][65], wys[NW][kW][65];   // padded: the 7 rows sit in distinct banks
+  __shared__ V cs[NW][64];
+  __shared__ int cl[NW][64];
+  const SpreadItem it = items[blockIdx.x];
   for (int c = threadIdx.x; c < T * T; c += blockDim.x) tile[c] = cmk<V>(0, 0);
-  __syncthreads();
-  for (int k = threadIdx.x; k < cb.cell_count; k += blockDim.x) {
-    const CellSpan sp = spans[cb.cell_begin + k];
-    const int lx = sp.lcell % kCellBlock, ly = sp.lcell / kCellBlock;
-    for (int q0 = 0; q0 < kW; q0 += QB) {
-      R ar[QB][kW], ai[QB][kW];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int base = it.pt_begin + w * 64;
+  const int np = max(0, min(64, it.pt_end - base));
+  if (lane < np) {
+    const int j = base + lane;
+    R wx[kW], wy[kW];
+    es_weights(pt_ex[j], beta, wx);
+    es_weights(pt_ey[j], beta, wy);
 #pragma unroll
-      for (int a = 0; a < QB; ++a)
-#pragma unroll
-        for (int qx = 0; qx < kW; ++qx) {
-          ar[a][qx] = 0;
-          ai[a][qx] = 0;
-        }
-      for (int j = sp.pt_begin; j < sp.pt_end; ++j) {
-        R wx[kW];
-        es_weights(pt_ex[j], beta, wx);
-        const R ey = pt_ey[j];
-        const double2 r = Vres[j];
-        const R cr = (R)r.x, ci = (R)(-r.y);
-#pragma unroll
-        for (int a = 0; a < QB; ++a) {
-          if (q0 + a >= kW) break;
-          const R wy = es_tap(ey, beta, q0 + a);
-          const R yr = wy * cr, yi = wy * ci;
-#pragma unroll
-          for (int qx = 0; qx < kW; ++qx) {
-            ar[a][qx] = fma(wx[qx], yr, ar[a][qx]);
-            ai[a][qx] = fma(wx[qx], yi, ai[a][qx]);
-          }
-        }
-      }
-#pragma unroll
-      for (int a = 0; a < QB; ++a) {
-        if (q0 + a >= kW) break;
-#pragma unroll
-        for (int qx = 0; qx < kW; ++qx) {
-          V* dst = &tile[(ly + q0 + a) * T + lx + qx];
-          atomicAdd(&dst->x, ar[a][qx]);
-          atomicAdd(&dst->y, ai[a][qx]);
-        }
-      }
+    for (int q = 0; q < kW; ++q) {
+      wxs[w][q][lane] = wx[q];
+      wys[w][q][lane] = wy[q];
     }
+    const double2 r = Vres[j];
+    cs[w][lane] = cmk<V>((R)r.x, (R)(-r.y));
+    const int cell = pt_cell[j];
+    cl[w][lane] = ((cell >> 16) - it.gy0) * T + ((cell & 0xffff) - it.gx0);
   }
   __syncthreads();
-  V* g = gridB + cb.grid_off;
+  if (lane < kW * kW && np > 0) {
+    const int a = lane / kW, b = lane - a * kW;
+    const int off = a * T + b;
+    R ar = 0, ai = 0;
+    int cur = cl[w][0];
+    for (int p = 0; p < np; ++p) {
+      const int c = cl[w][p];
+      if (c != cur) {
+        atomicAdd(&tile[cur + off].x, ar);
+        atomicAdd(&tile[cur + off].y, ai);
+        ar = 0;
+        ai = 0;
+        cur = c;
+      }
+      const R wt = wxs[w][b][p] * wys[w][a][p];
+      const V v = cs[w][p];
+      ar = fma(wt, v.x, ar);
+      ai = fma(wt, v.y, ai);
+    }
+    atomicAdd(&tile[cur + off].x, ar);
+    atomicAdd(&tile[cur + off].y, ai);
+  }
+  __syncthreads();
+  V* g = gridB + it.grid_off;
   for (int c = threadIdx.x; c < T * T; c += blockDim.x) {
     const V val = tile[c];
     if (val.x != 0 || val.y != 0) {
       const int ty = c / T, tx = c - ty * T;
-      const int gy = (cb.gy0 + ty) % cb.nf2;
-      const int gx = (cb.gx0 + tx) % cb.nf1;
-      V* dst = g + (long long)gy * cb.nf1 + gx;
+      const int gy = (it.gy0 + ty) % it.nf2;
+      const int gx = (it.gx0 + tx) % it.nf1;
+      V* dst = g + (long long)gy * it.nf1 + gx;
       unsafeAtomicAdd(&dst->x, val.x);
       unsafeAtomicAdd(&dst->y, val.y);
     }
