@@ -138,16 +138,22 @@ typedef struct smili_movie_desc {
 
 /* Tunables of the spreading/interpolation kernels (0 = default). */
 typedef struct smili_tune {
-  int32_t bin;          /* bin edge in fine-grid cells (8..32, default 16) */
+  int32_t bin;          /* bin edge in fine-grid cells, a multiple of 8 in
+                           8..64 (default 16) */
   int32_t sub_points;   /* max points per interpolation subproblem (32..65536,
                            default 256) */
-  int32_t col_group;    /* columns per column-FFT work-group (1..4, default 2) */
+  int32_t col_group;    /* columns per column-FFT work-group (1..16; 0 = default:
+                           the largest of 4, 2, 1 whose LDS fits 64 KB) */
   int32_t chunk;        /* data items per chi-square work-group (default 2048) */
   int32_t use_graph;    /* >= 0: capture one evaluation in a hipGraph and replay it;
                            < 0: launch the kernels one by one */
   int32_t precision;    /* bits of the NUFFT arithmetic (spreading,
                            interpolation, FFT): 64 (default, also 0) or 32;
                            chi-square and regularizers are FP64 in both */
+  int32_t branches;     /* 1: every stage in order on one stream; 2: the
+                           regularizers and final sums on a second stream
+                           beside the transforms; 0 (default): 2 for plans
+                           of at least 1e6 uv points, else 1 */
 } smili_tune;
 
 int smili_abi_version(void);

@@ -175,7 +175,9 @@ struct MovieSpec {
 
 struct Plan {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;            // main stream: the transforms and chi-square
+  hipStream_t side = nullptr;              // regularizers and the final sums, forked per evaluation
+  hipEvent_t ev_fork = nullptr, ev_reg = nullptr, ev_chisq = nullptr, ev_fin = nullptr;
   smili_tune tune{};
   int bin = 16, sub_points = 256, col_group = 2, chunk = 2048, rows_per_block = 1;
   bool fp64 = false;                       // NUFFT arithmetic in FP64 instead of FP32
@@ -190,6 +192,7 @@ struct Plan {
   std::vector<std::vector<int>> winpix;   // per unit: window pixel linear index
   size_t npix_total = 0, npt = 0;
   bool any_window = false, any_moments = false;
+  bool two_branches = false;
 
   std::vector<FftItem> h_t2rows, h_cols, h_t1rows;
   std::vector<SubProb> h_subs;
@@ -212,7 +215,8 @@ struct Plan {
   DevBuf<double2> d_V;                     // float2 or double2 visibilities (sized for double2)
   DevBuf<double2> d_Vres;
   DevBuf<double2> d_gridA, d_gridB;        // float2 or double2 grids (sized for double2)
-  DevBuf<double> d_I2d, d_grad2d;
+  DevBuf<double> d_I2d, d_grad2d, d_greg;
+  DevBuf<int> d_pixinv;                    // 2-D pixel -> window index (-1 outside), per unit
   DevBuf<double> d_x, d_grad, d_terms;
   DevBuf<double> d_cpart, d_rpart, d_mpart;
   DevBuf<UnitMoments> d_mom;
@@ -242,8 +246,11 @@ struct Plan {
   ~Plan() {
     for (auto& e : prof_ev)
       if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {ev_fork, ev_reg, ev_chisq, ev_fin})
+      if (e) (void)hipEventDestroy(e);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -306,19 +313,23 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   if (t) tune = *t;
   bin = tune.bin > 0 ? tune.bin : 16;
   sub_points = tune.sub_points > 0 ? tune.sub_points : 256;
-  col_group = tune.col_group > 0 ? tune.col_group : 2;
+  col_group = tune.col_group;             // 0: the largest of 4, 2, 1 that fits the LDS budget
   chunk = tune.chunk > 0 ? tune.chunk : 2048;
   require(tune.precision == 0 || tune.precision == 32 || tune.precision == 64,
           "precision must be 0 (default, FP64), 32 or 64");
   fp64 = tune.precision != 32;
   require(bin >= 8 && bin <= 64 && bin % kCellBlock == 0, "bin must be a multiple of 8 in [8, 64]");
   require(sub_points >= 32 && sub_points <= 65536, "sub_points must be in [32, 65536]");
-  require(col_group >= 1 && col_group <= 16, "col_group must be in [1, 16]");
+  require(col_group >= 0 && col_group <= 16, "col_group must be in [0, 16]");
   require(chunk >= 64, "chunk must be >= 64");
   tune.use_graph = (tune.use_graph < 0) ? 0 : 1;
+  require(tune.branches >= 0 && tune.branches <= 2, "branches must be 0, 1 or 2");
 
   HIP_CHECK(hipGetDevice(&device));
   HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  HIP_CHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+  for (hipEvent_t* e : {&ev_fork, &ev_reg, &ev_chisq, &ev_fin})
+    HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
 
   // FFT plans and twiddles
   std::map<int, int> plan_of;
@@ -542,6 +553,12 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     npt += m;
   }
   require(npt < (1u << 30), "too many uv points");
+  // two branches pay off once the stages are long enough to hide the
+  // cross-stream dependencies of the graph; below that, riding the
+  // regularizers and sums on other launches is faster (measured: a single
+  // 1.3e5-point problem 0.107 ms fused vs 0.137 ms forked; a 64-frame movie
+  // of 3e5 points 0.268 vs 0.260 ms; a survey of 7.6e6 points 5.4 vs 4.6 ms)
+  two_branches = tune.branches == 2 || (tune.branches == 0 && (npt >= 200000 || us.size() >= 16));
 
   // problems: remapped data items and chunks
   std::vector<FcvItem> fcv;
@@ -647,6 +664,12 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     shift_chunks(DT_CA, ca0);
   }
 
+  // column work-groups: the largest of 4, 2, 1 columns whose LDS fits
+  if (col_group == 0) {
+    const size_t csz = fp64 ? sizeof(double2) : sizeof(float2);
+    col_group = 4;
+    while (col_group > 1 && (2ull * col_group * (max_nf2 + 1) + max_nf2) * csz > 65536) col_group /= 2;
+  }
   // regularizer items, row/column items
   const int reg_chunk = 512;
   std::vector<int> unit_reg_begin(units.size() + 1, 0);
@@ -767,6 +790,13 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   d_x.alloc(npix_total);
   d_x.zero();
   d_grad.alloc(npix_total);
+  d_greg.alloc(npix_total);
+  {
+    std::vector<int> inv(std::max<long long>(img_total, 1), -1);
+    for (size_t ui = 0; ui < units.size(); ++ui)
+      for (int k = 0; k < units[ui].npix; ++k) inv[units[ui].img_off + winpix[ui][k]] = k;
+    d_pixinv.upload(inv);
+  }
   d_terms.alloc(nterms_total);
   d_cpart.alloc(std::max<size_t>(h_chunks.size(), 1));
   d_rpart.alloc(std::max<size_t>(h_regs.size(), 1) * kRegTerms);
@@ -808,7 +838,7 @@ static size_t xcd_grid(size_t nitems) { return (nitems + kXcds - 1) / kXcds * kX
   } while (0)
 
 template <typename R>
-static void launch_forward_t(Plan& P) {
+static void launch_forward_t(Plan& P, bool with_scatter) {
   using V = typename Cx<R>::T;
   hipStream_t stream = P.stream;
   const bool prof = P.prof;
@@ -822,7 +852,7 @@ static void launch_forward_t(Plan& P) {
   const R* ey = reinterpret_cast<const R*>(P.fp64 ? (const void*)P.d_eyd.p : (const void*)P.d_ey.p);
   V* gA = reinterpret_cast<V*>(P.d_gridA.p);
   V* Vv = reinterpret_cast<V*>(P.d_V.p);
-  if (P.any_window)
+  if (P.any_window && with_scatter)
     LAUNCH(SMILI_S_SCATTER, k_scatter, P.h_regs.size(), kThreads, 0, P.d_regs.p, P.d_units.p, P.d_pix.p, P.d_x.p,
            P.d_I2d.p);
   if (P.npt == 0) return;
@@ -834,8 +864,11 @@ static void launch_forward_t(Plan& P) {
          Vv, P.d_Vres.p, P.bin, beta);
 }
 
+// rows: 0 = spreading and column pass only, 1 = + row pass into grad2d,
+// 2 = + row pass adding the chi-square gradient to gradreg into grad.
+// fin: the final sums ride on the spreading launch.
 template <typename R>
-static void launch_adjoint_t(Plan& P, bool with_rows) {
+static void launch_adjoint_t(Plan& P, int rows, const FinArgs* fin = nullptr) {
   using V = typename Cx<R>::T;
   hipStream_t stream = P.stream;
   const bool prof = P.prof;
@@ -850,49 +883,127 @@ static void launch_adjoint_t(Plan& P, bool with_rows) {
   const R* ey = reinterpret_cast<const R*>(P.fp64 ? (const void*)P.d_eyd.p : (const void*)P.d_ey.p);
   V* gA = reinterpret_cast<V*>(P.d_gridA.p);
   V* gB = reinterpret_cast<V*>(P.d_gridB.p);
-  LAUNCH(SMILI_S_SPREAD, k_spread<R>, P.h_spread.size(), kThreads, 0, P.d_spread.p, P.d_cell.p, ex, ey,
-         P.d_Vres.p, gB, beta);
+  if (fin)
+    LAUNCH(SMILI_S_SPREAD, k_spread_fin<R>, P.h_spread.size() + P.probs.size(), kThreads, 0,
+           (int)P.h_spread.size(), P.d_spread.p, P.d_cell.p, ex, ey, P.d_Vres.p, gB, beta, *fin);
+  else
+    LAUNCH(SMILI_S_SPREAD, k_spread<R>, P.h_spread.size(), kThreads, 0, P.d_spread.p, P.d_cell.p, ex, ey,
+           P.d_Vres.p, gB, beta);
   LAUNCH(SMILI_S_T1_COLS, k_t1_cols<R>, xcd_grid(P.h_cols.size()), P.threads_cols, P.lds_cols, P.d_cols.p,
          (int)P.h_cols.size(), tw, gB, gA);
-  if (with_rows)
+  if (rows)
     LAUNCH(SMILI_S_T1_ROWS, k_t1_rows<R>, P.h_t1rows.size(), P.threads_rows, P.lds_rows, P.d_t1rows.p, tw, corr,
-           gA, P.d_grad2d.p);
+           gA, P.d_grad2d.p, P.d_grad.p, P.d_greg.p, P.d_pixinv.p, rows == 2 ? 1 : 0);
+}
+
+// the row pass alone (after the regularizer branch has joined)
+template <typename R>
+static void launch_rows_t(Plan& P) {
+  using V = typename Cx<R>::T;
+  hipStream_t stream = P.stream;
+  const bool prof = P.prof;
+  auto& prof_ev = P.prof_ev;
+  auto& prof_used = P.prof_used;
+  int& launches = P.launches;
+  if (P.npt == 0) return;
+  const V* tw = reinterpret_cast<const V*>(P.fp64 ? (const void*)P.d_twd.p : (const void*)P.d_tw.p);
+  const R* corr = reinterpret_cast<const R*>(P.fp64 ? (const void*)P.d_corrd.p : (const void*)P.d_corr.p);
+  V* gA = reinterpret_cast<V*>(P.d_gridA.p);
+  LAUNCH(SMILI_S_T1_ROWS, k_t1_rows<R>, P.h_t1rows.size(), P.threads_rows, P.lds_rows, P.d_t1rows.p, tw, corr,
+         gA, P.d_grad2d.p, P.d_grad.p, P.d_greg.p, P.d_pixinv.p, 1);
 }
 
 void Plan::launch_forward() {
-  if (fp64) launch_forward_t<double>(*this);
-  else launch_forward_t<float>(*this);
+  if (fp64) launch_forward_t<double>(*this, true);
+  else launch_forward_t<float>(*this, true);
 }
 
 void Plan::launch_adjoint() {
-  if (fp64) launch_adjoint_t<double>(*this, true);
-  else launch_adjoint_t<float>(*this, true);
+  if (fp64) launch_adjoint_t<double>(*this, 1);
+  else launch_adjoint_t<float>(*this, 1);
 }
 
+// One evaluation.  Default (one stream):
+//   scatter | moments frame_mean | t2_rows t2_cols interp |
+//   chisq + reg (one launch) | spread + finalize (one launch) | t1_cols | t1_rows
+// The regularizers depend on x (and the scattered image) only and ride on
+// the chi-square launch; the final sums need the chi-square and
+// regularizer partials and ride on the spreading launch.  With two
+// branches (large plans) the regularizers and the sums run on a second
+// stream instead:
+//   main:  scatter | t2_rows t2_cols interp chisq | spread t1_cols | t1_rows
+//   side:          | moments frame_mean reg        | finalize (after chisq)
+// Both forms are captured into one graph; with per-stage profiling every
+// stage runs in order on the main stream.
 void Plan::launch_eval() {
   launches = 0;
-  launch_forward();
-  if (any_moments) {
-    LAUNCH(SMILI_S_MOMENTS, k_moments, h_regs.size(), kThreads, 0, d_regs.p, d_units.p, d_pix.p, d_x.p, d_mpart.p);
-    LAUNCH(SMILI_S_MOMENTS_REDUCE, k_moments_reduce, units.size(), 64, 0, d_units.p, d_unit_reg_begin.p, d_mpart.p, d_mom.p);
+  const bool fork = two_branches && !prof;
+  const hipStream_t main_stream = stream;
+  const hipStream_t side_stream = fork ? side : stream;
+  const RegArgs reg_args{d_regs.p, d_units.p, d_pix.p, d_x.p, d_I2d.p, d_mom.p, d_movies.p,
+                         d_greg.p, d_grad.p, d_rpart.p};
+  const FinArgs fin_args{d_probs.p, d_units.p, d_regs.p, d_cpart.p, d_rpart.p, d_mom.p, d_terms.p};
+  const bool fused = !two_branches && npt > 0;
+  if (any_window)
+    LAUNCH(SMILI_S_SCATTER, k_scatter, h_regs.size(), kThreads, 0, d_regs.p, d_units.p, d_pix.p, d_x.p, d_I2d.p);
+  if (fork) {
+    HIP_CHECK(hipEventRecord(ev_fork, main_stream));
+    HIP_CHECK(hipStreamWaitEvent(side_stream, ev_fork, 0));
+  }
+  {
+    const hipStream_t stream = side_stream;
+    if (any_moments) {
+      LAUNCH(SMILI_S_MOMENTS, k_moments, h_regs.size(), kThreads, 0, d_regs.p, d_units.p, d_pix.p, d_x.p,
+             d_mpart.p);
+      LAUNCH(SMILI_S_MOMENTS_REDUCE, k_moments_reduce, units.size(), 64, 0, d_units.p, d_unit_reg_begin.p,
+             d_mpart.p, d_mom.p);
+    }
+    if (any_ri)
+      for (size_t mi = 0; mi < movies.size(); ++mi)
+        LAUNCH(SMILI_S_FRAME_MEAN, k_frame_mean, (movies[mi].npix + kThreads - 1) / kThreads, kThreads, 0,
+               d_movies.p + mi, d_x.p);
+    if (!fused) LAUNCH(SMILI_S_REG, k_reg, h_regs.size(), kThreads, 0, reg_args);
+    if (fork) HIP_CHECK(hipEventRecord(ev_reg, side_stream));
+  }
+  if (fp64) launch_forward_t<double>(*this, false);
+  else launch_forward_t<float>(*this, false);
+  if (npt > 0) {
+    const int nch = (int)h_chunks.size();
+    const size_t grid = h_chunks.size() + (fused ? h_regs.size() : 0);
+    if (fp64) {
+      if (fused)
+        LAUNCH(SMILI_S_CHISQ, k_chisq_reg<double2>, grid, kThreads, 0, nch, d_chunks.p, d_slots.p, d_fcv.p, d_amp.p,
+               d_cp.p, d_ca.p, d_V.p, d_Vres.p, d_cpart.p, reg_args);
+      else
+        LAUNCH(SMILI_S_CHISQ, k_chisq<double2>, grid, kThreads, 0, d_chunks.p, d_slots.p, d_fcv.p, d_amp.p,
+               d_cp.p, d_ca.p, d_V.p, d_Vres.p, d_cpart.p);
+    } else {
+      const float2* Vf = reinterpret_cast<const float2*>(d_V.p);
+      if (fused)
+        LAUNCH(SMILI_S_CHISQ, k_chisq_reg<float2>, grid, kThreads, 0, nch, d_chunks.p, d_slots.p, d_fcv.p, d_amp.p,
+               d_cp.p, d_ca.p, Vf, d_Vres.p, d_cpart.p, reg_args);
+      else
+        LAUNCH(SMILI_S_CHISQ, k_chisq<float2>, grid, kThreads, 0, d_chunks.p, d_slots.p, d_fcv.p, d_amp.p,
+               d_cp.p, d_ca.p, Vf, d_Vres.p, d_cpart.p);
+    }
+  }
+  if (fork) {
+    HIP_CHECK(hipEventRecord(ev_chisq, main_stream));
+    HIP_CHECK(hipStreamWaitEvent(side_stream, ev_chisq, 0));
+  }
+  if (!fused) {
+    const hipStream_t stream = side_stream;
+    LAUNCH(SMILI_S_FINALIZE, k_finalize, probs.size(), kThreads, 0, fin_args);
+    if (fork) HIP_CHECK(hipEventRecord(ev_fin, side_stream));
   }
   if (npt > 0) {
-    if (fp64)
-      LAUNCH(SMILI_S_CHISQ, k_chisq<double2>, h_chunks.size(), kThreads, 0, d_chunks.p, d_slots.p, d_fcv.p, d_amp.p,
-             d_cp.p, d_ca.p, d_V.p, d_Vres.p, d_cpart.p);
-    else
-      LAUNCH(SMILI_S_CHISQ, k_chisq<float2>, h_chunks.size(), kThreads, 0, d_chunks.p, d_slots.p, d_fcv.p, d_amp.p,
-             d_cp.p, d_ca.p, reinterpret_cast<const float2*>(d_V.p), d_Vres.p, d_cpart.p);
-    launch_adjoint();
+    if (fp64) launch_adjoint_t<double>(*this, 0, fused ? &fin_args : nullptr);
+    else launch_adjoint_t<float>(*this, 0, fused ? &fin_args : nullptr);
   }
-  if (any_ri)
-    for (size_t mi = 0; mi < movies.size(); ++mi)
-      LAUNCH(SMILI_S_FRAME_MEAN, k_frame_mean, (movies[mi].npix + kThreads - 1) / kThreads, kThreads, 0,
-             d_movies.p + mi, d_x.p);
-  LAUNCH(SMILI_S_REG, k_reg, h_regs.size(), kThreads, 0, d_regs.p, d_units.p, d_pix.p, d_x.p, d_I2d.p, d_grad2d.p,
-         d_mom.p, d_movies.p, d_grad.p, d_rpart.p);
-  LAUNCH(SMILI_S_FINALIZE, k_finalize, probs.size(), kThreads, 0, d_probs.p, d_units.p, d_regs.p, d_cpart.p,
-         d_rpart.p, d_mom.p, d_terms.p);
+  if (fork) HIP_CHECK(hipStreamWaitEvent(main_stream, ev_reg, 0));
+  if (fp64) launch_rows_t<double>(*this);
+  else launch_rows_t<float>(*this);
+  if (fork) HIP_CHECK(hipStreamWaitEvent(main_stream, ev_fin, 0));
 }
 
 void Plan::eval() {

@@ -57,21 +57,34 @@ __device__ __forceinline__ void lds_add2(double2* R, int s, double re, double im
 }
 
 template <typename VT>
-__global__ void __launch_bounds__(kThreads)
-k_chisq(const Chunk* __restrict__ chunks, const int* __restrict__ slots,
-        const FcvItem* __restrict__ fcv, const AmpItem* __restrict__ amp,
-        const CpItem* __restrict__ cp, const CaItem* __restrict__ ca,
-        const VT* __restrict__ V, double2* __restrict__ Vres,
-        double* __restrict__ partial) {
+__device__ void chisq_block(int bid, const Chunk* __restrict__ chunks, const int* __restrict__ slots,
+                            const FcvItem* __restrict__ fcv, const AmpItem* __restrict__ amp,
+                            const CpItem* __restrict__ cp, const CaItem* __restrict__ ca,
+                            const VT* __restrict__ V, double2* __restrict__ Vres,
+                            double* __restrict__ partial) {
   __shared__ double2 Vl[kMaxSlots];
   __shared__ double2 Rl[kMaxSlots];
+  __shared__ double2 Al[kMaxSlots];      // per-point quantities shared by the chunk's data
   __shared__ double red[kThreads / 64];
-  const Chunk ch = chunks[blockIdx.x];
+  const Chunk ch = chunks[bid];
   const int* sl = slots + ch.slot_off;
+  // stage the chunk's model visibilities and, once per point rather than
+  // once per reference: amp |V| and 1/|V|; closure phase arg V and 1/|V|^2;
+  // closure amplitude log|V|^2 and 1/|V|^2
   for (int s = threadIdx.x; s < ch.nslots; s += blockDim.x) {
-    const VT m = V[sl[s]];
-    Vl[s] = make_double2((double)m.x, (double)m.y);
+    const VT mv = V[sl[s]];
+    const double2 m = make_double2((double)mv.x, (double)mv.y);
+    Vl[s] = m;
     Rl[s] = make_double2(0.0, 0.0);
+    const double a2 = m.x * m.x + m.y * m.y;
+    if (ch.type == DT_AMP) {
+      const double mod = sqrt(a2);
+      Al[s] = make_double2(mod, 1.0 / mod);
+    } else if (ch.type == DT_CP) {
+      Al[s] = make_double2(atan2(m.y, m.x), 1.0 / a2);
+    } else if (ch.type == DT_CA) {
+      Al[s] = make_double2(log(a2), 1.0 / a2);
+    }
   }
   __syncthreads();
   double acc = 0.0;
@@ -92,16 +105,16 @@ k_chisq(const Chunk* __restrict__ chunks, const int* __restrict__ slots,
       const AmpItem d = amp[ch.start + i];
       const int s = d.slot & 0x7fff;
       const double2 m = Vl[s];
-      const double mod = sqrt(m.x * m.x + m.y * m.y);
-      const double r = d.amp - mod;
+      const double2 al = Al[s];
+      const double r = d.amp - al.x;
       acc += r * r * d.w;
-      const double f = -2.0 * r * d.w / mod;
+      const double f = -2.0 * r * d.w * al.y;
       lds_add2(Rl, s, f * m.x, f * m.y);
     }
   } else if (ch.type == DT_CP) {
     for (int i = threadIdx.x; i < ch.count; i += blockDim.x) {
       const CpItem d = cp[ch.start + i];
-      double2 m[3];
+      double2 m[3], al[3];
       double sg[3], model = 0.0;
       int s[3];
 #pragma unroll
@@ -109,7 +122,8 @@ k_chisq(const Chunk* __restrict__ chunks, const int* __restrict__ slots,
         s[k] = d.slot[k] & 0x7fff;
         sg[k] = (d.slot[k] & kSlotNeg) ? -1.0 : 1.0;
         m[k] = Vl[s[k]];
-        model += atan2(m[k].y, m[k].x) * sg[k];
+        al[k] = Al[s[k]];
+        model += al[k].x * sg[k];
       }
       const double r = d.cp - model;
       double sr, cr;
@@ -118,30 +132,29 @@ k_chisq(const Chunk* __restrict__ chunks, const int* __restrict__ slots,
       const double f = -2.0 * sr * d.w;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const double a = f / (m[k].x * m[k].x + m[k].y * m[k].y) * sg[k];
+        const double a = f * al[k].y * sg[k];
         lds_add2(Rl, s[k], -a * m[k].y, a * m[k].x);
       }
     }
   } else {
     for (int i = threadIdx.x; i < ch.count; i += blockDim.x) {
       const CaItem d = ca[ch.start + i];
-      double2 m[4];
-      double a2[4];
+      double2 m[4], al[4];
       int s[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         s[k] = d.slot[k] & 0x7fff;       // signs are ignored (fftlib.f90:549-553)
         m[k] = Vl[s[k]];
-        a2[k] = m[k].x * m[k].x + m[k].y * m[k].y;
+        al[k] = Al[s[k]];
       }
       // log|V| = 0.5 log|V|^2
-      const double model = 0.5 * (log(a2[0]) + log(a2[1]) - log(a2[2]) - log(a2[3]));
+      const double model = 0.5 * (al[0].x + al[1].x - al[2].x - al[3].x);
       const double r = d.ca - model;
       acc += r * r * d.w;
       const double f = -2.0 * r * d.w;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const double a = (k < 2 ? f : -f) / a2[k];
+        const double a = (k < 2 ? f : -f) * al[k].y;
         lds_add2(Rl, s[k], a * m[k].x, a * m[k].y);
       }
     }
@@ -155,7 +168,17 @@ k_chisq(const Chunk* __restrict__ chunks, const int* __restrict__ slots,
   }
   double v[1] = {acc};
   block_sum<1>(v, red);
-  if (threadIdx.x == 0) partial[blockIdx.x] = v[0];
+  if (threadIdx.x == 0) partial[bid] = v[0];
+}
+
+template <typename VT>
+__global__ void __launch_bounds__(kThreads)
+k_chisq(const Chunk* __restrict__ chunks, const int* __restrict__ slots,
+        const FcvItem* __restrict__ fcv, const AmpItem* __restrict__ amp,
+        const CpItem* __restrict__ cp, const CaItem* __restrict__ ca,
+        const VT* __restrict__ V, double2* __restrict__ Vres,
+        double* __restrict__ partial) {
+  chisq_block<VT>(blockIdx.x, chunks, slots, fcv, amp, cp, ca, V, Vres, partial);
 }
 
 // ---------------------------------------------------------------------------
@@ -306,18 +329,37 @@ k_frame_mean(const MovieDev* __restrict__ movies, const double* __restrict__ x) 
   Mv.iavg[k] = a;
 }
 
-// Per-pixel regularizers + final gradient assembly:
-// grad = gradchisq + reg_scale * gradreg (fftim2d.f90:553-554); for a movie
-// frame also the Rt / Ri terms of calc_cost_dynamical (image3d.f90:358-388):
-// grad = gradchisq + (gradreg / nz + graddyn).
-__global__ void __launch_bounds__(kThreads)
-k_reg(const RegItem* __restrict__ items, const UnitDev* __restrict__ units,
-      const int* __restrict__ pix, const double* __restrict__ x,
-      const double* __restrict__ I2dbuf, const double* __restrict__ grad2d,
-      const UnitMoments* __restrict__ mom, const MovieDev* __restrict__ movies,
-      double* __restrict__ grad, double* __restrict__ rpart) {
+// Per-pixel regularizers: gradreg = reg_scale * gradient (reg_scale = 1/nz
+// for a movie frame, calc_cost_reg3d image3d.f90:255) plus, for a movie
+// frame, the Rt / Ri terms of calc_cost_dynamical (image3d.f90:358-388).
+// It depends on x only, so it runs beside the transforms; k_t1_rows forms
+// grad = gradchisq + gradreg (fftim2d.f90:553-554).
+struct RegArgs {
+  const RegItem* items;
+  const UnitDev* units;
+  const int* pix;
+  const double* x;
+  const double* I2dbuf;
+  const UnitMoments* mom;
+  const MovieDev* movies;
+  double* greg;
+  double* grad;
+  double* rpart;
+};
+
+__device__ void reg_block(int bid, const RegArgs& A) {
+  const RegItem* __restrict__ items = A.items;
+  const UnitDev* __restrict__ units = A.units;
+  const int* __restrict__ pix = A.pix;
+  const double* __restrict__ x = A.x;
+  const double* __restrict__ I2dbuf = A.I2dbuf;
+  const UnitMoments* __restrict__ mom = A.mom;
+  const MovieDev* __restrict__ movies = A.movies;
+  double* __restrict__ greg = A.greg;
+  double* __restrict__ grad = A.grad;
+  double* __restrict__ rpart = A.rpart;
   __shared__ double red[kRegTerms * (kThreads / 64)];
-  const RegItem it = items[blockIdx.x];
+  const RegItem it = items[bid];
   const UnitDev U = units[it.unit];
   const double* I2d = U.identity ? (x + U.x_off) : (I2dbuf + U.img_off);
   double v[kRegTerms];
@@ -428,12 +470,31 @@ k_reg(const RegItem* __restrict__ items, const UnitDev* __restrict__ units,
         gt += w * (2.0 * d) / lz;
       }
     }
-    const double gc = (U.m > 0) ? grad2d[U.img_off + p] : 0.0;
-    grad[U.x_off + k] = gc + (U.reg_scale * g + gt);
+    // units with uv points: k_t1_rows adds the chi-square gradient; units
+    // without (movie frames with no data) have none
+    const double gr = U.reg_scale * g + gt;
+    if (U.m > 0) greg[U.x_off + k] = gr;
+    else grad[U.x_off + k] = 0.0 + gr;
   }
   block_sum<kRegTerms>(v, red);
   if (threadIdx.x == 0)
-    for (int k = 0; k < kRegTerms; ++k) rpart[(long long)blockIdx.x * kRegTerms + k] = v[k];
+    for (int k = 0; k < kRegTerms; ++k) rpart[(long long)bid * kRegTerms + k] = v[k];
+}
+
+__global__ void __launch_bounds__(kThreads) k_reg(RegArgs A) { reg_block(blockIdx.x, A); }
+
+// The chi-square chunks and the regularizer items in one launch: the two
+// are independent (V vs x), so the regularizer work-groups fill the CUs the
+// chunks leave idle instead of adding a launch to the dependent chain.
+template <typename VT>
+__global__ void __launch_bounds__(kThreads)
+k_chisq_reg(int nchunks, const Chunk* __restrict__ chunks, const int* __restrict__ slots,
+            const FcvItem* __restrict__ fcv, const AmpItem* __restrict__ amp,
+            const CpItem* __restrict__ cp, const CaItem* __restrict__ ca,
+            const VT* __restrict__ V, double2* __restrict__ Vres,
+            double* __restrict__ partial, RegArgs A) {
+  if ((int)blockIdx.x < nchunks) chisq_block<VT>(blockIdx.x, chunks, slots, fcv, amp, cp, ca, V, Vres, partial);
+  else reg_block(blockIdx.x - nchunks, A);
 }
 
 // ---------------------------------------------------------------------------
@@ -504,14 +565,35 @@ __device__ void finalize_problem(const ProbDev& P, const UnitDev* __restrict__ u
   }
 }
 
+struct FinArgs {
+  const ProbDev* probs;
+  const UnitDev* units;
+  const RegItem* reg_items;
+  const double* cpart;
+  const double* rpart;
+  const UnitMoments* mom;
+  double* terms;
+};
+
 // one block per problem
-__global__ void __launch_bounds__(kThreads)
-k_finalize(const ProbDev* __restrict__ probs, const UnitDev* __restrict__ units,
-           const RegItem* __restrict__ reg_items, const double* __restrict__ cpart,
-           const double* __restrict__ rpart, const UnitMoments* __restrict__ mom,
-           double* __restrict__ terms) {
+__global__ void __launch_bounds__(kThreads) k_finalize(FinArgs F) {
   __shared__ double red[16 * (kThreads / 64)];
-  finalize_problem(probs[blockIdx.x], units, reg_items, cpart, rpart, mom, terms, red);
+  finalize_problem(F.probs[blockIdx.x], F.units, F.reg_items, F.cpart, F.rpart, F.mom, F.terms, red);
+}
+
+// Spreading work items followed by one finalize work-group per problem: the
+// sums need only the chi-square and regularizer partials, complete before
+// this launch, so they overlap the spreading instead of ending the chain.
+template <typename R>
+__global__ void __launch_bounds__(kThreads)
+k_spread_fin(int nspread, const SpreadItem* __restrict__ items, const int* __restrict__ pt_cell,
+             const R* __restrict__ pt_ex, const R* __restrict__ pt_ey, const double2* __restrict__ Vres,
+             typename Cx<R>::T* __restrict__ gridB, R beta, FinArgs F) {
+  __shared__ double red[16 * (kThreads / 64)];
+  if ((int)blockIdx.x < nspread)
+    spread_block<R>(blockIdx.x, items, pt_cell, pt_ex, pt_ey, Vres, gridB, beta);
+  else
+    finalize_problem(F.probs[blockIdx.x - nspread], F.units, F.reg_items, F.cpart, F.rpart, F.mom, F.terms, red);
 }
 
 }  // namespace smili

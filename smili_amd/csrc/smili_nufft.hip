@@ -330,10 +330,10 @@ k_interp(const SubProb* __restrict__ sps, const typename Cx<R>::T* __restrict__ 
 // (k_t1_cols clears what it consumes).
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ void __launch_bounds__(kThreads)
-k_spread(const SpreadItem* __restrict__ items, const int* __restrict__ pt_cell,
-         const R* __restrict__ pt_ex, const R* __restrict__ pt_ey,
-         const double2* __restrict__ Vres, typename Cx<R>::T* __restrict__ gridB, R beta) {
+__device__ void spread_block(int bid, const SpreadItem* __restrict__ items, const int* __restrict__ pt_cell,
+                             const R* __restrict__ pt_ex, const R* __restrict__ pt_ey,
+                             const double2* __restrict__ Vres, typename Cx<R>::T* __restrict__ gridB,
+                             R beta) {
   using V = typename Cx<R>::T;
   constexpr int T = kCellBlock + kW - 1;
   constexpr int NW = kThreads / 64;
@@ -341,7 +341,7 @@ k_spread(const SpreadItem* __restrict__ items, const int* __restrict__ pt_cell,
   __shared__ R wxs[NW][kW][65], wys[NW][kW][65];   // padded: the 7 rows sit in distinct banks
   __shared__ V cs[NW][64];
   __shared__ int cl[NW][64];
-  const SpreadItem it = items[blockIdx.x];
+  const SpreadItem it = items[bid];
   for (int c = threadIdx.x; c < T * T; c += blockDim.x) tile[c] = cmk<V>(0, 0);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int base = it.pt_begin + w * 64;
@@ -399,6 +399,14 @@ k_spread(const SpreadItem* __restrict__ items, const int* __restrict__ pt_cell,
   }
 }
 
+template <typename R>
+__global__ void __launch_bounds__(kThreads)
+k_spread(const SpreadItem* __restrict__ items, const int* __restrict__ pt_cell,
+         const R* __restrict__ pt_ex, const R* __restrict__ pt_ey,
+         const double2* __restrict__ Vres, typename Cx<R>::T* __restrict__ gridB, R beta) {
+  spread_block<R>(blockIdx.x, items, pt_cell, pt_ex, pt_ey, Vres, gridB, beta);
+}
+
 // ---------------------------------------------------------------------------
 // Type-1 step 2: FFT along y of `count` full grid columns; keeps the ny
 // output rows that are image modes (compact, into H = gridA) and clears the
@@ -440,13 +448,15 @@ k_t1_cols(const FftItem* __restrict__ items, int nitems, const typename Cx<R>::T
 
 // ---------------------------------------------------------------------------
 // Type-1 step 3: FFT along x of `count` compact rows, keep the nx modes,
-// deconvolve, real part -> grad2d (FP64).
+// deconvolve, real part -> the chi-square gradient (FP64): either stored as
+// a 2-D image (grad2d) or added to the regularizer gradient into grad.
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ void __launch_bounds__(kThreads)
 k_t1_rows(const FftItem* __restrict__ items, const typename Cx<R>::T* __restrict__ tw,
           const R* __restrict__ corr, const typename Cx<R>::T* __restrict__ H,
-          double* __restrict__ grad2d) {
+          double* __restrict__ grad2d, double* __restrict__ grad, const double* __restrict__ greg,
+          const int* __restrict__ pixinv, int assemble) {
   using V = typename Cx<R>::T;
   extern __shared__ __align__(16) unsigned char smraw[];
   V* sm = reinterpret_cast<V*>(smraw);
@@ -468,8 +478,16 @@ k_t1_rows(const FftItem* __restrict__ items, const typename Cx<R>::T* __restrict
     const int iy = it.first + b;
     int c = ix - hx;
     if (c < 0) c += n;
-    grad2d[it.img_off + (long long)iy * it.nx + ix] =
-        (double)Rv[b * n + c].x * (double)cx[ix] * (double)cy[iy];
+    const double val = (double)Rv[b * n + c].x * (double)cx[ix] * (double)cy[iy];
+    const long long p = (long long)iy * it.nx + ix;
+    if (!assemble) {
+      grad2d[it.img_off + p] = val;
+    } else {
+      // gradient assembly: grad = gradchisq + gradreg (fftim2d.f90:553-554),
+      // gradreg from k_reg; window pixels through the inverse pixel map
+      const int k = it.identity ? (int)p : pixinv[it.img_off + p];
+      if (k >= 0) grad[it.x_off + k] = val + greg[it.x_off + k];
+    }
   }
 }
 
