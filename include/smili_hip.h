@@ -153,7 +153,9 @@ typedef struct smili_tune {
   int32_t branches;     /* 1: every stage in order on one stream; 2: the
                            regularizers and final sums on a second stream
                            beside the transforms; 0 (default): 2 for plans
-                           of at least 1e6 uv points, else 1 */
+                           of at least 2e5 uv points or 16 images, else 1 */
+  int32_t fft_radix;    /* largest power-of-two radix of the grid FFTs: 2, 4
+                           or 8 (0 = default, 4) */
 } smili_tune;
 
 int smili_abi_version(void);

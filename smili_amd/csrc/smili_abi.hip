@@ -181,7 +181,7 @@ struct Plan {
   smili_tune tune{};
   int bin = 16, sub_points = 256, col_group = 2, chunk = 2048, rows_per_block = 1;
   bool fp64 = false;                       // NUFFT arithmetic in FP64 instead of FP32
-  int threads_rows = 64, threads_cols = 128;
+  int threads_rows = 64, threads_cols = 128, fft_radix = 4;
 
   std::vector<UnitDev> units;
   std::vector<ProbDev> probs;
@@ -322,6 +322,8 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   require(sub_points >= 32 && sub_points <= 65536, "sub_points must be in [32, 65536]");
   require(col_group >= 0 && col_group <= 16, "col_group must be in [0, 16]");
   require(chunk >= 64, "chunk must be >= 64");
+  require(tune.fft_radix == 0 || tune.fft_radix == 2 || tune.fft_radix == 4 || tune.fft_radix == 8,
+          "fft_radix must be 0, 2, 4 or 8");
   tune.use_graph = (tune.use_graph < 0) ? 0 : 1;
   require(tune.branches >= 0 && tune.branches <= 2, "branches must be 0, 1 or 2");
 
@@ -676,8 +678,12 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   // FFT work-groups: about 512 points per row group (one 64-thread wave of
   // radix-8 butterflies), 64 threads per column of a column group
   rows_per_block = std::max(1, std::min(8, 512 / std::max(max_nf1, 1)));
-  threads_rows = 64;
-  threads_cols = std::min(kThreads, 64 * col_group);
+  // FFT radix and work-group sizes: one thread per butterfly of a stage
+  // (n / maxr per transform), rounded up to whole waves, at most 256
+  fft_radix = tune.fft_radix > 0 ? tune.fft_radix : 4;
+  auto wave_round = [](long long t) { return (int)std::min<long long>(kThreads, std::max<long long>(64, (t + 63) / 64 * 64)); };
+  threads_rows = wave_round((long long)rows_per_block * max_nf1 / fft_radix);
+  threads_cols = wave_round((long long)col_group * max_nf2 / fft_radix);
   for (size_t ui = 0; ui < units.size(); ++ui) {
     const UnitDev& U = units[ui];
     unit_reg_begin[ui] = (int)h_regs.size();
@@ -692,6 +698,7 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
       f.tw2 = plans[U.fft2].tw_off;
       f.cx_off = U.cx_off; f.cy_off = U.cy_off;
       f.x_off = U.x_off; f.identity = U.identity;
+      f.maxr = fft_radix;
       f.grid_off = U.grid_off; f.img_off = U.img_off;
       return f;
     };

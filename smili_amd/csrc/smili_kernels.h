@@ -66,6 +66,7 @@ struct FftItem {
   int first, count, nx, ny, nf1, nf2;
   int tw1, tw2;                      // twiddle tables of length nf1, nf2
   int cx_off, cy_off, x_off, identity;
+  int maxr, pad;                     // largest power-of-two radix of the LDS FFT
   long long grid_off, img_off;
 };
 struct SubProb { int bin_x, bin_y, start, count, nf1, nf2; long long grid_off; };

@@ -97,15 +97,18 @@ template <typename V> __device__ __forceinline__ void dft8(V* v) {
 // ---------------------------------------------------------------------------
 // LDS Stockham FFT (autosort, +i sign) of nb transforms of length n stored
 // at a + b*stride.  Uses scratch s of the same shape and the twiddle table
-// tw[t] = exp(+2 pi i t / n) staged in LDS.  Radices in the order of the
-// host plan: all 8s, then 4, 2, 3, 5.  Returns the buffer that holds the
-// result.  All threads of the block must call it.
+// tw[t] = exp(+2 pi i t / n) staged in LDS.  Radices: the largest power of
+// two up to maxr that divides what is left, then 3, 5 -- a smaller maxr
+// means more stages but more butterflies per stage (more threads busy on a
+// short transform).  Returns the buffer that holds the result.  All
+// threads of the block must call it.
 // ---------------------------------------------------------------------------
 template <typename V>
-__device__ V* lds_fft(V* a, V* s, int nb, int stride, int n, const V* tw) {
+__device__ V* lds_fft(V* a, V* s, int nb, int stride, int n, const V* tw, int maxr) {
   int Ns = 1, rem = n;
   while (rem > 1) {
-    const int r = (rem % 8 == 0) ? 8 : (rem % 4 == 0) ? 4 : (rem % 2 == 0) ? 2 : (rem % 3 == 0) ? 3 : 5;
+    const int r = (maxr >= 8 && rem % 8 == 0) ? 8 : (maxr >= 4 && rem % 4 == 0) ? 4 : (rem % 2 == 0) ? 2
+                : (rem % 3 == 0) ? 3 : 5;
     rem /= r;
     const int nbf = n / r;
     const int total = nbf * nb;
@@ -219,7 +222,7 @@ k_t2_rows(const FftItem* __restrict__ items, const typename Cx<R>::T* __restrict
     A[t] = val;
   }
   __syncthreads();
-  V* Rv = lds_fft(A, B, nb, n, n, W);
+  V* Rv = lds_fft(A, B, nb, n, n, W, it.maxr);
   const int hy = it.ny / 2;
   for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
     const int b = t / n, c = t - b * n;
@@ -258,7 +261,7 @@ k_t2_cols(const FftItem* __restrict__ items, int nitems, const typename Cx<R>::T
     A[b * stride + r] = val;
   }
   __syncthreads();
-  V* Rv = lds_fft(A, B, nb, stride, n, W);
+  V* Rv = lds_fft(A, B, nb, stride, n, W, it.maxr);
   for (int t = threadIdx.x; t < nb * n; t += blockDim.x) {
     const int r = t / nb, b = t - r * nb;
     g[(long long)r * it.nf1 + it.first + b] = Rv[b * stride + r];
@@ -435,7 +438,7 @@ k_t1_cols(const FftItem* __restrict__ items, int nitems, const typename Cx<R>::T
     *src = cmk<V>(0, 0);
   }
   __syncthreads();
-  V* Rv = lds_fft(A, B, nb, stride, n, W);
+  V* Rv = lds_fft(A, B, nb, stride, n, W, it.maxr);
   const int hy = it.ny / 2;
   V* h = H + it.grid_off;
   for (int t = threadIdx.x; t < nb * it.ny; t += blockDim.x) {
@@ -469,7 +472,7 @@ k_t1_rows(const FftItem* __restrict__ items, const typename Cx<R>::T* __restrict
   const V* h = H + it.grid_off + (long long)it.first * n;
   for (int t = threadIdx.x; t < nb * n; t += blockDim.x) A[t] = h[t];
   __syncthreads();
-  V* Rv = lds_fft(A, B, nb, n, n, W);
+  V* Rv = lds_fft(A, B, nb, n, n, W, it.maxr);
   const R* cx = corr + it.cx_off;
   const R* cy = corr + it.cy_off;
   const int hx = it.nx / 2;
