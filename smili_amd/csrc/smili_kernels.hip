@@ -590,10 +590,13 @@ k_spread_fin(int nspread, const SpreadItem* __restrict__ items, const int* __res
              const R* __restrict__ pt_ex, const R* __restrict__ pt_ey, const double2* __restrict__ Vres,
              typename Cx<R>::T* __restrict__ gridB, R beta, FinArgs F) {
   __shared__ double red[16 * (kThreads / 64)];
-  if ((int)blockIdx.x < nspread)
-    spread_block<R>(blockIdx.x, items, pt_cell, pt_ex, pt_ey, Vres, gridB, beta);
+  // the finalize work-groups first: dispatched with the launch, they
+  // finish under the spreading work instead of trailing it
+  const int nfin = (int)gridDim.x - nspread;
+  if ((int)blockIdx.x < nfin)
+    finalize_problem(F.probs[blockIdx.x], F.units, F.reg_items, F.cpart, F.rpart, F.mom, F.terms, red);
   else
-    finalize_problem(F.probs[blockIdx.x - nspread], F.units, F.reg_items, F.cpart, F.rpart, F.mom, F.terms, red);
+    spread_block<R>(blockIdx.x - nfin, items, pt_cell, pt_ex, pt_ey, Vres, gridB, beta);
 }
 
 }  // namespace smili
