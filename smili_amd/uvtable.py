@@ -281,3 +281,34 @@ def map_clean(table, image, restore=False, errorweight=-2, istokes=0, ifreq=0, t
     if hasattr(out, "update_fits"):
         out.update_fits()
     return out
+
+
+# ----------------------------------------------------------------------------
+# CSV tables (vistable.py:3461-3481, bstable.py:1224-1244, catable.py:1196-1216)
+# ----------------------------------------------------------------------------
+def _read_table(filename, **args):
+    import pandas as pd
+    t = pd.read_csv(filename, **args)
+    if "utc" in t.columns:
+        t["utc"] = pd.to_datetime(t["utc"])
+    return t
+
+
+def read_vistable(filename, **args):
+    """A visibility table from a CSV file (pd.read_csv; utc parsed to
+    datetimes), as smili.uvdata.read_vistable (vistable.py:3461-3481).
+    Columns the imager reads: u, v, amp, phase (deg), sigma (and frmidx for
+    movies)."""
+    return _read_table(filename, **args)
+
+
+def read_bstable(filename, **args):
+    """A closure-phase table from a CSV file (bstable.py:1224-1244); the
+    imager reads u12, v12, u23, v23, u31, v31, amp, phase (deg), sigma."""
+    return _read_table(filename, **args)
+
+
+def read_catable(filename, **args):
+    """A closure-amplitude table from a CSV file (catable.py:1196-1216); the
+    imager reads u1..v4, logamp, logsigma."""
+    return _read_table(filename, **args)
