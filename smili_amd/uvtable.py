@@ -9,7 +9,10 @@ of SMILI's VisTable methods (smili/uvdata/uvtable/vistable.py):
 * ``fit_beam`` (2539-2625): Gaussian beam from the weighted second moments
   of the uv coverage;
 * ``map_beam / map_residual / map_clean`` (2627-2800): dirty beam and
-  residual map by a type-1 transform of the weighted visibilities.
+  residual map by a type-1 transform of the weighted visibilities;
+* ``make_bstable / make_catable`` (743-975, 1532-1842) and the station
+  helpers, from smili_amd.closures: closure-phase and log
+  closure-amplitude tables built from a visibility table.
 
 Tables are DataFrames or mappings of columns (u, v, amp, phase, sigma; and
 frmidx for movies); images have ``header`` (nx, ny, dx, dy in degrees,
@@ -24,6 +27,7 @@ import numpy as np
 from scipy.spatial import cKDTree
 
 from . import fftlib
+from .closures import check_snr, make_bstable, make_catable, reorder_st_snr, station_dic, uvsort  # noqa: F401
 from .imaging.prep import get_uvlist
 from .imaging.prep3d import get_uvlist_loop
 
