@@ -1,0 +1,1250 @@
+// Warm-start U-Net on gfx950: BF16 activations and weights in HBM (NHWC),
+// FP32 accumulation on the matrix cores (v_mfma_f32_16x16x32_bf16), forward
+// and backward kernels and an Adam optimiser, all on one HIP stream.
+//
+// The network has no counterpart in the reference; its definition (layers,
+// rounding points, loss) is stated in include/smili_unet.h.
+//
+// Convolution kernels (implicit GEMM, 3x3, zero padding):
+//  * k_conv3x3: output tile of 8x16 pixels x 32/64 output channels per
+//    256-thread work-group; each 32-channel slice of the input is staged
+//    once in LDS as a 10x18 halo tile and read by all 9 taps.  The fused
+//    gather reads the nearest-x2 upsample of a coarser tensor and
+//    concatenates a skip tensor (decoder inputs), so neither is
+//    materialised.  Weights are the MFMA A operand (rows = output
+//    channels), the halo the B operand (columns = pixels): each lane then
+//    holds 4 consecutive channels of one pixel for the store.  Used for the
+//    forward pass (bias + ReLU, bf16 out) and for the input gradient
+//    (weights flipped and transposed, fp32 out).
+//  * k_conv3x3_wgrad: dW[co][tap][ci] = sum_px dP[px][co] X[px+off][ci];
+//    pixels are the reduction axis, so dP and the input halo are staged
+//    transposed ([channel][pixel]) in LDS, the halo in three copies
+//    shifted by dx so every 8-pixel fragment is a 16-byte aligned read.
+//    Work-groups split the pixel tiles and add their partial sums with
+//    float atomics.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/smili_unet.h"
+
+typedef uint16_t bf16_t;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int kBase = 32;
+constexpr int kThreads = 256;
+constexpr int kTileH = 8, kTileW = 16;
+constexpr int kHaloH = kTileH + 2, kHaloW = kTileW + 2;
+
+__device__ __forceinline__ float bf2f(uint32_t h) { return __uint_as_float(h << 16); }
+// float -> bfloat16, round to nearest even
+__device__ __forceinline__ uint32_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+
+// Item of a block so that consecutive items share an XCD (blocks are
+// dealt round-robin over the 8 XCDs); -1 past the end.
+__device__ __forceinline__ int xcd_item(int nitems) {
+  const int b = blockIdx.x, per = gridDim.x >> 3;
+  const int item = (b & 7) * per + (b >> 3);
+  return item < nitems ? item : -1;
+}
+inline int round8(int n) { return (n + 7) & ~7; }
+
+// ---------------------------------------------------------------------------
+// 3x3 convolution, forward / input gradient
+// ---------------------------------------------------------------------------
+struct ConvArgs {
+  const bf16_t* x1;  // channels [0, c1): full resolution, or half if up1
+  const bf16_t* x2;  // channels [c1, c1 + c2): full resolution (may be null)
+  const bf16_t* w;   // [cout][9][c1 + c2]
+  const float* bias; // [cout] (mode 0)
+  bf16_t* ybf;       // mode 0 output
+  float* yf;         // mode 1 / 2 output
+  int c1, c2, up1;
+  int n, h, w_, cout;
+  int mode;          // 0: bias + ReLU -> bf16; 1: fp32 store; 2: fp32 add
+  int tiles_x, tiles_y, ncob;
+};
+
+__device__ __forceinline__ int halo_off(int p, int g) {
+  // 64 bytes per halo pixel; the 16-byte channel groups are swizzled so
+  // that 16 consecutive pixels of one group spread over the banks
+  return p * 32 + ((g ^ ((p >> 1) & 3)) << 3);
+}
+
+__device__ __forceinline__ const bf16_t* gather_src(const ConvArgs& a, int b, int yy, int xx, int c) {
+  if (c < a.c1) {
+    if (a.up1)
+      return a.x1 + ((size_t)(b * (a.h >> 1) + (yy >> 1)) * (a.w_ >> 1) + (xx >> 1)) * a.c1 + c;
+    return a.x1 + ((size_t)(b * a.h + yy) * a.w_ + xx) * a.c1 + c;
+  }
+  return a.x2 + ((size_t)(b * a.h + yy) * a.w_ + xx) * a.c2 + (c - a.c1);
+}
+
+template <int NCB>  // output-channel blocks of 16 per work-group
+__global__ __launch_bounds__(kThreads) void k_conv3x3(ConvArgs a) {
+  __shared__ __attribute__((aligned(16))) bf16_t halo[kHaloH * kHaloW * 32];
+  const int item = xcd_item(a.n * a.tiles_y * a.tiles_x * a.ncob);
+  if (item < 0) return;
+  const int cob = item % a.ncob;
+  int t = item / a.ncob;
+  const int tx = t % a.tiles_x;
+  t /= a.tiles_x;
+  const int ty = t % a.tiles_y, b = t / a.tiles_y;
+  const int y0 = ty * kTileH, x0 = tx * kTileW, co0 = cob * NCB * 16;
+  const int cin = a.c1 + a.c2;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ln = lane & 15, lg = lane >> 4;
+
+  f32x4 acc[2][NCB];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < NCB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  for (int c0 = 0; c0 < cin; c0 += 32) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < kHaloH * kHaloW * 4; e += kThreads) {
+      const int p = e >> 2, g = e & 3;
+      const int hy = p / kHaloW, hx = p - hy * kHaloW;
+      const int yy = y0 + hy - 1, xx = x0 + hx - 1;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_)
+        v = *reinterpret_cast<const uint4*>(gather_src(a, b, yy, xx, c0 + g * 8));
+      *reinterpret_cast<uint4*>(halo + halo_off(p, g)) = v;
+    }
+    __syncthreads();
+    const bf16_t* wrow = a.w + (size_t)(co0 + ln) * 9 * cin + c0 + lg * 8;
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+      const int dy = tap / 3, dx = tap - 3 * (tap / 3);
+      bf16x8 bx[2];
+#pragma unroll
+      for (int pb = 0; pb < 2; ++pb) {
+        const int p = (wv * 2 + pb + dy) * kHaloW + ln + dx;
+        bx[pb] = *reinterpret_cast<const bf16x8*>(halo + halo_off(p, lg));
+      }
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb) {
+        const bf16x8 aw = *reinterpret_cast<const bf16x8*>(wrow + (size_t)cb * 16 * 9 * cin + tap * cin);
+#pragma unroll
+        for (int pb = 0; pb < 2; ++pb)
+          acc[pb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aw, bx[pb], acc[pb][cb], 0, 0, 0);
+      }
+    }
+  }
+  // D[row = channel 4*lg + i][col = pixel ln]
+#pragma unroll
+  for (int pb = 0; pb < 2; ++pb) {
+    const int y = y0 + wv * 2 + pb, x = x0 + ln;
+    const size_t o = ((size_t)(b * a.h + y) * a.w_ + x) * a.cout;
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb) {
+      const int co = co0 + cb * 16 + lg * 4;
+      const f32x4 v = acc[pb][cb];
+      if (a.mode == 0) {
+        float r[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) r[i] = fmaxf(v[i] + a.bias[co + i], 0.f);
+        uint2 pk;
+        pk.x = f2bf(r[0]) | (f2bf(r[1]) << 16);
+        pk.y = f2bf(r[2]) | (f2bf(r[3]) << 16);
+        *reinterpret_cast<uint2*>(a.ybf + o + co) = pk;
+      } else {
+        float4* dst = reinterpret_cast<float4*>(a.yf + o + co);
+        float4 r = make_float4(v[0], v[1], v[2], v[3]);
+        if (a.mode == 2) {
+          const float4 q = *dst;
+          r.x += q.x; r.y += q.y; r.z += q.z; r.w += q.w;
+        }
+        *dst = r;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 convolution, weight gradient
+// ---------------------------------------------------------------------------
+struct WgradArgs {
+  const bf16_t* dp;  // [n][h][w][cout]
+  const bf16_t* x1;
+  const bf16_t* x2;
+  float* dw;         // [cout][9][c1 + c2], accumulated
+  int c1, c2, up1;
+  int n, h, w_, cout;
+  int tiles_x, tiles_y, ncob, nchunk, tpg, ngroups;
+};
+
+template <int NCB>
+__global__ __launch_bounds__(kThreads) void k_conv3x3_wgrad(WgradArgs a) {
+  __shared__ __attribute__((aligned(16))) bf16_t sdp[NCB * 16][kTileH * kTileW];
+  __shared__ __attribute__((aligned(16))) bf16_t sx[3][32][kHaloH][kTileW];
+  const int item = xcd_item(a.ncob * a.nchunk * a.ngroups);
+  if (item < 0) return;
+  const int grp = item % a.ngroups;
+  const int rest = item / a.ngroups;
+  const int chunk = rest % a.nchunk, cob = rest / a.nchunk;
+  const int co0 = cob * NCB * 16, ci0 = chunk * 32;
+  const int cin = a.c1 + a.c2;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ln = lane & 15, lg = lane >> 4;
+  const int cib = wv & 1, tap0 = wv >> 1;   // this wave: ci block, taps tap0 + 2j
+  const int ntap = tap0 == 0 ? 5 : 4;
+  ConvArgs g;
+  g.x1 = a.x1; g.x2 = a.x2; g.c1 = a.c1; g.c2 = a.c2; g.up1 = a.up1;
+  g.h = a.h; g.w_ = a.w_;
+
+  f32x4 acc[5][NCB];
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+#pragma unroll
+    for (int j = 0; j < NCB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  const int ntiles = a.n * a.tiles_y * a.tiles_x;
+  const int tb = grp * a.tpg, te = min(ntiles, tb + a.tpg);
+  for (int tile = tb; tile < te; ++tile) {
+    const int tx = tile % a.tiles_x;
+    const int ty = (tile / a.tiles_x) % a.tiles_y, b = tile / (a.tiles_x * a.tiles_y);
+    const int y0 = ty * kTileH, x0 = tx * kTileW;
+    __syncthreads();
+    // dP tile, transposed to [co][px]
+    for (int e = threadIdx.x; e < kTileH * kTileW * NCB * 2; e += kThreads) {
+      const int px = e / (NCB * 2), q = e - px * (NCB * 2);
+      const int y = y0 + (px >> 4), x = x0 + (px & 15);
+      const uint4 v = *reinterpret_cast<const uint4*>(
+          a.dp + ((size_t)(b * a.h + y) * a.w_ + x) * a.cout + co0 + q * 8);
+      const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int i = 0; i < 8; ++i) sdp[q * 8 + i][px] = (bf16_t)(w4[i >> 1] >> ((i & 1) * 16));
+    }
+    // input halo slice, transposed to [ci][row][col] in three dx-shifted copies
+    for (int e = threadIdx.x; e < kHaloH * kHaloW * 4; e += kThreads) {
+      const int p = e >> 2, q = e & 3;
+      const int hy = p / kHaloW, hx = p - hy * kHaloW;
+      const int yy = y0 + hy - 1, xx = x0 + hx - 1;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_)
+        v = *reinterpret_cast<const uint4*>(gather_src(g, b, yy, xx, ci0 + q * 8));
+      const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int dx = 0; dx < 3; ++dx) {
+        const int col = hx - dx;
+        if (col >= 0 && col < kTileW) {
+#pragma unroll
+          for (int i = 0; i < 8; ++i) sx[dx][q * 8 + i][hy][col] = (bf16_t)(w4[i >> 1] >> ((i & 1) * 16));
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      // k = 32 pixels: group lg covers row 2s + lg/2, columns 8 (lg & 1) .. +7
+      const int row = 2 * s + (lg >> 1), cb0 = 8 * (lg & 1);
+      bf16x8 ad[NCB];
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb)
+        ad[cb] = *reinterpret_cast<const bf16x8*>(&sdp[cb * 16 + ln][row * kTileW + cb0]);
+#pragma unroll
+      for (int j = 0; j < 5; ++j) {
+        if (j < ntap) {
+          const int tap = tap0 + 2 * j;
+          const int dy = tap / 3, dx = tap - 3 * (tap / 3);
+          const bf16x8 bx = *reinterpret_cast<const bf16x8*>(&sx[dx][cib * 16 + ln][row + dy][cb0]);
+#pragma unroll
+          for (int cb = 0; cb < NCB; ++cb)
+            acc[j][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ad[cb], bx, acc[j][cb], 0, 0, 0);
+        }
+      }
+    }
+  }
+  // D[row = co 4*lg + i][col = ci ln]
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    if (j < ntap) {
+      const int tap = tap0 + 2 * j;
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int co = co0 + cb * 16 + lg * 4 + i, ci = ci0 + cib * 16 + ln;
+          unsafeAtomicAdd(a.dw + ((size_t)co * 9 + tap) * cin + ci, acc[j][cb][i]);
+        }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// first convolution (2 -> 32 channels, fp32 input and weights)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_conv_in(const float* __restrict__ x, const float* __restrict__ w,
+                                                      const float* __restrict__ bias, bf16_t* __restrict__ y,
+                                                      int n, int h, int wd) {
+  // one thread per (pixel, group of 8 output channels)
+  __shared__ float sw[kBase * 18 + kBase];
+  for (int i = threadIdx.x; i < kBase * 18; i += kThreads) sw[i] = w[i];
+  for (int i = threadIdx.x; i < kBase; i += kThreads) sw[kBase * 18 + i] = bias[i];
+  __syncthreads();
+  const size_t work = (size_t)n * h * wd * (kBase / 8);
+  for (size_t e = (size_t)blockIdx.x * kThreads + threadIdx.x; e < work; e += (size_t)gridDim.x * kThreads) {
+    const int q = e % (kBase / 8);
+    const size_t p = e / (kBase / 8);
+    const int xx = p % wd, yy = (p / wd) % h, b = p / ((size_t)wd * h);
+    float in[18];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      const int sy = yy + t / 3 - 1, sx = xx + t % 3 - 1;
+      const bool ok = sy >= 0 && sy < h && sx >= 0 && sx < wd;
+      const float2 v = ok ? *reinterpret_cast<const float2*>(x + ((size_t)(b * h + sy) * wd + sx) * 2)
+                          : make_float2(0.f, 0.f);
+      in[2 * t] = v.x;
+      in[2 * t + 1] = v.y;
+    }
+    uint32_t out[4];
+#pragma unroll
+    for (int k = 0; k < 8; k += 2) {
+      const int co = q * 8 + k;
+      float r0 = sw[kBase * 18 + co], r1 = sw[kBase * 18 + co + 1];
+#pragma unroll
+      for (int j = 0; j < 18; ++j) {
+        r0 = fmaf(sw[co * 18 + j], in[j], r0);
+        r1 = fmaf(sw[(co + 1) * 18 + j], in[j], r1);
+      }
+      out[k / 2] = f2bf(fmaxf(r0, 0.f)) | (f2bf(fmaxf(r1, 0.f)) << 16);
+    }
+    *reinterpret_cast<uint4*>(y + p * kBase + q * 8) = make_uint4(out[0], out[1], out[2], out[3]);
+  }
+}
+
+// dW[co][tap][ci] of the first convolution: each block writes the sums of
+// its pixels to part[block][576], reduced by k_sum_rows
+__global__ __launch_bounds__(kThreads) void k_conv_in_wgrad(const bf16_t* __restrict__ dp,
+                                                            const float* __restrict__ x, float* __restrict__ part,
+                                                            int n, int h, int wd, int px_per_block) {
+  const int npx = n * h * wd;
+  const int pb = blockIdx.x * px_per_block, pe = min(npx, pb + px_per_block);
+  float acc[3] = {0.f, 0.f, 0.f};
+  for (int p = pb; p < pe; ++p) {
+    const int xx = p % wd, yy = (p / wd) % h, b = p / (wd * h);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const int e = threadIdx.x + r * kThreads;
+      if (e < kBase * 18) {
+        const int co = e / 18, j = e - co * 18, t = j >> 1, ci = j & 1;
+        const int sy = yy + t / 3 - 1, sx = xx + t % 3 - 1;
+        if (sy >= 0 && sy < h && sx >= 0 && sx < wd)
+          acc[r] = fmaf(bf2f(dp[(size_t)p * kBase + co]), x[((size_t)(b * h + sy) * wd + sx) * 2 + ci], acc[r]);
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const int e = threadIdx.x + r * kThreads;
+    if (e < kBase * 18) part[(size_t)blockIdx.x * kBase * 18 + e] = acc[r];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// elementwise kernels
+// ---------------------------------------------------------------------------
+// 2x2 average pool, bf16 -> bf16 (8 channels per thread)
+__global__ void k_pool(const bf16_t* __restrict__ x, bf16_t* __restrict__ y, int n, int h, int wd, int c) {
+  const int ho = h >> 1, wo = wd >> 1, cg = c >> 3;
+  const size_t total = (size_t)n * ho * wo * cg;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const int q = e % cg;
+    const size_t p = e / cg;
+    const int xo = p % wo, yo = (p / wo) % ho, b = p / ((size_t)wo * ho);
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int yy = 2 * yo + (k >> 1), xx = 2 * xo + (k & 1);
+      const uint4 v = *reinterpret_cast<const uint4*>(x + ((size_t)(b * h + yy) * wd + xx) * c + q * 8);
+      const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int i = 0; i < 8; ++i) s[i] += bf2f((w4[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu);
+    }
+    uint32_t o[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) o[i] = f2bf(0.25f * s[2 * i]) | (f2bf(0.25f * s[2 * i + 1]) << 16);
+    *reinterpret_cast<uint4*>(y + p * c + q * 8) = make_uint4(o[0], o[1], o[2], o[3]);
+  }
+}
+
+// dX[2y+i][2x+j] += dY[y][x] / 4 (fp32, 4 channels per thread)
+__global__ void k_pool_bwd_add(const float* __restrict__ dy, float* __restrict__ dx, int n, int h, int wd, int c) {
+  const int ho = h >> 1, wo = wd >> 1, cg = c >> 2;
+  const size_t total = (size_t)n * ho * wo * cg;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const int q = e % cg;
+    const size_t p = e / cg;
+    const int xo = p % wo, yo = (p / wo) % ho, b = p / ((size_t)wo * ho);
+    float4 g = *reinterpret_cast<const float4*>(dy + p * c + q * 4);
+    g.x *= 0.25f; g.y *= 0.25f; g.z *= 0.25f; g.w *= 0.25f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int yy = 2 * yo + (k >> 1), xx = 2 * xo + (k & 1);
+      float4* d = reinterpret_cast<float4*>(dx + ((size_t)(b * h + yy) * wd + xx) * c + q * 4);
+      float4 r = *d;
+      r.x += g.x; r.y += g.y; r.z += g.z; r.w += g.w;
+      *d = r;
+    }
+  }
+}
+
+// Splits the input gradient of a decoder convolution [n][h][w][c1 + c2]:
+// the upsampled part sums over 2x2 into dx1 [n][h/2][w/2][c1], the skip
+// part is stored into dx2 [n][h][w][c2].
+__global__ void k_split_up(const float* __restrict__ dcat, float* __restrict__ dx1, float* __restrict__ dx2,
+                           int n, int h, int wd, int c1, int c2) {
+  const int ho = h >> 1, wo = wd >> 1, ct = c1 + c2;
+  const size_t n1 = (size_t)n * ho * wo * (c1 >> 2), n2 = (size_t)n * h * wd * (c2 >> 2);
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n1 + n2; e += (size_t)gridDim.x * blockDim.x) {
+    if (e < n1) {
+      const int q = e % (c1 >> 2);
+      const size_t p = e / (c1 >> 2);
+      const int xo = p % wo, yo = (p / wo) % ho, b = p / ((size_t)wo * ho);
+      float4 s = make_float4(0, 0, 0, 0);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int yy = 2 * yo + (k >> 1), xx = 2 * xo + (k & 1);
+        const float4 v = *reinterpret_cast<const float4*>(dcat + ((size_t)(b * h + yy) * wd + xx) * ct + q * 4);
+        s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+      }
+      *reinterpret_cast<float4*>(dx1 + p * c1 + q * 4) = s;
+    } else {
+      const size_t f = e - n1;
+      const int q = f % (c2 >> 2);
+      const size_t p = f / (c2 >> 2);
+      *reinterpret_cast<float4*>(dx2 + p * c2 + q * 4) =
+          *reinterpret_cast<const float4*>(dcat + p * ct + c1 + q * 4);
+    }
+  }
+}
+
+// dP = bf16(dA * (Y > 0)) and the block's partial sums
+// part[block][c] = sum_px dA * (Y > 0) (fp32, before rounding) of the bias
+// gradient, reduced by k_sum_rows (atomics of every block on the same c
+// addresses would serialise).  8 channels per thread; blockDim is a
+// multiple of c / 8.
+__global__ __launch_bounds__(kThreads) void k_relu_mask(const float* __restrict__ da, const bf16_t* __restrict__ y,
+                                                        bf16_t* __restrict__ dp, float* __restrict__ part,
+                                                        size_t npx, int c) {
+  __shared__ float red[kThreads][9];
+  const int cg = c >> 3;
+  const int q = threadIdx.x % cg;
+  const int pstride = (gridDim.x * kThreads) / cg;
+  float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (size_t p = (size_t)(blockIdx.x * kThreads + threadIdx.x) / cg; p < npx; p += pstride) {
+    const size_t o = p * c + q * 8;
+    const uint4 yv = *reinterpret_cast<const uint4*>(y + o);
+    const uint32_t y4[4] = {yv.x, yv.y, yv.z, yv.w};
+    const float4 a0 = *reinterpret_cast<const float4*>(da + o);
+    const float4 a1 = *reinterpret_cast<const float4*>(da + o + 4);
+    const float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+    uint32_t o4[4];
+#pragma unroll
+    for (int i = 0; i < 8; i += 2) {
+      const float g0 = bf2f(y4[i >> 1] & 0xFFFFu) > 0.f ? av[i] : 0.f;
+      const float g1 = bf2f(y4[i >> 1] >> 16) > 0.f ? av[i + 1] : 0.f;
+      s[i] += g0;
+      s[i + 1] += g1;
+      o4[i >> 1] = f2bf(g0) | (f2bf(g1) << 16);
+    }
+    *reinterpret_cast<uint4*>(dp + o) = make_uint4(o4[0], o4[1], o4[2], o4[3]);
+  }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) red[threadIdx.x][i] = s[i];
+  __syncthreads();
+  if (threadIdx.x < cg) {
+    float t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int r = threadIdx.x; r < kThreads; r += cg)
+#pragma unroll
+      for (int i = 0; i < 8; ++i) t[i] += red[r][i];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) part[(size_t)blockIdx.x * c + q * 8 + i] = t[i];
+  }
+}
+
+// out[j] += sum_r part[r][j]: a block takes 4 columns, its 64 row slices
+// sum rows r = slice, slice + 64, ... and are combined in LDS
+__global__ __launch_bounds__(256) void k_sum_rows(const float* __restrict__ part, int nrow, int ncol,
+                                                  float* __restrict__ out) {
+  __shared__ float red[64][4];
+  const int c = threadIdx.x & 3, sl = threadIdx.x >> 2;
+  const int j = blockIdx.x * 4 + c;
+  float s = 0.f;
+  if (j < ncol) {
+#pragma unroll 4
+    for (int r = sl; r < nrow; r += 64) s += part[(size_t)r * ncol + j];
+  }
+  red[sl][c] = s;
+  __syncthreads();
+  if (threadIdx.x < 4 && j < ncol) {
+    float v = 0.f;
+    for (int k = 0; k < 64; ++k) v += red[k][threadIdx.x];
+    out[j] += v;
+  }
+}
+
+// Head: z = w . d0 + b, image max(z, 0).  With a target: loss partial
+// sums (double) of (z - t)^2, dz = 2 (z - t) / N, dd0 = dz w, and the
+// head's gradients (block sums, then atomics).
+__global__ __launch_bounds__(kThreads) void k_head(const bf16_t* __restrict__ d0, const float* __restrict__ hw,
+                                                   float* __restrict__ out, const float* __restrict__ tgt,
+                                                   float* __restrict__ dd0, float* __restrict__ ghw,
+                                                   double* __restrict__ loss, size_t npx, float scale) {
+  __shared__ float sw[kBase + 1];
+  __shared__ float red[kThreads / 64][kBase + 1];
+  __shared__ double lred[kThreads / 64];
+  if (threadIdx.x <= kBase) sw[threadIdx.x] = hw[threadIdx.x];
+  __syncthreads();
+  float gw[kBase + 1];
+#pragma unroll
+  for (int c = 0; c <= kBase; ++c) gw[c] = 0.f;
+  double lsum = 0.0;
+  for (size_t p = (size_t)blockIdx.x * kThreads + threadIdx.x; p < npx; p += (size_t)gridDim.x * kThreads) {
+    float a[kBase];
+    const uint4* src = reinterpret_cast<const uint4*>(d0 + p * kBase);
+#pragma unroll
+    for (int q = 0; q < kBase / 8; ++q) {
+      const uint4 v = src[q];
+      const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int i = 0; i < 8; ++i) a[q * 8 + i] = bf2f((w4[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu);
+    }
+    float z = sw[kBase];
+#pragma unroll
+    for (int c = 0; c < kBase; ++c) z = fmaf(sw[c], a[c], z);
+    if (out) out[p] = fmaxf(z, 0.f);
+    if (tgt) {
+      const float diff = z - tgt[p];
+      lsum += (double)diff * diff;
+      const float dz = 2.f * diff * scale;
+      float4* dst = reinterpret_cast<float4*>(dd0 + p * kBase);
+#pragma unroll
+      for (int q = 0; q < kBase / 4; ++q)
+        dst[q] = make_float4(dz * sw[4 * q], dz * sw[4 * q + 1], dz * sw[4 * q + 2], dz * sw[4 * q + 3]);
+#pragma unroll
+      for (int c = 0; c < kBase; ++c) gw[c] = fmaf(dz, a[c], gw[c]);
+      gw[kBase] += dz;
+    }
+  }
+  if (!tgt) return;
+  // wave sums, then block sums
+#pragma unroll
+  for (int c = 0; c <= kBase; ++c) {
+    float v = gw[c];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][c] = v;
+  }
+  double lv = lsum;
+  for (int off = 32; off > 0; off >>= 1) lv += __shfl_xor(lv, off);
+  if ((threadIdx.x & 63) == 0) lred[threadIdx.x >> 6] = lv;
+  __syncthreads();
+  if (threadIdx.x <= kBase) {
+    float v = 0.f;
+    for (int wv = 0; wv < kThreads / 64; ++wv) v += red[wv][threadIdx.x];
+    unsafeAtomicAdd(ghw + threadIdx.x, v);
+  }
+  if (threadIdx.x == 0) {
+    double v = 0.0;
+    for (int wv = 0; wv < kThreads / 64; ++wv) v += lred[wv];
+    unsafeAtomicAdd(loss, v);
+  }
+}
+
+__global__ void k_fill(float* __restrict__ p, size_t n, float v) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_d(double* __restrict__ p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+// Adam with decoupled weight decay over the flat parameter vector
+__global__ void k_adam(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
+                       const float* __restrict__ g, size_t n, const float* __restrict__ lr_arr, int s,
+                       float b1, float b2, float eps, float wd, float bc1, float bc2) {
+  const float lr = lr_arr[s];
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float mh = mi / bc1, vh = vi / bc2;
+    p[i] = p[i] - lr * (mh / (sqrtf(vh) + eps) + wd * p[i]);
+  }
+}
+
+// bf16 copies of one 3x3 weight tensor: W[co][tap][ci] and the flipped,
+// transposed Wf[ci][8 - tap][co] the input gradient convolves with.
+__global__ void k_cast_w(const float* __restrict__ src, bf16_t* __restrict__ w, bf16_t* __restrict__ wf,
+                         int cout, int cin) {
+  const int total = cout * 9 * cin;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    const int ci = i % cin, tap = (i / cin) % 9, co = i / (9 * cin);
+    const bf16_t h = (bf16_t)f2bf(src[i]);
+    w[i] = h;
+    wf[((size_t)ci * 9 + (8 - tap)) * cout + co] = h;
+  }
+}
+
+// Batch gather from the device-resident training set.  idx = 8 sample + k
+// with k a symmetry of the square (k >= 4: transpose, then k % 4 quarter
+// turns, out[i][j] = in[j][n-1-i] per turn; square images when k != 0),
+// applied to the input and the target alike.
+__global__ void k_gather(const float* __restrict__ din, const float* __restrict__ dtg,
+                         const int64_t* __restrict__ idx, float* __restrict__ xin, float* __restrict__ tgt,
+                         int nb, int h, int wd) {
+  const int npx = h * wd;
+  const size_t total = (size_t)nb * npx;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const int b = e / npx, p = e - (size_t)b * npx;
+    const int64_t s = idx[b] >> 3;
+    const int k = (int)(idx[b] & 7);
+    int i = p / wd, j = p - (p / wd) * wd;
+    for (int r = 0; r < (k & 3); ++r) {  // undo the quarter turns
+      const int ni = j, nj = h - 1 - i;
+      i = ni;
+      j = nj;
+    }
+    if (k >= 4) {
+      const int q = i;
+      i = j;
+      j = q;
+    }
+    const size_t src = (size_t)s * npx + (size_t)i * wd + j;
+    reinterpret_cast<float2*>(xin)[e] = reinterpret_cast<const float2*>(din)[src];
+    tgt[e] = dtg[src];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+thread_local std::string g_err;
+int fail(int code, const std::string& m) {
+  g_err = m;
+  return code;
+}
+#define HIPCHK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess) return fail(-2, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct Layer {
+  const char* name;
+  int cin, cout;
+  size_t woff, boff;     // into the flat parameters
+  size_t bfoff;          // into the bf16 weight arena (3x3 convs except conv_in)
+};
+
+struct Act {
+  bf16_t* y = nullptr;   // activation (bf16)
+  float* g = nullptr;    // its gradient (fp32)
+  int h = 0, w = 0, c = 0;
+};
+
+int layer_count() { return 14; }
+
+void layer_table(std::vector<Layer>& L) {
+  const int c0 = kBase, c1 = 2 * kBase, c2 = 4 * kBase, c3 = 8 * kBase;
+  const struct { const char* n; int ci, co; } spec[14] = {
+      {"conv_in", 2, c0},     {"enc0b", c0, c0},    {"enc1a", c0, c1},    {"enc1b", c1, c1},
+      {"enc2a", c1, c2},      {"enc2b", c2, c2},    {"enc3a", c2, c3},    {"enc3b", c3, c3},
+      {"dec2a", c3 + c2, c2}, {"dec2b", c2, c2},    {"dec1a", c2 + c1, c1}, {"dec1b", c1, c1},
+      {"dec0a", c1 + c0, c0}, {"dec0b", c0, c0}};
+  L.clear();
+  size_t o = 0, bo = 0;
+  for (int i = 0; i < 14; ++i) {
+    Layer l;
+    l.name = spec[i].n;
+    l.cin = spec[i].ci;
+    l.cout = spec[i].co;
+    l.woff = o;
+    o += (size_t)l.cout * 9 * l.cin;
+    l.boff = o;
+    o += l.cout;
+    l.bfoff = bo;
+    if (i > 0) bo += (size_t)l.cout * 9 * l.cin;
+    L.push_back(l);
+  }
+}
+
+size_t nparams_total() {
+  std::vector<Layer> L;
+  layer_table(L);
+  return L.back().boff + L.back().cout + kBase + 1;
+}
+size_t bf_total() {
+  std::vector<Layer> L;
+  layer_table(L);
+  const Layer& l = L.back();
+  return l.bfoff + (size_t)l.cout * 9 * l.cin;
+}
+
+}  // namespace
+
+struct smili_unet {
+  int B, H, W;
+  hipStream_t s = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+  std::vector<Layer> L;
+  size_t np = 0, nbf = 0;
+  float *p = nullptr, *m = nullptr, *v = nullptr, *g = nullptr;
+  bf16_t *wbf = nullptr, *wfl = nullptr;
+  float *xin = nullptr, *tgt = nullptr, *out = nullptr;
+  float *dtmp = nullptr;          // input gradient of decoder convolutions
+  bf16_t* dp = nullptr;           // masked output gradient
+  float* part = nullptr;          // per-block partial sums of gradient reductions
+  double* loss = nullptr;         // per step
+  int64_t nloss = 0;
+  float* lr = nullptr;
+  int64_t nlr = 0;
+  int64_t* idx = nullptr;
+  int64_t nidx = 0;
+  float *ds_in = nullptr, *ds_tg = nullptr;
+  int64_t ds_n = 0;
+  int64_t step = 0;
+  // activations
+  Act e0a, e0, p0, e1a, e1, p1, e2a, e2, p2, e3a, e3, d2a, d2, d1a, d1, d0a, d0;
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+int dalloc(smili_unet* n, void** p, size_t bytes) {
+  HIPCHK(hipMalloc(p, bytes ? bytes : 16));
+  n->allocs.push_back(*p);
+  return 0;
+}
+
+int act_alloc(smili_unet* n, Act& a, int h, int w, int c) {
+  a.h = h; a.w = w; a.c = c;
+  const size_t e = (size_t)n->B * h * w * c;
+  int r = dalloc(n, (void**)&a.y, e * sizeof(bf16_t));
+  if (r) return r;
+  return dalloc(n, (void**)&a.g, e * sizeof(float));
+}
+
+inline int grid_for(size_t work, int per_block = kThreads, int cap = 8192) {
+  size_t b = (work + per_block - 1) / per_block;
+  if (b < 1) b = 1;
+  if (b > (size_t)cap) b = cap;
+  return (int)b;
+}
+
+const bf16_t* wbf_of(smili_unet* n, int li) { return n->wbf + n->L[li].bfoff; }
+const bf16_t* wfl_of(smili_unet* n, int li) { return n->wfl + n->L[li].bfoff; }
+
+void launch_conv(smili_unet* n, int nb, const Act* x1, int up1, const Act* x2, int li, int mode, bf16_t* ybf,
+                 float* yf, int h, int w, int cout, const bf16_t* wt, const float* bias) {
+  ConvArgs a;
+  a.x1 = x1->y;
+  a.c1 = x1->c;
+  a.up1 = up1;
+  a.x2 = x2 ? x2->y : nullptr;
+  a.c2 = x2 ? x2->c : 0;
+  a.w = wt;
+  a.bias = bias;
+  a.ybf = ybf;
+  a.yf = yf;
+  a.n = nb;
+  a.h = h;
+  a.w_ = w;
+  a.cout = cout;
+  a.mode = mode;
+  a.tiles_x = w / kTileW;
+  a.tiles_y = h / kTileH;
+  const int ncb = (cout % 64 == 0) ? 4 : 2;
+  a.ncob = cout / (ncb * 16);
+  const int items = nb * a.tiles_x * a.tiles_y * a.ncob;
+  if (ncb == 4)
+    hipLaunchKernelGGL(k_conv3x3<4>, dim3(round8(items)), dim3(kThreads), 0, n->s, a);
+  else
+    hipLaunchKernelGGL(k_conv3x3<2>, dim3(round8(items)), dim3(kThreads), 0, n->s, a);
+  (void)li;
+}
+
+// forward 3x3 conv of layer li: y = relu(conv(x1 [up] ++ x2) + b)
+void conv_fwd(smili_unet* n, int nb, int li, const Act* x1, int up1, const Act* x2, Act* y) {
+  launch_conv(n, nb, x1, up1, x2, li, 0, y->y, nullptr, y->h, y->w, y->c, wbf_of(n, li), n->p + n->L[li].boff);
+}
+
+void conv_wgrad(smili_unet* n, int nb, int li, const Act* x1, int up1, const Act* x2, int h, int w, int cout) {
+  WgradArgs a;
+  a.dp = n->dp;
+  a.x1 = x1->y;
+  a.c1 = x1->c;
+  a.up1 = up1;
+  a.x2 = x2 ? x2->y : nullptr;
+  a.c2 = x2 ? x2->c : 0;
+  a.dw = n->g + n->L[li].woff;
+  a.n = nb;
+  a.h = h;
+  a.w_ = w;
+  a.cout = cout;
+  a.tiles_x = w / kTileW;
+  a.tiles_y = h / kTileH;
+  const int ncb = (cout % 64 == 0) ? 4 : 2;
+  a.ncob = cout / (ncb * 16);
+  a.nchunk = (a.c1 + a.c2) / 32;
+  const int ntiles = nb * a.tiles_x * a.tiles_y;
+  const int base = a.ncob * a.nchunk;
+  int tpg = (int)(((size_t)ntiles * base + 2047) / 2048);
+  if (tpg < 1) tpg = 1;
+  a.tpg = tpg;
+  a.ngroups = (ntiles + tpg - 1) / tpg;
+  const int items = base * a.ngroups;
+  if (ncb == 4)
+    hipLaunchKernelGGL(k_conv3x3_wgrad<4>, dim3(round8(items)), dim3(kThreads), 0, n->s, a);
+  else
+    hipLaunchKernelGGL(k_conv3x3_wgrad<2>, dim3(round8(items)), dim3(kThreads), 0, n->s, a);
+}
+
+constexpr int kMaskBlocks = 1024;   // rows of the partial-sum scratch
+constexpr int kInPxPerBlock = 128;
+
+void relu_mask(smili_unet* n, int nb, const Act* y, const float* da, int li) {
+  const size_t npx = (size_t)nb * y->h * y->w;
+  const int blocks = grid_for(npx * (y->c / 8), kThreads, kMaskBlocks);
+  hipLaunchKernelGGL(k_relu_mask, dim3(blocks), dim3(kThreads), 0, n->s, da, y->y, n->dp, n->part, npx, y->c);
+  hipLaunchKernelGGL(k_sum_rows, dim3((y->c + 3) / 4), dim3(256), 0, n->s, n->part, blocks, y->c,
+                     n->g + n->L[li].boff);
+}
+
+// Backward through ReLU + conv li (input x1 [up] ++ x2, output y with
+// gradient da).  The input gradient goes to dx (fp32, [n][h][w][cin] at
+// the output's resolution) unless dx is null.
+void conv_bwd(smili_unet* n, int nb, int li, const Act* y, const float* da, const Act* x1, int up1, const Act* x2,
+              float* dx) {
+  relu_mask(n, nb, y, da, li);
+  conv_wgrad(n, nb, li, x1, up1, x2, y->h, y->w, y->c);
+  if (dx) {
+    Act dpa;
+    dpa.y = n->dp;
+    dpa.c = y->c;
+    launch_conv(n, nb, &dpa, 0, nullptr, li, 1, nullptr, dx, y->h, y->w, n->L[li].cin, wfl_of(n, li), nullptr);
+  }
+}
+
+void split_up(smili_unet* n, int nb, const float* dcat, int h, int w, int c1, int c2, float* dx1, float* dx2) {
+  const size_t work = (size_t)nb * (h / 2) * (w / 2) * (c1 / 4) + (size_t)nb * h * w * (c2 / 4);
+  hipLaunchKernelGGL(k_split_up, dim3(grid_for(work)), dim3(kThreads), 0, n->s, dcat, dx1, dx2, nb, h, w, c1, c2);
+}
+
+void pool_fwd(smili_unet* n, int nb, const Act* x, Act* y) {
+  const size_t work = (size_t)nb * y->h * y->w * (y->c / 8);
+  hipLaunchKernelGGL(k_pool, dim3(grid_for(work)), dim3(kThreads), 0, n->s, x->y, y->y, nb, x->h, x->w, x->c);
+}
+
+void pool_bwd_add(smili_unet* n, int nb, const float* dy, Act* x) {
+  const size_t work = (size_t)nb * (x->h / 2) * (x->w / 2) * (x->c / 4);
+  hipLaunchKernelGGL(k_pool_bwd_add, dim3(grid_for(work)), dim3(kThreads), 0, n->s, dy, x->g, nb, x->h, x->w, x->c);
+}
+
+void forward(smili_unet* n, int nb) {
+  const Layer& l0 = n->L[0];
+  hipLaunchKernelGGL(k_conv_in, dim3(grid_for((size_t)nb * n->H * n->W * (kBase / 8))), dim3(kThreads), 0, n->s, n->xin,
+                     n->p + l0.woff, n->p + l0.boff, n->e0a.y, nb, n->H, n->W);
+  conv_fwd(n, nb, 1, &n->e0a, 0, nullptr, &n->e0);
+  pool_fwd(n, nb, &n->e0, &n->p0);
+  conv_fwd(n, nb, 2, &n->p0, 0, nullptr, &n->e1a);
+  conv_fwd(n, nb, 3, &n->e1a, 0, nullptr, &n->e1);
+  pool_fwd(n, nb, &n->e1, &n->p1);
+  conv_fwd(n, nb, 4, &n->p1, 0, nullptr, &n->e2a);
+  conv_fwd(n, nb, 5, &n->e2a, 0, nullptr, &n->e2);
+  pool_fwd(n, nb, &n->e2, &n->p2);
+  conv_fwd(n, nb, 6, &n->p2, 0, nullptr, &n->e3a);
+  conv_fwd(n, nb, 7, &n->e3a, 0, nullptr, &n->e3);
+  conv_fwd(n, nb, 8, &n->e3, 1, &n->e2, &n->d2a);
+  conv_fwd(n, nb, 9, &n->d2a, 0, nullptr, &n->d2);
+  conv_fwd(n, nb, 10, &n->d2, 1, &n->e1, &n->d1a);
+  conv_fwd(n, nb, 11, &n->d1a, 0, nullptr, &n->d1);
+  conv_fwd(n, nb, 12, &n->d1, 1, &n->e0, &n->d0a);
+  conv_fwd(n, nb, 13, &n->d0a, 0, nullptr, &n->d0);
+}
+
+size_t head_off(smili_unet* n) { return n->L.back().boff + n->L.back().cout; }
+
+void head(smili_unet* n, int nb, bool train, double* loss) {
+  const size_t npx = (size_t)nb * n->H * n->W;
+  const float scale = 1.f / (float)npx;
+  hipLaunchKernelGGL(k_head, dim3(grid_for(npx, kThreads, 1024)), dim3(kThreads), 0, n->s, n->d0.y,
+                     n->p + head_off(n), train ? nullptr : n->out, train ? n->tgt : nullptr,
+                     train ? n->d0.g : nullptr, train ? n->g + head_off(n) : nullptr, loss, npx, scale);
+}
+
+void backward(smili_unet* n, int nb) {
+  // decoder
+  conv_bwd(n, nb, 13, &n->d0, n->d0.g, &n->d0a, 0, nullptr, n->d0a.g);
+  conv_bwd(n, nb, 12, &n->d0a, n->d0a.g, &n->d1, 1, &n->e0, n->dtmp);
+  split_up(n, nb, n->dtmp, n->d0a.h, n->d0a.w, n->d1.c, n->e0.c, n->d1.g, n->e0.g);
+  conv_bwd(n, nb, 11, &n->d1, n->d1.g, &n->d1a, 0, nullptr, n->d1a.g);
+  conv_bwd(n, nb, 10, &n->d1a, n->d1a.g, &n->d2, 1, &n->e1, n->dtmp);
+  split_up(n, nb, n->dtmp, n->d1a.h, n->d1a.w, n->d2.c, n->e1.c, n->d2.g, n->e1.g);
+  conv_bwd(n, nb, 9, &n->d2, n->d2.g, &n->d2a, 0, nullptr, n->d2a.g);
+  conv_bwd(n, nb, 8, &n->d2a, n->d2a.g, &n->e3, 1, &n->e2, n->dtmp);
+  split_up(n, nb, n->dtmp, n->d2a.h, n->d2a.w, n->e3.c, n->e2.c, n->e3.g, n->e2.g);
+  // encoder
+  conv_bwd(n, nb, 7, &n->e3, n->e3.g, &n->e3a, 0, nullptr, n->e3a.g);
+  conv_bwd(n, nb, 6, &n->e3a, n->e3a.g, &n->p2, 0, nullptr, n->p2.g);
+  pool_bwd_add(n, nb, n->p2.g, &n->e2);
+  conv_bwd(n, nb, 5, &n->e2, n->e2.g, &n->e2a, 0, nullptr, n->e2a.g);
+  conv_bwd(n, nb, 4, &n->e2a, n->e2a.g, &n->p1, 0, nullptr, n->p1.g);
+  pool_bwd_add(n, nb, n->p1.g, &n->e1);
+  conv_bwd(n, nb, 3, &n->e1, n->e1.g, &n->e1a, 0, nullptr, n->e1a.g);
+  conv_bwd(n, nb, 2, &n->e1a, n->e1a.g, &n->p0, 0, nullptr, n->p0.g);
+  pool_bwd_add(n, nb, n->p0.g, &n->e0);
+  conv_bwd(n, nb, 1, &n->e0, n->e0.g, &n->e0a, 0, nullptr, n->e0a.g);
+  // first convolution: weight gradient only
+  relu_mask(n, nb, &n->e0a, n->e0a.g, 0);
+  const size_t npx = (size_t)nb * n->H * n->W;
+  const int blocks = (int)((npx + kInPxPerBlock - 1) / kInPxPerBlock);
+  hipLaunchKernelGGL(k_conv_in_wgrad, dim3(blocks), dim3(kThreads), 0, n->s, n->dp, n->xin, n->part, nb, n->H,
+                     n->W, kInPxPerBlock);
+  hipLaunchKernelGGL(k_sum_rows, dim3((kBase * 18 + 3) / 4), dim3(256), 0, n->s, n->part, blocks, kBase * 18,
+                     n->g + n->L[0].woff);
+}
+
+void cast_weights(smili_unet* n) {
+  for (int li = 1; li < layer_count(); ++li) {
+    const Layer& l = n->L[li];
+    hipLaunchKernelGGL(k_cast_w, dim3(grid_for((size_t)l.cout * 9 * l.cin)), dim3(kThreads), 0, n->s,
+                       n->p + l.woff, n->wbf + l.bfoff, n->wfl + l.bfoff, l.cout, l.cin);
+  }
+}
+
+void zero_grad(smili_unet* n) {
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n->np)), dim3(kThreads), 0, n->s, n->g, n->np, 0.f);
+}
+
+int ensure_loss(smili_unet* n, int64_t count) {
+  if (count <= n->nloss) return 0;
+  if (n->loss) {
+    HIPCHK(hipFree(n->loss));
+    for (auto& a : n->allocs)
+      if (a == n->loss) a = nullptr;
+  }
+  n->loss = nullptr;
+  int r = dalloc(n, (void**)&n->loss, count * sizeof(double));
+  if (r) return r;
+  n->nloss = count;
+  return 0;
+}
+
+double conv_flops_fwd(smili_unet* n) {
+  double f = 0.0;
+  const int hs[14] = {0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0};
+  for (int li = 0; li < 14; ++li) {
+    const double px = (double)n->B * (n->H >> hs[li]) * (n->W >> hs[li]);
+    f += 2.0 * px * n->L[li].cout * 9.0 * n->L[li].cin;
+  }
+  return f;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* smili_unet_last_error(void) { return g_err.c_str(); }
+
+int64_t smili_unet_nparams(void) { return (int64_t)nparams_total(); }
+
+void smili_unet_destroy(smili_unet* n) {
+  if (!n) return;
+  if (n->s) (void)hipStreamSynchronize(n->s);
+  for (void* a : n->allocs)
+    if (a) (void)hipFree(a);
+  if (n->ev0) (void)hipEventDestroy(n->ev0);
+  if (n->ev1) (void)hipEventDestroy(n->ev1);
+  if (n->ev2) (void)hipEventDestroy(n->ev2);
+  if (n->s) (void)hipStreamDestroy(n->s);
+  delete n;
+}
+
+int smili_unet_create(int32_t batch, int32_t height, int32_t width, smili_unet** out) {
+  if (!out) return fail(-1, "out is null");
+  *out = nullptr;
+  if (batch < 1 || batch > 4096) return fail(-1, "batch must be in 1..4096");
+  if (height < 64 || height % 64 || width < 128 || width % 128)
+    return fail(-1, "height must be a multiple of 64 and width a multiple of 128");
+  if ((int64_t)batch * height * width > (int64_t)1 << 28) return fail(-1, "batch x height x width too large");
+  smili_unet* n = new smili_unet();
+  n->B = batch;
+  n->H = height;
+  n->W = width;
+  layer_table(n->L);
+  n->np = nparams_total();
+  n->nbf = bf_total();
+  int r = 0;
+  auto chk = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess && !r) r = fail(-2, std::string(what) + ": " + hipGetErrorString(e));
+  };
+  chk(hipStreamCreateWithFlags(&n->s, hipStreamNonBlocking), "hipStreamCreate");
+  chk(hipEventCreate(&n->ev0), "hipEventCreate");
+  chk(hipEventCreate(&n->ev1), "hipEventCreate");
+  chk(hipEventCreate(&n->ev2), "hipEventCreate");
+  const int c0 = kBase, c1 = 2 * kBase, c2 = 4 * kBase, c3 = 8 * kBase;
+  const int H = height, W = width;
+  const size_t B = batch;
+  if (!r) r = dalloc(n, (void**)&n->p, n->np * 4);
+  if (!r) r = dalloc(n, (void**)&n->m, n->np * 4);
+  if (!r) r = dalloc(n, (void**)&n->v, n->np * 4);
+  if (!r) r = dalloc(n, (void**)&n->g, n->np * 4);
+  if (!r) r = dalloc(n, (void**)&n->wbf, n->nbf * 2);
+  if (!r) r = dalloc(n, (void**)&n->wfl, n->nbf * 2);
+  if (!r) r = dalloc(n, (void**)&n->xin, B * H * W * 2 * 4);
+  if (!r) r = dalloc(n, (void**)&n->tgt, B * H * W * 4);
+  if (!r) r = dalloc(n, (void**)&n->out, B * H * W * 4);
+  // largest decoder input gradient: level 0, c1 + c0 channels
+  size_t dt = B * H * W * (c1 + c0);
+  dt = std::max(dt, B * (H / 2) * (W / 2) * (size_t)(c2 + c1));
+  dt = std::max(dt, B * (H / 4) * (W / 4) * (size_t)(c3 + c2));
+  if (!r) r = dalloc(n, (void**)&n->dtmp, dt * 4);
+  if (!r) r = dalloc(n, (void**)&n->dp, B * H * W * c0 * 2);
+  // partial sums: the larger of the bias reductions (kMaskBlocks rows of at
+  // most c3 channels) and the first convolution's weight gradient
+  const size_t nin_blocks = (B * H * W + kInPxPerBlock - 1) / kInPxPerBlock;
+  if (!r) r = dalloc(n, (void**)&n->part, std::max((size_t)kMaskBlocks * c3, nin_blocks * kBase * 18) * 4);
+  const struct { Act* a; int s, c; } acts[17] = {
+      {&n->e0a, 0, c0}, {&n->e0, 0, c0}, {&n->p0, 1, c0}, {&n->e1a, 1, c1}, {&n->e1, 1, c1},
+      {&n->p1, 2, c1},  {&n->e2a, 2, c2}, {&n->e2, 2, c2}, {&n->p2, 3, c2}, {&n->e3a, 3, c3},
+      {&n->e3, 3, c3},  {&n->d2a, 2, c2}, {&n->d2, 2, c2}, {&n->d1a, 1, c1}, {&n->d1, 1, c1},
+      {&n->d0a, 0, c0}, {&n->d0, 0, c0}};
+  for (int i = 0; i < 17 && !r; ++i) r = act_alloc(n, *acts[i].a, H >> acts[i].s, W >> acts[i].s, acts[i].c);
+  if (!r) r = ensure_loss(n, 16);
+  if (!r) {
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n->np)), dim3(kThreads), 0, n->s, n->p, n->np, 0.f);
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n->np)), dim3(kThreads), 0, n->s, n->m, n->np, 0.f);
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n->np)), dim3(kThreads), 0, n->s, n->v, n->np, 0.f);
+    cast_weights(n);
+    chk(hipGetLastError(), "launch");
+    chk(hipStreamSynchronize(n->s), "hipStreamSynchronize");
+  }
+  if (r) {
+    smili_unet_destroy(n);
+    return r;
+  }
+  *out = n;
+  return 0;
+}
+
+int smili_unet_set_params(smili_unet* n, const float* params) {
+  if (!n || !params) return fail(-1, "null argument");
+  for (size_t i = 0; i < n->np; ++i)
+    if (!std::isfinite(params[i])) return fail(-1, "non-finite parameter at index " + std::to_string(i));
+  HIPCHK(hipMemcpyAsync(n->p, params, n->np * 4, hipMemcpyHostToDevice, n->s));
+  cast_weights(n);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(n->s));
+  return 0;
+}
+
+int smili_unet_get_params(smili_unet* n, float* params) {
+  if (!n || !params) return fail(-1, "null argument");
+  HIPCHK(hipMemcpyAsync(params, n->p, n->np * 4, hipMemcpyDeviceToHost, n->s));
+  HIPCHK(hipStreamSynchronize(n->s));
+  return 0;
+}
+
+int smili_unet_forward(smili_unet* n, const float* input, int32_t count, float* output) {
+  if (!n || !input || !output) return fail(-1, "null argument");
+  if (count < 1) return fail(-1, "n must be >= 1");
+  const size_t npx = (size_t)n->H * n->W;
+  for (int32_t b0 = 0; b0 < count; b0 += n->B) {
+    const int nb = std::min<int>(n->B, count - b0);
+    HIPCHK(hipMemcpyAsync(n->xin, input + (size_t)b0 * npx * 2, nb * npx * 2 * 4, hipMemcpyHostToDevice, n->s));
+    forward(n, nb);
+    head(n, nb, false, nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(output + (size_t)b0 * npx, n->out, nb * npx * 4, hipMemcpyDeviceToHost, n->s));
+  }
+  HIPCHK(hipStreamSynchronize(n->s));
+  return 0;
+}
+
+int smili_unet_loss_grad(smili_unet* n, const float* input, const float* target, int32_t count, double* loss,
+                         float* grad) {
+  if (!n || !input || !target || !loss) return fail(-1, "null argument");
+  if (count < 1 || count > n->B) return fail(-1, "n must be in 1..batch");
+  const size_t npx = (size_t)n->H * n->W;
+  HIPCHK(hipMemcpyAsync(n->xin, input, count * npx * 2 * 4, hipMemcpyHostToDevice, n->s));
+  HIPCHK(hipMemcpyAsync(n->tgt, target, count * npx * 4, hipMemcpyHostToDevice, n->s));
+  zero_grad(n);
+  hipLaunchKernelGGL(k_fill_d, dim3(1), dim3(64), 0, n->s, n->loss, (size_t)1);
+  forward(n, count);
+  head(n, count, true, n->loss);
+  backward(n, count);
+  HIPCHK(hipGetLastError());
+  double l = 0.0;
+  HIPCHK(hipMemcpyAsync(&l, n->loss, sizeof(double), hipMemcpyDeviceToHost, n->s));
+  if (grad) HIPCHK(hipMemcpyAsync(grad, n->g, n->np * 4, hipMemcpyDeviceToHost, n->s));
+  HIPCHK(hipStreamSynchronize(n->s));
+  *loss = l / ((double)count * npx);
+  return 0;
+}
+
+int smili_unet_set_data(smili_unet* n, const float* inputs, const float* targets, int64_t count) {
+  if (!n || !inputs || !targets) return fail(-1, "null argument");
+  if (count < 1) return fail(-1, "count must be >= 1");
+  const size_t npx = (size_t)n->H * n->W;
+  HIPCHK(hipStreamSynchronize(n->s));
+  for (void** p : {(void**)&n->ds_in, (void**)&n->ds_tg}) {
+    if (*p) {
+      HIPCHK(hipFree(*p));
+      for (auto& a : n->allocs)
+        if (a == *p) a = nullptr;
+      *p = nullptr;
+    }
+  }
+  int r = dalloc(n, (void**)&n->ds_in, count * npx * 2 * 4);
+  if (!r) r = dalloc(n, (void**)&n->ds_tg, count * npx * 4);
+  if (r) return r;
+  HIPCHK(hipMemcpyAsync(n->ds_in, inputs, count * npx * 2 * 4, hipMemcpyHostToDevice, n->s));
+  HIPCHK(hipMemcpyAsync(n->ds_tg, targets, count * npx * 4, hipMemcpyHostToDevice, n->s));
+  HIPCHK(hipStreamSynchronize(n->s));
+  n->ds_n = count;
+  return 0;
+}
+
+int smili_unet_reset_optimizer(smili_unet* n) {
+  if (!n) return fail(-1, "null argument");
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n->np)), dim3(kThreads), 0, n->s, n->m, n->np, 0.f);
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n->np)), dim3(kThreads), 0, n->s, n->v, n->np, 0.f);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(n->s));
+  n->step = 0;
+  return 0;
+}
+
+static int train_steps(smili_unet* n, int64_t nsteps, const float* hyper) {
+  const size_t npx = (size_t)n->H * n->W;
+  const float b1 = hyper[0], b2 = hyper[1], eps = hyper[2], wd = hyper[3];
+  for (int64_t s = 0; s < nsteps; ++s) {
+    hipLaunchKernelGGL(k_gather, dim3(grid_for((size_t)n->B * npx)), dim3(kThreads), 0, n->s, n->ds_in, n->ds_tg,
+                       n->idx + s * n->B, n->xin, n->tgt, n->B, n->H, n->W);
+    zero_grad(n);
+    forward(n, n->B);
+    head(n, n->B, true, n->loss + s);
+    backward(n, n->B);
+    n->step += 1;
+    const float bc1 = 1.f - powf(b1, (float)n->step), bc2 = 1.f - powf(b2, (float)n->step);
+    hipLaunchKernelGGL(k_adam, dim3(grid_for(n->np)), dim3(kThreads), 0, n->s, n->p, n->m, n->v, n->g, n->np,
+                       n->lr, (int)s, b1, b2, eps, wd, bc1, bc2);
+    cast_weights(n);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+static int upload_steps(smili_unet* n, const int64_t* index, int64_t nsteps, const float* lr) {
+  const int64_t ni = nsteps * n->B;
+  for (int64_t i = 0; i < ni; ++i) {
+    if (index[i] < 0 || (index[i] >> 3) >= n->ds_n) return fail(-1, "index out of range at " + std::to_string(i));
+    if ((index[i] & 7) != 0 && n->H != n->W)
+      return fail(-1, "symmetries other than the identity need square images (index " + std::to_string(i) + ")");
+  }
+  HIPCHK(hipStreamSynchronize(n->s));
+  if (ni > n->nidx) {
+    if (n->idx) {
+      HIPCHK(hipFree(n->idx));
+      for (auto& a : n->allocs)
+        if (a == n->idx) a = nullptr;
+      n->idx = nullptr;
+    }
+    int r = dalloc(n, (void**)&n->idx, ni * 8);
+    if (r) return r;
+    n->nidx = ni;
+  }
+  if (nsteps > n->nlr) {
+    if (n->lr) {
+      HIPCHK(hipFree(n->lr));
+      for (auto& a : n->allocs)
+        if (a == n->lr) a = nullptr;
+      n->lr = nullptr;
+    }
+    int r = dalloc(n, (void**)&n->lr, nsteps * 4);
+    if (r) return r;
+    n->nlr = nsteps;
+  }
+  int r = ensure_loss(n, nsteps);
+  if (r) return r;
+  HIPCHK(hipMemcpyAsync(n->idx, index, ni * 8, hipMemcpyHostToDevice, n->s));
+  HIPCHK(hipMemcpyAsync(n->lr, lr, nsteps * 4, hipMemcpyHostToDevice, n->s));
+  hipLaunchKernelGGL(k_fill_d, dim3(grid_for(nsteps)), dim3(kThreads), 0, n->s, n->loss, (size_t)nsteps);
+  return 0;
+}
+
+int smili_unet_train(smili_unet* n, const int64_t* index, int64_t nsteps, const float* lr, const float* hyper,
+                     double* losses) {
+  if (!n || !index || !lr || !hyper) return fail(-1, "null argument");
+  if (nsteps < 1) return fail(-1, "nsteps must be >= 1");
+  if (!n->ds_n) return fail(-1, "no training set uploaded");
+  for (int64_t s = 0; s < nsteps; ++s)
+    if (!std::isfinite(lr[s]) || lr[s] < 0.f) return fail(-1, "invalid learning rate at step " + std::to_string(s));
+  int r = upload_steps(n, index, nsteps, lr);
+  if (r) return r;
+  r = train_steps(n, nsteps, hyper);
+  if (r) return r;
+  if (losses) HIPCHK(hipMemcpyAsync(losses, n->loss, nsteps * sizeof(double), hipMemcpyDeviceToHost, n->s));
+  HIPCHK(hipStreamSynchronize(n->s));
+  if (losses) {
+    const double norm = (double)n->B * n->H * n->W;
+    for (int64_t s = 0; s < nsteps; ++s) losses[s] /= norm;
+  }
+  return 0;
+}
+
+int smili_unet_time_steps(smili_unet* n, int32_t reps, float* ms) {
+  if (!n || !ms) return fail(-1, "null argument");
+  if (reps < 1) return fail(-1, "reps must be >= 1");
+  if (n->ds_n < n->B) return fail(-1, "the uploaded set must hold at least one batch");
+  std::vector<int64_t> index((size_t)reps * n->B);
+  for (size_t i = 0; i < index.size(); ++i) index[i] = (int64_t)(i % n->B) * 8;
+  std::vector<float> lr(reps, 0.f);
+  int r = upload_steps(n, index.data(), reps, lr.data());
+  if (r) return r;
+  const float hyper[4] = {0.9f, 0.999f, 1e-8f, 0.f};
+  HIPCHK(hipStreamSynchronize(n->s));
+  HIPCHK(hipEventRecord(n->ev0, n->s));
+  r = train_steps(n, reps, hyper);
+  if (r) return r;
+  HIPCHK(hipEventRecord(n->ev1, n->s));
+  // forward passes alone
+  for (int i = 0; i < reps; ++i) forward(n, n->B);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(n->ev2, n->s));
+  HIPCHK(hipEventSynchronize(n->ev2));
+  float t_train = 0.f, t_fwd = 0.f;
+  HIPCHK(hipEventElapsedTime(&t_train, n->ev0, n->ev1));
+  HIPCHK(hipEventElapsedTime(&t_fwd, n->ev1, n->ev2));
+  ms[0] = t_train / reps;
+  ms[1] = t_fwd / reps;
+  return 0;
+}
+
+int smili_unet_flops(smili_unet* n, double* flop) {
+  if (!n || !flop) return fail(-1, "null argument");
+  const double f = conv_flops_fwd(n);
+  // training: forward + input gradient (all but conv_in) + weight gradient
+  const double f_in = 2.0 * n->B * n->H * n->W * kBase * 9.0 * 2;
+  flop[0] = 3.0 * f - f_in;
+  flop[1] = f;
+  return 0;
+}
+
+}  // extern "C"

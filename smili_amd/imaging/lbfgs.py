@@ -41,7 +41,9 @@ class SolveInfo:
     success: bool
     wall_s: float
     eval_s: float
-    history: list = field(default_factory=list)
+    history: list = field(default_factory=list)      # cost after each iteration
+    history_t: list = field(default_factory=list)    # seconds since the start, after each iteration
+    stopped: bool = False                            # reached stop_cost
 
 
 def _summary(terms, inormfactr, x, nz=1):
@@ -72,14 +74,15 @@ def _summary(terms, inormfactr, x, nz=1):
 
 
 def solve(problem: CostProblem, x0, niter=1000, nonneg=True, m=None, factr=None, pgtol=None,
-          nprint=500, inormfactr=1.0, callback=None, verbose=True):
-    """L-BFGS-B over ``problem.cost_grad`` from x0 (scaled units).
-    Returns (x, SolveInfo)."""
+          nprint=500, inormfactr=1.0, callback=None, verbose=True, stop_cost=None):
+    """L-BFGS-B over ``problem.cost_grad`` from x0 (scaled units); with
+    ``stop_cost`` it also ends at the first iterate whose cost is at or
+    below it.  Returns (x, SolveInfo)."""
     m = lbfgsbprms["m"] if m is None else m
     factr = lbfgsbprms["factr"] if factr is None else factr
     pgtol = lbfgsbprms["pgtol"] if pgtol is None else pgtol
-    state = {"nfev": 0, "eval_s": 0.0, "it": 0, "last": None}
-    hist = []
+    state = {"nfev": 0, "eval_s": 0.0, "it": 0, "last": None, "stopped": False}
+    hist, hist_t = [], []
     nz = getattr(problem, "Nz", 1)
 
     def fg(x):
@@ -93,11 +96,15 @@ def solve(problem: CostProblem, x0, niter=1000, nonneg=True, m=None, factr=None,
     def cb(xk):
         state["it"] += 1
         hist.append(state["last"]["cost"])
+        hist_t.append(time.perf_counter() - t0)
         if verbose and nprint > 0 and state["it"] % nprint == 0:
             print("Iteration :%5d/%5d" % (state["it"], niter))
             print(_summary(state["last"], inormfactr, xk, nz))
         if callback is not None:
             callback(state["it"], xk, state["last"])
+        if stop_cost is not None and state["last"]["cost"] <= stop_cost:
+            state["stopped"] = True
+            raise StopIteration
 
     bounds = [(0.0, None)] * np.asarray(x0).size if nonneg else None
     t0 = time.perf_counter()
@@ -118,7 +125,7 @@ def solve(problem: CostProblem, x0, niter=1000, nonneg=True, m=None, factr=None,
         print(res.message)
     info = SolveInfo(niter=int(res.nit), nfev=int(state["nfev"]), cost=float(c), terms=terms,
                      message=str(res.message), success=bool(res.success), wall_s=wall,
-                     eval_s=state["eval_s"], history=hist)
+                     eval_s=state["eval_s"], history=hist, history_t=hist_t, stopped=state["stopped"])
     return res.x, info
 
 

@@ -7,7 +7,11 @@ visibilities are exact closed forms and need no transform:
   V(u, v) = F exp(-2 pi^2 s^2 (u^2 + v^2)) exp(+2 pi i (u x0 + v y0)),
   s = t / (2 sqrt(2 ln 2));
 * thin ring of diameter d:  V = F J0(pi d rho),  rho = |(u, v)|; blurring by a
-  Gaussian multiplies by the Gaussian's envelope.
+  Gaussian multiplies by the Gaussian's envelope;
+* uniform disk of diameter d:  V = F 2 J1(pi d rho) / (pi d rho);
+* crescent: a disk minus a smaller disk of the same surface brightness
+  shifted inside it (a disk's transform times exp(+2 pi i (u sx + v sy))
+  for the shift).
 
 The +2 pi i sign is SMILI's (iflag=0 in src/wrapfinufft/wrapfinufft.f90:108-111)
 with sky positions x = dx (ix - Nxref) (smili/imdata/imdata.py:743-744).
@@ -22,7 +26,7 @@ import math
 from dataclasses import dataclass
 
 import numpy as np
-from scipy.special import j0
+from scipy.special import j0, j1
 
 UAS = math.pi / 180.0 / 3600.0 / 1e6      # 1 micro-arcsecond in radians
 C_LIGHT = 299792458.0
@@ -47,6 +51,38 @@ class Ring:
     blur_fwhm: float  # rad
 
 
+@dataclass
+class Disk:
+    flux: float
+    diameter: float   # rad
+    blur_fwhm: float = 0.0
+    x0: float = 0.0
+    y0: float = 0.0
+
+
+@dataclass
+class Crescent:
+    """Disk of diameter d_out minus a disk of diameter d_in < d_out shifted
+    by (sx, sy) (|shift| <= (d_out - d_in) / 2 keeps it inside), equal
+    surface brightness; total flux `flux`."""
+    flux: float
+    d_out: float
+    d_in: float
+    sx: float
+    sy: float
+    blur_fwhm: float = 0.0
+    x0: float = 0.0
+    y0: float = 0.0
+
+
+def _disk_env(d, rho):
+    a = math.pi * d * rho
+    out = np.ones_like(a)
+    nz = a > 1e-12
+    out[nz] = 2.0 * j1(a[nz]) / a[nz]
+    return out
+
+
 def _sig(fwhm):
     return fwhm / (2.0 * math.sqrt(2.0 * math.log(2.0)))
 
@@ -62,6 +98,17 @@ def sky_vis(sky, u, v):
             s = _sig(c.fwhm)
             out += c.flux * np.exp(-2 * math.pi ** 2 * s * s * rho2) * \
                 np.exp(2j * math.pi * (u * c.x0 + v * c.y0))
+        elif isinstance(c, (Disk, Crescent)):
+            rho = np.sqrt(rho2)
+            s = _sig(c.blur_fwhm)
+            env = np.exp(-2 * math.pi ** 2 * s * s * rho2) * np.exp(2j * math.pi * (u * c.x0 + v * c.y0))
+            if isinstance(c, Disk):
+                out += c.flux * _disk_env(c.diameter, rho) * env
+            else:
+                q = (c.d_in / c.d_out) ** 2
+                f_out = c.flux / (1.0 - q)
+                out += (f_out * _disk_env(c.d_out, rho)
+                        - f_out * q * _disk_env(c.d_in, rho) * np.exp(2j * math.pi * (u * c.sx + v * c.sy))) * env
         else:
             s = _sig(c.blur_fwhm)
             out += c.flux * j0(math.pi * c.diameter * np.sqrt(rho2)) * \
@@ -85,6 +132,8 @@ def sky_image(sky, Nx, Ny, dx, dy, Nxref=None, Nyref=None, oversample=4):
             s = _sig(c.fwhm)
             img += c.flux / (2 * math.pi * s * s) * \
                 np.exp(-((X - c.x0) ** 2 + (Y - c.y0) ** 2) / (2 * s * s))
+        elif isinstance(c, (Disk, Crescent)):
+            raise NotImplementedError("sky_image renders Gaussians and rings only")
         else:
             # ring as an annulus profile: radial Gaussian of the blur width
             s = _sig(c.blur_fwhm)

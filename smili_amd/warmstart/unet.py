@@ -1,0 +1,178 @@
+"""ctypes binding of libsmili_unet.so (C ABI in include/smili_unet.h).
+
+The network, its parameter layout and its rounding points are defined in
+include/smili_unet.h.  There is no fallback: without the library every
+call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_HERE, "libsmili_unet.so")
+_lib = None
+
+_pf = C.POINTER(C.c_float)
+_pd = C.POINTER(C.c_double)
+_pl = C.POINTER(C.c_int64)
+
+
+class UNetError(RuntimeError):
+    pass
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise UNetError(f"{LIB_PATH} is missing: build it with python -m smili_amd.build")
+    lib = C.CDLL(LIB_PATH)
+    lib.smili_unet_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]
+    lib.smili_unet_destroy.argtypes = [C.c_void_p]
+    lib.smili_unet_destroy.restype = None
+    lib.smili_unet_nparams.restype = C.c_int64
+    lib.smili_unet_set_params.argtypes = [C.c_void_p, _pf]
+    lib.smili_unet_get_params.argtypes = [C.c_void_p, _pf]
+    lib.smili_unet_forward.argtypes = [C.c_void_p, _pf, C.c_int32, _pf]
+    lib.smili_unet_loss_grad.argtypes = [C.c_void_p, _pf, _pf, C.c_int32, _pd, _pf]
+    lib.smili_unet_set_data.argtypes = [C.c_void_p, _pf, _pf, C.c_int64]
+    lib.smili_unet_train.argtypes = [C.c_void_p, _pl, C.c_int64, _pf, _pf, _pd]
+    lib.smili_unet_reset_optimizer.argtypes = [C.c_void_p]
+    lib.smili_unet_time_steps.argtypes = [C.c_void_p, C.c_int32, _pf]
+    lib.smili_unet_flops.argtypes = [C.c_void_p, _pd]
+    lib.smili_unet_last_error.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise UNetError(f"smili_unet error {rc}: {load().smili_unet_last_error().decode()}")
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None and a.shape != shape:
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+def nparams():
+    return int(load().smili_unet_nparams())
+
+
+class UNet:
+    """Handle of one network on the current HIP device: batches of up to
+    ``batch`` images of height x width (multiples of 64 and 128)."""
+
+    def __init__(self, batch=32, height=128, width=128, params=None):
+        lib = load()
+        h = C.c_void_p()
+        _check(lib.smili_unet_create(int(batch), int(height), int(width), C.byref(h)))
+        self._h = h
+        self.batch, self.height, self.width = int(batch), int(height), int(width)
+        if params is not None:
+            self.params = params
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().smili_unet_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self):
+        out = np.empty(nparams(), np.float32)
+        _check(load().smili_unet_get_params(self._h, out.ctypes.data_as(_pf)))
+        return out
+
+    @params.setter
+    def params(self, p):
+        p = _f32(p, (nparams(),))
+        _check(load().smili_unet_set_params(self._h, p.ctypes.data_as(_pf)))
+
+    def forward(self, x):
+        """x[n, h, w, 2] -> output[n, h, w] (float32, >= 0)."""
+        x = _f32(x)
+        if x.ndim != 4 or x.shape[1:] != (self.height, self.width, 2):
+            raise ValueError(f"input must be [n, {self.height}, {self.width}, 2], got {x.shape}")
+        out = np.empty(x.shape[:3], np.float32)
+        _check(load().smili_unet_forward(self._h, x.ctypes.data_as(_pf), x.shape[0], out.ctypes.data_as(_pf)))
+        return out
+
+    def loss_grad(self, x, target):
+        """Mean squared error of n <= batch images and its gradient."""
+        x = _f32(x)
+        n = x.shape[0]
+        t = _f32(target, (n, self.height, self.width))
+        if x.shape != (n, self.height, self.width, 2):
+            raise ValueError(f"input must be [n, {self.height}, {self.width}, 2]")
+        loss = C.c_double()
+        g = np.empty(nparams(), np.float32)
+        _check(load().smili_unet_loss_grad(self._h, x.ctypes.data_as(_pf), t.ctypes.data_as(_pf), n,
+                                           C.byref(loss), g.ctypes.data_as(_pf)))
+        return loss.value, g
+
+    def set_data(self, inputs, targets):
+        x = _f32(inputs)
+        n = x.shape[0]
+        t = _f32(targets, (n, self.height, self.width))
+        if x.shape != (n, self.height, self.width, 2):
+            raise ValueError(f"inputs must be [n, {self.height}, {self.width}, 2]")
+        _check(load().smili_unet_set_data(self._h, x.ctypes.data_as(_pf), t.ctypes.data_as(_pf), n))
+        self.ndata = n
+
+    def train(self, index, lr, beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=0.0):
+        """Adam steps over batches index[step, batch] of the uploaded set;
+        returns the per-step losses."""
+        idx = np.ascontiguousarray(index, dtype=np.int64)
+        if idx.ndim != 2 or idx.shape[1] != self.batch:
+            raise ValueError(f"index must be [nsteps, {self.batch}]")
+        ns = idx.shape[0]
+        lr = _f32(np.broadcast_to(np.asarray(lr, np.float32), (ns,)))
+        hyper = _f32([beta1, beta2, eps, weight_decay])
+        losses = np.empty(ns, np.float64)
+        _check(load().smili_unet_train(self._h, idx.ctypes.data_as(_pl), ns, lr.ctypes.data_as(_pf),
+                                       hyper.ctypes.data_as(_pf), losses.ctypes.data_as(_pd)))
+        return losses
+
+    def reset_optimizer(self):
+        _check(load().smili_unet_reset_optimizer(self._h))
+
+    def time_steps(self, reps=10):
+        """(ms per training step, ms per forward pass) of a full batch."""
+        ms = np.zeros(2, np.float32)
+        _check(load().smili_unet_time_steps(self._h, int(reps), ms.ctypes.data_as(_pf)))
+        return float(ms[0]), float(ms[1])
+
+    def flops(self):
+        """(3x3-convolution FLOP of a training step, of a forward pass)."""
+        f = np.zeros(2, np.float64)
+        _check(load().smili_unet_flops(self._h, f.ctypes.data_as(_pd)))
+        return float(f[0]), float(f[1])
+
+    def save(self, path, **meta):
+        np.savez_compressed(path, params=self.params, height=self.height, width=self.width,
+                            **{k: np.asarray(v) for k, v in meta.items()})
+
+    @classmethod
+    def from_file(cls, path, batch=1):
+        z = np.load(path)
+        net = cls(batch, int(z["height"]), int(z["width"]))
+        net.params = z["params"]
+        return net, {k: z[k] for k in z.files if k not in ("params", "height", "width")}
