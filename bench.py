@@ -10,7 +10,11 @@ evaluation (calc_cost) with x already in HBM.
 chi-square + L1 + TSV + Rt as one joint cost (fftim3d.calc_cost); one step =
 one evaluation of the whole movie.  ``--workload survey``: 512 independent
 problems of 64^2..256^2 evaluated together; one step = one evaluation of all
-512 (the value counts problem evaluations per second).
+512 (the value counts problem evaluations per second).  ``--workload
+warmstart``: time-to-image of the learned warm start against the plain
+L-BFGS-B imager on held-out 128^2 cases (L1 + TSV); one step = one case,
+the value is the median speed-up, and the roofline is the MFMA one of the
+network's dominant training kernel.
 
 Prints one JSON line.  Besides the throughput it reports:
   roofline      -- the dominant kernel's algorithmic bytes per launch over its
@@ -291,6 +295,89 @@ def _roofline(g, kws, fp64, ms, steps, workload="single_large"):
             "stage_ms": {k: round(v, 5) for k, v in prof.items() if v > 0}}
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def _unet_roofline(params, batch=32, reps=20):
+    """MFMA roofline of the network's dominant kernel launch in a training
+    step (batch 32, 128^2): every 3x3 layer's forward, input-gradient and
+    weight-gradient launch timed with HIP events on the handle's stream."""
+    import numpy as np
+    from smili_amd.warmstart.unet import LAYERS, UNet
+    net = UNet(batch, 128, 128, params=params)
+    x = np.random.default_rng(0).standard_normal((batch, 128, 128, 2)).astype(np.float32)
+    net.set_data(x, np.zeros((batch, 128, 128), np.float32))
+    step_ms, fwd_ms = net.time_steps(10)
+    step_flop, fwd_flop = net.flops()
+    launches = []
+    for li in range(1, len(LAYERS)):
+        for ps, pname in ((0, "forward"), (1, "input_grad"), (2, "weight_grad")):
+            ms, fl = net.time_layer(li, ps, reps)
+            launches.append((ms, fl, LAYERS[li][0], pname))
+    net.close()
+    ms, fl, lname, pname = max(launches)
+    kern = "k_conv3x3_wgrad" if pname == "weight_grad" else "k_conv3x3"
+    achieved = fl / (ms * 1e-3) / 1e12
+    return {"bound": "mfma", "achieved": round(achieved, 2), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None,
+            "kernel": f"{kern} ({lname}, {pname}, batch {batch}, 128^2)", "kernel_ms": round(ms, 4),
+            "kernel_flop": fl, "train_step_ms": round(step_ms, 3),
+            "train_step_TFLOPs": round(step_flop / (step_ms * 1e-3) / 1e12, 1), "forward_ms": round(fwd_ms, 3),
+            "forward_TFLOPs": round(fwd_flop / (fwd_ms * 1e-3) / 1e12, 1),
+            "launches": {f"{n}/{p}": round(f / (m * 1e-3) / 1e12, 1) for m, f, n, p in launches}}
+
+
+def bench_warmstart(args):
+    import numpy as np
+    import torch
+    torch.cuda.set_device(0)
+    from smili_amd.warmstart import data as D
+    from smili_amd.warmstart import solve as S
+    from smili_amd.warmstart.unet import DEFAULT_WEIGHTS, UNet
+    net, meta = UNet.from_file(DEFAULT_WEIGHTS, batch=1)
+    base = 2_000_000 + 1000 * args.seed          # held out: training seeds are below 1e6
+    for s in range(base - args.warmup, base):
+        S.time_to_image(net, s)
+    rows = [S.time_to_image(net, s) for s in range(base, base + args.steps)]
+    sm = S.summary(rows)
+    roof = _unet_roofline(net.params)
+    cpu = None
+    if not args.no_cpu:
+        case = D.make_case(base)
+        prep = D.prepare_case(case)
+        per, n, el, thr = _cpu_time_per_eval(dict(prep.kwargs, x0=prep.x0), args.cpu_budget)
+        nfev = float(np.median([r["plain_iters"] for r in rows])) * 1.05
+        cpu = {"value": round(per * nfev, 3), "unit": "s per image (plain L-BFGS-B to the matched objective)",
+               "cores": thr, "kind": "port",
+               "sample": f"{n} evaluations of one 128^2 case ({el:.1f} s, C + OpenMP FP64 restatement of the "
+                         f"cost) times the median plain evaluation count to the matched objective"}
+    net.close()
+    line = {
+        "metric": "time-to-image speedup, learned warm start vs plain L-BFGS-B (128^2, matched objective 1e-6)",
+        "value": round(sm["median_time_ratio"], 3) if sm["median_time_ratio"] else None,
+        "unit": "x",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * sm["median_warm_s"], 2) if sm["median_warm_s"] else None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16 network (f32 accumulation) + f64 imaging cost",
+        "data": "synthetic",
+        "config": {"workload": "warmstart: held-out seeds of the training distribution (rings, crescents, 1-5 "
+                               "Gaussians, disks; 0.3-2 Jy), 128x128 at 1 uas, 6-10 stations at 230 GHz, "
+                               "full-visibility chi-square + L1 + TSV, L-BFGS-B m=10",
+                   "seeds": [base, base + args.steps], "reached": sm["reached"],
+                   "median_iter_ratio": sm["median_iter_ratio"], "median_plain_s": sm["median_plain_s"],
+                   "median_warm_s": sm["median_warm_s"], "max_nrmse": sm["max_nrmse"],
+                   "proposals_used": sm["proposals_used"], "n_train": int(meta.get("n_train", 0))},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -302,7 +389,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp64",
                     help="NUFFT arithmetic (chi-square and regularizers are FP64 in both)")
-    ap.add_argument("--workload", choices=("single_large", "movie", "survey"), default="single_large")
+    ap.add_argument("--workload", choices=("single_large", "movie", "survey", "warmstart"), default="single_large")
     ap.add_argument("--pmc-run", action="store_true",
                     help="only evaluations plus the calibration copy (run under rocprofv3 --pmc)")
     ap.add_argument("--pmc-ingest", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -313,9 +400,16 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the objective+gradient path has no multi-GPU "
-              "component in this build (see DESIGN.md for how it would shard); run with --gpus 1",
-              file=sys.stderr)
+              "component in this build (it would shard uv points by timestamp and all-reduce the "
+              "gradient); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "warmstart":
+        if "--steps" not in sys.argv:
+            args.steps = 20
+        if "--warmup" not in sys.argv:
+            args.warmup = 2
+        bench_warmstart(args)
+        return
 
     import numpy as np
     import torch

@@ -74,6 +74,14 @@ int smili_unet_reset_optimizer(smili_unet* net);
  * ms[0] = mean step, ms[1] = mean forward pass. */
 int smili_unet_time_steps(smili_unet* net, int32_t reps, float* ms);
 
+/* Times `reps` launches of one 3x3 convolution kernel of layer `layer`
+ * (1..13, the parameter order above; 0 is the float32 first convolution)
+ * on the activations of the last forward pass: pass 0 forward, 1 input
+ * gradient, 2 weight gradient (this one adds into the gradient buffer).
+ * ms = mean launch time (HIP events on the handle's stream), flop = FLOP
+ * of one launch for a full batch. */
+int smili_unet_time_layer(smili_unet* net, int32_t layer, int32_t pass, int32_t reps, float* ms, double* flop);
+
 /* Work of one training step / forward pass of a full batch:
  * flop[0] = 3x3-convolution FLOP of a training step, flop[1] = of a
  * forward pass. */

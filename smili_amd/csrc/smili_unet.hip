@@ -20,8 +20,9 @@
 //    pixels are the reduction axis, so dP and the input halo are staged
 //    transposed ([channel][pixel]) in LDS, the halo in three copies
 //    shifted by dx so every 8-pixel fragment is a 16-byte aligned read.
-//    Work-groups split the pixel tiles and add their partial sums with
-//    float atomics.
+//    Work-groups split the pixel tiles; each writes its partial sums to a
+//    row of a scratch array that k_sum_rows reduces (float atomics of every
+//    group on the same addresses would serialise).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -92,9 +93,27 @@ __device__ __forceinline__ const bf16_t* gather_src(const ConvArgs& a, int b, in
   return a.x2 + ((size_t)(b * a.h + yy) * a.w_ + xx) * a.c2 + (c - a.c1);
 }
 
-template <int NCB>  // output-channel blocks of 16 per work-group
-__global__ __launch_bounds__(kThreads) void k_conv3x3(ConvArgs a) {
-  __shared__ __attribute__((aligned(16))) bf16_t halo[kHaloH * kHaloW * 32];
+// Work-group tile: 8 rows x TW columns of output pixels (TW = 16 or 32) x
+// NCB * 16 output channels; wave w computes rows 2w, 2w + 1 (TW / 8 blocks
+// of 16 pixels) for all NCB channel blocks.  Per 32-channel slice of the
+// input, the halo (10 x (TW + 2) pixels) and the slice's weights
+// (NCB * 16 x 9 taps x 32) sit in LDS; the next slice's halo is loaded into
+// registers while the current one feeds the matrix cores (the weights,
+// L2-resident, are copied at staging; a second work-group per CU covers
+// that latency).
+constexpr int kWStride = 9 * 32 + 8;   // bf16 per output channel in LDS: 592 bytes, bank-spread
+
+template <int TW, int NCB>
+__global__ __launch_bounds__(kThreads, 2) void k_conv3x3(ConvArgs a) {
+  constexpr int HW = TW + 2;                      // halo width
+  constexpr int NHALO = kHaloH * HW * 4;          // 16-byte halo pieces
+  constexpr int NCO = NCB * 16;
+  constexpr int NWT = NCO * 9 * 4;                // 16-byte weight pieces
+  constexpr int RH = (NHALO + kThreads - 1) / kThreads;
+  constexpr int RW = (NWT + kThreads - 1) / kThreads;
+  constexpr int NPB = TW / 8;                     // 16-pixel blocks per wave (2 rows of TW)
+  __shared__ __attribute__((aligned(16))) bf16_t halo[kHaloH * HW * 32];
+  __shared__ __attribute__((aligned(16))) bf16_t wl[NCO * kWStride];
   const int item = xcd_item(a.n * a.tiles_y * a.tiles_x * a.ncob);
   if (item < 0) return;
   const int cob = item % a.ncob;
@@ -102,52 +121,79 @@ __global__ __launch_bounds__(kThreads) void k_conv3x3(ConvArgs a) {
   const int tx = t % a.tiles_x;
   t /= a.tiles_x;
   const int ty = t % a.tiles_y, b = t / a.tiles_y;
-  const int y0 = ty * kTileH, x0 = tx * kTileW, co0 = cob * NCB * 16;
+  const int y0 = ty * kTileH, x0 = tx * TW, co0 = cob * NCO;
   const int cin = a.c1 + a.c2;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int ln = lane & 15, lg = lane >> 4;
 
-  f32x4 acc[2][NCB];
+  uint4 ph[RH];
+  auto fetch = [&](int c0) {
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+    for (int r = 0; r < RH; ++r) {
+      const int e = threadIdx.x + r * kThreads;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (e < NHALO) {
+        const int p = e >> 2, g = e & 3;
+        const int hy = p / HW, hx = p - hy * HW;
+        const int yy = y0 + hy - 1, xx = x0 + hx - 1;
+        if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_)
+          v = *reinterpret_cast<const uint4*>(gather_src(a, b, yy, xx, c0 + g * 8));
+      }
+      ph[r] = v;
+    }
+  };
+  auto stage = [&](int c0) {
+#pragma unroll
+    for (int r = 0; r < RH; ++r) {
+      const int e = threadIdx.x + r * kThreads;
+      if (e < NHALO) *reinterpret_cast<uint4*>(halo + halo_off(e >> 2, e & 3)) = ph[r];
+    }
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      const int e = threadIdx.x + r * kThreads;
+      if (e < NWT) {
+        const int q = e & 3, ct = e >> 2, co = ct / 9, tap = ct - co * 9;   // ct = co * 9 + tap
+        *reinterpret_cast<uint4*>(wl + co * kWStride + tap * 32 + q * 8) =
+            *reinterpret_cast<const uint4*>(a.w + ((size_t)(co0 * 9 + ct)) * cin + c0 + q * 8);
+      }
+    }
+  };
+
+  f32x4 acc[NPB][NCB];
+#pragma unroll
+  for (int i = 0; i < NPB; ++i)
 #pragma unroll
     for (int j = 0; j < NCB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
+  fetch(0);
   for (int c0 = 0; c0 < cin; c0 += 32) {
     __syncthreads();
-    for (int e = threadIdx.x; e < kHaloH * kHaloW * 4; e += kThreads) {
-      const int p = e >> 2, g = e & 3;
-      const int hy = p / kHaloW, hx = p - hy * kHaloW;
-      const int yy = y0 + hy - 1, xx = x0 + hx - 1;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_)
-        v = *reinterpret_cast<const uint4*>(gather_src(a, b, yy, xx, c0 + g * 8));
-      *reinterpret_cast<uint4*>(halo + halo_off(p, g)) = v;
-    }
+    stage(c0);
     __syncthreads();
-    const bf16_t* wrow = a.w + (size_t)(co0 + ln) * 9 * cin + c0 + lg * 8;
+    if (c0 + 32 < cin) fetch(c0 + 32);
 #pragma unroll
     for (int tap = 0; tap < 9; ++tap) {
       const int dy = tap / 3, dx = tap - 3 * (tap / 3);
-      bf16x8 bx[2];
+      bf16x8 aw[NCB];
 #pragma unroll
-      for (int pb = 0; pb < 2; ++pb) {
-        const int p = (wv * 2 + pb + dy) * kHaloW + ln + dx;
-        bx[pb] = *reinterpret_cast<const bf16x8*>(halo + halo_off(p, lg));
-      }
+      for (int cb = 0; cb < NCB; ++cb)
+        aw[cb] = *reinterpret_cast<const bf16x8*>(wl + (cb * 16 + ln) * kWStride + tap * 32 + lg * 8);
 #pragma unroll
-      for (int cb = 0; cb < NCB; ++cb) {
-        const bf16x8 aw = *reinterpret_cast<const bf16x8*>(wrow + (size_t)cb * 16 * 9 * cin + tap * cin);
+      for (int pb = 0; pb < NPB; ++pb) {
+        // block pb: row 2 wv + pb / (TW / 16), columns 16 (pb % (TW / 16)) ..
+        const int row = wv * 2 + pb / (TW / 16), col = (pb % (TW / 16)) * 16;
+        const int p = (row + dy) * HW + col + ln + dx;
+        const bf16x8 bx = *reinterpret_cast<const bf16x8*>(halo + halo_off(p, lg));
 #pragma unroll
-        for (int pb = 0; pb < 2; ++pb)
-          acc[pb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aw, bx[pb], acc[pb][cb], 0, 0, 0);
+        for (int cb = 0; cb < NCB; ++cb)
+          acc[pb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aw[cb], bx, acc[pb][cb], 0, 0, 0);
       }
     }
   }
   // D[row = channel 4*lg + i][col = pixel ln]
 #pragma unroll
-  for (int pb = 0; pb < 2; ++pb) {
-    const int y = y0 + wv * 2 + pb, x = x0 + ln;
+  for (int pb = 0; pb < NPB; ++pb) {
+    const int y = y0 + wv * 2 + pb / (TW / 16), x = x0 + (pb % (TW / 16)) * 16 + ln;
     const size_t o = ((size_t)(b * a.h + y) * a.w_ + x) * a.cout;
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb) {
@@ -177,11 +223,31 @@ __global__ __launch_bounds__(kThreads) void k_conv3x3(ConvArgs a) {
 // ---------------------------------------------------------------------------
 // 3x3 convolution, weight gradient
 // ---------------------------------------------------------------------------
+// 8 x 8 transpose of bfloat16: v[j] holds channels 0..7 of pixel j, w[i]
+// receives pixels 0..7 of channel i
+__device__ __forceinline__ void transpose8(const uint4 (&v)[8], uint4 (&w)[8]) {
+  uint32_t s[8][4];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    s[j][0] = v[j].x; s[j][1] = v[j].y; s[j][2] = v[j].z; s[j][3] = v[j].w;
+  }
+  uint32_t o[8][4];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t lo = s[2 * k][i >> 1], hi = s[2 * k + 1][i >> 1];
+      o[i][k] = (i & 1) ? ((lo >> 16) | (hi & 0xFFFF0000u)) : ((lo & 0xFFFFu) | (hi << 16));
+    }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) w[i] = make_uint4(o[i][0], o[i][1], o[i][2], o[i][3]);
+}
+
 struct WgradArgs {
   const bf16_t* dp;  // [n][h][w][cout]
   const bf16_t* x1;
   const bf16_t* x2;
-  float* dw;         // [cout][9][c1 + c2], accumulated
+  float* dw;         // [ngroups][cout][9][c1 + c2]: one row of partial sums per pixel group
   int c1, c2, up1;
   int n, h, w_, cout;
   int tiles_x, tiles_y, ncob, nchunk, tpg, ngroups;
@@ -219,33 +285,38 @@ __global__ __launch_bounds__(kThreads) void k_conv3x3_wgrad(WgradArgs a) {
     const int ty = (tile / a.tiles_x) % a.tiles_y, b = tile / (a.tiles_x * a.tiles_y);
     const int y0 = ty * kTileH, x0 = tx * kTileW;
     __syncthreads();
-    // dP tile, transposed to [co][px]
-    for (int e = threadIdx.x; e < kTileH * kTileW * NCB * 2; e += kThreads) {
-      const int px = e / (NCB * 2), q = e - px * (NCB * 2);
-      const int y = y0 + (px >> 4), x = x0 + (px & 15);
-      const uint4 v = *reinterpret_cast<const uint4*>(
-          a.dp + ((size_t)(b * a.h + y) * a.w_ + x) * a.cout + co0 + q * 8);
-      const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+    // dP tile, transposed to [co][px]: a unit is 8 pixels (one row, columns
+    // 8 pg .. 8 pg + 7) x 8 channels, transposed in registers
+    for (int u = threadIdx.x; u < 16 * NCB * 2; u += kThreads) {
+      const int pg = u & 15, q = u >> 4;
+      const int y = y0 + (pg >> 1), xb = x0 + (pg & 1) * 8;
+      uint4 v[8];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) sdp[q * 8 + i][px] = (bf16_t)(w4[i >> 1] >> ((i & 1) * 16));
+      for (int j = 0; j < 8; ++j)
+        v[j] = *reinterpret_cast<const uint4*>(a.dp + ((size_t)(b * a.h + y) * a.w_ + xb + j) * a.cout + co0 + q * 8);
+      uint4 w[8];
+      transpose8(v, w);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) *reinterpret_cast<uint4*>(&sdp[q * 8 + i][pg * 8]) = w[i];
     }
-    // input halo slice, transposed to [ci][row][col] in three dx-shifted copies
-    for (int e = threadIdx.x; e < kHaloH * kHaloW * 4; e += kThreads) {
-      const int p = e >> 2, q = e & 3;
-      const int hy = p / kHaloW, hx = p - hy * kHaloW;
-      const int yy = y0 + hy - 1, xx = x0 + hx - 1;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_)
-        v = *reinterpret_cast<const uint4*>(gather_src(g, b, yy, xx, ci0 + q * 8));
-      const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+    // input halo slice, transposed to [ci][row][col] in three copies shifted
+    // by dx: a unit is (dx, halo row, 8 destination columns, 8 channels)
+    for (int u = threadIdx.x; u < 3 * kHaloH * 2 * 4; u += kThreads) {
+      const int q = u & 3, cg = (u >> 2) & 1, r = u >> 3;
+      const int hy = r % kHaloH, dx = r / kHaloH;
+      const int yy = y0 + hy - 1;
+      uint4 v[8];
 #pragma unroll
-      for (int dx = 0; dx < 3; ++dx) {
-        const int col = hx - dx;
-        if (col >= 0 && col < kTileW) {
-#pragma unroll
-          for (int i = 0; i < 8; ++i) sx[dx][q * 8 + i][hy][col] = (bf16_t)(w4[i >> 1] >> ((i & 1) * 16));
-        }
+      for (int j = 0; j < 8; ++j) {
+        const int xx = x0 + cg * 8 + dx + j - 1;
+        v[j] = make_uint4(0, 0, 0, 0);
+        if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_)
+          v[j] = *reinterpret_cast<const uint4*>(gather_src(g, b, yy, xx, ci0 + q * 8));
       }
+      uint4 w[8];
+      transpose8(v, w);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) *reinterpret_cast<uint4*>(&sx[dx][q * 8 + i][hy][cg * 8]) = w[i];
     }
     __syncthreads();
 #pragma unroll
@@ -279,7 +350,7 @@ __global__ __launch_bounds__(kThreads) void k_conv3x3_wgrad(WgradArgs a) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int co = co0 + cb * 16 + lg * 4 + i, ci = ci0 + cib * 16 + ln;
-          unsafeAtomicAdd(a.dw + ((size_t)co * 9 + tap) * cin + ci, acc[j][cb][i]);
+          a.dw[(size_t)grp * a.cout * 9 * cin + ((size_t)co * 9 + tap) * cin + ci] = acc[j][cb][i];
         }
     }
   }
@@ -327,32 +398,61 @@ __global__ __launch_bounds__(kThreads) void k_conv_in(const float* __restrict__ 
   }
 }
 
-// dW[co][tap][ci] of the first convolution: each block writes the sums of
-// its pixels to part[block][576], reduced by k_sum_rows
+// dW[co][tap][ci] of the first convolution: a block stages the masked
+// output gradients (32 channels) and the 3x3 input neighbourhoods (18
+// values) of 256 pixels in LDS; thread t accumulates channel t / 8 against
+// (tap, ci) pairs j = t % 8, +8, +16 (< 18) and writes the block's sums to
+// part[block][576], reduced by k_sum_rows
+constexpr int kInPx = 256;
 __global__ __launch_bounds__(kThreads) void k_conv_in_wgrad(const bf16_t* __restrict__ dp,
                                                             const float* __restrict__ x, float* __restrict__ part,
                                                             int n, int h, int wd, int px_per_block) {
-  const int npx = n * h * wd;
-  const int pb = blockIdx.x * px_per_block, pe = min(npx, pb + px_per_block);
+  __shared__ float sg[kInPx][kBase + 1];
+  __shared__ float sx[kInPx][19];
+  const size_t npx = (size_t)n * h * wd;
+  const int co = threadIdx.x >> 3, jg = threadIdx.x & 7;
   float acc[3] = {0.f, 0.f, 0.f};
-  for (int p = pb; p < pe; ++p) {
-    const int xx = p % wd, yy = (p / wd) % h, b = p / (wd * h);
+  const size_t pb = (size_t)blockIdx.x * px_per_block, pe = min(npx, pb + px_per_block);
+  for (size_t p0 = pb; p0 < pe; p0 += kInPx) {
+    __syncthreads();
+    {
+      const size_t p = p0 + threadIdx.x;
+      if (p < pe) {
+        const uint4* src = reinterpret_cast<const uint4*>(dp + p * kBase);
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      const int e = threadIdx.x + r * kThreads;
-      if (e < kBase * 18) {
-        const int co = e / 18, j = e - co * 18, t = j >> 1, ci = j & 1;
-        const int sy = yy + t / 3 - 1, sx = xx + t % 3 - 1;
-        if (sy >= 0 && sy < h && sx >= 0 && sx < wd)
-          acc[r] = fmaf(bf2f(dp[(size_t)p * kBase + co]), x[((size_t)(b * h + sy) * wd + sx) * 2 + ci], acc[r]);
+        for (int q = 0; q < kBase / 8; ++q) {
+          const uint4 v = src[q];
+          const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+          for (int i = 0; i < 8; ++i) sg[threadIdx.x][q * 8 + i] = bf2f((w4[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu);
+        }
+        const int xx = p % wd, yy = (p / wd) % h, b = p / ((size_t)wd * h);
+#pragma unroll
+        for (int tp = 0; tp < 9; ++tp) {
+          const int sy = yy + tp / 3 - 1, sx_ = xx + tp % 3 - 1;
+          const bool ok = sy >= 0 && sy < h && sx_ >= 0 && sx_ < wd;
+          const float2 v = ok ? *reinterpret_cast<const float2*>(x + ((size_t)(b * h + sy) * wd + sx_) * 2)
+                              : make_float2(0.f, 0.f);
+          sx[threadIdx.x][2 * tp] = v.x;
+          sx[threadIdx.x][2 * tp + 1] = v.y;
+        }
+      } else {
+        for (int i = 0; i < kBase; ++i) sg[threadIdx.x][i] = 0.f;
+        for (int i = 0; i < 18; ++i) sx[threadIdx.x][i] = 0.f;
       }
     }
+    __syncthreads();
+    for (int k = 0; k < kInPx; ++k) {
+      const float gv = sg[k][co];
+      acc[0] = fmaf(gv, sx[k][jg], acc[0]);
+      acc[1] = fmaf(gv, sx[k][jg + 8], acc[1]);
+      if (jg < 2) acc[2] = fmaf(gv, sx[k][jg + 16], acc[2]);
+    }
   }
-#pragma unroll
-  for (int r = 0; r < 3; ++r) {
-    const int e = threadIdx.x + r * kThreads;
-    if (e < kBase * 18) part[(size_t)blockIdx.x * kBase * 18 + e] = acc[r];
-  }
+  float* row = part + (size_t)blockIdx.x * kBase * 18 + co * 18;
+  row[jg] = acc[0];
+  row[jg + 8] = acc[1];
+  if (jg < 2) row[jg + 16] = acc[2];
 }
 
 // ---------------------------------------------------------------------------
@@ -477,23 +577,26 @@ __global__ __launch_bounds__(kThreads) void k_relu_mask(const float* __restrict_
   }
 }
 
-// out[j] += sum_r part[r][j]: a block takes 4 columns, its 64 row slices
-// sum rows r = slice, slice + 64, ... and are combined in LDS
+// out[j] += sum_r part[r][j]: a block takes CB columns (consecutive
+// threads, coalesced), its 256 / CB row slices sum rows r = slice,
+// slice + 256 / CB, ... and are combined in LDS
+template <int CB>
 __global__ __launch_bounds__(256) void k_sum_rows(const float* __restrict__ part, int nrow, int ncol,
                                                   float* __restrict__ out) {
-  __shared__ float red[64][4];
-  const int c = threadIdx.x & 3, sl = threadIdx.x >> 2;
-  const int j = blockIdx.x * 4 + c;
+  constexpr int NS = 256 / CB;
+  __shared__ float red[NS][CB];
+  const int c = threadIdx.x % CB, sl = threadIdx.x / CB;
+  const size_t j = (size_t)blockIdx.x * CB + c;
   float s = 0.f;
-  if (j < ncol) {
-#pragma unroll 4
-    for (int r = sl; r < nrow; r += 64) s += part[(size_t)r * ncol + j];
+  if (j < (size_t)ncol) {
+#pragma unroll 8
+    for (int r = sl; r < nrow; r += NS) s += part[(size_t)r * ncol + j];
   }
   red[sl][c] = s;
   __syncthreads();
-  if (threadIdx.x < 4 && j < ncol) {
+  if (threadIdx.x < CB && j < (size_t)ncol) {
     float v = 0.f;
-    for (int k = 0; k < 64; ++k) v += red[k][threadIdx.x];
+    for (int k = 0; k < NS; ++k) v += red[k][threadIdx.x];
     out[j] += v;
   }
 }
@@ -709,7 +812,9 @@ struct smili_unet {
   float *xin = nullptr, *tgt = nullptr, *out = nullptr;
   float *dtmp = nullptr;          // input gradient of decoder convolutions
   bf16_t* dp = nullptr;           // masked output gradient
-  float* part = nullptr;          // per-block partial sums of gradient reductions
+  float* part = nullptr;          // per-block partial sums of bias / first-layer reductions
+  float* wpart = nullptr;         // per-pixel-group partial sums of 3x3 weight gradients
+  size_t nwpart = 0;
   double* loss = nullptr;         // per step
   int64_t nloss = 0;
   float* lr = nullptr;
@@ -767,21 +872,58 @@ void launch_conv(smili_unet* n, int nb, const Act* x1, int up1, const Act* x2, i
   a.w_ = w;
   a.cout = cout;
   a.mode = mode;
-  a.tiles_x = w / kTileW;
+  const int tw = (w % 32 == 0) ? 32 : 16;
+  a.tiles_x = w / tw;
   a.tiles_y = h / kTileH;
   const int ncb = (cout % 64 == 0) ? 4 : 2;
   a.ncob = cout / (ncb * 16);
-  const int items = nb * a.tiles_x * a.tiles_y * a.ncob;
-  if (ncb == 4)
-    hipLaunchKernelGGL(k_conv3x3<4>, dim3(round8(items)), dim3(kThreads), 0, n->s, a);
+  const dim3 grid(round8(nb * a.tiles_x * a.tiles_y * a.ncob));
+  if (tw == 32 && ncb == 4)
+    hipLaunchKernelGGL((k_conv3x3<32, 4>), grid, dim3(kThreads), 0, n->s, a);
+  else if (tw == 32)
+    hipLaunchKernelGGL((k_conv3x3<32, 2>), grid, dim3(kThreads), 0, n->s, a);
+  else if (ncb == 4)
+    hipLaunchKernelGGL((k_conv3x3<16, 4>), grid, dim3(kThreads), 0, n->s, a);
   else
-    hipLaunchKernelGGL(k_conv3x3<2>, dim3(round8(items)), dim3(kThreads), 0, n->s, a);
+    hipLaunchKernelGGL((k_conv3x3<16, 2>), grid, dim3(kThreads), 0, n->s, a);
   (void)li;
 }
 
 // forward 3x3 conv of layer li: y = relu(conv(x1 [up] ++ x2) + b)
 void conv_fwd(smili_unet* n, int nb, int li, const Act* x1, int up1, const Act* x2, Act* y) {
   launch_conv(n, nb, x1, up1, x2, li, 0, y->y, nullptr, y->h, y->w, y->c, wbf_of(n, li), n->p + n->L[li].boff);
+}
+
+// Split of a weight gradient over pixel groups: about kWgradBlocks
+// work-groups in all, each (output block, input slice, pixel group).
+constexpr int kWgradBlocks = 1024;
+struct WgradPlan {
+  int ncb, ncob, nchunk, ntiles, tpg, ngroups;
+  int want;   // upper bound of ngroups for this layer at any batch up to this one
+  size_t row() const { return (size_t)ncob * ncb * 16 * 9 * nchunk * 32; }
+};
+WgradPlan wgrad_plan(int nb, int h, int w, int cout, int cin) {
+  WgradPlan p;
+  p.ncb = (cout % 64 == 0) ? 4 : 2;
+  p.ncob = cout / (p.ncb * 16);
+  p.nchunk = cin / 32;
+  p.ntiles = nb * (h / kTileH) * (w / kTileW);
+  const int base = p.ncob * p.nchunk;
+  int want = std::max(1, (kWgradBlocks + base / 2) / base);
+  want = std::min(want, p.ntiles);
+  p.want = want;
+  p.tpg = (p.ntiles + want - 1) / want;
+  p.ngroups = (p.ntiles + p.tpg - 1) / p.tpg;
+  return p;
+}
+
+void sum_rows(smili_unet* n, const float* part, int nrow, size_t ncol, float* out) {
+  if (ncol >= 4096)
+    hipLaunchKernelGGL(k_sum_rows<64>, dim3((unsigned)((ncol + 63) / 64)), dim3(256), 0, n->s, part, nrow,
+                       (int)ncol, out);
+  else
+    hipLaunchKernelGGL(k_sum_rows<4>, dim3((unsigned)((ncol + 3) / 4)), dim3(256), 0, n->s, part, nrow, (int)ncol,
+                       out);
 }
 
 void conv_wgrad(smili_unet* n, int nb, int li, const Act* x1, int up1, const Act* x2, int h, int w, int cout) {
@@ -792,38 +934,34 @@ void conv_wgrad(smili_unet* n, int nb, int li, const Act* x1, int up1, const Act
   a.up1 = up1;
   a.x2 = x2 ? x2->y : nullptr;
   a.c2 = x2 ? x2->c : 0;
-  a.dw = n->g + n->L[li].woff;
+  a.dw = n->wpart;
   a.n = nb;
   a.h = h;
   a.w_ = w;
   a.cout = cout;
   a.tiles_x = w / kTileW;
   a.tiles_y = h / kTileH;
-  const int ncb = (cout % 64 == 0) ? 4 : 2;
-  a.ncob = cout / (ncb * 16);
-  a.nchunk = (a.c1 + a.c2) / 32;
-  const int ntiles = nb * a.tiles_x * a.tiles_y;
-  const int base = a.ncob * a.nchunk;
-  int tpg = (int)(((size_t)ntiles * base + 2047) / 2048);
-  if (tpg < 1) tpg = 1;
-  a.tpg = tpg;
-  a.ngroups = (ntiles + tpg - 1) / tpg;
-  const int items = base * a.ngroups;
-  if (ncb == 4)
+  const WgradPlan pl = wgrad_plan(nb, h, w, cout, a.c1 + a.c2);
+  a.ncob = pl.ncob;
+  a.nchunk = pl.nchunk;
+  a.tpg = pl.tpg;
+  a.ngroups = pl.ngroups;
+  const int items = pl.ncob * pl.nchunk * pl.ngroups;
+  if (pl.ncb == 4)
     hipLaunchKernelGGL(k_conv3x3_wgrad<4>, dim3(round8(items)), dim3(kThreads), 0, n->s, a);
   else
     hipLaunchKernelGGL(k_conv3x3_wgrad<2>, dim3(round8(items)), dim3(kThreads), 0, n->s, a);
+  sum_rows(n, n->wpart, pl.ngroups, pl.row(), n->g + n->L[li].woff);
 }
 
 constexpr int kMaskBlocks = 1024;   // rows of the partial-sum scratch
-constexpr int kInPxPerBlock = 128;
+constexpr int kInPxPerBlock = 1024;
 
 void relu_mask(smili_unet* n, int nb, const Act* y, const float* da, int li) {
   const size_t npx = (size_t)nb * y->h * y->w;
   const int blocks = grid_for(npx * (y->c / 8), kThreads, kMaskBlocks);
   hipLaunchKernelGGL(k_relu_mask, dim3(blocks), dim3(kThreads), 0, n->s, da, y->y, n->dp, n->part, npx, y->c);
-  hipLaunchKernelGGL(k_sum_rows, dim3((y->c + 3) / 4), dim3(256), 0, n->s, n->part, blocks, y->c,
-                     n->g + n->L[li].boff);
+  sum_rows(n, n->part, blocks, y->c, n->g + n->L[li].boff);
 }
 
 // Backward through ReLU + conv li (input x1 [up] ++ x2, output y with
@@ -916,8 +1054,7 @@ void backward(smili_unet* n, int nb) {
   const int blocks = (int)((npx + kInPxPerBlock - 1) / kInPxPerBlock);
   hipLaunchKernelGGL(k_conv_in_wgrad, dim3(blocks), dim3(kThreads), 0, n->s, n->dp, n->xin, n->part, nb, n->H,
                      n->W, kInPxPerBlock);
-  hipLaunchKernelGGL(k_sum_rows, dim3((kBase * 18 + 3) / 4), dim3(256), 0, n->s, n->part, blocks, kBase * 18,
-                     n->g + n->L[0].woff);
+  sum_rows(n, n->part, blocks, kBase * 18, n->g + n->L[0].woff);
 }
 
 void cast_weights(smili_unet* n) {
@@ -1020,6 +1157,14 @@ int smili_unet_create(int32_t batch, int32_t height, int32_t width, smili_unet**
   // most c3 channels) and the first convolution's weight gradient
   const size_t nin_blocks = (B * H * W + kInPxPerBlock - 1) / kInPxPerBlock;
   if (!r) r = dalloc(n, (void**)&n->part, std::max((size_t)kMaskBlocks * c3, nin_blocks * kBase * 18) * 4);
+  // weight-gradient partial rows: the largest split over the 3x3 layers
+  // (ngroups <= min(want, tiles) <= its value at the full batch)
+  const int lev[14] = {0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0};
+  for (int li = 1; li < layer_count(); ++li) {
+    const WgradPlan pl = wgrad_plan(batch, H >> lev[li], W >> lev[li], n->L[li].cout, n->L[li].cin);
+    n->nwpart = std::max(n->nwpart, (size_t)pl.want * pl.row());
+  }
+  if (!r) r = dalloc(n, (void**)&n->wpart, n->nwpart * 4);
   const struct { Act* a; int s, c; } acts[17] = {
       {&n->e0a, 0, c0}, {&n->e0, 0, c0}, {&n->p0, 1, c0}, {&n->e1a, 1, c1}, {&n->e1, 1, c1},
       {&n->p1, 2, c1},  {&n->e2a, 2, c2}, {&n->e2, 2, c2}, {&n->p2, 3, c2}, {&n->e3a, 3, c3},
@@ -1234,6 +1379,46 @@ int smili_unet_time_steps(smili_unet* n, int32_t reps, float* ms) {
   HIPCHK(hipEventElapsedTime(&t_fwd, n->ev1, n->ev2));
   ms[0] = t_train / reps;
   ms[1] = t_fwd / reps;
+  return 0;
+}
+
+int smili_unet_time_layer(smili_unet* n, int32_t layer, int32_t pass, int32_t reps, float* ms, double* flop) {
+  if (!n || !ms || !flop) return fail(-1, "null argument");
+  if (layer < 1 || layer >= layer_count()) return fail(-1, "layer must be in 1..13");
+  if (pass < 0 || pass > 2) return fail(-1, "pass must be 0, 1 or 2");
+  if (reps < 1) return fail(-1, "reps must be >= 1");
+  // inputs and output of each 3x3 layer, as in forward()
+  const Act* io[14][3] = {{nullptr, nullptr, nullptr},
+                          {&n->e0a, nullptr, &n->e0},  {&n->p0, nullptr, &n->e1a}, {&n->e1a, nullptr, &n->e1},
+                          {&n->p1, nullptr, &n->e2a},  {&n->e2a, nullptr, &n->e2}, {&n->p2, nullptr, &n->e3a},
+                          {&n->e3a, nullptr, &n->e3},  {&n->e3, &n->e2, &n->d2a},  {&n->d2a, nullptr, &n->d2},
+                          {&n->d2, &n->e1, &n->d1a},   {&n->d1a, nullptr, &n->d1}, {&n->d1, &n->e0, &n->d0a},
+                          {&n->d0a, nullptr, &n->d0}};
+  const Act* x1 = io[layer][0];
+  const Act* x2 = io[layer][1];
+  const Act* y = io[layer][2];
+  const int up1 = x2 ? 1 : 0;
+  Act dpa;
+  dpa.y = n->dp;
+  dpa.c = y->c;
+  forward(n, n->B);
+  HIPCHK(hipEventRecord(n->ev0, n->s));
+  for (int r = 0; r < reps; ++r) {
+    if (pass == 0)
+      conv_fwd(n, n->B, layer, x1, up1, x2, const_cast<Act*>(y));
+    else if (pass == 1)
+      launch_conv(n, n->B, &dpa, 0, nullptr, layer, 1, nullptr, n->dtmp, y->h, y->w, n->L[layer].cin,
+                  wfl_of(n, layer), nullptr);
+    else
+      conv_wgrad(n, n->B, layer, x1, up1, x2, y->h, y->w, y->c);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(n->ev1, n->s));
+  HIPCHK(hipEventSynchronize(n->ev1));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, n->ev0, n->ev1));
+  *ms = t / reps;
+  *flop = 2.0 * n->B * y->h * y->w * (double)n->L[layer].cout * 9.0 * n->L[layer].cin;
   return 0;
 }
 

@@ -63,12 +63,14 @@ def warm_solve(net, case, cfg=D.CFG, stop_cost=None, tune=None):
     prep = D.prepare_case(case, cfg)
     with CostProblem(**prep.kwargs, tune=tune) as prob:
         x0, tm = propose(net, case, prep, prob, cfg)
+        f_prior = prob.cost_grad(prep.x0)[0]
+        f_prop = prob.cost_grad(x0)[0] if x0 is not None else float("inf")
         used = "proposal"
-        if x0 is None or prob.cost_grad(x0)[0] >= prob.cost_grad(prep.x0)[0]:
+        if f_prop >= f_prior:
             x0, used = prep.x0, "prior"
         t_setup = time.perf_counter() - t0
         x, info = lbfgs.solve(prob, x0, niter=cfg["niter"], m=cfg["m"], verbose=False, stop_cost=stop_cost)
-    return _img(x, prep), dict(info=info, setup_s=t_setup, start=used, **tm)
+    return _img(x, prep), dict(info=info, setup_s=t_setup, start=used, f_prior=f_prior, f_proposal=f_prop, **tm)
 
 
 def _reach(info, target):
@@ -99,7 +101,9 @@ def time_to_image(net, seed, cfg=D.CFG, rel=1e-6):
     iw = w["info"]
     kw, tw = _reach(iw, target)
     nrmse = float(np.linalg.norm(img_w - img_p) / max(np.linalg.norm(img_p), 1e-300))
-    return dict(seed=int(seed), kind=case.kind, nvis=int(case.u.size), plain_iters=kp, plain_s=t_setup + tp,
+    return dict(seed=int(seed), kind=case.kind, nvis=int(case.u.size),
+                gap_prior=w["f_prior"] / ip.cost - 1.0, gap_proposal=w["f_proposal"] / ip.cost - 1.0,
+                plain_iters=kp, plain_s=t_setup + tp,
                 plain_total_iters=ip.niter, plain_cost=ip.cost, warm_iters=kw,
                 warm_s=(w["setup_s"] + tw) if tw is not None else None, warm_wall_s=wall_w, warm_cost=iw.cost,
                 warm_start=w["start"], reached=kw is not None, nrmse=nrmse,
@@ -116,6 +120,8 @@ def summary(rows):
                 median_plain_s=float(np.median([r["plain_s"] for r in rows])),
                 median_warm_s=float(np.median([r["warm_s"] for r in ok])) if ok else None,
                 max_nrmse=float(max(r["nrmse"] for r in ok)) if ok else None,
+                median_gap_prior=float(np.median([r["gap_prior"] for r in rows])),
+                median_gap_proposal=float(np.median([r["gap_proposal"] for r in rows])),
                 proposals_used=sum(r["warm_start"] == "proposal" for r in rows))
 
 

@@ -23,11 +23,9 @@ from .unet import UNet
 
 def init_params(rng):
     """He-normal 3x3 weights, zero biases, head weights N(0, 1/32)."""
-    from .unet import nparams
+    from .unet import LAYERS, nparams
     parts = []
-    specs = [(2, 32), (32, 32), (32, 64), (64, 64), (64, 128), (128, 128), (128, 256), (256, 256),
-             (384, 128), (128, 128), (192, 64), (64, 64), (96, 32), (32, 32)]
-    for ci, co in specs:
+    for _, ci, co in LAYERS:
         parts.append(rng.standard_normal(co * 9 * ci) * math.sqrt(2.0 / (9 * ci)))
         parts.append(np.zeros(co))
     parts.append(rng.standard_normal(32) * math.sqrt(1.0 / 32))
@@ -95,14 +93,16 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--holdout", type=float, default=0.05)
     ap.add_argument("--weight-decay", type=float, default=0.0)
+    ap.add_argument("--packed", action="store_true", help="write the packed (bfloat16 3x3 weights) form")
     a = ap.parse_args(argv)
     x, t, meta = gen.load(a.data, log=lambda s: print(s, flush=True))
     print(f"[train] loaded {x.shape[0]} samples from {a.data}", flush=True)
     net, rep = train(x, t, a.epochs, a.batch, a.lr, a.warmup, a.holdout, weight_decay=a.weight_decay,
                      log=lambda s: print(s, flush=True))
     from .data import CFG
-    net.save(a.out, cfg=json.dumps(CFG), seeds=meta["seed"], n_train=rep["n_train"], epochs=rep["epochs"],
-             train_loss=rep["train_loss"], val=rep["val"], seconds=rep["seconds"])
+    net.save(a.out, packed=a.packed, cfg=json.dumps(CFG), seeds=meta["seed"], n_train=rep["n_train"],
+             epochs=rep["epochs"], train_loss=rep["train_loss"][::max(1, rep["steps"] // 2000)], val=rep["val"],
+             seconds=rep["seconds"])
     print(f"[train] saved {a.out}", flush=True)
     return 0
 

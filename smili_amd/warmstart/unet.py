@@ -24,6 +24,51 @@ class UNetError(RuntimeError):
     pass
 
 
+# (name, input channels, output channels) of the 3x3 convolutions in
+# parameter order (include/smili_unet.h); then the head w[32], b[1]
+LAYERS = (("conv_in", 2, 32), ("enc0b", 32, 32), ("enc1a", 32, 64), ("enc1b", 64, 64),
+          ("enc2a", 64, 128), ("enc2b", 128, 128), ("enc3a", 128, 256), ("enc3b", 256, 256),
+          ("dec2a", 384, 128), ("dec2b", 128, 128), ("dec1a", 192, 64), ("dec1b", 64, 64),
+          ("dec0a", 96, 32), ("dec0b", 32, 32))
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "unet128_l1tsv.npz")
+
+
+def _bf16_bits(a):
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def pack(params):
+    """Weights file arrays: the 3x3 weights of layers 1..13 as bfloat16 bits
+    (the precision the kernels compute in), everything else float32."""
+    p = np.asarray(params, np.float32)
+    w3, f32, o = [], [], 0
+    for i, (_, ci, co) in enumerate(LAYERS):
+        nw = co * 9 * ci
+        (f32 if i == 0 else w3).append(p[o:o + nw])
+        f32.append(p[o + nw:o + nw + co])
+        o += nw + co
+    f32.append(p[o:])
+    return {"w3_bf16": _bf16_bits(np.concatenate(w3)), "f32": np.concatenate(f32)}
+
+
+def unpack(w3_bf16, f32):
+    w3 = (np.asarray(w3_bf16, np.uint16).astype(np.uint32) << 16).view(np.float32)
+    out, a, b = [], 0, 0
+    for i, (_, ci, co) in enumerate(LAYERS):
+        nw = co * 9 * ci
+        if i == 0:
+            out.append(f32[b:b + nw])
+            b += nw
+        else:
+            out.append(w3[a:a + nw])
+            a += nw
+        out.append(f32[b:b + co])
+        b += co
+    out.append(f32[b:])
+    return np.concatenate(out).astype(np.float32)
+
+
 def load():
     global _lib
     if _lib is not None:
@@ -44,6 +89,7 @@ def load():
     lib.smili_unet_reset_optimizer.argtypes = [C.c_void_p]
     lib.smili_unet_time_steps.argtypes = [C.c_void_p, C.c_int32, _pf]
     lib.smili_unet_flops.argtypes = [C.c_void_p, _pd]
+    lib.smili_unet_time_layer.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _pf, _pd]
     lib.smili_unet_last_error.restype = C.c_char_p
     _lib = lib
     return lib
@@ -160,19 +206,32 @@ class UNet:
         _check(load().smili_unet_time_steps(self._h, int(reps), ms.ctypes.data_as(_pf)))
         return float(ms[0]), float(ms[1])
 
+    def time_layer(self, layer, pass_=0, reps=20):
+        """(mean ms of one launch, FLOP of one launch) of a 3x3 convolution
+        kernel: pass 0 forward, 1 input gradient, 2 weight gradient."""
+        ms = np.zeros(1, np.float32)
+        fl = np.zeros(1, np.float64)
+        _check(load().smili_unet_time_layer(self._h, int(layer), int(pass_), int(reps), ms.ctypes.data_as(_pf),
+                                            fl.ctypes.data_as(_pd)))
+        return float(ms[0]), float(fl[0])
+
     def flops(self):
         """(3x3-convolution FLOP of a training step, of a forward pass)."""
         f = np.zeros(2, np.float64)
         _check(load().smili_unet_flops(self._h, f.ctypes.data_as(_pd)))
         return float(f[0]), float(f[1])
 
-    def save(self, path, **meta):
-        np.savez_compressed(path, params=self.params, height=self.height, width=self.width,
+    def save(self, path, packed=False, **meta):
+        arrs = pack(self.params) if packed else {"params": self.params}
+        np.savez_compressed(path, height=self.height, width=self.width, **arrs,
                             **{k: np.asarray(v) for k, v in meta.items()})
 
     @classmethod
-    def from_file(cls, path, batch=1):
-        z = np.load(path)
+    def from_file(cls, path=None, batch=1):
+        """(network, metadata) from a weights file (full float32 parameters
+        or the packed form); the default is the trained warm-start network."""
+        z = np.load(DEFAULT_WEIGHTS if path is None else path)
         net = cls(batch, int(z["height"]), int(z["width"]))
-        net.params = z["params"]
-        return net, {k: z[k] for k in z.files if k not in ("params", "height", "width")}
+        net.params = z["params"] if "params" in z.files else unpack(z["w3_bf16"], z["f32"])
+        skip = ("params", "height", "width", "w3_bf16", "f32")
+        return net, {k: z[k] for k in z.files if k not in skip}

@@ -23,7 +23,7 @@ def header_functions():
 def test_exports_every_declared_symbol():
     lib = unet.load()
     names = header_functions()
-    assert len(names) == 13
+    assert len(names) == 14
     for n in names:
         assert hasattr(lib, n), n
 
@@ -57,3 +57,24 @@ def test_wrapper_fails_loudly_without_library(monkeypatch, tmp_path):
     monkeypatch.setattr(unet, "_lib", None)
     with pytest.raises(unet.UNetError):
         unet.nparams()
+
+
+def test_packed_weights_round_trip():
+    """The packed weights form keeps the first convolution, biases and head
+    in float32 and rounds the other 3x3 weights to bfloat16 exactly as the
+    kernels do."""
+    import numpy as np
+    rng = np.random.default_rng(0)
+    p = U.init_params(rng)
+    p[U.nparams() - 40:] = rng.standard_normal(40)
+    q = unet.unpack(**unet.pack(p))
+    assert q.dtype == np.float32 and q.shape == p.shape
+    ref = p.copy()
+    o = 0
+    for i, (_, ci, co) in enumerate(U.layer_specs()):
+        nw = co * 9 * ci
+        if i > 0:
+            ref[o:o + nw] = U.bf16(p[o:o + nw])
+        o += nw + co
+    assert np.array_equal(q, ref)
+    assert [(n, ci, co) for n, ci, co in unet.LAYERS] == U.layer_specs()
