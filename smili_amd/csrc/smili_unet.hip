@@ -18,8 +18,9 @@
 //    (weights flipped and transposed, fp32 out).
 //  * k_conv3x3_wgrad: dW[co][tap][ci] = sum_px dP[px][co] X[px+off][ci];
 //    pixels are the reduction axis, so dP and the input halo are staged
-//    transposed ([channel][pixel]) in LDS, the halo in three copies
-//    shifted by dx so every 8-pixel fragment is a 16-byte aligned read.
+//    transposed ([channel][pixel]) in LDS (8 x 8 blocks transposed in
+//    registers); a tap's 8-pixel fragment, shifted by dx columns, is cut
+//    from one aligned 16-byte read and the next word.
 //    Work-groups split the pixel tiles; each writes its partial sums to a
 //    row of a scratch array that k_sum_rows reduces (float atomics of every
 //    group on the same addresses would serialise).
@@ -255,8 +256,9 @@ struct WgradArgs {
 
 template <int NCB>
 __global__ __launch_bounds__(kThreads) void k_conv3x3_wgrad(WgradArgs a) {
+  constexpr int SXW = 24;   // halo row in LDS: 18 columns padded to 48 bytes
   __shared__ __attribute__((aligned(16))) bf16_t sdp[NCB * 16][kTileH * kTileW];
-  __shared__ __attribute__((aligned(16))) bf16_t sx[3][32][kHaloH][kTileW];
+  __shared__ __attribute__((aligned(16))) bf16_t sx[32][kHaloH][SXW];
   const int item = xcd_item(a.ncob * a.nchunk * a.ngroups);
   if (item < 0) return;
   const int grp = item % a.ngroups;
@@ -299,24 +301,23 @@ __global__ __launch_bounds__(kThreads) void k_conv3x3_wgrad(WgradArgs a) {
 #pragma unroll
       for (int i = 0; i < 8; ++i) *reinterpret_cast<uint4*>(&sdp[q * 8 + i][pg * 8]) = w[i];
     }
-    // input halo slice, transposed to [ci][row][col] in three copies shifted
-    // by dx: a unit is (dx, halo row, 8 destination columns, 8 channels)
-    for (int u = threadIdx.x; u < 3 * kHaloH * 2 * 4; u += kThreads) {
-      const int q = u & 3, cg = (u >> 2) & 1, r = u >> 3;
-      const int hy = r % kHaloH, dx = r / kHaloH;
+    // input halo slice, transposed to [ci][row][col]: a unit is (halo row,
+    // 8 columns, 8 channels); columns 18..23 are padding
+    for (int u = threadIdx.x; u < kHaloH * 3 * 4; u += kThreads) {
+      const int q = u & 3, cg = (u >> 2) % 3, hy = u / 12;
       const int yy = y0 + hy - 1;
       uint4 v[8];
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        const int xx = x0 + cg * 8 + dx + j - 1;
+        const int hx = cg * 8 + j, xx = x0 + hx - 1;
         v[j] = make_uint4(0, 0, 0, 0);
-        if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_)
+        if (hx < kHaloW && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_)
           v[j] = *reinterpret_cast<const uint4*>(gather_src(g, b, yy, xx, ci0 + q * 8));
       }
       uint4 w[8];
       transpose8(v, w);
 #pragma unroll
-      for (int i = 0; i < 8; ++i) *reinterpret_cast<uint4*>(&sx[dx][q * 8 + i][hy][cg * 8]) = w[i];
+      for (int i = 0; i < 8; ++i) *reinterpret_cast<uint4*>(&sx[q * 8 + i][hy][cg * 8]) = w[i];
     }
     __syncthreads();
 #pragma unroll
@@ -332,7 +333,20 @@ __global__ __launch_bounds__(kThreads) void k_conv3x3_wgrad(WgradArgs a) {
         if (j < ntap) {
           const int tap = tap0 + 2 * j;
           const int dy = tap / 3, dx = tap - 3 * (tap / 3);
-          const bf16x8 bx = *reinterpret_cast<const bf16x8*>(&sx[dx][cib * 16 + ln][row + dy][cb0]);
+          // columns cb0 + dx .. cb0 + dx + 7 from the aligned words at cb0
+          const bf16_t* rp = &sx[cib * 16 + ln][row + dy][cb0];
+          const uint4 s0 = *reinterpret_cast<const uint4*>(rp);
+          const uint32_t s4 = *reinterpret_cast<const uint32_t*>(rp + 8);
+          uint4 sel;
+          if (dx == 0) {
+            sel = s0;
+          } else if (dx == 2) {
+            sel = make_uint4(s0.y, s0.z, s0.w, s4);
+          } else {
+            sel = make_uint4((s0.x >> 16) | (s0.y << 16), (s0.y >> 16) | (s0.z << 16),
+                             (s0.z >> 16) | (s0.w << 16), (s0.w >> 16) | (s4 << 16));
+          }
+          const bf16x8 bx = __builtin_bit_cast(bf16x8, sel);
 #pragma unroll
           for (int cb = 0; cb < NCB; ++cb)
             acc[j][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ad[cb], bx, acc[j][cb], 0, 0, 0);
@@ -896,7 +910,7 @@ void conv_fwd(smili_unet* n, int nb, int li, const Act* x1, int up1, const Act* 
 
 // Split of a weight gradient over pixel groups: about kWgradBlocks
 // work-groups in all, each (output block, input slice, pixel group).
-constexpr int kWgradBlocks = 1024;
+constexpr int kWgradBlocks = 512;
 struct WgradPlan {
   int ncb, ncob, nchunk, ntiles, tpg, ngroups;
   int want;   // upper bound of ngroups for this layer at any batch up to this one

@@ -235,3 +235,36 @@ class UNet:
         net.params = z["params"] if "params" in z.files else unpack(z["w3_bf16"], z["f32"])
         skip = ("params", "height", "width", "w3_bf16", "f32")
         return net, {k: z[k] for k in z.files if k not in skip}
+
+
+def main(argv=None):
+    """Times training steps and forward passes of a randomly initialised
+    network on random inputs: ``python -m smili_amd.warmstart.unet
+    [--batch 32] [--size 128] [--reps 10]``."""
+    import argparse
+    import json
+    import math
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--size", type=int, default=128)
+    ap.add_argument("--reps", type=int, default=10)
+    a = ap.parse_args(argv)
+    rng = np.random.default_rng(0)
+    parts = []
+    for _, ci, co in LAYERS:
+        parts += [rng.standard_normal(co * 9 * ci) * math.sqrt(2.0 / (9 * ci)), np.zeros(co)]
+    parts += [rng.standard_normal(32) / math.sqrt(32), np.zeros(1)]
+    with UNet(a.batch, a.size, a.size, params=np.concatenate(parts)) as net:
+        x = rng.standard_normal((a.batch, a.size, a.size, 2)).astype(np.float32)
+        net.set_data(x, np.abs(x[..., 0]))
+        net.time_steps(2)
+        step_ms, fwd_ms = net.time_steps(a.reps)
+        f_step, f_fwd = net.flops()
+    print(json.dumps({"batch": a.batch, "size": a.size, "train_step_ms": round(step_ms, 4),
+                      "train_TFLOPs": round(f_step / step_ms / 1e9, 1), "forward_ms": round(fwd_ms, 4),
+                      "forward_TFLOPs": round(f_fwd / fwd_ms / 1e9, 1)}), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
