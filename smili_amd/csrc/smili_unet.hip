@@ -254,10 +254,13 @@ struct WgradArgs {
   int tiles_x, tiles_y, ncob, nchunk, tpg, ngroups;
 };
 
-template <int NCB>
+// Tile: 8 rows x TW columns (TW = 16 or 32); 8 TW / 32 k-steps of 32
+// pixels per staging round.
+template <int NCB, int TW>
 __global__ __launch_bounds__(kThreads) void k_conv3x3_wgrad(WgradArgs a) {
-  constexpr int SXW = 24;   // halo row in LDS: 18 columns padded to 48 bytes
-  __shared__ __attribute__((aligned(16))) bf16_t sdp[NCB * 16][kTileH * kTileW];
+  constexpr int SXW = TW + 8;        // halo row in LDS: TW + 2 columns padded to a multiple of 8
+  constexpr int NCG = SXW / 8;       // 8-column groups of a halo row
+  __shared__ __attribute__((aligned(16))) bf16_t sdp[NCB * 16][kTileH * TW];
   __shared__ __attribute__((aligned(16))) bf16_t sx[32][kHaloH][SXW];
   const int item = xcd_item(a.ncob * a.nchunk * a.ngroups);
   if (item < 0) return;
@@ -285,13 +288,14 @@ __global__ __launch_bounds__(kThreads) void k_conv3x3_wgrad(WgradArgs a) {
   for (int tile = tb; tile < te; ++tile) {
     const int tx = tile % a.tiles_x;
     const int ty = (tile / a.tiles_x) % a.tiles_y, b = tile / (a.tiles_x * a.tiles_y);
-    const int y0 = ty * kTileH, x0 = tx * kTileW;
+    const int y0 = ty * kTileH, x0 = tx * TW;
     __syncthreads();
-    // dP tile, transposed to [co][px]: a unit is 8 pixels (one row, columns
-    // 8 pg .. 8 pg + 7) x 8 channels, transposed in registers
-    for (int u = threadIdx.x; u < 16 * NCB * 2; u += kThreads) {
-      const int pg = u & 15, q = u >> 4;
-      const int y = y0 + (pg >> 1), xb = x0 + (pg & 1) * 8;
+    // dP tile, transposed to [co][px]: a unit is 8 pixels (pixel group pg:
+    // row pg / (TW / 8), columns 8 (pg % (TW / 8)) .. + 7) x 8 channels,
+    // transposed in registers
+    for (int u = threadIdx.x; u < TW * NCB * 2; u += kThreads) {
+      const int pg = u % TW, q = u / TW;
+      const int y = y0 + pg / (TW / 8), xb = x0 + (pg % (TW / 8)) * 8;
       uint4 v[8];
 #pragma unroll
       for (int j = 0; j < 8; ++j)
@@ -302,16 +306,16 @@ __global__ __launch_bounds__(kThreads) void k_conv3x3_wgrad(WgradArgs a) {
       for (int i = 0; i < 8; ++i) *reinterpret_cast<uint4*>(&sdp[q * 8 + i][pg * 8]) = w[i];
     }
     // input halo slice, transposed to [ci][row][col]: a unit is (halo row,
-    // 8 columns, 8 channels); columns 18..23 are padding
-    for (int u = threadIdx.x; u < kHaloH * 3 * 4; u += kThreads) {
-      const int q = u & 3, cg = (u >> 2) % 3, hy = u / 12;
+    // 8 columns, 8 channels); columns TW + 2 .. SXW - 1 are padding
+    for (int u = threadIdx.x; u < kHaloH * NCG * 4; u += kThreads) {
+      const int q = u & 3, cg = (u >> 2) % NCG, hy = u / (4 * NCG);
       const int yy = y0 + hy - 1;
       uint4 v[8];
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const int hx = cg * 8 + j, xx = x0 + hx - 1;
         v[j] = make_uint4(0, 0, 0, 0);
-        if (hx < kHaloW && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_)
+        if (hx < TW + 2 && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_)
           v[j] = *reinterpret_cast<const uint4*>(gather_src(g, b, yy, xx, ci0 + q * 8));
       }
       uint4 w[8];
@@ -321,13 +325,13 @@ __global__ __launch_bounds__(kThreads) void k_conv3x3_wgrad(WgradArgs a) {
     }
     __syncthreads();
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      // k = 32 pixels: group lg covers row 2s + lg/2, columns 8 (lg & 1) .. +7
-      const int row = 2 * s + (lg >> 1), cb0 = 8 * (lg & 1);
+    for (int s = 0; s < kTileH * TW / 32; ++s) {
+      // k = 32 pixels: group lg covers pixels 32 s + 8 lg .. + 7 of the tile
+      const int p0 = 32 * s + 8 * lg, row = p0 / TW, cb0 = p0 % TW;
       bf16x8 ad[NCB];
 #pragma unroll
       for (int cb = 0; cb < NCB; ++cb)
-        ad[cb] = *reinterpret_cast<const bf16x8*>(&sdp[cb * 16 + ln][row * kTileW + cb0]);
+        ad[cb] = *reinterpret_cast<const bf16x8*>(&sdp[cb * 16 + ln][p0]);
 #pragma unroll
       for (int j = 0; j < 5; ++j) {
         if (j < ntap) {
@@ -376,38 +380,46 @@ __global__ __launch_bounds__(kThreads) void k_conv3x3_wgrad(WgradArgs a) {
 __global__ __launch_bounds__(kThreads) void k_conv_in(const float* __restrict__ x, const float* __restrict__ w,
                                                       const float* __restrict__ bias, bf16_t* __restrict__ y,
                                                       int n, int h, int wd) {
-  // one thread per (pixel, group of 8 output channels)
-  __shared__ float sw[kBase * 18 + kBase];
-  for (int i = threadIdx.x; i < kBase * 18; i += kThreads) sw[i] = w[i];
-  for (int i = threadIdx.x; i < kBase; i += kThreads) sw[kBase * 18 + i] = bias[i];
+  // one thread per (pixel, group of 8 output channels); weights transposed
+  // in LDS to [tap * 2 + ci][co] so the 8 channels of one input value are
+  // two 16-byte reads
+  __shared__ __attribute__((aligned(16))) float sw[18 * kBase];
+  __shared__ float sb[kBase];
+  for (int i = threadIdx.x; i < kBase * 18; i += kThreads) sw[(i % 18) * kBase + i / 18] = w[i];
+  for (int i = threadIdx.x; i < kBase; i += kThreads) sb[i] = bias[i];
   __syncthreads();
   const size_t work = (size_t)n * h * wd * (kBase / 8);
   for (size_t e = (size_t)blockIdx.x * kThreads + threadIdx.x; e < work; e += (size_t)gridDim.x * kThreads) {
     const int q = e % (kBase / 8);
     const size_t p = e / (kBase / 8);
     const int xx = p % wd, yy = (p / wd) % h, b = p / ((size_t)wd * h);
-    float in[18];
+    float acc[8];
 #pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = sb[q * 8 + k];
+#pragma unroll 1
     for (int t = 0; t < 9; ++t) {
       const int sy = yy + t / 3 - 1, sx = xx + t % 3 - 1;
       const bool ok = sy >= 0 && sy < h && sx >= 0 && sx < wd;
       const float2 v = ok ? *reinterpret_cast<const float2*>(x + ((size_t)(b * h + sy) * wd + sx) * 2)
                           : make_float2(0.f, 0.f);
-      in[2 * t] = v.x;
-      in[2 * t + 1] = v.y;
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const float xv = c ? v.y : v.x;
+        const float4 w0 = *reinterpret_cast<const float4*>(&sw[(2 * t + c) * kBase + q * 8]);
+        const float4 w1 = *reinterpret_cast<const float4*>(&sw[(2 * t + c) * kBase + q * 8 + 4]);
+        acc[0] = fmaf(w0.x, xv, acc[0]);
+        acc[1] = fmaf(w0.y, xv, acc[1]);
+        acc[2] = fmaf(w0.z, xv, acc[2]);
+        acc[3] = fmaf(w0.w, xv, acc[3]);
+        acc[4] = fmaf(w1.x, xv, acc[4]);
+        acc[5] = fmaf(w1.y, xv, acc[5]);
+        acc[6] = fmaf(w1.z, xv, acc[6]);
+        acc[7] = fmaf(w1.w, xv, acc[7]);
+      }
     }
     uint32_t out[4];
 #pragma unroll
-    for (int k = 0; k < 8; k += 2) {
-      const int co = q * 8 + k;
-      float r0 = sw[kBase * 18 + co], r1 = sw[kBase * 18 + co + 1];
-#pragma unroll
-      for (int j = 0; j < 18; ++j) {
-        r0 = fmaf(sw[co * 18 + j], in[j], r0);
-        r1 = fmaf(sw[(co + 1) * 18 + j], in[j], r1);
-      }
-      out[k / 2] = f2bf(fmaxf(r0, 0.f)) | (f2bf(fmaxf(r1, 0.f)) << 16);
-    }
+    for (int k = 0; k < 4; ++k) out[k] = f2bf(fmaxf(acc[2 * k], 0.f)) | (f2bf(fmaxf(acc[2 * k + 1], 0.f)) << 16);
     *reinterpret_cast<uint4*>(y + p * kBase + q * 8) = make_uint4(out[0], out[1], out[2], out[3]);
   }
 }
@@ -705,16 +717,27 @@ __global__ void k_adam(float* __restrict__ p, float* __restrict__ m, float* __re
   }
 }
 
-// bf16 copies of one 3x3 weight tensor: W[co][tap][ci] and the flipped,
-// transposed Wf[ci][8 - tap][co] the input gradient convolves with.
-__global__ void k_cast_w(const float* __restrict__ src, bf16_t* __restrict__ w, bf16_t* __restrict__ wf,
-                         int cout, int cin) {
+// bf16 copies of the 3x3 weight tensors (layer blockIdx.y + 1 of the
+// table): W[co][tap][ci] and the flipped, transposed Wf[ci][8 - tap][co]
+// the input gradient convolves with.
+struct CastTable {
+  int n;
+  int cout[16], cin[16];
+  long long src[16], dst[16];   // offsets into the fp32 parameters / the bf16 arenas
+};
+__global__ void k_cast_w(const float* __restrict__ params, bf16_t* __restrict__ wbf, bf16_t* __restrict__ wfl,
+                         CastTable tab) {
+  const int l = blockIdx.y;
+  const int cout = tab.cout[l], cin = tab.cin[l];
   const int total = cout * 9 * cin;
+  const float* src = params + tab.src[l];
+  bf16_t* w = wbf + tab.dst[l];
+  bf16_t* wf = wfl + tab.dst[l];
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
     const int ci = i % cin, tap = (i / cin) % 9, co = i / (9 * cin);
-    const bf16_t h = (bf16_t)f2bf(src[i]);
-    w[i] = h;
-    wf[((size_t)ci * 9 + (8 - tap)) * cout + co] = h;
+    const bf16_t hv = (bf16_t)f2bf(src[i]);
+    w[i] = hv;
+    wf[((size_t)ci * 9 + (8 - tap)) * cout + co] = hv;
   }
 }
 
@@ -912,16 +935,17 @@ void conv_fwd(smili_unet* n, int nb, int li, const Act* x1, int up1, const Act* 
 // work-groups in all, each (output block, input slice, pixel group).
 constexpr int kWgradBlocks = 512;
 struct WgradPlan {
-  int ncb, ncob, nchunk, ntiles, tpg, ngroups;
+  int ncb, tw, ncob, nchunk, ntiles, tpg, ngroups;
   int want;   // upper bound of ngroups for this layer at any batch up to this one
   size_t row() const { return (size_t)ncob * ncb * 16 * 9 * nchunk * 32; }
 };
 WgradPlan wgrad_plan(int nb, int h, int w, int cout, int cin) {
   WgradPlan p;
   p.ncb = (cout % 64 == 0) ? 4 : 2;
+  p.tw = (w % 32 == 0) ? 32 : 16;
   p.ncob = cout / (p.ncb * 16);
   p.nchunk = cin / 32;
-  p.ntiles = nb * (h / kTileH) * (w / kTileW);
+  p.ntiles = nb * (h / kTileH) * (w / p.tw);
   const int base = p.ncob * p.nchunk;
   int want = std::max(1, (kWgradBlocks + base / 2) / base);
   want = std::min(want, p.ntiles);
@@ -953,18 +977,22 @@ void conv_wgrad(smili_unet* n, int nb, int li, const Act* x1, int up1, const Act
   a.h = h;
   a.w_ = w;
   a.cout = cout;
-  a.tiles_x = w / kTileW;
-  a.tiles_y = h / kTileH;
   const WgradPlan pl = wgrad_plan(nb, h, w, cout, a.c1 + a.c2);
+  a.tiles_x = w / pl.tw;
+  a.tiles_y = h / kTileH;
   a.ncob = pl.ncob;
   a.nchunk = pl.nchunk;
   a.tpg = pl.tpg;
   a.ngroups = pl.ngroups;
-  const int items = pl.ncob * pl.nchunk * pl.ngroups;
-  if (pl.ncb == 4)
-    hipLaunchKernelGGL(k_conv3x3_wgrad<4>, dim3(round8(items)), dim3(kThreads), 0, n->s, a);
+  const dim3 grid(round8(pl.ncob * pl.nchunk * pl.ngroups));
+  if (pl.ncb == 4 && pl.tw == 32)
+    hipLaunchKernelGGL((k_conv3x3_wgrad<4, 32>), grid, dim3(kThreads), 0, n->s, a);
+  else if (pl.ncb == 4)
+    hipLaunchKernelGGL((k_conv3x3_wgrad<4, 16>), grid, dim3(kThreads), 0, n->s, a);
+  else if (pl.tw == 32)
+    hipLaunchKernelGGL((k_conv3x3_wgrad<2, 32>), grid, dim3(kThreads), 0, n->s, a);
   else
-    hipLaunchKernelGGL(k_conv3x3_wgrad<2>, dim3(round8(items)), dim3(kThreads), 0, n->s, a);
+    hipLaunchKernelGGL((k_conv3x3_wgrad<2, 16>), grid, dim3(kThreads), 0, n->s, a);
   sum_rows(n, n->wpart, pl.ngroups, pl.row(), n->g + n->L[li].woff);
 }
 
@@ -1072,11 +1100,19 @@ void backward(smili_unet* n, int nb) {
 }
 
 void cast_weights(smili_unet* n) {
+  CastTable tab;
+  tab.n = layer_count() - 1;
+  size_t most = 0;
   for (int li = 1; li < layer_count(); ++li) {
     const Layer& l = n->L[li];
-    hipLaunchKernelGGL(k_cast_w, dim3(grid_for((size_t)l.cout * 9 * l.cin)), dim3(kThreads), 0, n->s,
-                       n->p + l.woff, n->wbf + l.bfoff, n->wfl + l.bfoff, l.cout, l.cin);
+    tab.cout[li - 1] = l.cout;
+    tab.cin[li - 1] = l.cin;
+    tab.src[li - 1] = (long long)l.woff;
+    tab.dst[li - 1] = (long long)l.bfoff;
+    most = std::max(most, (size_t)l.cout * 9 * l.cin);
   }
+  hipLaunchKernelGGL(k_cast_w, dim3(grid_for(most, kThreads, 256), tab.n), dim3(kThreads), 0, n->s, n->p, n->wbf,
+                     n->wfl, tab);
 }
 
 void zero_grad(smili_unet* n) {
