@@ -104,8 +104,11 @@ __device__ __forceinline__ const bf16_t* gather_src(const ConvArgs& a, int b, in
 // that latency).
 constexpr int kWStride = 9 * 32 + 8;   // bf16 per output channel in LDS: 592 bytes, bank-spread
 
+#ifndef SMILI_UNET_CONV_MIN_BLOCKS
+#define SMILI_UNET_CONV_MIN_BLOCKS 2
+#endif
 template <int TW, int NCB>
-__global__ __launch_bounds__(kThreads, 2) void k_conv3x3(ConvArgs a) {
+__global__ __launch_bounds__(kThreads, SMILI_UNET_CONV_MIN_BLOCKS) void k_conv3x3(ConvArgs a) {
   constexpr int HW = TW + 2;                      // halo width
   constexpr int NHALO = kHaloH * HW * 4;          // 16-byte halo pieces
   constexpr int NCO = NCB * 16;
@@ -933,7 +936,10 @@ void conv_fwd(smili_unet* n, int nb, int li, const Act* x1, int up1, const Act* 
 
 // Split of a weight gradient over pixel groups: about kWgradBlocks
 // work-groups in all, each (output block, input slice, pixel group).
-constexpr int kWgradBlocks = 512;
+#ifndef SMILI_UNET_WGRAD_BLOCKS
+#define SMILI_UNET_WGRAD_BLOCKS 512
+#endif
+constexpr int kWgradBlocks = SMILI_UNET_WGRAD_BLOCKS;
 struct WgradPlan {
   int ncb, tw, ncob, nchunk, ntiles, tpg, ngroups;
   int want;   // upper bound of ngroups for this layer at any batch up to this one
@@ -996,7 +1002,10 @@ void conv_wgrad(smili_unet* n, int nb, int li, const Act* x1, int up1, const Act
   sum_rows(n, n->wpart, pl.ngroups, pl.row(), n->g + n->L[li].woff);
 }
 
-constexpr int kMaskBlocks = 1024;   // rows of the partial-sum scratch
+#ifndef SMILI_UNET_MASK_BLOCKS
+#define SMILI_UNET_MASK_BLOCKS 1024
+#endif
+constexpr int kMaskBlocks = SMILI_UNET_MASK_BLOCKS;   // rows of the partial-sum scratch
 constexpr int kInPxPerBlock = 1024;
 
 void relu_mask(smili_unet* n, int nb, const Act* y, const float* da, int li) {
@@ -1391,6 +1400,9 @@ int smili_unet_train(smili_unet* n, const int64_t* index, int64_t nsteps, const 
   if (!n->ds_n) return fail(-1, "no training set uploaded");
   for (int64_t s = 0; s < nsteps; ++s)
     if (!std::isfinite(lr[s]) || lr[s] < 0.f) return fail(-1, "invalid learning rate at step " + std::to_string(s));
+  if (!(hyper[0] >= 0.f && hyper[0] < 1.f) || !(hyper[1] >= 0.f && hyper[1] < 1.f) || !(hyper[2] > 0.f) ||
+      !(hyper[3] >= 0.f) || !std::isfinite(hyper[2]) || !std::isfinite(hyper[3]))
+    return fail(-1, "hyper must be {beta1, beta2 in [0, 1), eps > 0, weight_decay >= 0}");
   int r = upload_steps(n, index, nsteps, lr);
   if (r) return r;
   r = train_steps(n, nsteps, hyper);

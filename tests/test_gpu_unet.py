@@ -152,6 +152,12 @@ def test_adam_step_and_training_reduces_loss(net_case):
         # first Adam step moves every parameter by lr * g / (|g| + eps)
         bad = np.abs(p1 - ref) > 1e-3 * lr + 1e-6 * np.abs(ref)
         assert bad.mean() <= 1e-4, bad.mean()
+        for bad in (dict(beta1=1.0), dict(beta2=-0.1), dict(eps=0.0), dict(weight_decay=-1.0)):
+            with pytest.raises(unet.UNetError, match="hyper"):
+                n2.train(8 * np.array([[0, 1]]), lr, **bad)
+        with pytest.raises(unet.UNetError, match="index out of range"):
+            n2.train(np.array([[0, 8 * 4]]), lr)
+        assert np.array_equal(n2.params, p1)
         idx = 8 * np.tile(np.array([[0, 1], [2, 3]]), (30, 1))
         losses = n2.train(idx, 2e-4)
         print("train losses", losses[:2], losses[-2:])

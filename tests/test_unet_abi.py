@@ -78,3 +78,4 @@ def test_packed_weights_round_trip():
         o += nw + co
     assert np.array_equal(q, ref)
     assert [(n, ci, co) for n, ci, co in unet.LAYERS] == U.layer_specs()
+
