@@ -81,7 +81,8 @@ struct ConvArgs {
 
 __device__ __forceinline__ int halo_off(int p, int g) {
   // 64 bytes per halo pixel; the 16-byte channel groups are swizzled so
-  // that 16 consecutive pixels of one group spread over the banks
+  // that 8 consecutive pixels of one group (the lanes a 16-byte LDS read
+  // serves together) cover all 32 banks
   return p * 32 + ((g ^ ((p >> 1) & 3)) << 3);
 }
 
@@ -263,8 +264,15 @@ template <int NCB, int TW>
 __global__ __launch_bounds__(kThreads) void k_conv3x3_wgrad(WgradArgs a) {
   constexpr int SXW = TW + 8;        // halo row in LDS: TW + 2 columns padded to a multiple of 8
   constexpr int NCG = SXW / 8;       // 8-column groups of a halo row
-  __shared__ __attribute__((aligned(16))) bf16_t sdp[NCB * 16][kTileH * TW];
-  __shared__ __attribute__((aligned(16))) bf16_t sx[32][kHaloH][SXW];
+  // rows padded by 16 bytes so the 8 channel rows a 16-byte fragment read
+  // serves together start on different banks (strides of 4 or 12 dwords
+  // modulo 32)
+  constexpr int SDP = kTileH * TW + 8;          // bf16 per dP channel row
+  constexpr int SXC = kHaloH * SXW + 8;         // bf16 per input channel
+  __shared__ __attribute__((aligned(16))) bf16_t sdp_[NCB * 16 * SDP];
+  __shared__ __attribute__((aligned(16))) bf16_t sx_[32 * SXC];
+  auto sdp = [&](int c, int px) -> bf16_t* { return sdp_ + c * SDP + px; };
+  auto sx = [&](int c, int hy, int col) -> bf16_t* { return sx_ + c * SXC + hy * SXW + col; };
   const int item = xcd_item(a.ncob * a.nchunk * a.ngroups);
   if (item < 0) return;
   const int grp = item % a.ngroups;
@@ -306,12 +314,14 @@ __global__ __launch_bounds__(kThreads) void k_conv3x3_wgrad(WgradArgs a) {
       uint4 w[8];
       transpose8(v, w);
 #pragma unroll
-      for (int i = 0; i < 8; ++i) *reinterpret_cast<uint4*>(&sdp[q * 8 + i][pg * 8]) = w[i];
+      for (int i = 0; i < 8; ++i) *reinterpret_cast<uint4*>(sdp(q * 8 + i, pg * 8)) = w[i];
     }
     // input halo slice, transposed to [ci][row][col]: a unit is (halo row,
-    // 8 columns, 8 channels); columns TW + 2 .. SXW - 1 are padding
+    // 8 columns, 8 channels); columns TW + 2 .. SXW - 1 are padding.
+    // Consecutive lanes take consecutive column groups, then rows, so the
+    // 16-byte stores of 8 lanes cover all banks.
     for (int u = threadIdx.x; u < kHaloH * NCG * 4; u += kThreads) {
-      const int q = u & 3, cg = (u >> 2) % NCG, hy = u / (4 * NCG);
+      const int cg = u % NCG, hy = (u / NCG) % kHaloH, q = u / (NCG * kHaloH);
       const int yy = y0 + hy - 1;
       uint4 v[8];
 #pragma unroll
@@ -324,7 +334,7 @@ __global__ __launch_bounds__(kThreads) void k_conv3x3_wgrad(WgradArgs a) {
       uint4 w[8];
       transpose8(v, w);
 #pragma unroll
-      for (int i = 0; i < 8; ++i) *reinterpret_cast<uint4*>(&sx[q * 8 + i][hy][cg * 8]) = w[i];
+      for (int i = 0; i < 8; ++i) *reinterpret_cast<uint4*>(sx(q * 8 + i, hy, cg * 8)) = w[i];
     }
     __syncthreads();
 #pragma unroll
@@ -334,14 +344,14 @@ __global__ __launch_bounds__(kThreads) void k_conv3x3_wgrad(WgradArgs a) {
       bf16x8 ad[NCB];
 #pragma unroll
       for (int cb = 0; cb < NCB; ++cb)
-        ad[cb] = *reinterpret_cast<const bf16x8*>(&sdp[cb * 16 + ln][p0]);
+        ad[cb] = *reinterpret_cast<const bf16x8*>(sdp(cb * 16 + ln, p0));
 #pragma unroll
       for (int j = 0; j < 5; ++j) {
         if (j < ntap) {
           const int tap = tap0 + 2 * j;
           const int dy = tap / 3, dx = tap - 3 * (tap / 3);
           // columns cb0 + dx .. cb0 + dx + 7 from the aligned words at cb0
-          const bf16_t* rp = &sx[cib * 16 + ln][row + dy][cb0];
+          const bf16_t* rp = sx(cib * 16 + ln, row + dy, cb0);
           const uint4 s0 = *reinterpret_cast<const uint4*>(rp);
           const uint32_t s4 = *reinterpret_cast<const uint32_t*>(rp + 8);
           uint4 sel;
