@@ -105,6 +105,12 @@ __device__ __forceinline__ const bf16_t* gather_src(const ConvArgs& a, int b, in
 // that latency).
 constexpr int kWStride = 9 * 32 + 8;   // bf16 per output channel in LDS: 592 bytes, bank-spread
 
+#ifndef SMILI_UNET_TW16_MAXCIN
+#define SMILI_UNET_TW16_MAXCIN (1 << 30)   // measured: 8 x 16 tiles everywhere are fastest
+#endif
+#ifndef SMILI_UNET_WGRAD_TW
+#define SMILI_UNET_WGRAD_TW 32
+#endif
 #ifndef SMILI_UNET_CONV_MIN_BLOCKS
 #define SMILI_UNET_CONV_MIN_BLOCKS 2
 #endif
@@ -922,7 +928,10 @@ void launch_conv(smili_unet* n, int nb, const Act* x1, int up1, const Act* x2, i
   a.w_ = w;
   a.cout = cout;
   a.mode = mode;
-  const int tw = (w % 32 == 0) ? 32 : 16;
+  // 8 x 16 tiles where the width needs it, or where the input has at most
+  // SMILI_UNET_TW16_MAXCIN channels (few slices: more, smaller work-groups
+  // hide the staging latency better than larger tiles)
+  const int tw = (w % 32 == 0 && x1->c + (x2 ? x2->c : 0) > SMILI_UNET_TW16_MAXCIN) ? 32 : 16;
   a.tiles_x = w / tw;
   a.tiles_y = h / kTileH;
   const int ncb = (cout % 64 == 0) ? 4 : 2;
@@ -958,7 +967,7 @@ struct WgradPlan {
 WgradPlan wgrad_plan(int nb, int h, int w, int cout, int cin) {
   WgradPlan p;
   p.ncb = (cout % 64 == 0) ? 4 : 2;
-  p.tw = (w % 32 == 0) ? 32 : 16;
+  p.tw = (w % 32 == 0 && SMILI_UNET_WGRAD_TW == 32) ? 32 : 16;
   p.ncob = cout / (p.ncb * 16);
   p.nchunk = cin / 32;
   p.ntiles = nb * (h / kTileH) * (w / p.tw);
