@@ -310,6 +310,8 @@ def _unet_roofline(params, batch=32, reps=20):
     step_ms, fwd_ms = net.time_steps(10)
     step_flop, fwd_flop = net.flops()
     level = (0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0)
+    # kernel instantiations as the library picks them: 8 x 16 forward /
+    # input-gradient tiles, 8 x 32 weight-gradient tiles where the width allows
     launches, kernels = [], {}
     for li in range(1, len(LAYERS)):
         name, cin, cout = LAYERS[li]
@@ -319,10 +321,10 @@ def _unet_roofline(params, batch=32, reps=20):
             launches.append((ms, fl, name, pname))
             # the kernel instantiation this launch runs (the library's choice)
             if ps == 2:
-                kern = f"k_conv3x3_wgrad<{4 if cout % 64 == 0 else 2}>"
+                kern = f"k_conv3x3_wgrad<{4 if cout % 64 == 0 else 2}, {32 if w % 32 == 0 else 16}>"
             else:
                 co = cout if ps == 0 else cin
-                kern = f"k_conv3x3<{32 if w % 32 == 0 else 16}, {4 if co % 64 == 0 else 2}>"
+                kern = f"k_conv3x3<16, {4 if co % 64 == 0 else 2}>"
             k = kernels.setdefault(kern, [0.0, 0.0, 0])
             k[0] += ms
             k[1] += fl

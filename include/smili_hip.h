@@ -144,7 +144,7 @@ typedef struct smili_tune {
                            default 256) */
   int32_t col_group;    /* columns per column-FFT work-group (1..16; 0 = default:
                            the largest of 4, 2, 1 whose LDS fits 64 KB) */
-  int32_t chunk;        /* data items per chi-square work-group (default 2048) */
+  int32_t chunk;        /* data items per chi-square work-group (default 4096) */
   int32_t use_graph;    /* >= 0: capture one evaluation in a hipGraph and replay it;
                            < 0: launch the kernels one by one */
   int32_t precision;    /* bits of the NUFFT arithmetic (spreading,
