@@ -185,7 +185,9 @@ class UNet:
 
     def train(self, index, lr, beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=0.0):
         """Adam steps over batches index[step, batch] of the uploaded set;
-        returns the per-step losses."""
+        an index is 8 * sample + k with k one of the 8 symmetries of the
+        square (smili_amd.warmstart.data.dihedral; 0 = identity), applied to
+        input and target on the device.  Returns the per-step losses."""
         idx = np.ascontiguousarray(index, dtype=np.int64)
         if idx.ndim != 2 or idx.shape[1] != self.batch:
             raise ValueError(f"index must be [nsteps, {self.batch}]")
