@@ -77,7 +77,9 @@ int smili_unet_time_steps(smili_unet* net, int32_t reps, float* ms);
 /* Times `reps` launches of one 3x3 convolution kernel of layer `layer`
  * (1..13, the parameter order above; 0 is the float32 first convolution)
  * on the activations of the last forward pass: pass 0 forward, 1 input
- * gradient, 2 weight gradient (this one adds into the gradient buffer).
+ * gradient (for the odd layers, whose input has no other consumer, fused
+ * with that input's ReLU mask and bias sums as in training), 2 weight
+ * gradient (this one adds into the gradient buffer).
  * ms = mean launch time (HIP events on the handle's stream), flop = FLOP
  * of one launch for a full batch. */
 int smili_unet_time_layer(smili_unet* net, int32_t layer, int32_t pass, int32_t reps, float* ms, double* flop);

@@ -71,11 +71,15 @@ struct ConvArgs {
   const bf16_t* x2;  // channels [c1, c1 + c2): full resolution (may be null)
   const bf16_t* w;   // [cout][9][c1 + c2]
   const float* bias; // [cout] (mode 0)
-  bf16_t* ybf;       // mode 0 output
+  bf16_t* ybf;       // mode 0 / 3 output
   float* yf;         // mode 1 / 2 output
+  const bf16_t* mask;  // mode 3: activation whose sign gates the result
+  float* part;         // mode 3: per-tile channel sums [tile][cout]
   int c1, c2, up1;
   int n, h, w_, cout;
-  int mode;          // 0: bias + ReLU -> bf16; 1: fp32 store; 2: fp32 add
+  int mode;          // 0: bias + ReLU -> bf16; 1: fp32 store; 2: fp32 add;
+                     // 3: input gradient through the ReLU of `mask` -> bf16 dP of
+                     //    the layer below, with its bias-gradient partial sums
   int tiles_x, tiles_y, ncob;
 };
 
@@ -202,6 +206,11 @@ __global__ __launch_bounds__(kThreads, SMILI_UNET_CONV_MIN_BLOCKS) void k_conv3x
     }
   }
   // D[row = channel 4*lg + i][col = pixel ln]
+  float csum[NCB][4];
+#pragma unroll
+  for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) csum[cb][i] = 0.f;
 #pragma unroll
   for (int pb = 0; pb < NPB; ++pb) {
     const int y = y0 + wv * 2 + pb / (TW / 16), x = x0 + (pb % (TW / 16)) * 16 + ln;
@@ -210,7 +219,20 @@ __global__ __launch_bounds__(kThreads, SMILI_UNET_CONV_MIN_BLOCKS) void k_conv3x
     for (int cb = 0; cb < NCB; ++cb) {
       const int co = co0 + cb * 16 + lg * 4;
       const f32x4 v = acc[pb][cb];
-      if (a.mode == 0) {
+      if (a.mode == 3) {
+        const uint2 m = *reinterpret_cast<const uint2*>(a.mask + o + co);
+        const uint32_t mw[4] = {m.x & 0xFFFFu, m.x >> 16, m.y & 0xFFFFu, m.y >> 16};
+        float r[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          r[i] = bf2f(mw[i]) > 0.f ? v[i] : 0.f;
+          csum[cb][i] += r[i];
+        }
+        uint2 pk;
+        pk.x = f2bf(r[0]) | (f2bf(r[1]) << 16);
+        pk.y = f2bf(r[2]) | (f2bf(r[3]) << 16);
+        *reinterpret_cast<uint2*>(a.ybf + o + co) = pk;
+      } else if (a.mode == 0) {
         float r[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) r[i] = fmaxf(v[i] + a.bias[co + i], 0.f);
@@ -227,6 +249,27 @@ __global__ __launch_bounds__(kThreads, SMILI_UNET_CONV_MIN_BLOCKS) void k_conv3x
         }
         *dst = r;
       }
+    }
+  }
+  if (a.mode == 3) {
+    // channel sums over the tile: the 16 lanes of a channel group hold
+    // different pixels, the 4 waves different rows
+    __shared__ float red[4][NCO];
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        float s = csum[cb][i];
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        s += __shfl_xor(s, 4);
+        s += __shfl_xor(s, 8);
+        if (ln == 0) red[wv][cb * 16 + lg * 4 + i] = s;
+      }
+    __syncthreads();
+    if (threadIdx.x < NCO) {
+      const float s = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+      a.part[(size_t)(item / a.ncob) * a.cout + co0 + threadIdx.x] = s;
     }
   }
 }
@@ -867,7 +910,8 @@ struct smili_unet {
   bf16_t *wbf = nullptr, *wfl = nullptr;
   float *xin = nullptr, *tgt = nullptr, *out = nullptr;
   float *dtmp = nullptr;          // input gradient of decoder convolutions
-  bf16_t* dp = nullptr;           // masked output gradient
+  bf16_t* dp = nullptr;           // masked output gradient (two buffers: a
+  bf16_t* dp2 = nullptr;          // layer's dP and the dP it makes for the layer below)
   float* part = nullptr;          // per-block partial sums of bias / first-layer reductions
   float* wpart = nullptr;         // per-pixel-group partial sums of 3x3 weight gradients
   size_t nwpart = 0;
@@ -911,9 +955,14 @@ inline int grid_for(size_t work, int per_block = kThreads, int cap = 8192) {
 const bf16_t* wbf_of(smili_unet* n, int li) { return n->wbf + n->L[li].bfoff; }
 const bf16_t* wfl_of(smili_unet* n, int li) { return n->wfl + n->L[li].bfoff; }
 
-void launch_conv(smili_unet* n, int nb, const Act* x1, int up1, const Act* x2, int li, int mode, bf16_t* ybf,
-                 float* yf, int h, int w, int cout, const bf16_t* wt, const float* bias) {
+// Launches one 3x3 convolution; returns its number of pixel tiles (the rows
+// of mode 3's partial sums).
+int launch_conv(smili_unet* n, int nb, const Act* x1, int up1, const Act* x2, int li, int mode, bf16_t* ybf,
+                float* yf, int h, int w, int cout, const bf16_t* wt, const float* bias,
+                const bf16_t* mask = nullptr, float* part = nullptr) {
   ConvArgs a;
+  a.mask = mask;
+  a.part = part;
   a.x1 = x1->y;
   a.c1 = x1->c;
   a.up1 = up1;
@@ -946,6 +995,7 @@ void launch_conv(smili_unet* n, int nb, const Act* x1, int up1, const Act* x2, i
   else
     hipLaunchKernelGGL((k_conv3x3<16, 2>), grid, dim3(kThreads), 0, n->s, a);
   (void)li;
+  return nb * a.tiles_x * a.tiles_y;
 }
 
 // forward 3x3 conv of layer li: y = relu(conv(x1 [up] ++ x2) + b)
@@ -989,9 +1039,10 @@ void sum_rows(smili_unet* n, const float* part, int nrow, size_t ncol, float* ou
                        out);
 }
 
-void conv_wgrad(smili_unet* n, int nb, int li, const Act* x1, int up1, const Act* x2, int h, int w, int cout) {
+void conv_wgrad(smili_unet* n, int nb, int li, const Act* x1, int up1, const Act* x2, int h, int w, int cout,
+                const bf16_t* dp) {
   WgradArgs a;
-  a.dp = n->dp;
+  a.dp = dp;
   a.x1 = x1->y;
   a.c1 = x1->c;
   a.up1 = up1;
@@ -1027,26 +1078,43 @@ void conv_wgrad(smili_unet* n, int nb, int li, const Act* x1, int up1, const Act
 constexpr int kMaskBlocks = SMILI_UNET_MASK_BLOCKS;   // rows of the partial-sum scratch
 constexpr int kInPxPerBlock = 1024;
 
-void relu_mask(smili_unet* n, int nb, const Act* y, const float* da, int li) {
+void relu_mask(smili_unet* n, int nb, const Act* y, const float* da, int li, bf16_t* dp) {
   const size_t npx = (size_t)nb * y->h * y->w;
   const int blocks = grid_for(npx * (y->c / 8), kThreads, kMaskBlocks);
-  hipLaunchKernelGGL(k_relu_mask, dim3(blocks), dim3(kThreads), 0, n->s, da, y->y, n->dp, n->part, npx, y->c);
+  hipLaunchKernelGGL(k_relu_mask, dim3(blocks), dim3(kThreads), 0, n->s, da, y->y, dp, n->part, npx, y->c);
   sum_rows(n, n->part, blocks, y->c, n->g + n->L[li].boff);
 }
 
 // Backward through ReLU + conv li (input x1 [up] ++ x2, output y with
 // gradient da).  The input gradient goes to dx (fp32, [n][h][w][cin] at
 // the output's resolution) unless dx is null.
-void conv_bwd(smili_unet* n, int nb, int li, const Act* y, const float* da, const Act* x1, int up1, const Act* x2,
-              float* dx) {
-  relu_mask(n, nb, y, da, li);
-  conv_wgrad(n, nb, li, x1, up1, x2, y->h, y->w, y->c);
-  if (dx) {
-    Act dpa;
-    dpa.y = n->dp;
-    dpa.c = y->c;
-    launch_conv(n, nb, &dpa, 0, nullptr, li, 1, nullptr, dx, y->h, y->w, n->L[li].cin, wfl_of(n, li), nullptr);
+// This layer's dP comes from dp_in (made by the fused epilogue of the layer
+// above, its bias gradient already summed) or is made here from the output
+// gradient da.  The input gradient goes to dx (fp32), or -- with fuse, the
+// input activation when this layer is its only consumer -- through fuse's
+// ReLU straight into the dP of layer li - 1 (in the other dP buffer) with
+// that layer's bias gradient; the new dP is returned.
+bf16_t* conv_bwd(smili_unet* n, int nb, int li, const Act* y, const float* da, bf16_t* dp_in, const Act* x1,
+                 int up1, const Act* x2, float* dx, const Act* fuse = nullptr) {
+  bf16_t* dp = dp_in;
+  if (!dp) {
+    dp = n->dp;
+    relu_mask(n, nb, y, da, li, dp);
   }
+  conv_wgrad(n, nb, li, x1, up1, x2, y->h, y->w, y->c, dp);
+  Act dpa;
+  dpa.y = dp;
+  dpa.c = y->c;
+  if (fuse) {
+    bf16_t* out = (dp == n->dp) ? n->dp2 : n->dp;
+    const int ntiles = launch_conv(n, nb, &dpa, 0, nullptr, li, 3, out, nullptr, y->h, y->w, n->L[li].cin,
+                                   wfl_of(n, li), nullptr, fuse->y, n->part);
+    sum_rows(n, n->part, ntiles, n->L[li].cin, n->g + n->L[li - 1].boff);
+    return out;
+  }
+  if (dx)
+    launch_conv(n, nb, &dpa, 0, nullptr, li, 1, nullptr, dx, y->h, y->w, n->L[li].cin, wfl_of(n, li), nullptr);
+  return nullptr;
 }
 
 void split_up(smili_unet* n, int nb, const float* dcat, int h, int w, int c1, int c2, float* dx1, float* dx2) {
@@ -1098,31 +1166,34 @@ void head(smili_unet* n, int nb, bool train, double* loss) {
 
 void backward(smili_unet* n, int nb) {
   // decoder
-  conv_bwd(n, nb, 13, &n->d0, n->d0.g, &n->d0a, 0, nullptr, n->d0a.g);
-  conv_bwd(n, nb, 12, &n->d0a, n->d0a.g, &n->d1, 1, &n->e0, n->dtmp);
+  // (the second convolution of each level is the only consumer of the
+  // first one's output: its input gradient goes straight to the first
+  // one's dP)
+  bf16_t* dp = conv_bwd(n, nb, 13, &n->d0, n->d0.g, nullptr, &n->d0a, 0, nullptr, nullptr, &n->d0a);
+  conv_bwd(n, nb, 12, &n->d0a, nullptr, dp, &n->d1, 1, &n->e0, n->dtmp);
   split_up(n, nb, n->dtmp, n->d0a.h, n->d0a.w, n->d1.c, n->e0.c, n->d1.g, n->e0.g);
-  conv_bwd(n, nb, 11, &n->d1, n->d1.g, &n->d1a, 0, nullptr, n->d1a.g);
-  conv_bwd(n, nb, 10, &n->d1a, n->d1a.g, &n->d2, 1, &n->e1, n->dtmp);
+  dp = conv_bwd(n, nb, 11, &n->d1, n->d1.g, nullptr, &n->d1a, 0, nullptr, nullptr, &n->d1a);
+  conv_bwd(n, nb, 10, &n->d1a, nullptr, dp, &n->d2, 1, &n->e1, n->dtmp);
   split_up(n, nb, n->dtmp, n->d1a.h, n->d1a.w, n->d2.c, n->e1.c, n->d2.g, n->e1.g);
-  conv_bwd(n, nb, 9, &n->d2, n->d2.g, &n->d2a, 0, nullptr, n->d2a.g);
-  conv_bwd(n, nb, 8, &n->d2a, n->d2a.g, &n->e3, 1, &n->e2, n->dtmp);
+  dp = conv_bwd(n, nb, 9, &n->d2, n->d2.g, nullptr, &n->d2a, 0, nullptr, nullptr, &n->d2a);
+  conv_bwd(n, nb, 8, &n->d2a, nullptr, dp, &n->e3, 1, &n->e2, n->dtmp);
   split_up(n, nb, n->dtmp, n->d2a.h, n->d2a.w, n->e3.c, n->e2.c, n->e3.g, n->e2.g);
   // encoder
-  conv_bwd(n, nb, 7, &n->e3, n->e3.g, &n->e3a, 0, nullptr, n->e3a.g);
-  conv_bwd(n, nb, 6, &n->e3a, n->e3a.g, &n->p2, 0, nullptr, n->p2.g);
+  dp = conv_bwd(n, nb, 7, &n->e3, n->e3.g, nullptr, &n->e3a, 0, nullptr, nullptr, &n->e3a);
+  conv_bwd(n, nb, 6, &n->e3a, nullptr, dp, &n->p2, 0, nullptr, n->p2.g);
   pool_bwd_add(n, nb, n->p2.g, &n->e2);
-  conv_bwd(n, nb, 5, &n->e2, n->e2.g, &n->e2a, 0, nullptr, n->e2a.g);
-  conv_bwd(n, nb, 4, &n->e2a, n->e2a.g, &n->p1, 0, nullptr, n->p1.g);
+  dp = conv_bwd(n, nb, 5, &n->e2, n->e2.g, nullptr, &n->e2a, 0, nullptr, nullptr, &n->e2a);
+  conv_bwd(n, nb, 4, &n->e2a, nullptr, dp, &n->p1, 0, nullptr, n->p1.g);
   pool_bwd_add(n, nb, n->p1.g, &n->e1);
-  conv_bwd(n, nb, 3, &n->e1, n->e1.g, &n->e1a, 0, nullptr, n->e1a.g);
-  conv_bwd(n, nb, 2, &n->e1a, n->e1a.g, &n->p0, 0, nullptr, n->p0.g);
+  dp = conv_bwd(n, nb, 3, &n->e1, n->e1.g, nullptr, &n->e1a, 0, nullptr, nullptr, &n->e1a);
+  conv_bwd(n, nb, 2, &n->e1a, nullptr, dp, &n->p0, 0, nullptr, n->p0.g);
   pool_bwd_add(n, nb, n->p0.g, &n->e0);
-  conv_bwd(n, nb, 1, &n->e0, n->e0.g, &n->e0a, 0, nullptr, n->e0a.g);
+  // layer 1's input gradient is the first convolution's dP (and bias gradient)
+  dp = conv_bwd(n, nb, 1, &n->e0, n->e0.g, nullptr, &n->e0a, 0, nullptr, nullptr, &n->e0a);
   // first convolution: weight gradient only
-  relu_mask(n, nb, &n->e0a, n->e0a.g, 0);
   const size_t npx = (size_t)nb * n->H * n->W;
   const int blocks = (int)((npx + kInPxPerBlock - 1) / kInPxPerBlock);
-  hipLaunchKernelGGL(k_conv_in_wgrad, dim3(blocks), dim3(kThreads), 0, n->s, n->dp, n->xin, n->part, nb, n->H,
+  hipLaunchKernelGGL(k_conv_in_wgrad, dim3(blocks), dim3(kThreads), 0, n->s, dp, n->xin, n->part, nb, n->H,
                      n->W, kInPxPerBlock);
   sum_rows(n, n->part, blocks, kBase * 18, n->g + n->L[0].woff);
 }
@@ -1231,10 +1302,16 @@ int smili_unet_create(int32_t batch, int32_t height, int32_t width, smili_unet**
   dt = std::max(dt, B * (H / 4) * (W / 4) * (size_t)(c3 + c2));
   if (!r) r = dalloc(n, (void**)&n->dtmp, dt * 4);
   if (!r) r = dalloc(n, (void**)&n->dp, B * H * W * c0 * 2);
+  if (!r) r = dalloc(n, (void**)&n->dp2, B * H * W * c0 * 2);
   // partial sums: the larger of the bias reductions (kMaskBlocks rows of at
   // most c3 channels) and the first convolution's weight gradient
   const size_t nin_blocks = (B * H * W + kInPxPerBlock - 1) / kInPxPerBlock;
-  if (!r) r = dalloc(n, (void**)&n->part, std::max((size_t)kMaskBlocks * c3, nin_blocks * kBase * 18) * 4);
+  // (and the fused input gradients' per-tile sums: at most one row of c0
+  // channels per 8 x 16 tile of the full-resolution level)
+  const size_t fused_rows = B * (H / kTileH) * (W / 16) * (size_t)c0;
+  if (!r)
+    r = dalloc(n, (void**)&n->part,
+               std::max(std::max((size_t)kMaskBlocks * c3, nin_blocks * kBase * 18), fused_rows) * 4);
   // weight-gradient partial rows: the largest split over the 3x3 layers
   // (ngroups <= min(want, tiles) <= its value at the full batch)
   const int lev[14] = {0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0};
@@ -1487,11 +1564,14 @@ int smili_unet_time_layer(smili_unet* n, int32_t layer, int32_t pass, int32_t re
   for (int r = 0; r < reps; ++r) {
     if (pass == 0)
       conv_fwd(n, n->B, layer, x1, up1, x2, const_cast<Act*>(y));
+    else if (pass == 1 && (layer & 1))  // fused with the ReLU of its input, as in training
+      launch_conv(n, n->B, &dpa, 0, nullptr, layer, 3, n->dp2, nullptr, y->h, y->w, n->L[layer].cin,
+                  wfl_of(n, layer), nullptr, x1->y, n->part);
     else if (pass == 1)
       launch_conv(n, n->B, &dpa, 0, nullptr, layer, 1, nullptr, n->dtmp, y->h, y->w, n->L[layer].cin,
                   wfl_of(n, layer), nullptr);
     else
-      conv_wgrad(n, n->B, layer, x1, up1, x2, y->h, y->w, y->c);
+      conv_wgrad(n, n->B, layer, x1, up1, x2, y->h, y->w, y->c, n->dp);
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(n->ev1, n->s));
