@@ -156,6 +156,8 @@ typedef struct smili_tune {
                            of at least 2e5 uv points or 16 images, else 1 */
   int32_t fft_radix;    /* largest power-of-two radix of the grid FFTs: 2, 4
                            or 8 (0 = default, 4) */
+  int32_t chisq_threads; /* threads per chi-square (and regularizer)
+                           work-group: 256 or 512 (0 = default, 256) */
 } smili_tune;
 
 int smili_abi_version(void);

@@ -218,7 +218,7 @@ class CostProblem(_Handle):
     already phase-shifted; w* normalised weights.  A regularizer is on iff
     its lambda > 0; a data block is on iff it is given and non-empty.
     ``tune`` sets smili_tune fields (bin, sub_points, col_group, chunk,
-    use_graph, precision).
+    use_graph, precision, branches, fft_radix, chisq_threads).
     """
 
     def __init__(self, tune=None, **kw):

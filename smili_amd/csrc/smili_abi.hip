@@ -182,6 +182,7 @@ struct Plan {
   int bin = 16, sub_points = 256, col_group = 2, chunk = 4096, rows_per_block = 1;
   bool fp64 = false;                       // NUFFT arithmetic in FP64 instead of FP32
   int threads_rows = 64, threads_cols = 128, fft_radix = 4;
+  int chisq_threads = kThreads;            // work-group size of the chi-square (+ regularizer) launch
 
   std::vector<UnitDev> units;
   std::vector<ProbDev> probs;
@@ -324,6 +325,9 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   require(chunk >= 64, "chunk must be >= 64");
   require(tune.fft_radix == 0 || tune.fft_radix == 2 || tune.fft_radix == 4 || tune.fft_radix == 8,
           "fft_radix must be 0, 2, 4 or 8");
+  require(tune.chisq_threads == 0 || tune.chisq_threads == 256 || tune.chisq_threads == kChisqThreadsMax,
+          "chisq_threads must be 0, 256 or 512");
+  chisq_threads = tune.chisq_threads > 0 ? tune.chisq_threads : kThreads;
   tune.use_graph = (tune.use_graph < 0) ? 0 : 1;
   require(tune.branches >= 0 && tune.branches <= 2, "branches must be 0, 1 or 2");
 
@@ -979,18 +983,18 @@ void Plan::launch_eval() {
     const size_t grid = h_chunks.size() + (fused ? h_regs.size() : 0);
     if (fp64) {
       if (fused)
-        LAUNCH(SMILI_S_CHISQ, k_chisq_reg<double2>, grid, kThreads, 0, nch, d_chunks.p, d_slots.p, d_fcv.p, d_amp.p,
-               d_cp.p, d_ca.p, d_V.p, d_Vres.p, d_cpart.p, reg_args);
+        LAUNCH(SMILI_S_CHISQ, k_chisq_reg<double2>, grid, chisq_threads, 0, nch, d_chunks.p, d_slots.p, d_fcv.p,
+               d_amp.p, d_cp.p, d_ca.p, d_V.p, d_Vres.p, d_cpart.p, reg_args);
       else
-        LAUNCH(SMILI_S_CHISQ, k_chisq<double2>, grid, kThreads, 0, d_chunks.p, d_slots.p, d_fcv.p, d_amp.p,
+        LAUNCH(SMILI_S_CHISQ, k_chisq<double2>, grid, chisq_threads, 0, d_chunks.p, d_slots.p, d_fcv.p, d_amp.p,
                d_cp.p, d_ca.p, d_V.p, d_Vres.p, d_cpart.p);
     } else {
       const float2* Vf = reinterpret_cast<const float2*>(d_V.p);
       if (fused)
-        LAUNCH(SMILI_S_CHISQ, k_chisq_reg<float2>, grid, kThreads, 0, nch, d_chunks.p, d_slots.p, d_fcv.p, d_amp.p,
+        LAUNCH(SMILI_S_CHISQ, k_chisq_reg<float2>, grid, chisq_threads, 0, nch, d_chunks.p, d_slots.p, d_fcv.p, d_amp.p,
                d_cp.p, d_ca.p, Vf, d_Vres.p, d_cpart.p, reg_args);
       else
-        LAUNCH(SMILI_S_CHISQ, k_chisq<float2>, grid, kThreads, 0, d_chunks.p, d_slots.p, d_fcv.p, d_amp.p,
+        LAUNCH(SMILI_S_CHISQ, k_chisq<float2>, grid, chisq_threads, 0, d_chunks.p, d_slots.p, d_fcv.p, d_amp.p,
                d_cp.p, d_ca.p, Vf, d_Vres.p, d_cpart.p);
     }
   }

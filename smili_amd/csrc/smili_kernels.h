@@ -9,7 +9,8 @@
 
 namespace smili {
 
-constexpr int kThreads = 256;        // work-group size of every kernel
+constexpr int kThreads = 256;        // work-group size of every kernel but the chi-square ones
+constexpr int kChisqThreadsMax = 512;  // chi-square (+ regularizer) work-groups: 256 or 512
 constexpr int kW = 7;                // ES kernel width (eps 1e-6)
 constexpr int kMaxRadixStages = 12;
 constexpr int kRegTerms = 7;         // l1, tv, tsv, kl, gs, rt, ri partial sums per item

@@ -21,23 +21,23 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// red must hold NV * (kThreads / 64) doubles; blockDim.x is a multiple of
-// 64 and at most kThreads.  The result is valid in thread 0.
-template <int NV>
+// red must hold NV * NW doubles; blockDim.x is a multiple of 64 and at
+// most 64 NW.  The result is valid in thread 0.
+template <int NV, int NW = kThreads / 64>
 __device__ __forceinline__ void block_sum(double* v, double* red) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int nw = blockDim.x >> 6;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     double s = wave_sum(v[k]);
-    if (lane == 0) red[k * (kThreads / 64) + wid] = s;
+    if (lane == 0) red[k * NW + wid] = s;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       double s = 0.0;
-      for (int w = 0; w < nw; ++w) s += red[k * (kThreads / 64) + w];
+      for (int w = 0; w < nw; ++w) s += red[k * NW + w];
       v[k] = s;
     }
   }
@@ -65,7 +65,7 @@ __device__ void chisq_block(int bid, const Chunk* __restrict__ chunks, const int
   __shared__ double2 Vl[kMaxSlots];
   __shared__ double2 Rl[kMaxSlots];
   __shared__ double2 Al[kMaxSlots];      // per-point quantities shared by the chunk's data
-  __shared__ double red[kThreads / 64];
+  __shared__ double red[kChisqThreadsMax / 64];
   const Chunk ch = chunks[bid];
   const int* sl = slots + ch.slot_off;
   // stage the chunk's model visibilities and, once per point rather than
@@ -167,12 +167,12 @@ __device__ void chisq_block(int bid, const Chunk* __restrict__ chunks, const int
     unsafeAtomicAdd(&Vres[j].y, r.y);
   }
   double v[1] = {acc};
-  block_sum<1>(v, red);
+  block_sum<1, kChisqThreadsMax / 64>(v, red);
   if (threadIdx.x == 0) partial[bid] = v[0];
 }
 
 template <typename VT>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kChisqThreadsMax)
 k_chisq(const Chunk* __restrict__ chunks, const int* __restrict__ slots,
         const FcvItem* __restrict__ fcv, const AmpItem* __restrict__ amp,
         const CpItem* __restrict__ cp, const CaItem* __restrict__ ca,
@@ -358,7 +358,7 @@ __device__ void reg_block(int bid, const RegArgs& A) {
   double* __restrict__ greg = A.greg;
   double* __restrict__ grad = A.grad;
   double* __restrict__ rpart = A.rpart;
-  __shared__ double red[kRegTerms * (kThreads / 64)];
+  __shared__ double red[kRegTerms * (kChisqThreadsMax / 64)];
   const RegItem it = items[bid];
   const UnitDev U = units[it.unit];
   const double* I2d = U.identity ? (x + U.x_off) : (I2dbuf + U.img_off);
@@ -476,7 +476,7 @@ __device__ void reg_block(int bid, const RegArgs& A) {
     if (U.m > 0) greg[U.x_off + k] = gr;
     else grad[U.x_off + k] = 0.0 + gr;
   }
-  block_sum<kRegTerms>(v, red);
+  block_sum<kRegTerms, kChisqThreadsMax / 64>(v, red);
   if (threadIdx.x == 0)
     for (int k = 0; k < kRegTerms; ++k) rpart[(long long)bid * kRegTerms + k] = v[k];
 }
@@ -487,7 +487,7 @@ __global__ void __launch_bounds__(kThreads) k_reg(RegArgs A) { reg_block(blockId
 // are independent (V vs x), so the regularizer work-groups fill the CUs the
 // chunks leave idle instead of adding a launch to the dependent chain.
 template <typename VT>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kChisqThreadsMax)
 k_chisq_reg(int nchunks, const Chunk* __restrict__ chunks, const int* __restrict__ slots,
             const FcvItem* __restrict__ fcv, const AmpItem* __restrict__ amp,
             const CpItem* __restrict__ cp, const CaItem* __restrict__ ca,

@@ -151,7 +151,7 @@ def test_graph_and_eager_agree_and_repeat(gpu_lib):
 
 
 @pytest.mark.parametrize("tune", [dict(bin=8, sub_points=64), dict(bin=32, sub_points=512, col_group=4),
-                                  dict(chunk=64)])
+                                  dict(chunk=64), dict(chisq_threads=512), dict(chisq_threads=512, chunk=64)])
 def test_tunables_do_not_change_results(gpu_lib, tune):
     rng = np.random.default_rng(21)
     p, x = random_problem(rng, 40, 40, M=700)
@@ -160,6 +160,17 @@ def test_tunables_do_not_change_results(gpu_lib, tune):
         c2, gr2, _ = h.cost_grad(x)
     assert abs(c1 - c2) <= 1e-6 * abs(c1)
     assert rel_l2(gr2, gr1) < 1e-6
+
+
+@pytest.mark.parametrize("tune", [dict(chisq_threads=128), dict(chisq_threads=1024), dict(fft_radix=3),
+                                  dict(chunk=8), dict(bin=12)])
+def test_invalid_tunables_are_rejected(gpu_lib, tune):
+    from smili_amd._lib import SmiliError
+    rng = np.random.default_rng(22)
+    p, _ = random_problem(rng, 16, 16, M=50)
+    with pytest.raises(SmiliError) as e:
+        _mk(p, tune=tune)
+    assert e.value.code == -1 and next(iter(tune)) in str(e.value)
 
 
 def test_points_on_the_period_boundary(gpu_lib):
