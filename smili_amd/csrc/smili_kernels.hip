@@ -56,6 +56,18 @@ __device__ __forceinline__ void lds_add2(double2* R, int s, double re, double im
   atomicAdd(&R[s].y, im);
 }
 
+// Closure data are stored by timestamp, and consecutive closures of one
+// timestamp share baselines: 64 of them in one wave would hit the same few
+// uv points, and LDS atomics on one address serialise.  Lane l of every wave
+// therefore takes the closures l Q, l Q + 1, ... (Q = ceil(count / 64)) of
+// the chunk, so a wave's lanes hold closures about Q apart.
+#ifndef SMILI_CHISQ_INTERLEAVE
+#define SMILI_CHISQ_INTERLEAVE 1
+#endif
+__device__ __forceinline__ int closure_item(int j, int q) {
+  return SMILI_CHISQ_INTERLEAVE ? (j & 63) * q + (j >> 6) : j;
+}
+
 template <typename VT>
 __device__ void chisq_block(int bid, const Chunk* __restrict__ chunks, const int* __restrict__ slots,
                             const FcvItem* __restrict__ fcv, const AmpItem* __restrict__ amp,
@@ -112,7 +124,10 @@ __device__ void chisq_block(int bid, const Chunk* __restrict__ chunks, const int
       lds_add2(Rl, s, f * m.x, f * m.y);
     }
   } else if (ch.type == DT_CP) {
-    for (int i = threadIdx.x; i < ch.count; i += blockDim.x) {
+    const int q = (ch.count + 63) >> 6;
+    for (int j = threadIdx.x; j < 64 * q; j += blockDim.x) {
+      const int i = closure_item(j, q);
+      if (i >= ch.count) continue;
       const CpItem d = cp[ch.start + i];
       double2 m[3], al[3];
       double sg[3], model = 0.0;
@@ -137,7 +152,10 @@ __device__ void chisq_block(int bid, const Chunk* __restrict__ chunks, const int
       }
     }
   } else {
-    for (int i = threadIdx.x; i < ch.count; i += blockDim.x) {
+    const int q = (ch.count + 63) >> 6;
+    for (int j = threadIdx.x; j < 64 * q; j += blockDim.x) {
+      const int i = closure_item(j, q);
+      if (i >= ch.count) continue;
       const CaItem d = ca[ch.start + i];
       double2 m[4], al[4];
       int s[4];
