@@ -570,47 +570,93 @@ __global__ void k_pool(const bf16_t* __restrict__ x, bf16_t* __restrict__ y, int
   }
 }
 
-// dX[2y+i][2x+j] += dY[y][x] / 4 (fp32, 4 channels per thread)
-__global__ void k_pool_bwd_add(const float* __restrict__ dy, float* __restrict__ dx, int n, int h, int wd, int c) {
+// Partial sums of a bias gradient: each thread holds s[4] for its channels
+// 4 q .. 4 q + 3 (q = threadIdx.x % cg, fixed because the grid stride is a
+// multiple of cg); the block's row part[blockIdx.x][c] (c = 4 cg) sums the
+// threads of each q.  Every block writes its row.
+__device__ __forceinline__ void block_row_sums(const float* s, float* part, int cg) {
+  __shared__ float red[kThreads][5];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) red[threadIdx.x][i] = s[i];
+  __syncthreads();
+  if ((int)threadIdx.x < cg) {
+    float t[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int r = threadIdx.x; r < kThreads; r += cg)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) t[i] += red[r][i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) part[(size_t)blockIdx.x * 4 * cg + 4 * threadIdx.x + i] = t[i];
+  }
+}
+
+// (dA * (Y > 0)) of 4 channels: bf16 into *dst, fp32 added to s
+__device__ __forceinline__ void mask4(float4 g, uint2 y, uint2* dst, float* s) {
+  const float gv[4] = {g.x, g.y, g.z, g.w};
+  const uint32_t yw[4] = {y.x & 0xFFFFu, y.x >> 16, y.y & 0xFFFFu, y.y >> 16};
+  float r[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    r[i] = bf2f(yw[i]) > 0.f ? gv[i] : 0.f;
+    s[i] += r[i];
+  }
+  *dst = make_uint2(f2bf(r[0]) | (f2bf(r[1]) << 16), f2bf(r[2]) | (f2bf(r[3]) << 16));
+}
+
+// Gradient of a pooled activation X (the output of a layer, bf16 y): its
+// skip gradient dskip (fp32) plus the pooling adjoint dY[y][x] / 4 of every
+// 2x2 block, through the layer's ReLU (y > 0) into the layer's dP (bf16), and
+// the per-block partial sums part[block][c] of its bias gradient.  4 channels
+// per thread.
+__global__ __launch_bounds__(kThreads) void k_pool_bwd_mask(const float* __restrict__ dy,
+                                                            const float* __restrict__ dskip,
+                                                            const bf16_t* __restrict__ y, bf16_t* __restrict__ dp,
+                                                            float* __restrict__ part, int n, int h, int wd, int c) {
   const int ho = h >> 1, wo = wd >> 1, cg = c >> 2;
   const size_t total = (size_t)n * ho * wo * cg;
+  float s[4] = {0.f, 0.f, 0.f, 0.f};
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
     const int q = e % cg;
     const size_t p = e / cg;
     const int xo = p % wo, yo = (p / wo) % ho, b = p / ((size_t)wo * ho);
-    float4 g = *reinterpret_cast<const float4*>(dy + p * c + q * 4);
-    g.x *= 0.25f; g.y *= 0.25f; g.z *= 0.25f; g.w *= 0.25f;
+    const float4 g = *reinterpret_cast<const float4*>(dy + p * c + q * 4);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int yy = 2 * yo + (k >> 1), xx = 2 * xo + (k & 1);
-      float4* d = reinterpret_cast<float4*>(dx + ((size_t)(b * h + yy) * wd + xx) * c + q * 4);
-      float4 r = *d;
-      r.x += g.x; r.y += g.y; r.z += g.z; r.w += g.w;
-      *d = r;
+      const size_t o = ((size_t)(b * h + yy) * wd + xx) * c + q * 4;
+      float4 a = *reinterpret_cast<const float4*>(dskip + o);
+      a.x += 0.25f * g.x; a.y += 0.25f * g.y; a.z += 0.25f * g.z; a.w += 0.25f * g.w;
+      mask4(a, *reinterpret_cast<const uint2*>(y + o), reinterpret_cast<uint2*>(dp + o), s);
     }
   }
+  block_row_sums(s, part, cg);
 }
 
-// Splits the input gradient of a decoder convolution [n][h][w][c1 + c2]:
-// the upsampled part sums over 2x2 into dx1 [n][h/2][w/2][c1], the skip
-// part is stored into dx2 [n][h][w][c2].
-__global__ void k_split_up(const float* __restrict__ dcat, float* __restrict__ dx1, float* __restrict__ dx2,
-                           int n, int h, int wd, int c1, int c2) {
+// Adjoint of the decoder input [up(x1) ++ x2] (dcat fp32, c1 + c2
+// channels): the x1 part sums each 2x2 block (nearest upsampling) and goes
+// through x1's ReLU (y1 > 0) into the dP of x1's layer (bf16 dp1) with the
+// per-block partial sums part[block][c1] of its bias gradient; the x2 part
+// (the skip connection) is copied to dx2 (fp32; the pooling adjoint adds to
+// it later).  4 channels per thread.
+__global__ __launch_bounds__(kThreads) void k_split_up(const float* __restrict__ dcat, const bf16_t* __restrict__ y1,
+                                                       bf16_t* __restrict__ dp1, float* __restrict__ part,
+                                                       float* __restrict__ dx2, int n, int h, int wd, int c1, int c2) {
   const int ho = h >> 1, wo = wd >> 1, ct = c1 + c2;
   const size_t n1 = (size_t)n * ho * wo * (c1 >> 2), n2 = (size_t)n * h * wd * (c2 >> 2);
+  float s[4] = {0.f, 0.f, 0.f, 0.f};
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n1 + n2; e += (size_t)gridDim.x * blockDim.x) {
     if (e < n1) {
       const int q = e % (c1 >> 2);
       const size_t p = e / (c1 >> 2);
       const int xo = p % wo, yo = (p / wo) % ho, b = p / ((size_t)wo * ho);
-      float4 s = make_float4(0, 0, 0, 0);
+      float4 a = make_float4(0, 0, 0, 0);
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const int yy = 2 * yo + (k >> 1), xx = 2 * xo + (k & 1);
         const float4 v = *reinterpret_cast<const float4*>(dcat + ((size_t)(b * h + yy) * wd + xx) * ct + q * 4);
-        s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+        a.x += v.x; a.y += v.y; a.z += v.z; a.w += v.w;
       }
-      *reinterpret_cast<float4*>(dx1 + p * c1 + q * 4) = s;
+      mask4(a, *reinterpret_cast<const uint2*>(y1 + p * c1 + q * 4), reinterpret_cast<uint2*>(dp1 + p * c1 + q * 4),
+            s);
     } else {
       const size_t f = e - n1;
       const int q = f % (c2 >> 2);
@@ -619,51 +665,9 @@ __global__ void k_split_up(const float* __restrict__ dcat, float* __restrict__ d
           *reinterpret_cast<const float4*>(dcat + p * ct + c1 + q * 4);
     }
   }
+  block_row_sums(s, part, c1 >> 2);
 }
 
-// dP = bf16(dA * (Y > 0)) and the block's partial sums
-// part[block][c] = sum_px dA * (Y > 0) (fp32, before rounding) of the bias
-// gradient, reduced by k_sum_rows (atomics of every block on the same c
-// addresses would serialise).  8 channels per thread; blockDim is a
-// multiple of c / 8.
-__global__ __launch_bounds__(kThreads) void k_relu_mask(const float* __restrict__ da, const bf16_t* __restrict__ y,
-                                                        bf16_t* __restrict__ dp, float* __restrict__ part,
-                                                        size_t npx, int c) {
-  __shared__ float red[kThreads][9];
-  const int cg = c >> 3;
-  const int q = threadIdx.x % cg;
-  const int pstride = (gridDim.x * kThreads) / cg;
-  float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (size_t p = (size_t)(blockIdx.x * kThreads + threadIdx.x) / cg; p < npx; p += pstride) {
-    const size_t o = p * c + q * 8;
-    const uint4 yv = *reinterpret_cast<const uint4*>(y + o);
-    const uint32_t y4[4] = {yv.x, yv.y, yv.z, yv.w};
-    const float4 a0 = *reinterpret_cast<const float4*>(da + o);
-    const float4 a1 = *reinterpret_cast<const float4*>(da + o + 4);
-    const float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
-    uint32_t o4[4];
-#pragma unroll
-    for (int i = 0; i < 8; i += 2) {
-      const float g0 = bf2f(y4[i >> 1] & 0xFFFFu) > 0.f ? av[i] : 0.f;
-      const float g1 = bf2f(y4[i >> 1] >> 16) > 0.f ? av[i + 1] : 0.f;
-      s[i] += g0;
-      s[i + 1] += g1;
-      o4[i >> 1] = f2bf(g0) | (f2bf(g1) << 16);
-    }
-    *reinterpret_cast<uint4*>(dp + o) = make_uint4(o4[0], o4[1], o4[2], o4[3]);
-  }
-#pragma unroll
-  for (int i = 0; i < 8; ++i) red[threadIdx.x][i] = s[i];
-  __syncthreads();
-  if (threadIdx.x < cg) {
-    float t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int r = threadIdx.x; r < kThreads; r += cg)
-#pragma unroll
-      for (int i = 0; i < 8; ++i) t[i] += red[r][i];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) part[(size_t)blockIdx.x * c + q * 8 + i] = t[i];
-  }
-}
 
 // out[j] += sum_r part[r][j]: a block takes CB columns (consecutive
 // threads, coalesced), its 256 / CB row slices sum rows r = slice,
@@ -694,16 +698,24 @@ __global__ __launch_bounds__(256) void k_sum_rows(const float* __restrict__ part
 // head's gradients (block sums, then atomics).
 __global__ __launch_bounds__(kThreads) void k_head(const bf16_t* __restrict__ d0, const float* __restrict__ hw,
                                                    float* __restrict__ out, const float* __restrict__ tgt,
-                                                   float* __restrict__ dd0, float* __restrict__ ghw,
-                                                   double* __restrict__ loss, size_t npx, float scale) {
+                                                   bf16_t* __restrict__ dp, float* __restrict__ part,
+                                                   float* __restrict__ ghw, double* __restrict__ loss, size_t npx,
+                                                   float scale) {
+  // training (tgt != null): dz = 2 (z - t) / npx; the gradient dz * w of the
+  // last 3x3 layer's output goes through that layer's ReLU (d0 > 0) straight
+  // into its dP (bf16) and per-block sums of its bias gradient part[block][c]
+  // (fp32, before rounding); the head's own gradient and the loss are
+  // reduced per block and added atomically (33 + 1 values)
   __shared__ float sw[kBase + 1];
-  __shared__ float red[kThreads / 64][kBase + 1];
+  __shared__ float red[kThreads / 64][2 * kBase + 1];
   __shared__ double lred[kThreads / 64];
   if (threadIdx.x <= kBase) sw[threadIdx.x] = hw[threadIdx.x];
   __syncthreads();
-  float gw[kBase + 1];
+  float gw[kBase + 1], bs[kBase];
 #pragma unroll
   for (int c = 0; c <= kBase; ++c) gw[c] = 0.f;
+#pragma unroll
+  for (int c = 0; c < kBase; ++c) bs[c] = 0.f;
   double lsum = 0.0;
   for (size_t p = (size_t)blockIdx.x * kThreads + threadIdx.x; p < npx; p += (size_t)gridDim.x * kThreads) {
     float a[kBase];
@@ -723,10 +735,21 @@ __global__ __launch_bounds__(kThreads) void k_head(const bf16_t* __restrict__ d0
       const float diff = z - tgt[p];
       lsum += (double)diff * diff;
       const float dz = 2.f * diff * scale;
-      float4* dst = reinterpret_cast<float4*>(dd0 + p * kBase);
+      uint4* dst = reinterpret_cast<uint4*>(dp + p * kBase);
 #pragma unroll
-      for (int q = 0; q < kBase / 4; ++q)
-        dst[q] = make_float4(dz * sw[4 * q], dz * sw[4 * q + 1], dz * sw[4 * q + 2], dz * sw[4 * q + 3]);
+      for (int q = 0; q < kBase / 8; ++q) {
+        uint32_t o4[4];
+#pragma unroll
+        for (int i = 0; i < 8; i += 2) {
+          const int c = q * 8 + i;
+          const float g0 = a[c] > 0.f ? dz * sw[c] : 0.f;
+          const float g1 = a[c + 1] > 0.f ? dz * sw[c + 1] : 0.f;
+          bs[c] += g0;
+          bs[c + 1] += g1;
+          o4[i >> 1] = f2bf(g0) | (f2bf(g1) << 16);
+        }
+        dst[q] = make_uint4(o4[0], o4[1], o4[2], o4[3]);
+      }
 #pragma unroll
       for (int c = 0; c < kBase; ++c) gw[c] = fmaf(dz, a[c], gw[c]);
       gw[kBase] += dz;
@@ -735,8 +758,8 @@ __global__ __launch_bounds__(kThreads) void k_head(const bf16_t* __restrict__ d0
   if (!tgt) return;
   // wave sums, then block sums
 #pragma unroll
-  for (int c = 0; c <= kBase; ++c) {
-    float v = gw[c];
+  for (int c = 0; c < 2 * kBase + 1; ++c) {
+    float v = c <= kBase ? gw[c] : bs[c - kBase - 1];
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][c] = v;
   }
@@ -744,10 +767,13 @@ __global__ __launch_bounds__(kThreads) void k_head(const bf16_t* __restrict__ d0
   for (int off = 32; off > 0; off >>= 1) lv += __shfl_xor(lv, off);
   if ((threadIdx.x & 63) == 0) lred[threadIdx.x >> 6] = lv;
   __syncthreads();
-  if (threadIdx.x <= kBase) {
+  if (threadIdx.x < 2 * kBase + 1) {
     float v = 0.f;
     for (int wv = 0; wv < kThreads / 64; ++wv) v += red[wv][threadIdx.x];
-    unsafeAtomicAdd(ghw + threadIdx.x, v);
+    if (threadIdx.x <= kBase)
+      unsafeAtomicAdd(ghw + threadIdx.x, v);
+    else
+      part[(size_t)blockIdx.x * kBase + threadIdx.x - kBase - 1] = v;
   }
   if (threadIdx.x == 0) {
     double v = 0.0;
@@ -937,11 +963,13 @@ int dalloc(smili_unet* n, void** p, size_t bytes) {
   return 0;
 }
 
-int act_alloc(smili_unet* n, Act& a, int h, int w, int c) {
+// An activation; `grad`: also an fp32 gradient buffer (the pooled inputs
+// and the skip sources, whose gradients have two parts)
+int act_alloc(smili_unet* n, Act& a, int h, int w, int c, bool grad) {
   a.h = h; a.w = w; a.c = c;
   const size_t e = (size_t)n->B * h * w * c;
   int r = dalloc(n, (void**)&a.y, e * sizeof(bf16_t));
-  if (r) return r;
+  if (r || !grad) return r;
   return dalloc(n, (void**)&a.g, e * sizeof(float));
 }
 
@@ -1078,29 +1106,17 @@ void conv_wgrad(smili_unet* n, int nb, int li, const Act* x1, int up1, const Act
 constexpr int kMaskBlocks = SMILI_UNET_MASK_BLOCKS;   // rows of the partial-sum scratch
 constexpr int kInPxPerBlock = 1024;
 
-void relu_mask(smili_unet* n, int nb, const Act* y, const float* da, int li, bf16_t* dp) {
-  const size_t npx = (size_t)nb * y->h * y->w;
-  const int blocks = grid_for(npx * (y->c / 8), kThreads, kMaskBlocks);
-  hipLaunchKernelGGL(k_relu_mask, dim3(blocks), dim3(kThreads), 0, n->s, da, y->y, dp, n->part, npx, y->c);
-  sum_rows(n, n->part, blocks, y->c, n->g + n->L[li].boff);
-}
 
 // Backward through ReLU + conv li (input x1 [up] ++ x2, output y with
 // gradient da).  The input gradient goes to dx (fp32, [n][h][w][cin] at
 // the output's resolution) unless dx is null.
-// This layer's dP comes from dp_in (made by the fused epilogue of the layer
-// above, its bias gradient already summed) or is made here from the output
-// gradient da.  The input gradient goes to dx (fp32), or -- with fuse, the
-// input activation when this layer is its only consumer -- through fuse's
-// ReLU straight into the dP of layer li - 1 (in the other dP buffer) with
-// that layer's bias gradient; the new dP is returned.
-bf16_t* conv_bwd(smili_unet* n, int nb, int li, const Act* y, const float* da, bf16_t* dp_in, const Act* x1,
-                 int up1, const Act* x2, float* dx, const Act* fuse = nullptr) {
-  bf16_t* dp = dp_in;
-  if (!dp) {
-    dp = n->dp;
-    relu_mask(n, nb, y, da, li, dp);
-  }
+// Backward of 3x3 layer li (input x1 [up] ++ x2, output y) from its dP:
+// the weight gradient, then the input gradient -- to dx (fp32), or, with
+// fuse (the input activation, when this layer is its only consumer),
+// through fuse's ReLU straight into the dP of layer li - 1 (in the other dP
+// buffer) with that layer's bias gradient; that dP is returned.
+bf16_t* conv_bwd(smili_unet* n, int nb, int li, const Act* y, bf16_t* dp, const Act* x1, int up1, const Act* x2,
+                 float* dx, const Act* fuse = nullptr) {
   conv_wgrad(n, nb, li, x1, up1, x2, y->h, y->w, y->c, dp);
   Act dpa;
   dpa.y = dp;
@@ -1112,14 +1128,19 @@ bf16_t* conv_bwd(smili_unet* n, int nb, int li, const Act* y, const float* da, b
     sum_rows(n, n->part, ntiles, n->L[li].cin, n->g + n->L[li - 1].boff);
     return out;
   }
-  if (dx)
-    launch_conv(n, nb, &dpa, 0, nullptr, li, 1, nullptr, dx, y->h, y->w, n->L[li].cin, wfl_of(n, li), nullptr);
+  launch_conv(n, nb, &dpa, 0, nullptr, li, 1, nullptr, dx, y->h, y->w, n->L[li].cin, wfl_of(n, li), nullptr);
   return nullptr;
 }
 
-void split_up(smili_unet* n, int nb, const float* dcat, int h, int w, int c1, int c2, float* dx1, float* dx2) {
-  const size_t work = (size_t)nb * (h / 2) * (w / 2) * (c1 / 4) + (size_t)nb * h * w * (c2 / 4);
-  hipLaunchKernelGGL(k_split_up, dim3(grid_for(work)), dim3(kThreads), 0, n->s, dcat, dx1, dx2, nb, h, w, c1, c2);
+// Splits the input gradient dcat of decoder layer li + 1 (input [up(x1) ++
+// x2], h x w): x1's part becomes the dP of layer li (x1's producer) in
+// n->dp, with that layer's bias gradient; x2's part goes to x2->g.
+void split_up(smili_unet* n, int nb, int li, const float* dcat, int h, int w, const Act* x1, Act* x2) {
+  const size_t work = (size_t)nb * (h / 2) * (w / 2) * (x1->c / 4) + (size_t)nb * h * w * (x2->c / 4);
+  const int blocks = grid_for(work, kThreads, kMaskBlocks);
+  hipLaunchKernelGGL(k_split_up, dim3(blocks), dim3(kThreads), 0, n->s, dcat, x1->y, n->dp, n->part, x2->g, nb, h, w,
+                     x1->c, x2->c);
+  sum_rows(n, n->part, blocks, x1->c, n->g + n->L[li].boff);
 }
 
 void pool_fwd(smili_unet* n, int nb, const Act* x, Act* y) {
@@ -1127,9 +1148,15 @@ void pool_fwd(smili_unet* n, int nb, const Act* x, Act* y) {
   hipLaunchKernelGGL(k_pool, dim3(grid_for(work)), dim3(kThreads), 0, n->s, x->y, y->y, nb, x->h, x->w, x->c);
 }
 
-void pool_bwd_add(smili_unet* n, int nb, const float* dy, Act* x) {
+// dP of layer li, whose output x is pooled (gradient dy) and also feeds a
+// skip connection (gradient x->g), into n->dp, with that layer's bias
+// gradient.
+void pool_bwd(smili_unet* n, int nb, int li, const float* dy, const Act* x) {
   const size_t work = (size_t)nb * (x->h / 2) * (x->w / 2) * (x->c / 4);
-  hipLaunchKernelGGL(k_pool_bwd_add, dim3(grid_for(work)), dim3(kThreads), 0, n->s, dy, x->g, nb, x->h, x->w, x->c);
+  const int blocks = grid_for(work, kThreads, kMaskBlocks);
+  hipLaunchKernelGGL(k_pool_bwd_mask, dim3(blocks), dim3(kThreads), 0, n->s, dy, x->g, x->y, n->dp, n->part, nb,
+                     x->h, x->w, x->c);
+  sum_rows(n, n->part, blocks, x->c, n->g + n->L[li].boff);
 }
 
 void forward(smili_unet* n, int nb) {
@@ -1156,40 +1183,47 @@ void forward(smili_unet* n, int nb) {
 
 size_t head_off(smili_unet* n) { return n->L.back().boff + n->L.back().cout; }
 
+// Inference: out = max(z, 0).  Training: the loss, the head's gradient, and
+// the dP (in n->dp) and bias gradient of the last 3x3 layer.
 void head(smili_unet* n, int nb, bool train, double* loss) {
   const size_t npx = (size_t)nb * n->H * n->W;
   const float scale = 1.f / (float)npx;
-  hipLaunchKernelGGL(k_head, dim3(grid_for(npx, kThreads, 1024)), dim3(kThreads), 0, n->s, n->d0.y,
-                     n->p + head_off(n), train ? nullptr : n->out, train ? n->tgt : nullptr,
-                     train ? n->d0.g : nullptr, train ? n->g + head_off(n) : nullptr, loss, npx, scale);
+  const int blocks = grid_for(npx, kThreads, 1024);
+  hipLaunchKernelGGL(k_head, dim3(blocks), dim3(kThreads), 0, n->s, n->d0.y, n->p + head_off(n),
+                     train ? nullptr : n->out, train ? n->tgt : nullptr, train ? n->dp : nullptr,
+                     train ? n->part : nullptr, train ? n->g + head_off(n) : nullptr, loss, npx, scale);
+  if (train) sum_rows(n, n->part, blocks, kBase, n->g + n->L[13].boff);
 }
 
 void backward(smili_unet* n, int nb) {
+  // Every dP is made by the step that completes its gradient: the head
+  // (layer 13), the input gradient of the second convolution of a level
+  // (the only consumer of the first one's output: layers 12, 10, 8, 6, 4,
+  // 2, 0), the upsampling adjoint (layers 11, 9, 7) and the pooling adjoint
+  // (layers 5, 3, 1, whose outputs also feed skip connections).  Those of
+  // the first kind go to n->dp2, the others to n->dp.
   // decoder
-  // (the second convolution of each level is the only consumer of the
-  // first one's output: its input gradient goes straight to the first
-  // one's dP)
-  bf16_t* dp = conv_bwd(n, nb, 13, &n->d0, n->d0.g, nullptr, &n->d0a, 0, nullptr, nullptr, &n->d0a);
-  conv_bwd(n, nb, 12, &n->d0a, nullptr, dp, &n->d1, 1, &n->e0, n->dtmp);
-  split_up(n, nb, n->dtmp, n->d0a.h, n->d0a.w, n->d1.c, n->e0.c, n->d1.g, n->e0.g);
-  dp = conv_bwd(n, nb, 11, &n->d1, n->d1.g, nullptr, &n->d1a, 0, nullptr, nullptr, &n->d1a);
-  conv_bwd(n, nb, 10, &n->d1a, nullptr, dp, &n->d2, 1, &n->e1, n->dtmp);
-  split_up(n, nb, n->dtmp, n->d1a.h, n->d1a.w, n->d2.c, n->e1.c, n->d2.g, n->e1.g);
-  dp = conv_bwd(n, nb, 9, &n->d2, n->d2.g, nullptr, &n->d2a, 0, nullptr, nullptr, &n->d2a);
-  conv_bwd(n, nb, 8, &n->d2a, nullptr, dp, &n->e3, 1, &n->e2, n->dtmp);
-  split_up(n, nb, n->dtmp, n->d2a.h, n->d2a.w, n->e3.c, n->e2.c, n->e3.g, n->e2.g);
+  bf16_t* dp = conv_bwd(n, nb, 13, &n->d0, n->dp, &n->d0a, 0, nullptr, nullptr, &n->d0a);
+  conv_bwd(n, nb, 12, &n->d0a, dp, &n->d1, 1, &n->e0, n->dtmp);
+  split_up(n, nb, 11, n->dtmp, n->d0a.h, n->d0a.w, &n->d1, &n->e0);
+  dp = conv_bwd(n, nb, 11, &n->d1, n->dp, &n->d1a, 0, nullptr, nullptr, &n->d1a);
+  conv_bwd(n, nb, 10, &n->d1a, dp, &n->d2, 1, &n->e1, n->dtmp);
+  split_up(n, nb, 9, n->dtmp, n->d1a.h, n->d1a.w, &n->d2, &n->e1);
+  dp = conv_bwd(n, nb, 9, &n->d2, n->dp, &n->d2a, 0, nullptr, nullptr, &n->d2a);
+  conv_bwd(n, nb, 8, &n->d2a, dp, &n->e3, 1, &n->e2, n->dtmp);
+  split_up(n, nb, 7, n->dtmp, n->d2a.h, n->d2a.w, &n->e3, &n->e2);
   // encoder
-  dp = conv_bwd(n, nb, 7, &n->e3, n->e3.g, nullptr, &n->e3a, 0, nullptr, nullptr, &n->e3a);
-  conv_bwd(n, nb, 6, &n->e3a, nullptr, dp, &n->p2, 0, nullptr, n->p2.g);
-  pool_bwd_add(n, nb, n->p2.g, &n->e2);
-  dp = conv_bwd(n, nb, 5, &n->e2, n->e2.g, nullptr, &n->e2a, 0, nullptr, nullptr, &n->e2a);
-  conv_bwd(n, nb, 4, &n->e2a, nullptr, dp, &n->p1, 0, nullptr, n->p1.g);
-  pool_bwd_add(n, nb, n->p1.g, &n->e1);
-  dp = conv_bwd(n, nb, 3, &n->e1, n->e1.g, nullptr, &n->e1a, 0, nullptr, nullptr, &n->e1a);
-  conv_bwd(n, nb, 2, &n->e1a, nullptr, dp, &n->p0, 0, nullptr, n->p0.g);
-  pool_bwd_add(n, nb, n->p0.g, &n->e0);
+  dp = conv_bwd(n, nb, 7, &n->e3, n->dp, &n->e3a, 0, nullptr, nullptr, &n->e3a);
+  conv_bwd(n, nb, 6, &n->e3a, dp, &n->p2, 0, nullptr, n->p2.g);
+  pool_bwd(n, nb, 5, n->p2.g, &n->e2);
+  dp = conv_bwd(n, nb, 5, &n->e2, n->dp, &n->e2a, 0, nullptr, nullptr, &n->e2a);
+  conv_bwd(n, nb, 4, &n->e2a, dp, &n->p1, 0, nullptr, n->p1.g);
+  pool_bwd(n, nb, 3, n->p1.g, &n->e1);
+  dp = conv_bwd(n, nb, 3, &n->e1, n->dp, &n->e1a, 0, nullptr, nullptr, &n->e1a);
+  conv_bwd(n, nb, 2, &n->e1a, dp, &n->p0, 0, nullptr, n->p0.g);
+  pool_bwd(n, nb, 1, n->p0.g, &n->e0);
   // layer 1's input gradient is the first convolution's dP (and bias gradient)
-  dp = conv_bwd(n, nb, 1, &n->e0, n->e0.g, nullptr, &n->e0a, 0, nullptr, nullptr, &n->e0a);
+  dp = conv_bwd(n, nb, 1, &n->e0, n->dp, &n->e0a, 0, nullptr, nullptr, &n->e0a);
   // first convolution: weight gradient only
   const size_t npx = (size_t)nb * n->H * n->W;
   const int blocks = (int)((npx + kInPxPerBlock - 1) / kInPxPerBlock);
@@ -1320,12 +1354,14 @@ int smili_unet_create(int32_t batch, int32_t height, int32_t width, smili_unet**
     n->nwpart = std::max(n->nwpart, (size_t)pl.want * pl.row());
   }
   if (!r) r = dalloc(n, (void**)&n->wpart, n->nwpart * 4);
-  const struct { Act* a; int s, c; } acts[17] = {
-      {&n->e0a, 0, c0}, {&n->e0, 0, c0}, {&n->p0, 1, c0}, {&n->e1a, 1, c1}, {&n->e1, 1, c1},
-      {&n->p1, 2, c1},  {&n->e2a, 2, c2}, {&n->e2, 2, c2}, {&n->p2, 3, c2}, {&n->e3a, 3, c3},
-      {&n->e3, 3, c3},  {&n->d2a, 2, c2}, {&n->d2, 2, c2}, {&n->d1a, 1, c1}, {&n->d1, 1, c1},
-      {&n->d0a, 0, c0}, {&n->d0, 0, c0}};
-  for (int i = 0; i < 17 && !r; ++i) r = act_alloc(n, *acts[i].a, H >> acts[i].s, W >> acts[i].s, acts[i].c);
+  const struct { Act* a; int s, c; bool g; } acts[17] = {
+      {&n->e0a, 0, c0, false}, {&n->e0, 0, c0, true},   {&n->p0, 1, c0, true},   {&n->e1a, 1, c1, false},
+      {&n->e1, 1, c1, true},   {&n->p1, 2, c1, true},   {&n->e2a, 2, c2, false}, {&n->e2, 2, c2, true},
+      {&n->p2, 3, c2, true},   {&n->e3a, 3, c3, false}, {&n->e3, 3, c3, false},  {&n->d2a, 2, c2, false},
+      {&n->d2, 2, c2, false},  {&n->d1a, 1, c1, false}, {&n->d1, 1, c1, false},  {&n->d0a, 0, c0, false},
+      {&n->d0, 0, c0, false}};
+  for (int i = 0; i < 17 && !r; ++i)
+    r = act_alloc(n, *acts[i].a, H >> acts[i].s, W >> acts[i].s, acts[i].c, acts[i].g);
   if (!r) r = ensure_loss(n, 16);
   if (!r) {
     hipLaunchKernelGGL(k_fill, dim3(grid_for(n->np)), dim3(kThreads), 0, n->s, n->p, n->np, 0.f);
