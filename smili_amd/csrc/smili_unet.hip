@@ -439,6 +439,9 @@ __global__ __launch_bounds__(kThreads) void k_conv3x3_wgrad(WgradArgs a) {
 // ---------------------------------------------------------------------------
 // first convolution (2 -> 32 channels, fp32 input and weights)
 // ---------------------------------------------------------------------------
+#ifndef SMILI_UNET_CONVIN_PRELOAD
+#define SMILI_UNET_CONVIN_PRELOAD 1
+#endif
 __global__ __launch_bounds__(kThreads) void k_conv_in(const float* __restrict__ x, const float* __restrict__ w,
                                                       const float* __restrict__ bias, bf16_t* __restrict__ y,
                                                       int n, int h, int wd) {
@@ -458,17 +461,36 @@ __global__ __launch_bounds__(kThreads) void k_conv_in(const float* __restrict__ 
     float acc[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) acc[k] = sb[q * 8 + k];
+#if SMILI_UNET_CONVIN_PRELOAD
+    // the 9 input pixels are loaded together (one latency, not nine); the
+    // weight offset is made opaque so that the 144 weights are read from LDS
+    // per pixel rather than hoisted into registers for the whole loop
+    float2 vin[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      const int sy = yy + t / 3 - 1, sx = xx + t % 3 - 1;
+      const bool ok = sy >= 0 && sy < h && sx >= 0 && sx < wd;
+      vin[t] = ok ? *reinterpret_cast<const float2*>(x + ((size_t)(b * h + sy) * wd + sx) * 2) : make_float2(0.f, 0.f);
+    }
+    int wq = q * 8;
+    asm volatile("" : "+v"(wq));
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      const float2 v = vin[t];
+#else
+    const int wq = q * 8;
 #pragma unroll 1
     for (int t = 0; t < 9; ++t) {
       const int sy = yy + t / 3 - 1, sx = xx + t % 3 - 1;
       const bool ok = sy >= 0 && sy < h && sx >= 0 && sx < wd;
       const float2 v = ok ? *reinterpret_cast<const float2*>(x + ((size_t)(b * h + sy) * wd + sx) * 2)
                           : make_float2(0.f, 0.f);
+#endif
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
         const float xv = c ? v.y : v.x;
-        const float4 w0 = *reinterpret_cast<const float4*>(&sw[(2 * t + c) * kBase + q * 8]);
-        const float4 w1 = *reinterpret_cast<const float4*>(&sw[(2 * t + c) * kBase + q * 8 + 4]);
+        const float4 w0 = *reinterpret_cast<const float4*>(&sw[(2 * t + c) * kBase + wq]);
+        const float4 w1 = *reinterpret_cast<const float4*>(&sw[(2 * t + c) * kBase + wq + 4]);
         acc[0] = fmaf(w0.x, xv, acc[0]);
         acc[1] = fmaf(w0.y, xv, acc[1]);
         acc[2] = fmaf(w0.z, xv, acc[2]);
@@ -486,61 +508,58 @@ __global__ __launch_bounds__(kThreads) void k_conv_in(const float* __restrict__ 
   }
 }
 
-// dW[co][tap][ci] of the first convolution: a block stages the masked
-// output gradients (32 channels) and the 3x3 input neighbourhoods (18
-// values) of 256 pixels in LDS; thread t accumulates channel t / 8 against
-// (tap, ci) pairs j = t % 8, +8, +16 (< 18) and writes the block's sums to
-// part[block][576], reduced by k_sum_rows
-constexpr int kInPx = 256;
+// dW[co][tap][ci] of the first convolution over the block's pixels,
+// straight from global memory: thread t takes the 4 output channels
+// 4 ((t % 16) >> 1) and input channel t % 2 for all 9 taps (36 sums) over
+// pixels t / 16, + 16, ... of the block's range (the 16 threads of a pixel
+// share its loads through L1); the 16 pixel lanes of each (channels, input)
+// pair are reduced with two shuffles and over the waves in LDS, and the
+// block writes part[block][576], reduced by k_sum_rows.
 __global__ __launch_bounds__(kThreads) void k_conv_in_wgrad(const bf16_t* __restrict__ dp,
                                                             const float* __restrict__ x, float* __restrict__ part,
                                                             int n, int h, int wd, int px_per_block) {
-  __shared__ float sg[kInPx][kBase + 1];
-  __shared__ float sx[kInPx][19];
+  static_assert(kBase == 32 && kThreads % 64 == 0, "16 (channels, input) pairs per pixel");
+  __shared__ float red[kThreads / 64][16][37];
+  const int pair = threadIdx.x & 15, cg = pair >> 1, ci = pair & 1;
+  float acc[4][9];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int tp = 0; tp < 9; ++tp) acc[k][tp] = 0.f;
   const size_t npx = (size_t)n * h * wd;
-  const int co = threadIdx.x >> 3, jg = threadIdx.x & 7;
-  float acc[3] = {0.f, 0.f, 0.f};
   const size_t pb = (size_t)blockIdx.x * px_per_block, pe = min(npx, pb + px_per_block);
-  for (size_t p0 = pb; p0 < pe; p0 += kInPx) {
-    __syncthreads();
-    {
-      const size_t p = p0 + threadIdx.x;
-      if (p < pe) {
-        const uint4* src = reinterpret_cast<const uint4*>(dp + p * kBase);
+  for (size_t p = pb + (threadIdx.x >> 4); p < pe; p += kThreads / 16) {
+    const uint2 g2 = *reinterpret_cast<const uint2*>(dp + p * kBase + cg * 4);
+    const float g[4] = {bf2f(g2.x & 0xFFFFu), bf2f(g2.x >> 16), bf2f(g2.y & 0xFFFFu), bf2f(g2.y >> 16)};
+    const int xx = p % wd, yy = (p / wd) % h, b = p / ((size_t)wd * h);
 #pragma unroll
-        for (int q = 0; q < kBase / 8; ++q) {
-          const uint4 v = src[q];
-          const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+    for (int tp = 0; tp < 9; ++tp) {
+      const int sy = yy + tp / 3 - 1, sx = xx + tp % 3 - 1;
+      const bool ok = sy >= 0 && sy < h && sx >= 0 && sx < wd;
+      const float v = ok ? x[((size_t)(b * h + sy) * wd + sx) * 2 + ci] : 0.f;
 #pragma unroll
-          for (int i = 0; i < 8; ++i) sg[threadIdx.x][q * 8 + i] = bf2f((w4[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu);
-        }
-        const int xx = p % wd, yy = (p / wd) % h, b = p / ((size_t)wd * h);
-#pragma unroll
-        for (int tp = 0; tp < 9; ++tp) {
-          const int sy = yy + tp / 3 - 1, sx_ = xx + tp % 3 - 1;
-          const bool ok = sy >= 0 && sy < h && sx_ >= 0 && sx_ < wd;
-          const float2 v = ok ? *reinterpret_cast<const float2*>(x + ((size_t)(b * h + sy) * wd + sx_) * 2)
-                              : make_float2(0.f, 0.f);
-          sx[threadIdx.x][2 * tp] = v.x;
-          sx[threadIdx.x][2 * tp + 1] = v.y;
-        }
-      } else {
-        for (int i = 0; i < kBase; ++i) sg[threadIdx.x][i] = 0.f;
-        for (int i = 0; i < 18; ++i) sx[threadIdx.x][i] = 0.f;
-      }
-    }
-    __syncthreads();
-    for (int k = 0; k < kInPx; ++k) {
-      const float gv = sg[k][co];
-      acc[0] = fmaf(gv, sx[k][jg], acc[0]);
-      acc[1] = fmaf(gv, sx[k][jg + 8], acc[1]);
-      if (jg < 2) acc[2] = fmaf(gv, sx[k][jg + 16], acc[2]);
+      for (int k = 0; k < 4; ++k) acc[k][tp] = fmaf(g[k], v, acc[k][tp]);
     }
   }
-  float* row = part + (size_t)blockIdx.x * kBase * 18 + co * 18;
-  row[jg] = acc[0];
-  row[jg + 8] = acc[1];
-  if (jg < 2) row[jg + 16] = acc[2];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int tp = 0; tp < 9; ++tp) {
+      float v = acc[k][tp];
+      v += __shfl_xor(v, 16);
+      v += __shfl_xor(v, 32);
+      if (lane < 16) red[wv][pair][k * 9 + tp] = v;
+    }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kBase * 18; i += kThreads) {
+    // i = co * 18 + tap * 2 + ci
+    const int co = i / 18, j = i - co * 18, tp = j >> 1, c = j & 1;
+    const int pr = (co >> 2) * 2 + c, r = (co & 3) * 9 + tp;
+    float s = 0.f;
+    for (int w = 0; w < kThreads / 64; ++w) s += red[w][pr][r];
+    part[(size_t)blockIdx.x * kBase * 18 + i] = s;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -696,89 +715,105 @@ __global__ __launch_bounds__(256) void k_sum_rows(const float* __restrict__ part
 // Head: z = w . d0 + b, image max(z, 0).  With a target: loss partial
 // sums (double) of (z - t)^2, dz = 2 (z - t) / N, dd0 = dz w, and the
 // head's gradients (block sums, then atomics).
+// Head z = w . d0 + b.  Four threads per pixel, 8 channels each (the dot
+// product is finished with two lane shuffles).  Inference: out = max(z, 0).
+// Training (tgt != null): dz = 2 (z - t) / npx; the gradient dz * w of the
+// last 3x3 layer's output goes through that layer's ReLU (d0 > 0) straight
+// into its dP (bf16) and per-block sums of its bias gradient part[block][c]
+// (fp32, before rounding); the head's own gradient (33 values) and the loss
+// are reduced per block and added atomically.
 __global__ __launch_bounds__(kThreads) void k_head(const bf16_t* __restrict__ d0, const float* __restrict__ hw,
                                                    float* __restrict__ out, const float* __restrict__ tgt,
                                                    bf16_t* __restrict__ dp, float* __restrict__ part,
                                                    float* __restrict__ ghw, double* __restrict__ loss, size_t npx,
                                                    float scale) {
-  // training (tgt != null): dz = 2 (z - t) / npx; the gradient dz * w of the
-  // last 3x3 layer's output goes through that layer's ReLU (d0 > 0) straight
-  // into its dP (bf16) and per-block sums of its bias gradient part[block][c]
-  // (fp32, before rounding); the head's own gradient and the loss are
-  // reduced per block and added atomically (33 + 1 values)
-  __shared__ float sw[kBase + 1];
+  static_assert(kBase == 32, "four threads of 8 channels per pixel");
   __shared__ float red[kThreads / 64][2 * kBase + 1];
   __shared__ double lred[kThreads / 64];
-  if (threadIdx.x <= kBase) sw[threadIdx.x] = hw[threadIdx.x];
-  __syncthreads();
-  float gw[kBase + 1], bs[kBase];
+  const int g = threadIdx.x & 3, c0 = 8 * g;
+  float sw[8];
 #pragma unroll
-  for (int c = 0; c <= kBase; ++c) gw[c] = 0.f;
+  for (int i = 0; i < 8; ++i) sw[i] = hw[c0 + i];
+  const float hb = hw[kBase];
+  float gw[8], bs[8], gb = 0.f;
 #pragma unroll
-  for (int c = 0; c < kBase; ++c) bs[c] = 0.f;
+  for (int i = 0; i < 8; ++i) gw[i] = bs[i] = 0.f;
   double lsum = 0.0;
-  for (size_t p = (size_t)blockIdx.x * kThreads + threadIdx.x; p < npx; p += (size_t)gridDim.x * kThreads) {
-    float a[kBase];
-    const uint4* src = reinterpret_cast<const uint4*>(d0 + p * kBase);
+  const size_t work = npx * 4;
+  for (size_t e = (size_t)blockIdx.x * kThreads + threadIdx.x; e < work; e += (size_t)gridDim.x * kThreads) {
+    const size_t p = e >> 2;
+    const uint4 v = *reinterpret_cast<const uint4*>(d0 + p * kBase + c0);
+    const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+    float a[8];
 #pragma unroll
-    for (int q = 0; q < kBase / 8; ++q) {
-      const uint4 v = src[q];
-      const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+    for (int i = 0; i < 8; ++i) a[i] = bf2f((w4[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu);
+    float z = 0.f;
 #pragma unroll
-      for (int i = 0; i < 8; ++i) a[q * 8 + i] = bf2f((w4[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu);
-    }
-    float z = sw[kBase];
-#pragma unroll
-    for (int c = 0; c < kBase; ++c) z = fmaf(sw[c], a[c], z);
-    if (out) out[p] = fmaxf(z, 0.f);
+    for (int i = 0; i < 8; ++i) z = fmaf(sw[i], a[i], z);
+    z += __shfl_xor(z, 1);
+    z += __shfl_xor(z, 2);
+    z += hb;
+    if (out && g == 0) out[p] = fmaxf(z, 0.f);
     if (tgt) {
       const float diff = z - tgt[p];
-      lsum += (double)diff * diff;
-      const float dz = 2.f * diff * scale;
-      uint4* dst = reinterpret_cast<uint4*>(dp + p * kBase);
-#pragma unroll
-      for (int q = 0; q < kBase / 8; ++q) {
-        uint32_t o4[4];
-#pragma unroll
-        for (int i = 0; i < 8; i += 2) {
-          const int c = q * 8 + i;
-          const float g0 = a[c] > 0.f ? dz * sw[c] : 0.f;
-          const float g1 = a[c + 1] > 0.f ? dz * sw[c + 1] : 0.f;
-          bs[c] += g0;
-          bs[c + 1] += g1;
-          o4[i >> 1] = f2bf(g0) | (f2bf(g1) << 16);
-        }
-        dst[q] = make_uint4(o4[0], o4[1], o4[2], o4[3]);
+      if (g == 0) {
+        lsum += (double)diff * diff;
       }
+      const float dz = 2.f * diff * scale;
+      uint32_t o4[4];
 #pragma unroll
-      for (int c = 0; c < kBase; ++c) gw[c] = fmaf(dz, a[c], gw[c]);
-      gw[kBase] += dz;
+      for (int i = 0; i < 8; i += 2) {
+        const float g0 = a[i] > 0.f ? dz * sw[i] : 0.f;
+        const float g1 = a[i + 1] > 0.f ? dz * sw[i + 1] : 0.f;
+        bs[i] += g0;
+        bs[i + 1] += g1;
+        o4[i >> 1] = f2bf(g0) | (f2bf(g1) << 16);
+      }
+      *reinterpret_cast<uint4*>(dp + p * kBase + c0) = make_uint4(o4[0], o4[1], o4[2], o4[3]);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) gw[i] = fmaf(dz, a[i], gw[i]);
+      if (g == 0) gb += dz;
     }
   }
   if (!tgt) return;
-  // wave sums, then block sums
+  // sums over the lanes of one channel group (lane % 4), then over waves
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-  for (int c = 0; c < 2 * kBase + 1; ++c) {
-    float v = c <= kBase ? gw[c] : bs[c - kBase - 1];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][c] = v;
+  for (int i = 0; i < 8; ++i) {
+    float x = gw[i], y = bs[i];
+#pragma unroll
+    for (int off = 4; off < 64; off <<= 1) {
+      x += __shfl_xor(x, off);
+      y += __shfl_xor(y, off);
+    }
+    if (lane < 4) {
+      red[wv][c0 + i] = x;
+      red[wv][kBase + 1 + c0 + i] = y;
+    }
   }
+  float xb = gb;
   double lv = lsum;
-  for (int off = 32; off > 0; off >>= 1) lv += __shfl_xor(lv, off);
-  if ((threadIdx.x & 63) == 0) lred[threadIdx.x >> 6] = lv;
+  for (int off = 32; off > 0; off >>= 1) {
+    xb += __shfl_xor(xb, off);
+    lv += __shfl_xor(lv, off);
+  }
+  if (lane == 0) {
+    red[wv][kBase] = xb;
+    lred[wv] = lv;
+  }
   __syncthreads();
   if (threadIdx.x < 2 * kBase + 1) {
-    float v = 0.f;
-    for (int wv = 0; wv < kThreads / 64; ++wv) v += red[wv][threadIdx.x];
+    float x = 0.f;
+    for (int w = 0; w < kThreads / 64; ++w) x += red[w][threadIdx.x];
     if (threadIdx.x <= kBase)
-      unsafeAtomicAdd(ghw + threadIdx.x, v);
+      unsafeAtomicAdd(ghw + threadIdx.x, x);
     else
-      part[(size_t)blockIdx.x * kBase + threadIdx.x - kBase - 1] = v;
+      part[(size_t)blockIdx.x * kBase + threadIdx.x - kBase - 1] = x;
   }
   if (threadIdx.x == 0) {
-    double v = 0.0;
-    for (int wv = 0; wv < kThreads / 64; ++wv) v += lred[wv];
-    unsafeAtomicAdd(loss, v);
+    double x = 0.0;
+    for (int w = 0; w < kThreads / 64; ++w) x += lred[w];
+    unsafeAtomicAdd(loss, x);
   }
 }
 
@@ -1104,7 +1139,7 @@ void conv_wgrad(smili_unet* n, int nb, int li, const Act* x1, int up1, const Act
 #define SMILI_UNET_MASK_BLOCKS 1024
 #endif
 constexpr int kMaskBlocks = SMILI_UNET_MASK_BLOCKS;   // rows of the partial-sum scratch
-constexpr int kInPxPerBlock = 1024;
+constexpr int kInPxPerBlock = 512;
 
 
 // Backward through ReLU + conv li (input x1 [up] ++ x2, output y with
@@ -1188,7 +1223,7 @@ size_t head_off(smili_unet* n) { return n->L.back().boff + n->L.back().cout; }
 void head(smili_unet* n, int nb, bool train, double* loss) {
   const size_t npx = (size_t)nb * n->H * n->W;
   const float scale = 1.f / (float)npx;
-  const int blocks = grid_for(npx, kThreads, 1024);
+  const int blocks = grid_for(npx * 4, kThreads, 1024);
   hipLaunchKernelGGL(k_head, dim3(blocks), dim3(kThreads), 0, n->s, n->d0.y, n->p + head_off(n),
                      train ? nullptr : n->out, train ? n->tgt : nullptr, train ? n->dp : nullptr,
                      train ? n->part : nullptr, train ? n->g + head_off(n) : nullptr, loss, npx, scale);
