@@ -115,6 +115,9 @@ constexpr int kWStride = 9 * 32 + 8;   // bf16 per output channel in LDS: 592 by
 #ifndef SMILI_UNET_WGRAD_TW
 #define SMILI_UNET_WGRAD_TW 32
 #endif
+#ifndef SMILI_UNET_NCB4_MIN_GROUPS
+#define SMILI_UNET_NCB4_MIN_GROUPS 1536   // measured: 0 (always 64) 1.753 ms per step, 1536 1.711, all-32 1.723
+#endif
 #ifndef SMILI_UNET_CONV_MIN_BLOCKS
 #define SMILI_UNET_CONV_MIN_BLOCKS 2
 #endif
@@ -1046,7 +1049,11 @@ int launch_conv(smili_unet* n, int nb, const Act* x1, int up1, const Act* x2, in
   const int tw = (w % 32 == 0 && x1->c + (x2 ? x2->c : 0) > SMILI_UNET_TW16_MAXCIN) ? 32 : 16;
   a.tiles_x = w / tw;
   a.tiles_y = h / kTileH;
-  const int ncb = (cout % 64 == 0) ? 4 : 2;
+  // 64 output channels per work-group where that still gives enough
+  // work-groups to fill the CUs, else 32 (more, smaller work-groups: the
+  // deep, small levels)
+  const int tiles = nb * a.tiles_x * a.tiles_y;
+  const int ncb = (cout % 64 == 0 && (long long)tiles * (cout / 64) >= SMILI_UNET_NCB4_MIN_GROUPS) ? 4 : 2;
   a.ncob = cout / (ncb * 16);
   const dim3 grid(round8(nb * a.tiles_x * a.tiles_y * a.ncob));
   if (tw == 32 && ncb == 4)
@@ -1058,7 +1065,7 @@ int launch_conv(smili_unet* n, int nb, const Act* x1, int up1, const Act* x2, in
   else
     hipLaunchKernelGGL((k_conv3x3<16, 2>), grid, dim3(kThreads), 0, n->s, a);
   (void)li;
-  return nb * a.tiles_x * a.tiles_y;
+  return tiles;
 }
 
 // forward 3x3 conv of layer li: y = relu(conv(x1 [up] ++ x2) + b)
