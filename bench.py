@@ -310,12 +310,15 @@ def _unet_roofline(params, batch=32, reps=20):
     step_ms, fwd_ms = net.time_steps(10)
     step_flop, fwd_flop = net.flops()
     level = (0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0)
-    # kernel instantiations as the library picks them: 8 x 16 forward /
-    # input-gradient tiles, 8 x 32 weight-gradient tiles where the width allows
+    # kernel instantiations as the library picks them (smili_unet.hip
+    # launch_conv, wgrad_plan): 8 x 16 forward / input-gradient tiles with 64
+    # output channels per work-group when that gives at least 1536
+    # work-groups, else 32; 8 x 32 weight-gradient tiles where the width allows
     launches, kernels = [], {}
     for li in range(1, len(LAYERS)):
         name, cin, cout = LAYERS[li]
         w = 128 >> level[li]
+        tiles = batch * (w // 8) * (w // 16)
         for ps, pname in ((0, "forward"), (1, "input_grad"), (2, "weight_grad")):
             ms, fl = net.time_layer(li, ps, reps)
             launches.append((ms, fl, name, pname))
@@ -324,7 +327,7 @@ def _unet_roofline(params, batch=32, reps=20):
                 kern = f"k_conv3x3_wgrad<{4 if cout % 64 == 0 else 2}, {32 if w % 32 == 0 else 16}>"
             else:
                 co = cout if ps == 0 else cin
-                kern = f"k_conv3x3<16, {4 if co % 64 == 0 else 2}>"
+                kern = f"k_conv3x3<16, {4 if co % 64 == 0 and tiles * (co // 64) >= 1536 else 2}>"
             k = kernels.setdefault(kern, [0.0, 0.0, 0])
             k[0] += ms
             k[1] += fl
