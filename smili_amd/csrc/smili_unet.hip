@@ -715,9 +715,6 @@ __global__ __launch_bounds__(256) void k_sum_rows(const float* __restrict__ part
   }
 }
 
-// Head: z = w . d0 + b, image max(z, 0).  With a target: loss partial
-// sums (double) of (z - t)^2, dz = 2 (z - t) / N, dd0 = dz w, and the
-// head's gradients (block sums, then atomics).
 // Head z = w . d0 + b.  Four threads per pixel, 8 channels each (the dot
 // product is finished with two lane shuffles).  Inference: out = max(z, 0).
 // Training (tgt != null): dz = 2 (z - t) / npx; the gradient dz * w of the
