@@ -319,32 +319,51 @@ def _unet_roofline(params, batch=32, reps=20):
         name, cin, cout = LAYERS[li]
         w = 128 >> level[li]
         tiles = batch * (w // 8) * (w // 16)
+        hw = batch * w * w
+        # activation bytes each launch must move at least (bf16 activations,
+        # fp32 input gradients); decoder inputs [up(x1) ++ x2] read x1 at
+        # half resolution
+        c2 = cout if li in (8, 10, 12) else cin
+        x_bytes = 2 * (hw * c2 + hw // 4 * (cin - c2))
+        dp_bytes = 2 * hw * cout
+        min_bytes = {0: x_bytes + 2 * hw * cout, 1: dp_bytes + 4 * hw * cin, 2: dp_bytes + x_bytes}
         for ps, pname in ((0, "forward"), (1, "input_grad"), (2, "weight_grad")):
             ms, fl = net.time_layer(li, ps, reps)
-            launches.append((ms, fl, name, pname))
+            # the time a launch cannot beat: its FLOP at the BF16 MFMA peak or
+            # its bytes at the HBM peak, whichever is longer
+            bound_ms = max(fl / (MFMA_BF16_PEAK_TFLOPS * 1e9), min_bytes[ps] / (HBM_PEAK_GBS * 1e6))
+            launches.append((ms, fl, name, pname, bound_ms))
             # the kernel instantiation this launch runs (the library's choice)
             if ps == 2:
                 kern = f"k_conv3x3_wgrad<{4 if cout % 64 == 0 else 2}, {32 if w % 32 == 0 else 16}>"
             else:
                 co = cout if ps == 0 else cin
                 kern = f"k_conv3x3<16, {4 if co % 64 == 0 and tiles * (co // 64) >= 1536 else 2}>"
-            k = kernels.setdefault(kern, [0.0, 0.0, 0])
+            k = kernels.setdefault(kern, [0.0, 0.0, 0, 0.0])
             k[0] += ms
             k[1] += fl
             k[2] += 1
+            k[3] += bound_ms
     net.close()
     # dominant kernel: the largest share of a training step's 3x3 time
-    kern, (ms_tot, fl_tot, nl) = max(kernels.items(), key=lambda kv: kv[1][0])
+    kern, (ms_tot, fl_tot, nl, bound_tot) = max(kernels.items(), key=lambda kv: kv[1][0])
     achieved = fl_tot / (ms_tot * 1e-3) / 1e12
     return {"bound": "mfma", "achieved": round(achieved, 2), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None,
             "kernel": f"{kern} ({nl} launches of a training step, batch {batch}, 128^2)",
             "kernel_ms": round(ms_tot / nl, 4), "kernel_flop": fl_tot / nl,
             "kernel_share": {k: round(v[0] / sum(x[0] for x in kernels.values()), 3) for k, v in kernels.items()},
+            # fraction of the attainable: per launch max(FLOP / MFMA peak,
+            # activation bytes / HBM peak), summed over the kernel's launches,
+            # over their measured time (32-channel levels are HBM-bound:
+            # 9 Cin Cout / (Cin + Cout) = 144 FLOP/B < 2.5 PFLOP/s / 8 TB/s)
+            "attainable_frac": round(bound_tot / ms_tot, 4),
+            "hbm_bound_launches": sum(1 for m, f, n, p, b in launches
+                                      if b > f / (MFMA_BF16_PEAK_TFLOPS * 1e9)),
             "train_step_ms": round(step_ms, 3),
             "train_step_TFLOPs": round(step_flop / (step_ms * 1e-3) / 1e12, 1), "forward_ms": round(fwd_ms, 3),
             "forward_TFLOPs": round(fwd_flop / (fwd_ms * 1e-3) / 1e12, 1),
-            "launches": {f"{n}/{p}": round(f / (m * 1e-3) / 1e12, 1) for m, f, n, p in launches}}
+            "launches": {f"{n}/{p}": round(f / (m * 1e-3) / 1e12, 1) for m, f, n, p, b in launches}}
 
 
 def bench_warmstart(args):
