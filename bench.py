@@ -377,7 +377,13 @@ def bench_warmstart(args):
     base = 2_000_000 + 1000 * args.seed          # held out: training seeds are below 1e6
     for s in range(base - args.warmup, base):
         S.time_to_image(net, s)
-    rows = [S.time_to_image(net, s) for s in range(base, base + args.steps)]
+    rows = []
+    for s in range(base, base + args.steps):
+        rows.append(S.time_to_image(net, s))
+        r = rows[-1]
+        # progress on stderr (stdout carries the one JSON line)
+        print(f"[warmstart] seed {s}: plain {r['plain_s']:.3f} s, warm "
+              f"{r['warm_s'] if r['warm_s'] is None else round(r['warm_s'], 3)} s", file=sys.stderr, flush=True)
     sm = S.summary(rows)
     roof = _unet_roofline(net.params)
     cpu = None
