@@ -332,6 +332,11 @@ k_interp(const SubProb* __restrict__ sps, const typename Cx<R>::T* __restrict__ 
 // block and of neighbouring blocks overlap).  The grid is zero on entry
 // (k_t1_cols clears what it consumes).
 // ---------------------------------------------------------------------------
+#ifndef SMILI_SPREAD_BATCH
+#define SMILI_SPREAD_BATCH 4
+#endif
+constexpr int kSpreadBatch = SMILI_SPREAD_BATCH;
+
 template <typename R>
 __device__ void spread_block(int bid, const SpreadItem* __restrict__ items, const int* __restrict__ pt_cell,
                              const R* __restrict__ pt_ex, const R* __restrict__ pt_ey,
@@ -370,8 +375,7 @@ __device__ void spread_block(int bid, const SpreadItem* __restrict__ items, cons
     const int off = a * T + b;
     R ar = 0, ai = 0;
     int cur = cl[w][0];
-    for (int p = 0; p < np; ++p) {
-      const int c = cl[w][p];
+    auto step = [&](int c, R wt, V v) {
       if (c != cur) {
         atomicAdd(&tile[cur + off].x, ar);
         atomicAdd(&tile[cur + off].y, ai);
@@ -379,11 +383,26 @@ __device__ void spread_block(int bid, const SpreadItem* __restrict__ items, cons
         ai = 0;
         cur = c;
       }
-      const R wt = wxs[w][b][p] * wys[w][a][p];
-      const V v = cs[w][p];
       ar = fma(wt, v.x, ar);
       ai = fma(wt, v.y, ai);
+    };
+    // kSpreadBatch points' LDS values are read together, then walked in
+    // order (the walk is otherwise one LDS latency per point)
+    int p = 0;
+    for (; p + kSpreadBatch <= np; p += kSpreadBatch) {
+      int c[kSpreadBatch];
+      R wt[kSpreadBatch];
+      V v[kSpreadBatch];
+#pragma unroll
+      for (int k = 0; k < kSpreadBatch; ++k) {
+        c[k] = cl[w][p + k];
+        wt[k] = wxs[w][b][p + k] * wys[w][a][p + k];
+        v[k] = cs[w][p + k];
+      }
+#pragma unroll
+      for (int k = 0; k < kSpreadBatch; ++k) step(c[k], wt[k], v[k]);
     }
+    for (; p < np; ++p) step(cl[w][p], wxs[w][b][p] * wys[w][a][p], cs[w][p]);
     atomicAdd(&tile[cur + off].x, ar);
     atomicAdd(&tile[cur + off].y, ai);
   }
