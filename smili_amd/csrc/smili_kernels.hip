@@ -519,6 +519,26 @@ k_chisq_reg(int nchunks, const Chunk* __restrict__ chunks, const int* __restrict
 // fixed-order sums of one problem's partials -> its 15 terms
 // ---------------------------------------------------------------------------
 
+// tfd, cen and sm costs of one unit (image.f90:111-135), added to tfd,
+// cen, smc.  Not inlined: the moments path is rare, and inlined its
+// registers would count against the spreading work that shares a launch
+// with the final sums.
+__device__ __noinline__ void unit_moment_costs(const UnitDev& U, const UnitMoments& M, double& tfd, double& cen,
+                                               double& smc) {
+  if (U.tfd_l > 0) tfd += U.reg_scale * U.tfd_l * (M.S - U.tfd_tgtfd) * (M.S - U.tfd_tgtfd);
+  if (U.cen_l > 0) {
+    const double sIp = kZeroEps + M.P;
+    const double sx = M.Px / sIp, sy = M.Py / sIp;
+    cen += U.reg_scale * U.cen_l * sqrt(sx * sx + sy * sy + kZeroEps);
+  }
+  if (U.sm_l > 0) {
+    const SmState S = sm_state(U, M);
+    const double c = (S.Sg[0] - S.Sp[0]) * (S.Sg[0] - S.Sp[0]) + (S.Sg[1] - S.Sp[1]) * (S.Sg[1] - S.Sp[1]) +
+                     2.0 * (S.Sg[2] - S.Sp[2]) * (S.Sg[2] - S.Sp[2]);
+    smc += U.reg_scale * U.sm_l * c;
+  }
+}
+
 __device__ void finalize_problem(const ProbDev& P, const UnitDev* __restrict__ units,
                                  const RegItem* __restrict__ reg_items, const double* cpart,
                                  const double* rpart, const UnitMoments* __restrict__ mom,
@@ -535,23 +555,8 @@ __device__ void finalize_problem(const ProbDev& P, const UnitDev* __restrict__ u
     v[13] += rpart[(long long)i * kRegTerms + 6];
   }
   // tfd / cen / sm costs per unit
-  for (int u = P.unit_begin + threadIdx.x; u < P.unit_end; u += blockDim.x) {
-    const UnitDev U = units[u];
-    if (!U.need_moments) continue;
-    const UnitMoments M = mom[u];
-    if (U.tfd_l > 0) v[9] += U.reg_scale * U.tfd_l * (M.S - U.tfd_tgtfd) * (M.S - U.tfd_tgtfd);
-    if (U.cen_l > 0) {
-      const double sIp = kZeroEps + M.P;
-      const double sx = M.Px / sIp, sy = M.Py / sIp;
-      v[10] += U.reg_scale * U.cen_l * sqrt(sx * sx + sy * sy + kZeroEps);
-    }
-    if (U.sm_l > 0) {
-      const SmState S = sm_state(U, M);
-      const double c = (S.Sg[0] - S.Sp[0]) * (S.Sg[0] - S.Sp[0]) + (S.Sg[1] - S.Sp[1]) * (S.Sg[1] - S.Sp[1]) +
-                       2.0 * (S.Sg[2] - S.Sp[2]) * (S.Sg[2] - S.Sp[2]);
-      v[11] += U.reg_scale * U.sm_l * c;
-    }
-  }
+  for (int u = P.unit_begin + threadIdx.x; u < P.unit_end; u += blockDim.x)
+    if (units[u].need_moments) unit_moment_costs(units[u], mom[u], v[9], v[10], v[11]);
   block_sum<14>(v, red);
   if (threadIdx.x == 0) {
     double* T = terms + P.terms_off;
