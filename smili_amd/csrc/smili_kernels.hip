@@ -68,7 +68,20 @@ __device__ __forceinline__ int closure_item(int j, int q) {
   return SMILI_CHISQ_INTERLEAVE ? (j & 63) * q + (j >> 6) : j;
 }
 
-template <typename VT>
+// The FP64 transcendental functions of the chi-square terms, out of line
+// for the LEAN kernels (below).
+__device__ __noinline__ double arg_nl(double y, double x) { return atan2(y, x); }
+__device__ __noinline__ double log_nl(double x) { return log(x); }
+__device__ __noinline__ void sincos_nl(double r, double& s, double& c) { sincos(r, &s, &c); }
+
+// LEAN: the rare or heavy math (FP64 transcendentals here, the tfd / cen /
+// sm gradients of the regularizers) is called out of line, so the kernel
+// needs about half the registers and a CU holds twice the work-groups.
+// Measured: faster for the standalone chi-square and regularizer launches
+// of large plans (movie 0.258 -> 0.252 ms); slower inside the fused
+// chi-square + regularizer launch of a single problem (C2 0.1045 ->
+// 0.1055 ms), which keeps the inline code.
+template <typename VT, bool LEAN>
 __device__ void chisq_block(int bid, const Chunk* __restrict__ chunks, const int* __restrict__ slots,
                             const FcvItem* __restrict__ fcv, const AmpItem* __restrict__ amp,
                             const CpItem* __restrict__ cp, const CaItem* __restrict__ ca,
@@ -93,9 +106,9 @@ __device__ void chisq_block(int bid, const Chunk* __restrict__ chunks, const int
       const double mod = sqrt(a2);
       Al[s] = make_double2(mod, 1.0 / mod);
     } else if (ch.type == DT_CP) {
-      Al[s] = make_double2(atan2(m.y, m.x), 1.0 / a2);
+      Al[s] = make_double2(LEAN ? arg_nl(m.y, m.x) : atan2(m.y, m.x), 1.0 / a2);
     } else if (ch.type == DT_CA) {
-      Al[s] = make_double2(log(a2), 1.0 / a2);
+      Al[s] = make_double2(LEAN ? log_nl(a2) : log(a2), 1.0 / a2);
     }
   }
   __syncthreads();
@@ -142,7 +155,10 @@ __device__ void chisq_block(int bid, const Chunk* __restrict__ chunks, const int
       }
       const double r = d.cp - model;
       double sr, cr;
-      sincos(r, &sr, &cr);
+      if (LEAN)
+        sincos_nl(r, sr, cr);
+      else
+        sincos(r, &sr, &cr);
       acc += 2.0 * (1.0 - cr) * d.w;
       const double f = -2.0 * sr * d.w;
 #pragma unroll
@@ -196,7 +212,7 @@ k_chisq(const Chunk* __restrict__ chunks, const int* __restrict__ slots,
         const CpItem* __restrict__ cp, const CaItem* __restrict__ ca,
         const VT* __restrict__ V, double2* __restrict__ Vres,
         double* __restrict__ partial) {
-  chisq_block<VT>(blockIdx.x, chunks, slots, fcv, amp, cp, ca, V, Vres, partial);
+  chisq_block<VT, true>(blockIdx.x, chunks, slots, fcv, amp, cp, ca, V, Vres, partial);
 }
 
 // ---------------------------------------------------------------------------
@@ -365,9 +381,37 @@ struct RegArgs {
   double* rpart;
 };
 
-__device__ void reg_block(int bid, const RegArgs& A) {
-  const RegItem* __restrict__ items = A.items;
-  const UnitDev* __restrict__ units = A.units;
+// Gradient of the tfd, cen and sm terms at one pixel (image.f90:111-135,
+// 760-1138) from the unit's moments, for the LEAN regularizer kernel (the
+// per-unit quantities are recomputed per pixel; these terms are rare).
+__device__ __noinline__ double moment_grad(const UnitDev& U, const UnitMoments& M, int ix, int iy, double I) {
+  double g = 0.0;
+  if (U.tfd_l > 0) g += U.tfd_l * 2.0 * (M.S - U.tfd_tgtfd);
+  if (U.cen_l > 0) {
+    const double cen_sIp = kZeroEps + M.P;
+    const double cen_sx = M.Px / cen_sIp, cen_sy = M.Py / cen_sIp;
+    const double cen_cost = sqrt(cen_sx * cen_sx + cen_sy * cen_sy + kZeroEps);
+    const bool alpha1 = fabs(U.cen_alpha - 1.0) < kZeroEps;
+    const double dix = (ix + 1) - U.nxref, diy = (iy + 1) - U.nyref;
+    const double sa = smabs(I);
+    const double gIp = alpha1 ? I / sa : I / sa * U.cen_alpha * pow(sa, U.cen_alpha - 1.0);
+    g += U.cen_l * (cen_sx * (dix - cen_sx) + cen_sy * (diy - cen_sy)) / cen_sIp / cen_cost * gIp;
+  }
+  if (U.sm_l > 0) {
+    const SmState S = sm_state(U, M);
+    const double xi = ix + 1, yi = iy + 1;
+    const double xg = (xi - S.xc) / S.Isum, yg = (yi - S.yc) / S.Isum;
+    const double g1 = (((xi - S.xc) * (xi - S.xc) - 2.0 * xg * S.mom[0]) - S.Sg[0]) / S.Isum;
+    const double g2 = (((yi - S.yc) * (yi - S.yc) - 2.0 * yg * S.mom[1]) - S.Sg[1]) / S.Isum;
+    const double g3 = ((xi - S.xc) * (yi - S.yc) - xg * S.mom[1] - yg * S.mom[0] - S.Sg[2]) / S.Isum;
+    g += U.sm_l * (2.0 * (S.Sg[0] - S.Sp[0]) * g1 + 2.0 * (S.Sg[1] - S.Sp[1]) * g2 +
+                   4.0 * (S.Sg[2] - S.Sp[2]) * g3);
+  }
+  return g;
+}
+
+template <bool LEAN>
+__device__ __forceinline__ void reg_block_unit(int bid, const RegArgs& A, const RegItem& it, const UnitDev& U) {
   const int* __restrict__ pix = A.pix;
   const double* __restrict__ x = A.x;
   const double* __restrict__ I2dbuf = A.I2dbuf;
@@ -377,16 +421,16 @@ __device__ void reg_block(int bid, const RegArgs& A) {
   double* __restrict__ grad = A.grad;
   double* __restrict__ rpart = A.rpart;
   __shared__ double red[kRegTerms * (kChisqThreadsMax / 64)];
-  const RegItem it = items[bid];
-  const UnitDev U = units[it.unit];
   const double* I2d = U.identity ? (x + U.x_off) : (I2dbuf + U.img_off);
+  // (LEAN: the moments' gradient is called per pixel instead)
+  const bool moments = !LEAN && U.need_moments;
   double v[kRegTerms];
 #pragma unroll
   for (int k = 0; k < kRegTerms; ++k) v[k] = 0.0;
   // global quantities of tfd / cen / sm
   double tfd_g = 0.0, cen_sx = 0.0, cen_sy = 0.0, cen_sIp = 1.0, cen_cost = 1.0;
   SmState S;
-  if (U.need_moments) {
+  if (moments) {
     const UnitMoments M = mom[it.unit];
     if (U.tfd_l > 0) tfd_g = U.tfd_l * 2.0 * (M.S - U.tfd_tgtfd);
     if (U.cen_l > 0) {
@@ -415,8 +459,9 @@ __device__ void reg_block(int bid, const RegArgs& A) {
     const int ix = p % U.nx, iy = p / U.nx;
     const double I = x[U.x_off + k];
     double g = 0.0;
-    if (U.tfd_l > 0) g += tfd_g;
-    if (U.cen_l > 0) {
+    if (LEAN && U.need_moments) g += moment_grad(U, mom[it.unit], ix, iy, I);
+    if (moments && U.tfd_l > 0) g += tfd_g;
+    if (moments && U.cen_l > 0) {
       const double dix = (ix + 1) - U.nxref, diy = (iy + 1) - U.nyref;
       const double sa = smabs(I);
       const double gIp = alpha1 ? I / sa : I / sa * U.cen_alpha * pow(sa, U.cen_alpha - 1.0);
@@ -456,7 +501,7 @@ __device__ void reg_block(int bid, const RegArgs& A) {
       v[2] += w * (dx * dx + dy * dy);
       g += w * tsv_grade(I2d, U.nx, U.ny, ix, iy);
     }
-    if (U.sm_l > 0) {
+    if (moments && U.sm_l > 0) {
       const double xi = ix + 1, yi = iy + 1;
       const double xg = (xi - S.xc) / S.Isum, yg = (yi - S.yc) / S.Isum;
       const double g1 = (((xi - S.xc) * (xi - S.xc) - 2.0 * xg * S.mom[0]) - S.Sg[0]) / S.Isum;
@@ -499,7 +544,20 @@ __device__ void reg_block(int bid, const RegArgs& A) {
     for (int k = 0; k < kRegTerms; ++k) rpart[(long long)bid * kRegTerms + k] = v[k];
 }
 
-__global__ void __launch_bounds__(kThreads) k_reg(RegArgs A) { reg_block(blockIdx.x, A); }
+// The unit's descriptor is read in place (LEAN) or copied to registers
+// first (the fused launch, measured faster so).
+template <bool LEAN>
+__device__ void reg_block(int bid, const RegArgs& A) {
+  const RegItem it = A.items[bid];
+  if (LEAN) {
+    reg_block_unit<true>(bid, A, it, A.units[it.unit]);
+  } else {
+    const UnitDev U = A.units[it.unit];
+    reg_block_unit<false>(bid, A, it, U);
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_reg(RegArgs A) { reg_block<true>(blockIdx.x, A); }
 
 // The chi-square chunks and the regularizer items in one launch: the two
 // are independent (V vs x), so the regularizer work-groups fill the CUs the
@@ -511,8 +569,8 @@ k_chisq_reg(int nchunks, const Chunk* __restrict__ chunks, const int* __restrict
             const CpItem* __restrict__ cp, const CaItem* __restrict__ ca,
             const VT* __restrict__ V, double2* __restrict__ Vres,
             double* __restrict__ partial, RegArgs A) {
-  if ((int)blockIdx.x < nchunks) chisq_block<VT>(blockIdx.x, chunks, slots, fcv, amp, cp, ca, V, Vres, partial);
-  else reg_block(blockIdx.x - nchunks, A);
+  if ((int)blockIdx.x < nchunks) chisq_block<VT, false>(blockIdx.x, chunks, slots, fcv, amp, cp, ca, V, Vres, partial);
+  else reg_block<false>(blockIdx.x - nchunks, A);
 }
 
 // ---------------------------------------------------------------------------
