@@ -36,6 +36,10 @@ struct Error : std::runtime_error {
                   std::string(#expr) + ": " + hipGetErrorString(_e));                    \
   } while (0)
 
+#ifndef SMILI_FFT_SCALE_ITEMS
+#define SMILI_FFT_SCALE_ITEMS 1
+#endif
+
 static void require(bool ok, const std::string& msg) {
   if (!ok) throw Error(SMILI_EINVAL, msg);
 }
@@ -682,6 +686,7 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   // FFT work-groups: about 512 points per row group (one 64-thread wave of
   // radix-8 butterflies), 64 threads per column of a column group
   rows_per_block = std::max(1, std::min(8, 512 / std::max(max_nf1, 1)));
+  const bool split_rows = SMILI_FFT_SCALE_ITEMS;
   // FFT radix and work-group sizes: one thread per butterfly of a stage
   // (n / maxr per transform), rounded up to whole waves, at most 256
   fft_radix = tune.fft_radix > 0 ? tune.fft_radix : 4;
@@ -707,12 +712,20 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
       return f;
     };
     if (U.m == 0) continue;     // a frame without data: no transforms, zero chi-square gradient
-    for (int s = 0; s < U.ny; s += rows_per_block) {
-      const FftItem r = fft_item(s, std::min(rows_per_block, U.ny - s));
+    // a grid smaller than the plan's largest takes proportionally more rows
+    // / columns per work-group: the same points per work-group, so the
+    // threads sized for the largest grid stay busy and the LDS sized for it
+    // still holds them (rows: rc nf1 <= rows_per_block max_nf1; columns:
+    // checked with the +1 padding of each column)
+    const int rc = split_rows ? std::max(1, rows_per_block * (max_nf1 / U.nf1)) : rows_per_block;
+    int cc = split_rows ? std::max(1, col_group * (max_nf2 / U.nf2)) : col_group;
+    while (cc > col_group && 2ll * cc * (U.nf2 + 1) + U.nf2 > 2ll * col_group * (max_nf2 + 1) + max_nf2) --cc;
+    for (int s = 0; s < U.ny; s += rc) {
+      const FftItem r = fft_item(s, std::min(rc, U.ny - s));
       h_t2rows.push_back(r);
       h_t1rows.push_back(r);
     }
-    for (int s = 0; s < U.nf1; s += col_group) h_cols.push_back(fft_item(s, std::min(col_group, U.nf1 - s)));
+    for (int s = 0; s < U.nf1; s += cc) h_cols.push_back(fft_item(s, std::min(cc, U.nf1 - s)));
   }
   unit_reg_begin[units.size()] = (int)h_regs.size();
   for (size_t pi = 0; pi < ps.size(); ++pi) {
