@@ -110,6 +110,29 @@ def time_to_image(net, seed, cfg=D.CFG, rel=1e-6):
                 net_s=w["net_s"], dirty_s=w["dirty_s"], fit_s=w["fit_s"])
 
 
+GAPS = (10.0, 1.0, 1e-1, 1e-2, 1e-3, 1e-4, 1e-5, 1e-6)
+
+
+def gap_curve(net, seed, cfg=D.CFG, gaps=GAPS):
+    """Iterations at which the plain solve (from the flat prior) and the
+    warm-started solve first reach objective f* (1 + gap), f* the plain
+    solve's final objective: {"plain": [...], "warm": [...]} per gap (None:
+    not reached)."""
+    from ..cost import CostProblem
+    from ..imaging import lbfgs
+    case = D.make_case(seed, cfg)
+    prep = D.prepare_case(case, cfg)
+    with CostProblem(**prep.kwargs) as prob:
+        _, ip = lbfgs.solve(prob, prep.x0, niter=cfg["niter"], m=cfg["m"], verbose=False)
+    fstar = ip.cost
+    _, w = warm_solve(net, case, cfg, stop_cost=fstar * (1.0 + min(gaps)))
+
+    def first(hist, g):
+        return next((k + 1 for k, c in enumerate(hist) if c <= fstar * (1.0 + g)), None)
+    return dict(seed=int(seed), plain=[first(ip.history, g) for g in gaps],
+                warm=[first(w["info"].history, g) for g in gaps])
+
+
 def summary(rows):
     ok = [r for r in rows if r["reached"]]
     it_ratio = [r["plain_iters"] / r["warm_iters"] for r in ok]
@@ -133,9 +156,20 @@ def main(argv=None):
     ap.add_argument("--weights", required=True)
     ap.add_argument("--seeds", default="1000000:1000020")
     ap.add_argument("--out", default=None)
+    ap.add_argument("--gap-curve", action="store_true",
+                    help="iterations to each objective gap (plain vs warm) instead of the time-to-image rows")
     a = ap.parse_args(argv)
     net, _ = UNet.from_file(a.weights, batch=1)
     lo, hi = (int(s) for s in a.seeds.split(":"))
+    if a.gap_curve:
+        curves = []
+        for s in range(lo, hi):
+            curves.append(gap_curve(net, s))
+            print(json.dumps(curves[-1]), flush=True)
+        med = {k: [float(np.nanmedian([np.nan if c[k][i] is None else c[k][i] for c in curves]))
+                   for i in range(len(GAPS))] for k in ("plain", "warm")}
+        print(json.dumps({"gaps": list(GAPS), "plain_median": med["plain"], "warm_median": med["warm"]}), flush=True)
+        return 0
     rows = []
     for s in range(lo, hi):
         r = time_to_image(net, s)

@@ -50,3 +50,14 @@ def test_warm_start_reaches_plain_objective_in_fewer_iterations(net):
     assert all(r["warm_start"] == "proposal" for r in rows)
     assert max(r["nrmse"] for r in rows) <= 5e-3
     assert sum(r["warm_iters"] for r in rows) < sum(r["plain_iters"] for r in rows)
+
+
+def test_gap_curves(net):
+    """Iterations to each objective gap: both solves reach every gap, in
+    order, and the warm start reaches the loose gaps no later."""
+    from smili_amd.warmstart.solve import GAPS, gap_curve
+    c = gap_curve(net, SEEDS[0])
+    print(c)
+    for k in ("plain", "warm"):
+        assert all(i is not None for i in c[k]) and c[k] == sorted(c[k])
+    assert c["warm"][GAPS.index(1.0)] <= c["plain"][GAPS.index(1.0)]
