@@ -40,6 +40,10 @@ struct Error : std::runtime_error {
 #define SMILI_FFT_SCALE_ITEMS 1
 #endif
 
+#ifndef SMILI_CHISQ_DIRECT
+#define SMILI_CHISQ_DIRECT 1
+#endif
+
 static void require(bool ok, const std::string& msg) {
   if (!ok) throw Error(SMILI_EINVAL, msg);
 }
@@ -608,8 +612,10 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     std::vector<int> cur_slots;
     auto close_chunk = [&](int type, long long start, long long count) {
       if (count <= 0) return;
+      const int direct = SMILI_CHISQ_DIRECT && (type == DT_FCV || type == DT_AMP) &&
+                         (long long)cur_slots.size() == count;
       h_chunks.push_back(Chunk{type, (int)pi, (int)start, (int)count, (int)h_slots.size(),
-                               (int)cur_slots.size(), 0, 0});
+                               (int)cur_slots.size(), direct, 0});
       for (int g : cur_slots) {
         h_slots.push_back(g);
         slot_of[g] = -1;

@@ -60,7 +60,9 @@ struct CaItem  { double ca, w; unsigned short slot[4]; };
 
 enum { DT_FCV = 0, DT_AMP = 1, DT_CP = 2, DT_CA = 3 };
 
-struct Chunk { int type, prob, start, count, slot_off, nslots, pad0, pad1; };
+// direct: every datum of the chunk has its own uv point (amplitudes and
+// full visibilities at distinct points), so it skips the LDS staging
+struct Chunk { int type, prob, start, count, slot_off, nslots, direct, pad1; };
 // FFT work item: `count` rows or columns starting at `first` of one unit,
 // with the unit's geometry embedded so a work-group needs one descriptor load
 struct FftItem {

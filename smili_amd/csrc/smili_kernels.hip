@@ -93,6 +93,41 @@ __device__ void chisq_block(int bid, const Chunk* __restrict__ chunks, const int
   __shared__ double red[kChisqThreadsMax / 64];
   const Chunk ch = chunks[bid];
   const int* sl = slots + ch.slot_off;
+  if (ch.direct) {
+    // one datum per uv point: no staging, the residual goes straight to
+    // the point (the same arithmetic as the staged path below)
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < ch.count; i += blockDim.x) {
+      if (ch.type == DT_FCV) {
+        const FcvItem d = fcv[ch.start + i];
+        const int j = sl[d.slot & 0x7fff];
+        const VT mv = V[j];
+        const double sg = (d.slot & kSlotNeg) ? -1.0 : 1.0;
+        const double rr = d.vr - (double)mv.x;
+        const double ri = d.vi - sg * (double)mv.y;
+        acc += (rr * rr + ri * ri) * d.w;
+        const double f = -2.0 * d.w;
+        unsafeAtomicAdd(&Vres[j].x, f * rr);
+        unsafeAtomicAdd(&Vres[j].y, f * ri * sg);
+      } else {
+        const AmpItem d = amp[ch.start + i];
+        const int j = sl[d.slot & 0x7fff];
+        const VT mv = V[j];
+        const double mx = (double)mv.x, my = (double)mv.y;
+        const double mod = sqrt(mx * mx + my * my);
+        const double r = d.amp - mod;
+        acc += r * r * d.w;
+        const double f = -2.0 * r * d.w * (1.0 / mod);
+        unsafeAtomicAdd(&Vres[j].x, f * mx);
+        unsafeAtomicAdd(&Vres[j].y, f * my);
+      }
+    }
+    double v[1] = {acc};
+    __shared__ double dred[kChisqThreadsMax / 64];
+    block_sum<1, kChisqThreadsMax / 64>(v, dred);
+    if (threadIdx.x == 0) partial[bid] = v[0];
+    return;
+  }
   // stage the chunk's model visibilities and, once per point rather than
   // once per reference: amp |V| and 1/|V|; closure phase arg V and 1/|V|^2;
   // closure amplitude log|V|^2 and 1/|V|^2
