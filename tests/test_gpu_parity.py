@@ -150,6 +150,21 @@ def test_graph_and_eager_agree_and_repeat(gpu_lib):
     assert rel_l2(gr1b, gr1) < 1e-6 and rel_l2(gr2, gr1) < 1e-6
 
 
+@pytest.mark.parametrize("precision", [32, 64])
+def test_distinct_point_chunks_match_oracle(gpu_lib, precision):
+    """Visibilities and amplitudes that each sit at their own uv point take
+    the chi-square kernel's direct path (no LDS staging of the points);
+    both paths (the closures here repeat points) match the oracle."""
+    rng = np.random.default_rng(31)
+    p, x = random_problem(rng, 32, 32, M=900)
+    perm = rng.permutation(900) + 1
+    sign = np.where(rng.uniform(size=900) < 0.5, -1, 1)
+    p.uvidxfcv = (perm[:200] * sign[:200]).astype(np.int32)
+    p.uvidxamp = (perm[200:350] * sign[200:350]).astype(np.int32)
+    with _mk(p, tune=dict(precision=precision)) as g:
+        _check_cost(p, x, g)
+
+
 @pytest.mark.parametrize("tune", [dict(bin=8, sub_points=64), dict(bin=32, sub_points=512, col_group=4),
                                   dict(chunk=64), dict(chisq_threads=512), dict(chisq_threads=512, chunk=64)])
 def test_tunables_do_not_change_results(gpu_lib, tune):
