@@ -44,6 +44,8 @@ struct Error : std::runtime_error {
 #define SMILI_CHISQ_DIRECT 1
 #endif
 
+constexpr long long kColGroupsMin = 256;   // column work-groups wanted (one per CU)
+
 static void require(bool ok, const std::string& msg) {
   if (!ok) throw Error(SMILI_EINVAL, msg);
 }
@@ -680,11 +682,18 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     shift_chunks(DT_CA, ca0);
   }
 
-  // column work-groups: the largest of 4, 2, 1 columns whose LDS fits
+  // column work-groups: the largest of 4, 2, 1 columns whose LDS fits and
+  // that still gives a work-group per CU (a single 512-column grid in FP32
+  // fits 4 columns, but 128 work-groups leave half the CUs idle)
   if (col_group == 0) {
     const size_t csz = fp64 ? sizeof(double2) : sizeof(float2);
+    long long ncols = 0;
+    for (const UnitDev& U : units)
+      if (U.m > 0) ncols += U.nf1;
     col_group = 4;
-    while (col_group > 1 && (2ull * col_group * (max_nf2 + 1) + max_nf2) * csz > 65536) col_group /= 2;
+    while (col_group > 1 && ((2ull * col_group * (max_nf2 + 1) + max_nf2) * csz > 65536 ||
+                             ncols / col_group < kColGroupsMin))
+      col_group /= 2;
   }
   // regularizer items, row/column items
   const int reg_chunk = 512;
