@@ -153,12 +153,15 @@ def imaging(initimage, imregion=None, vistable=None, amptable=None, bstable=None
     ``initimage`` is an image with ``header`` (nx, ny, dx, dy in degrees,
     nxref, nyref) and ``data[istokes, ifreq, iy, ix]`` (an IMFITS or
     ``smili_amd.imaging.image.Image``); tables are DataFrames or dicts with
-    the reference's column names.  Priors are images (their data[0, 0] is
+    the reference's column names.  ``tune``: smili_tune fields, or "auto"
+    (smili_amd.autotune picks them for this problem first).  Priors are images (their data[0, 0] is
     used, lbfgs.py:279-283).  Returns a copy of initimage holding the
     solution (lbfgs.py:565-576); with ``return_info`` also the SolveInfo.
     """
     if pulse_fwhm > 0:
         raise NotImplementedError("pulse_fwhm (deburring, lbfgs.py:1114-1194) is not supported")
+    if isinstance(tune, str) and tune != "auto":
+        raise ValueError('tune must be a dict of smili_tune fields, None or "auto"')
     hdr = initimage.header
     window = _window_of(imregion, initimage, istokes, ifreq)
 
@@ -173,6 +176,9 @@ def imaging(initimage, imregion=None, vistable=None, amptable=None, bstable=None
                    tfd_lambda, tfd_tgterror, cen_lambda, cen_alpha, sm_lambda,
                    np.deg2rad(sm_maj * _angconv(initimage)), np.deg2rad(sm_min * _angconv(initimage)), sm_phi,
                    totalflux, inorm)
+    if tune == "auto":
+        from ..autotune import autotune
+        tune, _ = autotune(lambda t: CostProblem(**prep.kwargs, tune=t), prep.x0)
     with CostProblem(**prep.kwargs, tune=tune) as prob:
         x, info = solve(prob, prep.x0, niter=niter, nonneg=nonneg, nprint=nprint,
                         inormfactr=prep.inormfactr, verbose=verbose)
