@@ -151,6 +151,25 @@ def test_graph_and_eager_agree_and_repeat(gpu_lib):
 
 
 @pytest.mark.parametrize("precision", [32, 64])
+def test_term_errors_against_dft(gpu_lib, precision):
+    """Every output term against the exact-DFT oracle: the chi-square terms
+    carry the ES kernel's error (eps 1e-6; FP32 grids add their rounding),
+    the regularizers none (they do not touch the transforms)."""
+    rng = np.random.default_rng(41 + precision)
+    p, x = random_problem(rng, 64, 64, M=1500, nfcv=600, namp=500, ncp=400, nca=400, noise=0.02)
+    with _mk(p, tune=dict(precision=precision)) as g:
+        c, gr, t = g.cost_grad(x)
+    c_ref, g_ref, t_ref = O.calc_cost(x, p)
+    errs = {k: abs(t[k] - t_ref[k]) / abs(t_ref[k]) for k in t_ref if t_ref[k] != 0}
+    print("precision", precision, " ".join("%s %.1e" % kv for kv in sorted(errs.items())))
+    for k, e in errs.items():
+        if k.startswith("chisq") or k == "cost":
+            assert e <= (2e-5 if precision == 32 else 5e-6), (k, e)
+        else:
+            assert e <= 1e-12, (k, e)
+
+
+@pytest.mark.parametrize("precision", [32, 64])
 def test_distinct_point_chunks_match_oracle(gpu_lib, precision):
     """Visibilities and amplitudes that each sit at their own uv point take
     the chi-square kernel's direct path (no LDS staging of the points);
