@@ -1,0 +1,38 @@
+"""The per-frame warm start of a movie: binning a network image onto a
+coarser movie grid (host), the per-frame network inputs (GPU transforms)."""
+import numpy as np
+import pytest
+
+from smili_amd.warmstart import movie as WM
+
+
+def test_bin_to_grid_keeps_flux_and_centre():
+    rng = np.random.default_rng(0)
+    img = np.kron(rng.uniform(size=(4, 4)), np.ones((2, 2)))     # constant on each 2 x 2 block
+    out = WM.bin_to_grid(img, 8, 2)
+    assert out.shape == (8, 8) and np.isclose(out.sum(), img.sum())
+    # block (b, c) of 2x2 network pixels lands on movie pixel (2 + b, 2 + c)
+    assert np.isclose(out[2, 2], img[0:2, 0:2].sum()) and np.isclose(out[5, 5], img[6:8, 6:8].sum())
+    assert out[:2].sum() == 0 and out[:, 6:].sum() == 0
+    # the reference pixels ((n + 1) / 2, 1-based) coincide: an image constant
+    # on the blocks keeps its centroid
+    yy, xx = np.mgrid[0:8, 0:8]
+    c_in = ((xx - 3.5) * 1.0 * img).sum() / img.sum()          # network pixel = 1
+    c_out = ((xx - 3.5) * 2.0 * out).sum() / out.sum()         # movie pixel = 2
+    assert np.isclose(c_in, c_out)
+    with pytest.raises(ValueError):
+        WM.bin_to_grid(img, 7, 2)
+
+
+@pytest.mark.gpu
+def test_frame_inputs_per_frame(gpu_lib):
+    from smili_amd.warmstart import data as D
+    cfg = dict(D.CFG, n=16)
+    rng = np.random.default_rng(1)
+    u = rng.uniform(-5e9, 5e9, 60)
+    v = rng.uniform(0, 5e9, 60)
+    vis = dict(u=u, v=v, amp=np.ones(60), phase=np.zeros(60), sigma=np.ones(60),
+               frmidx=np.repeat([0, 2], 30))
+    x = WM.frame_inputs(vis, 3, cfg)
+    assert x.shape == (3, 16, 16, 2)
+    assert np.all(x[1] == 0) and x[0, ..., 0].max() == pytest.approx(1.0)
