@@ -18,7 +18,7 @@ import time
 from dataclasses import dataclass, field
 
 import numpy as np
-from scipy.optimize import minimize
+from scipy.optimize import Bounds, minimize
 
 from ..cost import CostProblem
 from .prep import prepare
@@ -73,11 +73,68 @@ def _summary(terms, inormfactr, x, nz=1):
     return "\n".join(lines)
 
 
+def _setulb_loop(fg, x0, nonneg, m, factr, pgtol, maxiter, maxls, callback):
+    """L-BFGS-B through SciPy's own v3.0 routine ``setulb`` and its
+    reverse-communication protocol, the loop fftim2d.f90:286-351 runs over
+    lbfgsb.f's setulb: task FG -> evaluate f, g at x; task NEW_X -> one
+    iteration done (the callback, then the iteration limit); anything else
+    ends the loop.  ``scipy.optimize.minimize`` drives the same routine the
+    same way, but builds its bound arrays in a Python loop over every
+    variable (seconds at a million pixels); here they are set whole.
+    Returns (x, iterations, message, success) or None when this SciPy's
+    ``setulb`` does not take the call (then ``minimize`` is used)."""
+    try:
+        from scipy.optimize import _lbfgsb
+        from scipy.optimize._lbfgsb_py import status_messages, task_messages
+    except ImportError:
+        return None
+    n = x0.size
+    x = np.array(np.maximum(x0, 0.0) if nonneg else x0, dtype=np.float64)
+    nbd = np.full(n, 1 if nonneg else 0, np.int32)       # 1: lower bound only
+    low, upp = np.zeros(n), np.zeros(n)
+    f, g = 0.0, np.zeros(n)
+    wa = np.zeros(2 * m * n + 5 * n + 11 * m * m + 8 * m)
+    iwa = np.zeros(3 * n, np.int32)
+    task, ln_task = np.zeros(2, np.int32), np.zeros(2, np.int32)
+    lsave, isave, dsave = np.zeros(4, np.int32), np.zeros(44, np.int32), np.zeros(29)
+    nit, first = 0, True
+    while True:
+        try:
+            _lbfgsb.setulb(m, x, low, upp, nbd, f, g, factr, pgtol, wa, iwa, task, lsave, isave, dsave,
+                           maxls, ln_task)
+        except TypeError:
+            if first:
+                return None
+            raise
+        first = False
+        if task[0] == 3:                                     # FG
+            f, g = fg(x)
+            g = np.ascontiguousarray(g, dtype=np.float64)
+        elif task[0] == 1:                                   # NEW_X
+            nit += 1
+            xv = x.view()
+            xv.flags.writeable = False
+            try:
+                callback(xv)
+            except StopIteration:
+                task[0], task[1] = 5, 505
+            if nit >= maxiter:
+                task[0], task[1] = 5, 504
+        else:
+            break
+    if task[0] == 5 and task[1] == 505:
+        return x, nit, "`callback` raised `StopIteration`.", False      # minimize's wording
+    return x, nit, status_messages[task[0]] + ": " + task_messages[task[1]], bool(task[0] == 4)
+
+
 def solve(problem: CostProblem, x0, niter=1000, nonneg=True, m=None, factr=None, pgtol=None,
-          nprint=500, inormfactr=1.0, callback=None, verbose=True, stop_cost=None):
+          nprint=500, inormfactr=1.0, callback=None, verbose=True, stop_cost=None, driver="auto"):
     """L-BFGS-B over ``problem.cost_grad`` from x0 (scaled units); with
     ``stop_cost`` it also ends at the first iterate whose cost is at or
-    below it.  Returns (x, SolveInfo)."""
+    below it.  ``driver``: "setulb" (SciPy's routine driven directly),
+    "minimize" (through scipy.optimize.minimize; the same iterates), "auto"
+    (the first where this SciPy allows it).  ``callback(it, xk, terms)``
+    must not keep xk without copying it.  Returns (x, SolveInfo)."""
     m = lbfgsbprms["m"] if m is None else m
     factr = lbfgsbprms["factr"] if factr is None else factr
     pgtol = lbfgsbprms["pgtol"] if pgtol is None else pgtol
@@ -106,27 +163,35 @@ def solve(problem: CostProblem, x0, niter=1000, nonneg=True, m=None, factr=None,
             state["stopped"] = True
             raise StopIteration
 
-    bounds = [(0.0, None)] * np.asarray(x0).size if nonneg else None
+    x0 = np.asarray(x0, np.float64).ravel()
     t0 = time.perf_counter()
     if verbose:
         c, _, terms = problem.cost_grad(x0)
         print("Summary of the initial image")
         print(_summary(terms, inormfactr, x0, nz))
-    res = minimize(fg, np.asarray(x0, np.float64), jac=True, method="L-BFGS-B", bounds=bounds,
-                   callback=cb,
-                   options=dict(maxcor=int(m), ftol=float(factr) * np.finfo(float).eps, gtol=float(pgtol),
-                                maxiter=int(niter) + 1, maxfun=10 ** 9, maxls=20))
+    out = None if driver == "minimize" else _setulb_loop(fg, x0, nonneg, int(m), float(factr), float(pgtol),
+                                                          int(niter) + 1, 20, cb)
+    if out is None:
+        if driver == "setulb":
+            raise RuntimeError("this SciPy's L-BFGS-B routine does not take the direct call")
+        res = minimize(fg, x0, jac=True, method="L-BFGS-B",
+                       bounds=Bounds(np.zeros(x0.size), np.full(x0.size, np.inf)) if nonneg else None,
+                       callback=cb,
+                       options=dict(maxcor=int(m), ftol=float(factr) * np.finfo(float).eps, gtol=float(pgtol),
+                                    maxiter=int(niter) + 1, maxfun=10 ** 9, maxls=20))
+        out = res.x, int(res.nit), str(res.message), bool(res.success)
+    x, nit, message, success = out
     wall = time.perf_counter() - t0
-    c, _, terms = problem.cost_grad(res.x)
+    c, _, terms = problem.cost_grad(x)
     if verbose:
         print("Final Summary")
-        print("  Iteration ent at %5d/%5d" % (res.nit, niter))
-        print(_summary(terms, inormfactr, res.x, nz))
-        print(res.message)
-    info = SolveInfo(niter=int(res.nit), nfev=int(state["nfev"]), cost=float(c), terms=terms,
-                     message=str(res.message), success=bool(res.success), wall_s=wall,
+        print("  Iteration ent at %5d/%5d" % (nit, niter))
+        print(_summary(terms, inormfactr, x, nz))
+        print(message)
+    info = SolveInfo(niter=nit, nfev=int(state["nfev"]), cost=float(c), terms=terms,
+                     message=message, success=success, wall_s=wall,
                      eval_s=state["eval_s"], history=hist, history_t=hist_t, stopped=state["stopped"])
-    return res.x, info
+    return x, info
 
 
 def _window_of(imregion, initimage, istokes, ifreq):

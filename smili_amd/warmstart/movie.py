@@ -53,10 +53,8 @@ def bin_to_grid(img, n_out, factor):
     return out
 
 
-def propose_movie(net, vis, prep, prob, pix_uas, cfg=D.CFG, floor=0.0):
-    """(x0 of the movie in scaled units or None, timing dict).  ``floor``:
-    every pixel of a frame's proposal is raised to at least this fraction of
-    the frame's mean pixel (pixels at exactly zero sit on L-BFGS-B's bound)."""
+def propose_movie(net, vis, prep, prob, pix_uas, cfg=D.CFG):
+    """(x0 of the movie in scaled units or None, timing dict)."""
     t0 = time.perf_counter()
     nz, n_m = prep.Nt, prep.Nx
     factor = int(round(pix_uas / cfg["pix_uas"]))
@@ -72,7 +70,6 @@ def propose_movie(net, vis, prep, prob, pix_uas, cfg=D.CFG, floor=0.0):
     for z in range(nz):
         img = bin_to_grid(shapes[z], n_m, factor)[prep.yidx - 1, prep.xidx - 1]
         if img.sum() > 0:
-            img = np.maximum(img, floor * img.mean())
             s[z * npix:(z + 1) * npix] = img * (prior[z].sum() / img.sum())
     if not np.any(s > 0):
         return None, dict(dirty_s=t1 - t0, net_s=t2 - t1, fit_s=0.0)
@@ -84,14 +81,14 @@ def propose_movie(net, vis, prep, prob, pix_uas, cfg=D.CFG, floor=0.0):
     return a * s, dict(dirty_s=t1 - t0, net_s=t2 - t1, fit_s=time.perf_counter() - t2)
 
 
-def warm_solve_movie(net, vis, prep, pix_uas, niter=1000, m=10, stop_cost=None, tune=None, floor=0.0):
+def warm_solve_movie(net, vis, prep, pix_uas, niter=1000, m=10, stop_cost=None, tune=None):
     """Warm-started joint solve: (x scaled frame-major, info dict with the
     SolveInfo, the start used and the objectives of the prior and the
     proposal)."""
     from ..cost import MovieProblem
     from ..imaging import lbfgs
     with MovieProblem(**prep.kwargs, tune=tune) as prob:
-        x0, tm = propose_movie(net, vis, prep, prob, pix_uas, floor=floor)
+        x0, tm = propose_movie(net, vis, prep, prob, pix_uas)
         f_prior = prob.cost_grad(prep.x0)[0]
         f_prop = prob.cost_grad(x0)[0] if x0 is not None else float("inf")
         start = "proposal"
@@ -99,3 +96,56 @@ def warm_solve_movie(net, vis, prep, pix_uas, niter=1000, m=10, stop_cost=None, 
             x0, start = prep.x0, "prior"
         x, info = lbfgs.solve(prob, x0, niter=niter, m=m, verbose=False, stop_cost=stop_cost)
     return x, dict(info=info, start=start, f_prior=f_prior, f_proposal=f_prop, **tm)
+
+
+def time_to_image_movie(net, prep, vis, pix_uas, rel=1e-6, niter=1000, m=10, tune=None, truth=None):
+    """Plain joint solve from the movie's prior frames, then the warm-started
+    one stopped at the plain solve's final objective f* (1 + rel).  Times
+    are to the first iterate at or below that target: the plain one counts
+    the handle's set-up and the solve up to that iterate, the warm one
+    everything from the handle's set-up on (dirty maps, network, scale
+    fit, both start objectives, the solve).
+
+    Image distances (NRMSE over all frames): the warm movie against the
+    plain final one, and the plain solve's own iterate at the target
+    against its final one (how far apart two movies at the same objective
+    lie along the plain path); with ``truth`` ([Nt, Ny, Nx] in Jy/pixel),
+    also both movies against it."""
+    from ..cost import MovieProblem
+    from ..imaging import lbfgs
+    from .solve import _reach
+    kept = []                      # plain iterates within rel of the latest cost
+
+    def keep(it, xk, terms):
+        c = terms["cost"]
+        kept[:] = [(k, ck, xc) for k, ck, xc in kept if ck <= c * (1.0 + rel)]
+        kept.append((it, c, np.array(xk, copy=True)))
+
+    t0 = time.perf_counter()
+    with MovieProblem(**prep.kwargs, tune=tune) as prob:
+        t_setup = time.perf_counter() - t0
+        xp, ip = lbfgs.solve(prob, prep.x0, niter=niter, m=m, verbose=False, callback=keep)
+    target = ip.cost * (1.0 + rel)
+    kp, tp = _reach(ip, target)
+    x_tgt = next((xc for k, ck, xc in kept if ck <= target), xp)
+    t0 = time.perf_counter()
+    xw, w = warm_solve_movie(net, vis, prep, pix_uas, niter=niter, m=m, stop_cost=target, tune=tune)
+    wall_w = time.perf_counter() - t0
+    iw = w["info"]
+    kw, _ = _reach(iw, target)
+
+    def dist(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    out = dict(frames=int(prep.Nt), npix=int(xp.size), gap_prior=w["f_prior"] / ip.cost - 1.0,
+               gap_proposal=w["f_proposal"] / ip.cost - 1.0, start=w["start"],
+               plain_iters=kp, plain_s=t_setup + tp, plain_total_iters=ip.niter, plain_nfev=ip.nfev,
+               plain_total_s=t_setup + ip.wall_s, plain_cost=ip.cost,
+               warm_iters=kw, warm_nfev=iw.nfev, warm_s=wall_w if kw is not None else None,
+               warm_cost=iw.cost, reached=kw is not None, nrmse=dist(xw, xp), nrmse_plain_at_target=dist(x_tgt, xp),
+               dirty_s=w["dirty_s"], net_s=w["net_s"], fit_s=w["fit_s"])
+    if truth is not None:
+        tv = np.asarray(truth)[:, prep.yidx - 1, prep.xidx - 1].reshape(-1)
+        out["truth_err_plain"] = dist(xp / prep.inormfactr, tv)
+        out["truth_err_warm"] = dist(xw / prep.inormfactr, tv)
+    return out
