@@ -104,7 +104,9 @@ def time_to_image_movie(net, prep, vis, pix_uas, rel=1e-6, niter=1000, m=10, tun
     are to the first iterate at or below that target: the plain one counts
     the handle's set-up and the solve up to that iterate, the warm one
     everything from the handle's set-up on (dirty maps, network, scale
-    fit, both start objectives, the solve).
+    fit, both start objectives, the solve).  ``plain_curve`` /
+    ``warm_curve``: the iterations at which each solve first reaches
+    f* (1 + gap) for the gaps of ``solve.GAPS`` (None: not reached).
 
     Image distances (NRMSE over all frames): the warm movie against the
     plain final one, and the plain solve's own iterate at the target
@@ -113,7 +115,7 @@ def time_to_image_movie(net, prep, vis, pix_uas, rel=1e-6, niter=1000, m=10, tun
     also both movies against it."""
     from ..cost import MovieProblem
     from ..imaging import lbfgs
-    from .solve import _reach
+    from .solve import GAPS, _reach
     kept = []                      # plain iterates within rel of the latest cost
 
     def keep(it, xk, terms):
@@ -143,7 +145,9 @@ def time_to_image_movie(net, prep, vis, pix_uas, rel=1e-6, niter=1000, m=10, tun
                plain_total_s=t_setup + ip.wall_s, plain_cost=ip.cost,
                warm_iters=kw, warm_nfev=iw.nfev, warm_s=wall_w if kw is not None else None,
                warm_cost=iw.cost, reached=kw is not None, nrmse=dist(xw, xp), nrmse_plain_at_target=dist(x_tgt, xp),
-               dirty_s=w["dirty_s"], net_s=w["net_s"], fit_s=w["fit_s"])
+               dirty_s=w["dirty_s"], net_s=w["net_s"], fit_s=w["fit_s"],
+               gaps=list(GAPS), plain_curve=[_reach(ip, ip.cost * (1.0 + g))[0] for g in GAPS],
+               warm_curve=[_reach(iw, ip.cost * (1.0 + g))[0] for g in GAPS])
     if truth is not None:
         tv = np.asarray(truth)[:, prep.yidx - 1, prep.xidx - 1].reshape(-1)
         out["truth_err_plain"] = dist(xp / prep.inormfactr, tv)

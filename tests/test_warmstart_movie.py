@@ -56,8 +56,12 @@ def test_movie_warm_start_reaches_plain_objective_sooner(gpu_lib):
     finally:
         net.close()
     print({k: r[k] for k in ("gap_prior", "gap_proposal", "plain_iters", "warm_iters", "plain_s", "warm_s",
-                             "nrmse", "nrmse_plain_at_target", "truth_err_plain", "truth_err_warm")})
+                             "nrmse", "nrmse_plain_at_target", "truth_err_plain", "truth_err_warm",
+                             "plain_curve", "warm_curve")})
     assert r["start"] == "proposal" and r["gap_proposal"] < r["gap_prior"]
     assert r["reached"] and r["warm_iters"] < r["plain_iters"]
     assert r["truth_err_warm"] <= 1.2 * r["truth_err_plain"]
+    for k in ("plain_curve", "warm_curve"):
+        assert None not in r[k] and r[k] == sorted(r[k])
+    assert all(w <= p for w, p in zip(r["warm_curve"], r["plain_curve"]))
     assert r["nrmse"] <= 0.1
