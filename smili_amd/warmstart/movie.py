@@ -98,7 +98,8 @@ def warm_solve_movie(net, vis, prep, pix_uas, niter=1000, m=10, stop_cost=None, 
     return x, dict(info=info, start=start, f_prior=f_prior, f_proposal=f_prop, **tm)
 
 
-def time_to_image_movie(net, prep, vis, pix_uas, rel=1e-6, niter=1000, m=10, tune=None, truth=None):
+def time_to_image_movie(net, prep, vis, pix_uas, rel=1e-6, niter=1000, m=10, tune=None, truth=None,
+                        track=False):
     """Plain joint solve from the movie's prior frames, then the warm-started
     one stopped at the plain solve's final objective f* (1 + rel).  Times
     are to the first iterate at or below that target: the plain one counts
@@ -109,10 +110,11 @@ def time_to_image_movie(net, prep, vis, pix_uas, rel=1e-6, niter=1000, m=10, tun
     f* (1 + gap) for the gaps of ``solve.GAPS`` (None: not reached).
 
     Image distances (NRMSE over all frames): the warm movie against the
-    plain final one, and the plain solve's own iterate at the target
-    against its final one (how far apart two movies at the same objective
-    lie along the plain path); with ``truth`` ([Nt, Ny, Nx] in Jy/pixel),
-    also both movies against it."""
+    plain final one; with ``track``, also the plain solve's own iterate at
+    the target against its final one (how far apart two movies at the same
+    objective lie along the plain path; it copies every plain iterate, which
+    adds to the plain clock, so timing runs leave it off); with ``truth``
+    ([Nt, Ny, Nx] in Jy/pixel), both movies against it."""
     from ..cost import MovieProblem
     from ..imaging import lbfgs
     from .solve import GAPS, _reach
@@ -126,7 +128,7 @@ def time_to_image_movie(net, prep, vis, pix_uas, rel=1e-6, niter=1000, m=10, tun
     t0 = time.perf_counter()
     with MovieProblem(**prep.kwargs, tune=tune) as prob:
         t_setup = time.perf_counter() - t0
-        xp, ip = lbfgs.solve(prob, prep.x0, niter=niter, m=m, verbose=False, callback=keep)
+        xp, ip = lbfgs.solve(prob, prep.x0, niter=niter, m=m, verbose=False, callback=keep if track else None)
     target = ip.cost * (1.0 + rel)
     kp, tp = _reach(ip, target)
     x_tgt = next((xc for k, ck, xc in kept if ck <= target), xp)
@@ -144,7 +146,8 @@ def time_to_image_movie(net, prep, vis, pix_uas, rel=1e-6, niter=1000, m=10, tun
                plain_iters=kp, plain_s=t_setup + tp, plain_total_iters=ip.niter, plain_nfev=ip.nfev,
                plain_total_s=t_setup + ip.wall_s, plain_cost=ip.cost,
                warm_iters=kw, warm_nfev=iw.nfev, warm_s=wall_w if kw is not None else None,
-               warm_cost=iw.cost, reached=kw is not None, nrmse=dist(xw, xp), nrmse_plain_at_target=dist(x_tgt, xp),
+               warm_cost=iw.cost, reached=kw is not None, nrmse=dist(xw, xp),
+               nrmse_plain_at_target=dist(x_tgt, xp) if track else None,
                dirty_s=w["dirty_s"], net_s=w["net_s"], fit_s=w["fit_s"],
                gaps=list(GAPS), plain_curve=[_reach(ip, ip.cost * (1.0 + g))[0] for g in GAPS],
                warm_curve=[_reach(iw, ip.cost * (1.0 + g))[0] for g in GAPS])
