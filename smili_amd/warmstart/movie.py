@@ -54,7 +54,10 @@ def bin_to_grid(img, n_out, factor):
 
 
 def propose_movie(net, vis, prep, prob, pix_uas, cfg=D.CFG):
-    """(x0 of the movie in scaled units or None, timing dict)."""
+    """(x0 of the movie in scaled units or None, timing dict).  A frame
+    whose proposal is empty (no visibilities, or a network image without
+    flux inside the movie's window) keeps its prior frame; with no proposal
+    at all the result is None."""
     t0 = time.perf_counter()
     nz, n_m = prep.Nt, prep.Nx
     factor = int(round(pix_uas / cfg["pix_uas"]))
@@ -66,19 +69,21 @@ def propose_movie(net, vis, prep, prob, pix_uas, cfg=D.CFG):
     t2 = time.perf_counter()
     npix = prep.xidx.size
     prior = prep.x0.reshape(nz, npix)
-    s = np.zeros(nz * npix)
+    s = np.array(prior, dtype=np.float64).reshape(-1)
+    used = 0
     for z in range(nz):
         img = bin_to_grid(shapes[z], n_m, factor)[prep.yidx - 1, prep.xidx - 1]
-        if img.sum() > 0:
+        if img.sum() > 0 and x_in[z].any():
             s[z * npix:(z + 1) * npix] = img * (prior[z].sum() / img.sum())
-    if not np.any(s > 0):
-        return None, dict(dirty_s=t1 - t0, net_s=t2 - t1, fit_s=0.0)
+            used += 1
+    if used == 0:
+        return None, dict(dirty_s=t1 - t0, net_s=t2 - t1, fit_s=0.0, frames_proposed=0)
     a1, a2 = 1.0, 1.25
     g1 = float(prob.cost_grad(a1 * s)[1] @ s)
     g2 = float(prob.cost_grad(a2 * s)[1] @ s)
     a = a1 - g1 * (a2 - a1) / (g2 - g1) if g2 != g1 else a1
     a = float(np.clip(a, 0.2, 5.0))
-    return a * s, dict(dirty_s=t1 - t0, net_s=t2 - t1, fit_s=time.perf_counter() - t2)
+    return a * s, dict(dirty_s=t1 - t0, net_s=t2 - t1, fit_s=time.perf_counter() - t2, frames_proposed=used)
 
 
 def warm_solve_movie(net, vis, prep, pix_uas, niter=1000, m=10, stop_cost=None, tune=None):
@@ -148,7 +153,7 @@ def time_to_image_movie(net, prep, vis, pix_uas, rel=1e-6, niter=1000, m=10, tun
                warm_iters=kw, warm_nfev=iw.nfev, warm_s=wall_w if kw is not None else None,
                warm_cost=iw.cost, reached=kw is not None, nrmse=dist(xw, xp),
                nrmse_plain_at_target=dist(x_tgt, xp) if track else None,
-               dirty_s=w["dirty_s"], net_s=w["net_s"], fit_s=w["fit_s"],
+               dirty_s=w["dirty_s"], net_s=w["net_s"], fit_s=w["fit_s"], frames_proposed=w["frames_proposed"],
                gaps=list(GAPS), plain_curve=[_reach(ip, ip.cost * (1.0 + g))[0] for g in GAPS],
                warm_curve=[_reach(iw, ip.cost * (1.0 + g))[0] for g in GAPS])
     if truth is not None:

@@ -65,3 +65,41 @@ def test_movie_warm_start_reaches_plain_objective_sooner(gpu_lib):
         assert None not in r[k] and r[k] == sorted(r[k])
     assert all(w <= p for w, p in zip(r["warm_curve"], r["plain_curve"]))
     assert r["nrmse"] <= 0.1 and r["nrmse_plain_at_target"] < r["nrmse"]
+
+
+def test_propose_movie_bins_normalises_and_keeps_prior_of_empty_frames(monkeypatch):
+    """Host logic of the per-frame proposal with a stub network and a
+    quadratic objective f(x) = |x - t|^2 / 2: frames with inputs get their
+    binned network image at the prior frame's flux, a frame without
+    visibilities keeps its prior frame, and the common scale is the exact
+    minimiser of f along the stacked proposal (the secant is exact on a
+    quadratic)."""
+    from types import SimpleNamespace
+    from smili_amd.warmstart import data as D
+    rng = np.random.default_rng(4)
+    nz, n, n_m = 3, 8, 6
+    cfg = dict(D.CFG, n=n, pix_uas=1.0)
+    x_in = rng.uniform(size=(nz, n, n, 2)).astype(np.float32)
+    x_in[1] = 0.0                                                 # frame 1: no visibilities
+    shapes = rng.uniform(size=(nz, n, n)).astype(np.float32)
+    monkeypatch.setattr(WM, "frame_inputs", lambda vis, nz_, cfg_: x_in)
+    net = SimpleNamespace(forward=lambda x: shapes)
+    yy, xx = np.mgrid[1:n_m + 1, 1:n_m + 1]
+    prior = rng.uniform(0.5, 1.5, (nz, n_m * n_m))
+    prep = SimpleNamespace(Nt=nz, Nx=n_m, Ny=n_m, xidx=xx.ravel(), yidx=yy.ravel(), x0=prior.ravel())
+    target = rng.uniform(size=nz * n_m * n_m)
+    prob = SimpleNamespace(cost_grad=lambda x: (0.5 * np.sum((x - target) ** 2), x - target, {}))
+    x0, tm = WM.propose_movie(net, None, prep, prob, 2.0, cfg)
+    assert tm["frames_proposed"] == 2
+    s = np.empty((nz, n_m * n_m))
+    for z in range(nz):
+        img = WM.bin_to_grid(shapes[z].astype(np.float64), n_m, 2).ravel()
+        s[z] = prior[z] if z == 1 else img * prior[z].sum() / img.sum()
+    a = float(s.ravel() @ target / (s.ravel() @ s.ravel()))
+    np.testing.assert_allclose(x0, a * s.ravel(), rtol=1e-10)
+    # no frame with a proposal: no start
+    x_in[:] = 0.0
+    x0, tm = WM.propose_movie(net, None, prep, prob, 2.0, cfg)
+    assert x0 is None and tm["frames_proposed"] == 0
+    with pytest.raises(ValueError):
+        WM.propose_movie(net, None, prep, prob, 1.5, cfg)
