@@ -135,6 +135,8 @@ def solve(problem: CostProblem, x0, niter=1000, nonneg=True, m=None, factr=None,
     "minimize" (through scipy.optimize.minimize; the same iterates), "auto"
     (the first where this SciPy allows it).  ``callback(it, xk, terms)``
     must not keep xk without copying it.  Returns (x, SolveInfo)."""
+    if driver not in ("auto", "setulb", "minimize"):
+        raise ValueError(f'driver must be "auto", "setulb" or "minimize", not {driver!r}')
     m = lbfgsbprms["m"] if m is None else m
     factr = lbfgsbprms["factr"] if factr is None else factr
     pgtol = lbfgsbprms["pgtol"] if pgtol is None else pgtol

@@ -85,3 +85,8 @@ def test_callback_sees_read_only_iterates():
 
     lbfgs.solve(prob, np.full(300, 0.5), niter=5, verbose=False, driver="setulb", callback=cb)
     assert [s[0] for s in seen] == list(range(1, 7)) and not any(s[1] for s in seen)
+
+
+def test_unknown_driver_is_rejected():
+    with pytest.raises(ValueError):
+        lbfgs.solve(Quartic(10), np.ones(10), niter=2, verbose=False, driver="fortran")
