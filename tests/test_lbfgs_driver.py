@@ -90,3 +90,20 @@ def test_callback_sees_read_only_iterates():
 def test_unknown_driver_is_rejected():
     with pytest.raises(ValueError):
         lbfgs.solve(Quartic(10), np.ones(10), niter=2, verbose=False, driver="fortran")
+
+
+class WrongGradient(Quartic):
+    """The gradient of -f: no descent direction, so the line search fails
+    (lbfgsb.f's ABNORMAL_TERMINATION_IN_LNSRCH, lbfgsb.f:790-804)."""
+
+    def cost_grad(self, x):
+        f, g, t = super().cost_grad(x)
+        return f, -g, t
+
+
+def test_line_search_failure_is_surfaced():
+    prob = WrongGradient(200, seed=5)
+    a, b = _both(prob, np.full(200, 0.5), niter=50, m=5)
+    _same(a, b)
+    info = a[1]
+    assert not info.success and "ABNORMAL" in info.message and info.niter < 50
