@@ -44,6 +44,7 @@ class SolveInfo:
     history: list = field(default_factory=list)      # cost after each iteration
     history_t: list = field(default_factory=list)    # seconds since the start, after each iteration
     stopped: bool = False                            # reached stop_cost
+    start: str = "prior"                             # "proposal" when a warm start was used
 
 
 def _summary(terms, inormfactr, x, nz=1):

@@ -37,7 +37,8 @@ def imaging3d(initmovie, imregion=None, vistable=None, amptable=None, bstable=No
               rt_lambda=-1., rt_prior=None, ri_lambda=-1., ri_prior=None,
               rs_lambda=-1, rs_prior=None, rf_lambda=-1, rf_prior=None,
               niter=1000, nonneg=True, nprint=500, totalflux=None, inorm=1.,
-              istokes=0, ifreq=0, output='list', tune=None, verbose=True, return_info=False):
+              istokes=0, ifreq=0, output='list', tune=None, verbose=True, return_info=False,
+              warmstart=None):
     """smili.imaging.lbfgs3d.imaging3d on the GPU cost.
 
     ``initmovie`` has ``Nt`` and ``images[i]`` (IMFITS-like, or
@@ -47,6 +48,13 @@ def imaging3d(initmovie, imregion=None, vistable=None, amptable=None, bstable=No
     evaluates them), have no effect; ``cen_*`` likewise has no effect on a
     movie.  Returns a copy of initmovie holding the solution
     (lbfgs3d.py:825-832); with ``return_info`` also the SolveInfo.
+
+    ``warmstart``: a ``smili_amd.warmstart.unet.UNet`` handle (not in the
+    reference).  Every frame is then proposed by the network from that
+    frame's visibilities (``warmstart.movie.propose_movie``; the movie's
+    pixel must be an integer multiple of the network's) and the solve
+    starts from the proposal when its objective is below initmovie's;
+    ``info.start`` says which start was used.
     """
     Nt = initmovie.Nt
     initimage = initmovie.images[0]
@@ -68,9 +76,18 @@ def imaging3d(initmovie, imregion=None, vistable=None, amptable=None, bstable=No
                      cen_lambda, cen_alpha, rt_lambda, prior(rt_prior), ri_lambda, prior(ri_prior),
                      totalflux, inorm)
     with MovieProblem(**prep.kwargs, tune=tune) as prob:
-        x, info = solve(prob, prep.x0, niter=niter, nonneg=nonneg, nprint=nprint, m=lbfgsbprms["m"],
+        x0, start = prep.x0, "prior"
+        if warmstart is not None:
+            from ..warmstart.movie import propose_movie
+            xw, _ = propose_movie(warmstart, vistable, prep, prob, abs(hdr["dx"]) * 3.6e9)
+            if xw is not None and prob.cost_grad(xw)[0] < prob.cost_grad(prep.x0)[0]:
+                x0, start = xw, "proposal"
+            if verbose:
+                print("Warm start: %s" % start)
+        x, info = solve(prob, x0, niter=niter, nonneg=nonneg, nprint=nprint, m=lbfgsbprms["m"],
                         factr=lbfgsbprms["factr"], pgtol=lbfgsbprms["pgtol"],
                         inormfactr=prep.inormfactr, verbose=verbose)
+        info.start = start
     Iout = x / prep.inormfactr
     out = copy.deepcopy(initmovie)
     Npix = prep.xidx.size

@@ -103,3 +103,29 @@ def test_propose_movie_bins_normalises_and_keeps_prior_of_empty_frames(monkeypat
     assert x0 is None and tm["frames_proposed"] == 0
     with pytest.raises(ValueError):
         WM.propose_movie(net, None, prep, prob, 1.5, cfg)
+
+
+@pytest.mark.gpu
+def test_imaging3d_warmstart_keyword(gpu_lib):
+    """imaging3d(..., warmstart=net) on a 16-frame movie starts from the
+    per-frame proposal and converges to the plain solve's objective (1e-4
+    relative; both run to L-BFGS-B's own stop)."""
+    from smili_amd import workloads
+    from smili_amd.imaging import lbfgs3d
+    from smili_amd.imaging.image import Movie
+    from smili_amd.warmstart.unet import UNet
+    prep, meta = workloads.dynamic_movie(n_frames=16, n_vis=75_000)
+    frames0 = prep.x0.reshape(16, 128, 128) / prep.inormfactr
+    regs = dict(l1_lambda=1.0, tsv_lambda=1e4, rt_lambda=1.0)
+    movie = Movie.from_frames(frames0, dx=2.0, angunit="uas")
+    _, ip = lbfgs3d.imaging3d(movie, vistable=meta["vis"], verbose=False, return_info=True, **regs)
+    net, _ = UNet.from_file(batch=16)
+    try:
+        out, iw = lbfgs3d.imaging3d(movie, vistable=meta["vis"], verbose=False, return_info=True,
+                                    warmstart=net, **regs)
+    finally:
+        net.close()
+    print("plain", ip.niter, ip.cost, "warm", iw.start, iw.niter, iw.cost)
+    assert ip.start == "prior" and iw.start == "proposal"
+    assert iw.cost <= ip.cost * (1 + 1e-4)
+    assert out.frames().shape == (16, 128, 128)
