@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define SMILI_ABI_VERSION 1
+#define SMILI_ABI_VERSION 2
 #define SMILI_NTERMS 15
 
 /* status codes */
@@ -158,6 +158,11 @@ typedef struct smili_tune {
                            or 8 (0 = default, 4) */
   int32_t chisq_threads; /* threads per chi-square (and regularizer)
                            work-group: 256 or 512 (0 = default, 256) */
+  int32_t transform;    /* grid transforms of an image: 1 = LDS FFTs over
+                           the whole fine grid; 2 = pruned DFT products on
+                           the window of fine-grid cells its points touch;
+                           0 (default) = per image, whichever the cost
+                           model finds cheaper */
 } smili_tune;
 
 int smili_abi_version(void);
@@ -236,7 +241,7 @@ int smili_model_vis(smili_problem *p, const double *x, double *vre, double *vim)
 int smili_adjoint_resid(smili_problem *p, const double *vres_re,
                         const double *vres_im, double *out);
 
-/* Kernel stages of one evaluation, in launch order. */
+/* Kernel stages of one evaluation (launch order: see smili_stage comments). */
 enum smili_stage {
   SMILI_S_SCATTER = 0,     /* window pixels -> 2-D image (I1d_I2d_fwd) */
   SMILI_S_T2_ROWS,         /* deconvolve + zero-pad + FFT along x */
@@ -251,6 +256,13 @@ enum smili_stage {
   SMILI_S_REG,             /* regularizers + gradient assembly */
   SMILI_S_FINALIZE,        /* fixed-order sums -> the 15 terms */
   SMILI_S_FRAME_MEAN,      /* movies: mean normalised frame for Ri */
+  /* pruned-DFT transforms (images whose points touch a small window of the
+   * fine grid): they take the place of the four FFT passes, in the same
+   * positions of the launch order */
+  SMILI_S_DFT2_Y,          /* S = Ey I (deconvolution folded into Ey) */
+  SMILI_S_DFT2_X,          /* G = S Ex on the window */
+  SMILI_S_DFT1_X,          /* P = Gs Ex^T from the spread window */
+  SMILI_S_DFT1_Y,          /* Re(Ey^T P) + crop (+ gradient assembly) */
   SMILI_NSTAGES
 };
 

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -127,6 +128,73 @@ static std::vector<int> radices(int n) {
   return r;
 }
 
+// ---------------------------------------------------------------------------
+// pruned DFT: the window of fine-grid cells an image's points touch
+// ---------------------------------------------------------------------------
+// A scaled uv coordinate x (radians per pixel) sits at t = x nf / (2 pi)
+// fine-grid cells; its taps cover cells l0 .. l0 + w - 1, l0 = ceil(t - w/2).
+// The window is [c0, c0 + width) in unwrapped cells: the coordinates are
+// reduced to [-nf/2, nf/2) only when the points span more than a period
+// (then cells c and c + nf of the window are one grid cell, which the
+// periodic DFT phases account for).
+struct TapWindow {
+  long long c0 = 0;
+  int width = 0;
+  bool fold = false;
+};
+
+static long long first_tap(double t) { return (long long)std::ceil(t - 0.5 * kW); }
+static double fold_coord(double t, int nf) { return t - nf * std::floor(t / nf + 0.5); }
+
+static TapWindow tap_window(const double* x, int m, int nf) {
+  TapWindow w;
+  for (int pass = 0; pass < 2; ++pass) {
+    long long lo = 0, hi = 0;
+    for (int j = 0; j < m; ++j) {
+      require(std::isfinite(x[j]), "non-finite uv coordinate");
+      double t = x[j] * nf / (2.0 * M_PI);
+      if (pass) t = fold_coord(t, nf);
+      const long long l = first_tap(t);
+      if (j == 0 || l < lo) lo = l;
+      if (j == 0 || l > hi) hi = l;
+    }
+    w.c0 = lo;
+    w.width = (int)std::min<long long>(hi - lo + kW, 1LL << 30);
+    w.fold = pass == 1;
+    if (hi - lo + kW <= nf + kW - 1) break;
+  }
+  return w;
+}
+
+// Flops of the four DFT products of an nx x ny image on a gw x gh window
+// against those of the two full-grid FFTs; the products run near the
+// FMA rate, the FFTs are latency-bound at these sizes (factor 4).
+static bool dft_cheaper(int nx, int ny, int nf1, int nf2, int gw, int gh) {
+  const double dft = 8.0 * gh * (double)nx * (ny + 2.0 * gw);
+  const double g = (double)nf1 * nf2;
+  return dft <= 4.0 * 10.0 * g * std::log2(g);
+}
+
+// DFT matrix of one axis with the deconvolution factors c:
+// M0[k][a] = c(k) exp(+2 pi i (k - n/2)(c0 + a) / nf), k < n, a < width
+// (row-major [n][width]), followed by its transpose M1 = [width][n].
+// Phases are reduced exactly in integers before the FP64 sine and cosine.
+static size_t dft_axis(int n, int nf, long long c0, int width, const std::vector<double>& c,
+                       std::vector<double2>& out) {
+  const size_t base = out.size();
+  out.resize(base + 2 * (size_t)n * width);
+  for (int k = 0; k < n; ++k)
+    for (int a = 0; a < width; ++a) {
+      long long ph = ((long long)(k - n / 2) * (c0 + a)) % nf;
+      if (ph < 0) ph += nf;
+      const double ang = 2.0 * M_PI * (double)ph / nf;
+      const double2 v = make_double2(c[k] * std::cos(ang), c[k] * std::sin(ang));
+      out[base + (size_t)k * width + a] = v;
+      out[base + (size_t)n * width + (size_t)a * n + k] = v;
+    }
+  return base;
+}
+
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
@@ -206,6 +274,14 @@ struct Plan {
   bool two_branches = false;
 
   std::vector<FftItem> h_t2rows, h_cols, h_t1rows;
+  // pruned-DFT transforms: per unit whether it uses them, the four product
+  // work lists (S = Ey I, G = S Ex, P = Gs ExT, Re(EyT P)) and their matrices
+  std::vector<char> unit_dft;
+  bool any_fft = false, any_dft = false;
+  std::vector<DftItem> h_dft[4];
+  DevBuf<DftItem> d_dft[4];
+  DevBuf<double2> d_dftm;                  // float2 or double2 DFT matrices (sized for double2)
+  DevBuf<double2> d_H;                     // float2 or double2 S / P per DFT unit
   std::vector<SubProb> h_subs;
   std::vector<SpreadItem> h_spread;
   std::vector<Chunk> h_chunks;
@@ -397,6 +473,9 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   int max_nf1 = 0, max_nf2 = 0;
   units.resize(us.size());
   winpix.resize(us.size());
+  unit_dft.assign(us.size(), 0);
+  std::vector<TapWindow> win_x(us.size()), win_y(us.size());
+  std::vector<std::pair<int, int>> gdim(us.size());    // grid (or window) width, height
   npix_total = 0;
   for (size_t ui = 0; ui < us.size(); ++ui) {
     const smili_desc* d = us[ui].d;
@@ -406,21 +485,36 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     U.nf1 = fine_size(d->nx);
     U.nf2 = fine_size(d->ny);
     require(U.nf1 <= 4096 && U.nf2 <= 4096, "fine grid larger than 4096 is not supported");
-    max_nf1 = std::max(max_nf1, U.nf1);
-    max_nf2 = std::max(max_nf2, U.nf2);
     U.npix = d->npix;
+    U.m = us[ui].m;
     bool ident = (long long)d->npix == (long long)d->nx * d->ny;
     for (int i = 0; ident && i < d->npix; ++i)
       ident = d->xidx[i] == i % d->nx + 1 && d->yidx[i] == i / d->nx + 1;
     U.identity = ident ? 1 : 0;
     any_window = any_window || !ident;
-    U.m = us[ui].m;
     U.x_off = (int)npix_total;
     npix_total += d->npix;
     U.fft1 = get_plan(U.nf1);
     U.fft2 = get_plan(U.nf2);
+    // the unit's grid: the whole fine grid (FFT passes) or the window of
+    // cells its points' taps touch (pruned DFT products)
+    bool dft = false;
+    if (U.m > 0 && tune.transform != 1) {
+      win_x[ui] = tap_window(us[ui].u, U.m, U.nf1);
+      win_y[ui] = tap_window(us[ui].v, U.m, U.nf2);
+      dft = tune.transform == 2 ||
+            dft_cheaper(d->nx, d->ny, U.nf1, U.nf2, win_x[ui].width, win_y[ui].width);
+    }
+    unit_dft[ui] = dft;
+    gdim[ui] = dft ? std::make_pair(win_x[ui].width, win_y[ui].width) : std::make_pair(U.nf1, U.nf2);
+    if (U.m > 0 && !dft) {
+      max_nf1 = std::max(max_nf1, U.nf1);
+      max_nf2 = std::max(max_nf2, U.nf2);
+    }
+    any_dft = any_dft || dft;
+    any_fft = any_fft || (U.m > 0 && !dft);
     U.grid_off = grid_total;
-    grid_total += (long long)U.nf1 * U.nf2;
+    grid_total += U.m > 0 ? (long long)gdim[ui].first * gdim[ui].second : 0;
     U.img_off = img_total;
     img_total += (long long)d->nx * d->ny;
     U.cx_off = get_corr(d->nx, U.nf1);
@@ -480,12 +574,27 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     UnitDev& U = units[ui];
     const int m = us[ui].m;
     U.pt_off = (int)npt;
-    const int nbx = (U.nf1 + bin - 1) / bin, nby = (U.nf2 + bin - 1) / bin;
+    const int gw = gdim[ui].first, gh = gdim[ui].second;
+    const int nbx = (gw + bin - 1) / bin, nby = (gh + bin - 1) / bin;
     std::vector<int> l0x(m), l0y(m), bidx(m);
     std::vector<double> exu(m), eyu(m);
     for (int j = 0; j < m; ++j) {
       const double uu = us[ui].u[j], vv = us[ui].v[j];
       require(std::isfinite(uu) && std::isfinite(vv), "non-finite uv coordinate");
+      if (unit_dft[ui]) {
+        // cells relative to the window's first cell
+        auto wcoord = [&](double x, int nf, const TapWindow& w, int& l0, double& e) {
+          double t = x * nf / (2.0 * M_PI);
+          if (w.fold) t = fold_coord(t, nf);
+          const long long l = first_tap(t);
+          e = t - (double)l - 0.5 * kW;                   // in (-1, 0]
+          l0 = (int)(l - w.c0);
+        };
+        wcoord(uu, U.nf1, win_x[ui], l0x[j], exu[j]);
+        wcoord(vv, U.nf2, win_y[ui], l0y[j], eyu[j]);
+        bidx[j] = (l0y[j] / bin) * nbx + (l0x[j] / bin);
+        continue;
+      }
       auto coord = [&](double x, int nf, int& l0, double& e) {
         double tt = std::fmod(x * nf / (2.0 * M_PI), (double)nf);
         if (tt < 0) tt += nf;
@@ -546,8 +655,8 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
         si.gy0 = byk * kCellBlock;
         si.pt_begin = (int)U.pt_off + s;
         si.pt_end = (int)U.pt_off + e;
-        si.nf1 = U.nf1;
-        si.nf2 = U.nf2;
+        si.nf1 = gw;
+        si.nf2 = gh;
         si.grid_off = U.grid_off;
         h_spread.push_back(si);
         s = e;
@@ -556,8 +665,8 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     for (int b = 0; b < nbx * nby; ++b) {
       for (int s = count[b]; s < count[b + 1]; s += sub_points) {
         SubProb sp{};
-        sp.nf1 = U.nf1;
-        sp.nf2 = U.nf2;
+        sp.nf1 = gw;
+        sp.nf2 = gh;
         sp.grid_off = U.grid_off;
         sp.bin_x = b % nbx;
         sp.bin_y = b / nbx;
@@ -688,8 +797,8 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   if (col_group == 0) {
     const size_t csz = fp64 ? sizeof(double2) : sizeof(float2);
     long long ncols = 0;
-    for (const UnitDev& U : units)
-      if (U.m > 0) ncols += U.nf1;
+    for (size_t ui = 0; ui < units.size(); ++ui)
+      if (units[ui].m > 0 && !unit_dft[ui]) ncols += units[ui].nf1;
     col_group = 4;
     while (col_group > 1 && ((2ull * col_group * (max_nf2 + 1) + max_nf2) * csz > 65536 ||
                              ncols / col_group < kColGroupsMin))
@@ -727,6 +836,7 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
       return f;
     };
     if (U.m == 0) continue;     // a frame without data: no transforms, zero chi-square gradient
+    if (unit_dft[ui]) continue; // pruned DFT products (below)
     // a grid smaller than the plan's largest takes proportionally more rows
     // / columns per work-group: the same points per work-group, so the
     // threads sized for the largest grid stay busy and the LDS sized for it
@@ -743,6 +853,70 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     for (int s = 0; s < U.nf1; s += cc) h_cols.push_back(fft_item(s, std::min(cc, U.nf1 - s)));
   }
   unit_reg_begin[units.size()] = (int)h_regs.size();
+  // pruned-DFT products: matrices (shared by units of one geometry and
+  // window) and work items, ordered by output column block so that the
+  // items reading one block of B meet on one XCD (k_dft maps items to XCDs
+  // in contiguous ranges)
+  std::vector<double2> dftm;
+  long long h_total = 0;
+  {
+    std::map<std::tuple<int, int, long long, int>, size_t> axis_of;
+    auto get_axis = [&](int n, int nf, long long c0, int width) {
+      const auto key = std::make_tuple(n, nf, c0, width);
+      auto it = axis_of.find(key);
+      if (it != axis_of.end()) return it->second;
+      const size_t off = dft_axis(n, nf, c0, width, es_correction(n, nf), dftm);
+      axis_of[key] = off;
+      return off;
+    };
+    auto tiles = [&](int stage, int rows, const DftItem& base) {
+      for (int n0 = 0; n0 < base.N; n0 += 64)
+        for (int m0 = 0; m0 < base.M; m0 += rows) {
+          DftItem d = base;
+          d.m0 = m0;
+          d.n0 = n0;
+          h_dft[stage].push_back(d);
+        }
+    };
+    for (size_t ui = 0; ui < units.size(); ++ui) {
+      if (!unit_dft[ui]) continue;
+      const UnitDev& U = units[ui];
+      const int gw = gdim[ui].first, gh = gdim[ui].second;
+      const size_t ox = get_axis(U.nx, U.nf1, win_x[ui].c0, gw);   // Ex [nx][gw], ExT [gw][nx]
+      const size_t oy = get_axis(U.ny, U.nf2, win_y[ui].c0, gh);   // EyT [ny][gh], Ey [gh][ny]
+      const long long h_off = h_total;
+      h_total += (long long)gh * U.nx;
+      DftItem b{};
+      b.x_off = U.x_off;
+      b.identity = U.identity;
+      b.nx = U.nx;
+      b.img_off = U.img_off;
+      // S = Ey I
+      DftItem t = b;
+      t.M = gh; t.N = U.nx; t.K = U.ny; t.lda = U.ny; t.ldb = U.nx; t.ldc = U.nx;
+      t.a_off = (long long)(oy + (size_t)U.ny * gh);
+      t.bsel = U.identity ? 0 : 1;
+      t.b_off = U.identity ? U.x_off : U.img_off;
+      t.c_off = h_off;
+      tiles(0, ShapeT2Y::rows, t);
+      // G = S Ex (and the spreading window cleared)
+      t = b;
+      t.M = gh; t.N = gw; t.K = U.nx; t.lda = U.nx; t.ldb = gw; t.ldc = gw;
+      t.a_off = h_off; t.b_off = (long long)ox; t.c_off = U.grid_off;
+      tiles(1, ShapeT2X::rows, t);
+      // P = Gs ExT
+      t = b;
+      t.M = gh; t.N = U.nx; t.K = gw; t.lda = gw; t.ldb = U.nx; t.ldc = U.nx;
+      t.a_off = U.grid_off; t.b_off = (long long)(ox + (size_t)U.nx * gw); t.c_off = h_off;
+      tiles(2, ShapeT1X::rows, t);
+      // grad = Re(EyT P) (and the spreading window cleared)
+      t = b;
+      t.M = U.ny; t.N = U.nx; t.K = gh; t.lda = gh; t.ldb = U.nx;
+      t.a_off = (long long)oy; t.b_off = h_off;
+      t.z_off = U.grid_off; t.zn = gw * gh;
+      tiles(3, ShapeT1Y::rows, t);
+    }
+  }
   for (size_t pi = 0; pi < ps.size(); ++pi) {
     probs[pi].reg_begin = unit_reg_begin[ps[pi].unit_begin];
     probs[pi].reg_end = unit_reg_begin[ps[pi].unit_end];
@@ -856,6 +1030,16 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   d_cols.upload(h_cols);
   d_t1rows.upload(h_t1rows);
   d_regs.upload(h_regs);
+  for (int k = 0; k < 4; ++k) d_dft[k].upload(h_dft[k]);
+  if (fp64) {
+    d_dftm.upload(dftm);
+  } else {
+    std::vector<float2> f(dftm.size());
+    for (size_t i = 0; i < dftm.size(); ++i) f[i] = make_float2((float)dftm[i].x, (float)dftm[i].y);
+    d_dftm.alloc((f.size() + 1) / 2);
+    if (!f.empty()) HIP_CHECK(hipMemcpy(d_dftm.p, f.data(), f.size() * sizeof(float2), hipMemcpyHostToDevice));
+  }
+  d_H.alloc(h_total);
   HIP_CHECK(hipDeviceSynchronize());
 }
 
@@ -895,12 +1079,53 @@ static void launch_forward_t(Plan& P, bool with_scatter) {
     LAUNCH(SMILI_S_SCATTER, k_scatter, P.h_regs.size(), kThreads, 0, P.d_regs.p, P.d_units.p, P.d_pix.p, P.d_x.p,
            P.d_I2d.p);
   if (P.npt == 0) return;
-  LAUNCH(SMILI_S_T2_ROWS, k_t2_rows<R>, P.h_t2rows.size(), P.threads_rows, P.lds_rows, P.d_t2rows.p, tw, corr,
-         P.d_x.p, P.d_I2d.p, gA);
-  LAUNCH(SMILI_S_T2_COLS, k_t2_cols<R>, xcd_grid(P.h_cols.size()), P.threads_cols, P.lds_cols, P.d_cols.p,
-         (int)P.h_cols.size(), tw, gA);
+  if (P.any_fft) {
+    LAUNCH(SMILI_S_T2_ROWS, k_t2_rows<R>, P.h_t2rows.size(), P.threads_rows, P.lds_rows, P.d_t2rows.p, tw, corr,
+           P.d_x.p, P.d_I2d.p, gA);
+    LAUNCH(SMILI_S_T2_COLS, k_t2_cols<R>, xcd_grid(P.h_cols.size()), P.threads_cols, P.lds_cols, P.d_cols.p,
+           (int)P.h_cols.size(), tw, gA);
+  }
+  if (P.any_dft) {
+    const V* M = reinterpret_cast<const V*>(P.d_dftm.p);
+    V* H = reinterpret_cast<V*>(P.d_H.p);
+    LAUNCH(SMILI_S_DFT2_Y, (k_dft<R, ShapeT2Y::nw, ShapeT2Y::rb, ShapeT2Y::ks, true, DFT_STORE>), xcd_grid(P.h_dft[0].size()),
+           ShapeT2Y::threads, 0, P.d_dft[0].p, (int)P.h_dft[0].size(), M, (const V*)nullptr, P.d_x.p, P.d_I2d.p, H,
+           (V*)nullptr, (double*)nullptr, (double*)nullptr, (const double*)nullptr, (const int*)nullptr, 0);
+    LAUNCH(SMILI_S_DFT2_X, (k_dft<R, ShapeT2X::nw, ShapeT2X::rb, ShapeT2X::ks, false, DFT_STORE>),
+           xcd_grid(P.h_dft[1].size()), ShapeT2X::threads, 0, P.d_dft[1].p, (int)P.h_dft[1].size(), (const V*)H, M,
+           (const double*)nullptr, (const double*)nullptr, gA, (V*)nullptr, (double*)nullptr, (double*)nullptr,
+           (const double*)nullptr, (const int*)nullptr, 0);
+  }
   LAUNCH(SMILI_S_INTERP, k_interp<R>, P.h_subs.size(), kThreads, P.lds_tile, P.d_subs.p, gA, P.d_cell.p, ex, ey,
          Vv, P.d_Vres.p, P.bin, beta);
+}
+
+// the last pass of the type-1 transform (after the regularizer branch has
+// joined when `assemble`: it adds the chi-square gradient to gradreg)
+template <typename R>
+static void launch_rows_t(Plan& P, bool assemble = true) {
+  using V = typename Cx<R>::T;
+  hipStream_t stream = P.stream;
+  const bool prof = P.prof;
+  auto& prof_ev = P.prof_ev;
+  auto& prof_used = P.prof_used;
+  int& launches = P.launches;
+  if (P.npt == 0) return;
+  const V* tw = reinterpret_cast<const V*>(P.fp64 ? (const void*)P.d_twd.p : (const void*)P.d_tw.p);
+  const R* corr = reinterpret_cast<const R*>(P.fp64 ? (const void*)P.d_corrd.p : (const void*)P.d_corr.p);
+  V* gA = reinterpret_cast<V*>(P.d_gridA.p);
+  V* gB = reinterpret_cast<V*>(P.d_gridB.p);
+  if (P.any_fft)
+    LAUNCH(SMILI_S_T1_ROWS, k_t1_rows<R>, P.h_t1rows.size(), P.threads_rows, P.lds_rows, P.d_t1rows.p, tw, corr,
+           gA, P.d_grad2d.p, P.d_grad.p, P.d_greg.p, P.d_pixinv.p, assemble ? 1 : 0);
+  if (P.any_dft) {
+    const V* M = reinterpret_cast<const V*>(P.d_dftm.p);
+    const V* H = reinterpret_cast<const V*>(P.d_H.p);
+    LAUNCH(SMILI_S_DFT1_Y, (k_dft<R, ShapeT1Y::nw, ShapeT1Y::rb, ShapeT1Y::ks, false, DFT_GRAD>), xcd_grid(P.h_dft[3].size()),
+           ShapeT1Y::threads, 0, P.d_dft[3].p, (int)P.h_dft[3].size(), M, H, (const double*)nullptr,
+           (const double*)nullptr, (V*)nullptr, gB, P.d_grad2d.p, P.d_grad.p, P.d_greg.p,
+           P.d_pixinv.p, assemble ? 1 : 0);
+  }
 }
 
 // rows: 0 = spreading and column pass only, 1 = + row pass into grad2d,
@@ -928,28 +1153,18 @@ static void launch_adjoint_t(Plan& P, int rows, const FinArgs* fin = nullptr) {
   else
     LAUNCH(SMILI_S_SPREAD, k_spread<R>, P.h_spread.size(), kThreads, 0, P.d_spread.p, P.d_cell.p, ex, ey,
            P.d_Vres.p, gB, beta);
-  LAUNCH(SMILI_S_T1_COLS, k_t1_cols<R>, xcd_grid(P.h_cols.size()), P.threads_cols, P.lds_cols, P.d_cols.p,
-         (int)P.h_cols.size(), tw, gB, gA);
-  if (rows)
-    LAUNCH(SMILI_S_T1_ROWS, k_t1_rows<R>, P.h_t1rows.size(), P.threads_rows, P.lds_rows, P.d_t1rows.p, tw, corr,
-           gA, P.d_grad2d.p, P.d_grad.p, P.d_greg.p, P.d_pixinv.p, rows == 2 ? 1 : 0);
-}
-
-// the row pass alone (after the regularizer branch has joined)
-template <typename R>
-static void launch_rows_t(Plan& P) {
-  using V = typename Cx<R>::T;
-  hipStream_t stream = P.stream;
-  const bool prof = P.prof;
-  auto& prof_ev = P.prof_ev;
-  auto& prof_used = P.prof_used;
-  int& launches = P.launches;
-  if (P.npt == 0) return;
-  const V* tw = reinterpret_cast<const V*>(P.fp64 ? (const void*)P.d_twd.p : (const void*)P.d_tw.p);
-  const R* corr = reinterpret_cast<const R*>(P.fp64 ? (const void*)P.d_corrd.p : (const void*)P.d_corr.p);
-  V* gA = reinterpret_cast<V*>(P.d_gridA.p);
-  LAUNCH(SMILI_S_T1_ROWS, k_t1_rows<R>, P.h_t1rows.size(), P.threads_rows, P.lds_rows, P.d_t1rows.p, tw, corr,
-         gA, P.d_grad2d.p, P.d_grad.p, P.d_greg.p, P.d_pixinv.p, 1);
+  if (P.any_fft)
+    LAUNCH(SMILI_S_T1_COLS, k_t1_cols<R>, xcd_grid(P.h_cols.size()), P.threads_cols, P.lds_cols, P.d_cols.p,
+           (int)P.h_cols.size(), tw, gB, gA);
+  if (P.any_dft) {
+    const V* M = reinterpret_cast<const V*>(P.d_dftm.p);
+    V* H = reinterpret_cast<V*>(P.d_H.p);
+    LAUNCH(SMILI_S_DFT1_X, (k_dft<R, ShapeT1X::nw, ShapeT1X::rb, ShapeT1X::ks, false, DFT_STORE>), xcd_grid(P.h_dft[2].size()),
+           ShapeT1X::threads, 0, P.d_dft[2].p, (int)P.h_dft[2].size(), (const V*)gB, M, (const double*)nullptr,
+           (const double*)nullptr, H, (V*)nullptr, (double*)nullptr, (double*)nullptr, (const double*)nullptr,
+           (const int*)nullptr, 0);
+  }
+  if (rows) launch_rows_t<R>(P, rows == 2);
 }
 
 void Plan::launch_forward() {

@@ -72,6 +72,21 @@ struct FftItem {
   int maxr, pad;                     // largest power-of-two radix of the LDS FFT
   long long grid_off, img_off;
 };
+// Pruned-DFT product work item: one tile of C = A B (A complex, M x K,
+// leading dimension lda; B real or complex, K x N, ldb; C, ldc), rows
+// [m0, m0 + tile rows), columns [n0, n0 + 64).  bsel picks the real B
+// operand (0: x, 1: the scattered 2-D image).  The gradient epilogue writes
+// Re C[m][n] as image pixel (ix = n, iy = m) of the unit at x_off / img_off
+// and clears the zn cells of the unit's spreading window at z_off.
+struct DftItem {
+  int m0, n0, M, N, K, lda, ldb, ldc;
+  long long a_off, b_off, c_off, img_off, z_off;
+  int x_off, identity, nx, bsel, zn, pad;
+};
+// Grid of a unit as the interpolation and spreading kernels see it: nf1 x
+// nf2 cells with periodic wrap (the full fine grid of the FFT path, or the
+// compact window of fine-grid cells the unit's points touch in the pruned
+// DFT path, where no tap reaches past the window's edge).
 struct SubProb { int bin_x, bin_y, start, count, nf1, nf2; long long grid_off; };
 struct RegItem { int unit, start, count, pad; };
 // spreading: up to kSpreadPoints consecutive points (sorted by cell) of one

@@ -8,6 +8,7 @@
 // src/wrapfinufft/wrapfinufft.f90:94-206 (both transforms use the +i sign).
 #include "smili_kernels.h"
 #include "smili_nufft.hip"
+#include "smili_dft.hip"
 
 namespace smili {
 
