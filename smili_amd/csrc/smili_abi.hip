@@ -106,6 +106,19 @@ static std::vector<double> es_correction(int n, int nf) {
   return c;
 }
 
+// ES kernel weights of one point along one axis: e = t - l0 - w/2 in
+// (-1, 0] (t its fine-grid coordinate, l0 its first tap), z_q = (e + w/2 -
+// q) 2 / w, phi_q = exp(-beta z^2 / (1 + sqrt(1 - z^2))) (the
+// cancellation-free form of exp(beta (sqrt(1 - z^2) - 1))), q < w.
+static void es_taps(double e, double* wt) {
+  const double inv_half = 2.0 / kW;
+  for (int q = 0; q < kW; ++q) {
+    const double z = e * inv_half + (1.0 - q * inv_half);
+    const double z2 = z * z < 1.0 ? z * z : 1.0;
+    wt[q] = std::exp(-kBeta * z2 / (1.0 + std::sqrt(1.0 - z2)));
+  }
+}
+
 static int next_smooth(int n) {
   for (;; ++n) {
     int m = n;
@@ -297,8 +310,10 @@ struct Plan {
   DevBuf<int> d_pix;
   DevBuf<double> d_wgt;
   DevBuf<int> d_cell;
-  DevBuf<float> d_ex, d_ey;
-  DevBuf<double> d_exd, d_eyd;
+  DevBuf<double> d_wts;                    // float or double ES weights, [2 kW][npt] (sized for double)
+  DevBuf<double2> d_sres;                  // chi-square chunks' slot shares of the residual vectors
+  DevBuf<int4> d_pref;                     // per point: first slot, number of slots, offset in refs
+  DevBuf<int> d_refs;
   DevBuf<double2> d_V;                     // float2 or double2 visibilities (sized for double2)
   DevBuf<double2> d_Vres;
   DevBuf<double2> d_gridA, d_gridB;        // float2 or double2 grids (sized for double2)
@@ -566,7 +581,6 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
 
   // points: grid coordinates, binning (counting sort), subproblems
   std::vector<int> cell;
-  std::vector<float> ex, ey;
   std::vector<double> exd, eyd;
   std::vector<std::vector<int>> unit_rank(us.size());   // local uv index -> local sorted position
   npt = 0;
@@ -634,8 +648,6 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
       const int j = order[s];
       unit_rank[ui][j] = s;
       cell.push_back(l0x[j] | (l0y[j] << 16));
-      ex.push_back((float)exu[j]);
-      ey.push_back((float)eyu[j]);
       exd.push_back(exu[j]);
       eyd.push_back(eyu[j]);
     }
@@ -789,6 +801,20 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
       ca.push_back(CaItem{D.ca[i], D.wca / D.varca[i], {sl[0], sl[1], sl[2], sl[3]}});
     });
     shift_chunks(DT_CA, ca0);
+  }
+
+  // each point's references into the chunks' slot shares, in chunk order
+  std::vector<int4> pref(std::max<size_t>(npt, 1), make_int4(0, 0, 0, 0));
+  std::vector<int> refs;
+  {
+    std::vector<std::vector<int>> of(npt);
+    for (const Chunk& c : h_chunks)
+      for (int k = 0; k < c.nslots; ++k) of[h_slots[c.slot_off + k]].push_back(c.slot_off + k);
+    for (size_t j = 0; j < npt; ++j) {
+      const auto& r = of[j];
+      pref[j] = make_int4(r.empty() ? 0 : r[0], (int)r.size(), (int)refs.size(), 0);
+      for (size_t k = 1; k < r.size(); ++k) refs.push_back(r[k]);
+    }
   }
 
   // column work-groups: the largest of 4, 2, 1 columns whose LDS fits and
@@ -985,10 +1011,28 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   d_corrd.upload(corrd);
   d_pix.upload(pix);
   d_cell.upload(cell);
-  d_ex.upload(ex);
-  d_ey.upload(ey);
-  d_exd.upload(exd);
-  d_eyd.upload(eyd);
+  {
+    // ES weights, per tap
+    std::vector<double> w(2 * kW * npt);
+    double t[kW];
+    for (size_t j = 0; j < npt; ++j) {
+      es_taps(exd[j], t);
+      for (int q = 0; q < kW; ++q) w[q * npt + j] = t[q];
+      es_taps(eyd[j], t);
+      for (int q = 0; q < kW; ++q) w[(kW + q) * npt + j] = t[q];
+    }
+    if (fp64) {
+      d_wts.upload(w);
+    } else {
+      std::vector<float> f(w.begin(), w.end());
+      d_wts.alloc((f.size() + 1) / 2);
+      if (!f.empty()) HIP_CHECK(hipMemcpy(d_wts.p, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+  }
+  d_sres.alloc(std::max<size_t>(h_slots.size(), 1));
+  d_sres.zero();
+  d_pref.upload(pref);
+  d_refs.upload(refs.empty() ? std::vector<int>{0} : refs);
   d_V.alloc(std::max<size_t>(npt, 1));
   d_Vres.alloc(std::max<size_t>(npt, 1));
   d_Vres.zero();
@@ -1068,11 +1112,9 @@ static void launch_forward_t(Plan& P, bool with_scatter) {
   auto& prof_ev = P.prof_ev;
   auto& prof_used = P.prof_used;
   int& launches = P.launches;
-  const R beta = (R)kBeta;
   const V* tw = reinterpret_cast<const V*>(P.fp64 ? (const void*)P.d_twd.p : (const void*)P.d_tw.p);
   const R* corr = reinterpret_cast<const R*>(P.fp64 ? (const void*)P.d_corrd.p : (const void*)P.d_corr.p);
-  const R* ex = reinterpret_cast<const R*>(P.fp64 ? (const void*)P.d_exd.p : (const void*)P.d_ex.p);
-  const R* ey = reinterpret_cast<const R*>(P.fp64 ? (const void*)P.d_eyd.p : (const void*)P.d_ey.p);
+  const R* wts = reinterpret_cast<const R*>(P.d_wts.p);
   V* gA = reinterpret_cast<V*>(P.d_gridA.p);
   V* Vv = reinterpret_cast<V*>(P.d_V.p);
   if (P.any_window && with_scatter)
@@ -1096,8 +1138,8 @@ static void launch_forward_t(Plan& P, bool with_scatter) {
            (const double*)nullptr, (const double*)nullptr, gA, (V*)nullptr, (double*)nullptr, (double*)nullptr,
            (const double*)nullptr, (const int*)nullptr, 0);
   }
-  LAUNCH(SMILI_S_INTERP, k_interp<R>, P.h_subs.size(), kThreads, P.lds_tile, P.d_subs.p, gA, P.d_cell.p, ex, ey,
-         Vv, P.d_Vres.p, P.bin, beta);
+  LAUNCH(SMILI_S_INTERP, k_interp<R>, P.h_subs.size(), kThreads, P.lds_tile, P.d_subs.p, gA, P.d_cell.p, wts,
+         (long long)P.npt, Vv, P.bin);
 }
 
 // the last pass of the type-1 transform (after the regularizer branch has
@@ -1130,9 +1172,11 @@ static void launch_rows_t(Plan& P, bool assemble = true) {
 
 // rows: 0 = spreading and column pass only, 1 = + row pass into grad2d,
 // 2 = + row pass adding the chi-square gradient to gradreg into grad.
-// fin: the final sums ride on the spreading launch.
+// fin: the final sums ride on the spreading launch.  from_chisq: the
+// residuals are the chi-square chunks' slot shares (an evaluation), else
+// the caller's Vres (the bare adjoint).
 template <typename R>
-static void launch_adjoint_t(Plan& P, int rows, const FinArgs* fin = nullptr) {
+static void launch_adjoint_t(Plan& P, int rows, const FinArgs* fin = nullptr, bool from_chisq = true) {
   using V = typename Cx<R>::T;
   hipStream_t stream = P.stream;
   const bool prof = P.prof;
@@ -1140,19 +1184,19 @@ static void launch_adjoint_t(Plan& P, int rows, const FinArgs* fin = nullptr) {
   auto& prof_used = P.prof_used;
   int& launches = P.launches;
   if (P.npt == 0) return;
-  const R beta = (R)kBeta;
   const V* tw = reinterpret_cast<const V*>(P.fp64 ? (const void*)P.d_twd.p : (const void*)P.d_tw.p);
   const R* corr = reinterpret_cast<const R*>(P.fp64 ? (const void*)P.d_corrd.p : (const void*)P.d_corr.p);
-  const R* ex = reinterpret_cast<const R*>(P.fp64 ? (const void*)P.d_exd.p : (const void*)P.d_ex.p);
-  const R* ey = reinterpret_cast<const R*>(P.fp64 ? (const void*)P.d_eyd.p : (const void*)P.d_ey.p);
+  const R* wts = reinterpret_cast<const R*>(P.d_wts.p);
+  const ResidSrc res{P.d_sres.p, P.d_pref.p, P.d_refs.p, from_chisq ? nullptr : P.d_Vres.p};
+  const long long npt = (long long)P.npt;
   V* gA = reinterpret_cast<V*>(P.d_gridA.p);
   V* gB = reinterpret_cast<V*>(P.d_gridB.p);
   if (fin)
     LAUNCH(SMILI_S_SPREAD, k_spread_fin<R>, P.h_spread.size() + P.probs.size(), kThreads, 0,
-           (int)P.h_spread.size(), P.d_spread.p, P.d_cell.p, ex, ey, P.d_Vres.p, gB, beta, *fin);
+           (int)P.h_spread.size(), P.d_spread.p, P.d_cell.p, wts, npt, res, gB, *fin);
   else
-    LAUNCH(SMILI_S_SPREAD, k_spread<R>, P.h_spread.size(), kThreads, 0, P.d_spread.p, P.d_cell.p, ex, ey,
-           P.d_Vres.p, gB, beta);
+    LAUNCH(SMILI_S_SPREAD, k_spread<R>, P.h_spread.size(), kThreads, 0, P.d_spread.p, P.d_cell.p, wts, npt, res,
+           gB);
   if (P.any_fft)
     LAUNCH(SMILI_S_T1_COLS, k_t1_cols<R>, xcd_grid(P.h_cols.size()), P.threads_cols, P.lds_cols, P.d_cols.p,
            (int)P.h_cols.size(), tw, gB, gA);
@@ -1173,8 +1217,8 @@ void Plan::launch_forward() {
 }
 
 void Plan::launch_adjoint() {
-  if (fp64) launch_adjoint_t<double>(*this, 1);
-  else launch_adjoint_t<float>(*this, 1);
+  if (fp64) launch_adjoint_t<double>(*this, 1, nullptr, false);
+  else launch_adjoint_t<float>(*this, 1, nullptr, false);
 }
 
 // One evaluation.  Default (one stream):
@@ -1227,18 +1271,18 @@ void Plan::launch_eval() {
     if (fp64) {
       if (fused)
         LAUNCH(SMILI_S_CHISQ, k_chisq_reg<double2>, grid, chisq_threads, 0, nch, d_chunks.p, d_slots.p, d_fcv.p,
-               d_amp.p, d_cp.p, d_ca.p, d_V.p, d_Vres.p, d_cpart.p, reg_args);
+               d_amp.p, d_cp.p, d_ca.p, d_V.p, d_sres.p, d_cpart.p, reg_args);
       else
         LAUNCH(SMILI_S_CHISQ, k_chisq<double2>, grid, chisq_threads, 0, d_chunks.p, d_slots.p, d_fcv.p, d_amp.p,
-               d_cp.p, d_ca.p, d_V.p, d_Vres.p, d_cpart.p);
+               d_cp.p, d_ca.p, d_V.p, d_sres.p, d_cpart.p);
     } else {
       const float2* Vf = reinterpret_cast<const float2*>(d_V.p);
       if (fused)
         LAUNCH(SMILI_S_CHISQ, k_chisq_reg<float2>, grid, chisq_threads, 0, nch, d_chunks.p, d_slots.p, d_fcv.p, d_amp.p,
-               d_cp.p, d_ca.p, Vf, d_Vres.p, d_cpart.p, reg_args);
+               d_cp.p, d_ca.p, Vf, d_sres.p, d_cpart.p, reg_args);
       else
         LAUNCH(SMILI_S_CHISQ, k_chisq<float2>, grid, chisq_threads, 0, d_chunks.p, d_slots.p, d_fcv.p, d_amp.p,
-               d_cp.p, d_ca.p, Vf, d_Vres.p, d_cpart.p);
+               d_cp.p, d_ca.p, Vf, d_sres.p, d_cpart.p);
     }
   }
   if (fork) {

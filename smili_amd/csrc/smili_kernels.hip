@@ -48,14 +48,20 @@ __device__ __forceinline__ void block_sum(double* v, double* red) {
 // chi-square terms and residual vectors (fftlib.f90:384-655).  One work item
 // = `count` consecutive data of one type of one problem that reference
 // `nslots` distinct uv points.  Their model visibilities are staged in LDS
-// (FP64), the residual vectors are accumulated in LDS with FP64 LDS atomics
-// and flushed with one global FP64 atomic per distinct point.  The
+// (FP64) and each point's share of the residual vector is accumulated in
+// LDS, then written with plain stores to the chunk's own slot range of
+// `sres` (slot_off + slot).  No two chunks write the same slot, so there are
+// no global atomics; the spreading kernel sums a point's slots (its
+// references, in a fixed order) when it reads the residual.  The
 // chi-square of the item is written to partial[blockIdx.x].
+//
+// Every datum's contribution to one of its points is that point's fixed
+// vector times a datum scalar, so a point needs one real accumulator:
+//   amp:  (-2 r w / |V|) V                      (fftlib.f90:484-489)
+//   cp:   (-2 sin r w sgn / |V|^2) (-Im V, Re V) (fftlib.f90:644-652)
+//   ca:   (+-2 r w / |V|^2) V                    (fftlib.f90:556-571)
+// Full complex visibilities need two (fftlib.f90:425-433).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void lds_add2(double2* R, int s, double re, double im) {
-  atomicAdd(&R[s].x, re);
-  atomicAdd(&R[s].y, im);
-}
 
 // Closure data are stored by timestamp, and consecutive closures of one
 // timestamp share baselines: 64 of them in one wave would hit the same few
@@ -86,57 +92,56 @@ template <typename VT, bool LEAN>
 __device__ void chisq_block(int bid, const Chunk* __restrict__ chunks, const int* __restrict__ slots,
                             const FcvItem* __restrict__ fcv, const AmpItem* __restrict__ amp,
                             const CpItem* __restrict__ cp, const CaItem* __restrict__ ca,
-                            const VT* __restrict__ V, double2* __restrict__ Vres,
+                            const VT* __restrict__ V, double2* __restrict__ sres,
                             double* __restrict__ partial) {
   __shared__ double2 Vl[kMaxSlots];
-  __shared__ double2 Rl[kMaxSlots];
   __shared__ double2 Al[kMaxSlots];      // per-point quantities shared by the chunk's data
+  __shared__ double Sl[2][kMaxSlots];    // per-point accumulators
   __shared__ double red[kChisqThreadsMax / 64];
   const Chunk ch = chunks[bid];
   const int* sl = slots + ch.slot_off;
+  double2* out = sres + ch.slot_off;
   if (ch.direct) {
-    // one datum per uv point: no staging, the residual goes straight to
-    // the point (the same arithmetic as the staged path below)
+    // one datum per uv point: no staging, the datum's share goes straight
+    // to its slot (the same arithmetic as the staged path below)
     double acc = 0.0;
     for (int i = threadIdx.x; i < ch.count; i += blockDim.x) {
       if (ch.type == DT_FCV) {
         const FcvItem d = fcv[ch.start + i];
-        const int j = sl[d.slot & 0x7fff];
-        const VT mv = V[j];
+        const int s = d.slot & 0x7fff;
+        const VT mv = V[sl[s]];
         const double sg = (d.slot & kSlotNeg) ? -1.0 : 1.0;
         const double rr = d.vr - (double)mv.x;
         const double ri = d.vi - sg * (double)mv.y;
         acc += (rr * rr + ri * ri) * d.w;
         const double f = -2.0 * d.w;
-        unsafeAtomicAdd(&Vres[j].x, f * rr);
-        unsafeAtomicAdd(&Vres[j].y, f * ri * sg);
+        out[s] = make_double2(f * rr, f * ri * sg);
       } else {
         const AmpItem d = amp[ch.start + i];
-        const int j = sl[d.slot & 0x7fff];
-        const VT mv = V[j];
+        const int s = d.slot & 0x7fff;
+        const VT mv = V[sl[s]];
         const double mx = (double)mv.x, my = (double)mv.y;
         const double mod = sqrt(mx * mx + my * my);
         const double r = d.amp - mod;
         acc += r * r * d.w;
         const double f = -2.0 * r * d.w * (1.0 / mod);
-        unsafeAtomicAdd(&Vres[j].x, f * mx);
-        unsafeAtomicAdd(&Vres[j].y, f * my);
+        out[s] = make_double2(f * mx, f * my);
       }
     }
     double v[1] = {acc};
-    __shared__ double dred[kChisqThreadsMax / 64];
-    block_sum<1, kChisqThreadsMax / 64>(v, dred);
+    block_sum<1, kChisqThreadsMax / 64>(v, red);
     if (threadIdx.x == 0) partial[bid] = v[0];
     return;
   }
   // stage the chunk's model visibilities and, once per point rather than
-  // once per reference: amp |V| and 1/|V|; closure phase arg V and 1/|V|^2;
+  // once per reference: amp 1/|V|; closure phase arg V and 1/|V|^2;
   // closure amplitude log|V|^2 and 1/|V|^2
   for (int s = threadIdx.x; s < ch.nslots; s += blockDim.x) {
     const VT mv = V[sl[s]];
     const double2 m = make_double2((double)mv.x, (double)mv.y);
     Vl[s] = m;
-    Rl[s] = make_double2(0.0, 0.0);
+    Sl[0][s] = 0.0;
+    Sl[1][s] = 0.0;
     const double a2 = m.x * m.x + m.y * m.y;
     if (ch.type == DT_AMP) {
       const double mod = sqrt(a2);
@@ -159,18 +164,17 @@ __device__ void chisq_block(int bid, const Chunk* __restrict__ chunks, const int
       const double ri = d.vi - sg * m.y;               // conj for negative index
       acc += (rr * rr + ri * ri) * d.w;
       const double f = -2.0 * d.w;
-      lds_add2(Rl, s, f * rr, f * ri * sg);
+      atomicAdd(&Sl[0][s], f * rr);
+      atomicAdd(&Sl[1][s], f * ri * sg);
     }
   } else if (ch.type == DT_AMP) {
     for (int i = threadIdx.x; i < ch.count; i += blockDim.x) {
       const AmpItem d = amp[ch.start + i];
       const int s = d.slot & 0x7fff;
-      const double2 m = Vl[s];
       const double2 al = Al[s];
       const double r = d.amp - al.x;
       acc += r * r * d.w;
-      const double f = -2.0 * r * d.w * al.y;
-      lds_add2(Rl, s, f * m.x, f * m.y);
+      atomicAdd(&Sl[0][s], -2.0 * r * d.w * al.y);
     }
   } else if (ch.type == DT_CP) {
     const int q = (ch.count + 63) >> 6;
@@ -178,16 +182,13 @@ __device__ void chisq_block(int bid, const Chunk* __restrict__ chunks, const int
       const int i = closure_item(j, q);
       if (i >= ch.count) continue;
       const CpItem d = cp[ch.start + i];
-      double2 m[3], al[3];
       double sg[3], model = 0.0;
       int s[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         s[k] = d.slot[k] & 0x7fff;
         sg[k] = (d.slot[k] & kSlotNeg) ? -1.0 : 1.0;
-        m[k] = Vl[s[k]];
-        al[k] = Al[s[k]];
-        model += al[k].x * sg[k];
+        model += Al[s[k]].x * sg[k];
       }
       const double r = d.cp - model;
       double sr, cr;
@@ -198,10 +199,7 @@ __device__ void chisq_block(int bid, const Chunk* __restrict__ chunks, const int
       acc += 2.0 * (1.0 - cr) * d.w;
       const double f = -2.0 * sr * d.w;
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const double a = f * al[k].y * sg[k];
-        lds_add2(Rl, s[k], -a * m[k].y, a * m[k].x);
-      }
+      for (int k = 0; k < 3; ++k) atomicAdd(&Sl[0][s[k]], f * sg[k]);
     }
   } else {
     const int q = (ch.count + 63) >> 6;
@@ -209,32 +207,33 @@ __device__ void chisq_block(int bid, const Chunk* __restrict__ chunks, const int
       const int i = closure_item(j, q);
       if (i >= ch.count) continue;
       const CaItem d = ca[ch.start + i];
-      double2 m[4], al[4];
+      double l[4];
       int s[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         s[k] = d.slot[k] & 0x7fff;       // signs are ignored (fftlib.f90:549-553)
-        m[k] = Vl[s[k]];
-        al[k] = Al[s[k]];
+        l[k] = Al[s[k]].x;
       }
       // log|V| = 0.5 log|V|^2
-      const double model = 0.5 * (al[0].x + al[1].x - al[2].x - al[3].x);
+      const double model = 0.5 * (l[0] + l[1] - l[2] - l[3]);
       const double r = d.ca - model;
       acc += r * r * d.w;
       const double f = -2.0 * r * d.w;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const double a = (k < 2 ? f : -f) * al[k].y;
-        lds_add2(Rl, s[k], a * m[k].x, a * m[k].y);
-      }
+      for (int k = 0; k < 4; ++k) atomicAdd(&Sl[0][s[k]], k < 2 ? f : -f);
     }
   }
   __syncthreads();
+  // each point's share: its accumulator(s) times its fixed vector
   for (int s = threadIdx.x; s < ch.nslots; s += blockDim.x) {
-    const double2 r = Rl[s];
-    const int j = sl[s];
-    unsafeAtomicAdd(&Vres[j].x, r.x);
-    unsafeAtomicAdd(&Vres[j].y, r.y);
+    const double2 m = Vl[s];
+    const double a = Sl[0][s];
+    double2 r;
+    if (ch.type == DT_FCV) r = make_double2(a, Sl[1][s]);
+    else if (ch.type == DT_AMP) r = make_double2(a * m.x, a * m.y);
+    else if (ch.type == DT_CP) r = make_double2(-a * Al[s].y * m.y, a * Al[s].y * m.x);
+    else r = make_double2(a * Al[s].y * m.x, a * Al[s].y * m.y);
+    out[s] = r;
   }
   double v[1] = {acc};
   block_sum<1, kChisqThreadsMax / 64>(v, red);
@@ -246,9 +245,9 @@ __global__ void __launch_bounds__(kChisqThreadsMax)
 k_chisq(const Chunk* __restrict__ chunks, const int* __restrict__ slots,
         const FcvItem* __restrict__ fcv, const AmpItem* __restrict__ amp,
         const CpItem* __restrict__ cp, const CaItem* __restrict__ ca,
-        const VT* __restrict__ V, double2* __restrict__ Vres,
+        const VT* __restrict__ V, double2* __restrict__ sres,
         double* __restrict__ partial) {
-  chisq_block<VT, true>(blockIdx.x, chunks, slots, fcv, amp, cp, ca, V, Vres, partial);
+  chisq_block<VT, true>(blockIdx.x, chunks, slots, fcv, amp, cp, ca, V, sres, partial);
 }
 
 // ---------------------------------------------------------------------------
@@ -603,9 +602,9 @@ __global__ void __launch_bounds__(kChisqThreadsMax)
 k_chisq_reg(int nchunks, const Chunk* __restrict__ chunks, const int* __restrict__ slots,
             const FcvItem* __restrict__ fcv, const AmpItem* __restrict__ amp,
             const CpItem* __restrict__ cp, const CaItem* __restrict__ ca,
-            const VT* __restrict__ V, double2* __restrict__ Vres,
+            const VT* __restrict__ V, double2* __restrict__ sres,
             double* __restrict__ partial, RegArgs A) {
-  if ((int)blockIdx.x < nchunks) chisq_block<VT, false>(blockIdx.x, chunks, slots, fcv, amp, cp, ca, V, Vres, partial);
+  if ((int)blockIdx.x < nchunks) chisq_block<VT, false>(blockIdx.x, chunks, slots, fcv, amp, cp, ca, V, sres, partial);
   else reg_block<false>(blockIdx.x - nchunks, A);
 }
 
@@ -704,8 +703,8 @@ __global__ void __launch_bounds__(kThreads) k_finalize(FinArgs F) {
 template <typename R>
 __global__ void __launch_bounds__(kThreads)
 k_spread_fin(int nspread, const SpreadItem* __restrict__ items, const int* __restrict__ pt_cell,
-             const R* __restrict__ pt_ex, const R* __restrict__ pt_ey, const double2* __restrict__ Vres,
-             typename Cx<R>::T* __restrict__ gridB, R beta, FinArgs F) {
+             const R* __restrict__ wts, long long npt, ResidSrc res,
+             typename Cx<R>::T* __restrict__ gridB, FinArgs F) {
   __shared__ double red[16 * (kThreads / 64)];
   // the finalize work-groups first: dispatched with the launch, they
   // finish under the spreading work instead of trailing it
@@ -713,7 +712,7 @@ k_spread_fin(int nspread, const SpreadItem* __restrict__ items, const int* __res
   if ((int)blockIdx.x < nfin)
     finalize_problem(F.probs[blockIdx.x], F.units, F.reg_items, F.cpart, F.rpart, F.mom, F.terms, red);
   else
-    spread_block<R>(blockIdx.x - nfin, items, pt_cell, pt_ex, pt_ey, Vres, gridB, beta);
+    spread_block<R>(blockIdx.x - nfin, items, pt_cell, wts, npt, res, gridB);
 }
 
 }  // namespace smili

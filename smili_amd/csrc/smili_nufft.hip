@@ -153,25 +153,42 @@ __device__ __forceinline__ void stage_twiddles(V* dst, const V* __restrict__ tw,
   for (int t = threadIdx.x; t < n; t += blockDim.x) dst[t] = tw[off + t];
 }
 
-// ---------------------------------------------------------------------------
-// ES kernel weights.  e = d - w/2 in (-1, 0] where d is the distance from the
-// point to its first tap in cells; z_q = (d - q) * 2 / w; phi(z) =
-// exp(-beta z^2 / (1 + sqrt(1 - z^2))) (cancellation-free form of
-// exp(beta (sqrt(1 - z^2) - 1))).
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ float es_exp(float a) { return __expf(a); }
-__device__ __forceinline__ double es_exp(double a) { return exp(a); }
-
-template <typename R> __device__ __forceinline__ R es_tap(R e, R beta, int q) {
-  const R inv_half = (R)2 / (R)kW;
-  const R z = e * inv_half + ((R)1 - (R)q * inv_half);
-  const R z2 = z * z < (R)1 ? z * z : (R)1;
-  return es_exp(-beta * z2 / ((R)1 + sqrt((R)1 - z2)));
+// ES kernel weights.  The points are fixed for a plan's lifetime, so their
+// ES weights are computed once on the host (FP64) and stored per tap:
+// wts[q * npt + j] = phi_x,q of point j, wts[(kW + q) * npt + j] = phi_y,q.
+// Both transforms read them (coalesced over consecutive points) instead of
+// evaluating 14 exp / sqrt / divisions per point each.
+template <typename R>
+__device__ __forceinline__ void load_weights(const R* __restrict__ wts, long long npt, int j, R* wx, R* wy) {
+#pragma unroll
+  for (int q = 0; q < kW; ++q) {
+    wx[q] = wts[q * npt + j];
+    wy[q] = wts[(kW + q) * npt + j];
+  }
 }
 
-template <typename R> __device__ __forceinline__ void es_weights(R e, R beta, R* wt) {
-#pragma unroll
-  for (int q = 0; q < kW; ++q) wt[q] = es_tap(e, beta, q);
+// Residual of each point for the type-1 transform: the sum of the
+// chi-square chunks' slot shares that reference the point (pref[j] = first
+// slot, number of slots, offset of the others in refs; summed in that fixed
+// order), or, for the bare adjoint, the caller's Vres[j].
+struct ResidSrc {
+  const double2* sres;
+  const int4* pref;
+  const int* refs;
+  const double2* Vres;
+};
+
+__device__ __forceinline__ double2 point_resid(const ResidSrc& S, int j) {
+  if (S.Vres) return S.Vres[j];
+  const int4 p = S.pref[j];
+  double2 r = make_double2(0.0, 0.0);
+  if (p.y > 0) r = S.sres[p.x];
+  for (int e = 1; e < p.y; ++e) {
+    const double2 t = S.sres[S.refs[p.z + e - 1]];
+    r.x += t.x;
+    r.y += t.y;
+  }
+  return r;
 }
 
 // ---------------------------------------------------------------------------
@@ -270,15 +287,13 @@ k_t2_cols(const FftItem* __restrict__ items, int nitems, const typename Cx<R>::T
 
 // ---------------------------------------------------------------------------
 // Type-2 step 3: interpolation from an LDS-staged tile of the grid.  One work
-// item = up to sub_points points of one bin.  Also clears the residual
-// accumulators of its points for the chi-square kernel.
+// item = up to sub_points points of one bin.
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ void __launch_bounds__(kThreads)
 k_interp(const SubProb* __restrict__ sps, const typename Cx<R>::T* __restrict__ gridA,
-         const int* __restrict__ pt_cell, const R* __restrict__ pt_ex,
-         const R* __restrict__ pt_ey, typename Cx<R>::T* __restrict__ Vout,
-         double2* __restrict__ Vres, int bin, R beta) {
+         const int* __restrict__ pt_cell, const R* __restrict__ wts, long long npt,
+         typename Cx<R>::T* __restrict__ Vout, int bin) {
   using V = typename Cx<R>::T;
   extern __shared__ __align__(16) unsigned char smraw[];
   V* tile = reinterpret_cast<V*>(smraw);
@@ -299,8 +314,7 @@ k_interp(const SubProb* __restrict__ sps, const typename Cx<R>::T* __restrict__ 
     const int lx = (cell & 0xffff) - gx0;
     const int ly = (cell >> 16) - gy0;
     R wx[kW], wy[kW];
-    es_weights(pt_ex[j], beta, wx);
-    es_weights(pt_ey[j], beta, wy);
+    load_weights(wts, npt, j, wx, wy);
     R ar = 0, ai = 0;
 #pragma unroll
     for (int qy = 0; qy < kW; ++qy) {
@@ -316,7 +330,6 @@ k_interp(const SubProb* __restrict__ sps, const typename Cx<R>::T* __restrict__ 
       ai = fma(wy[qy], ri, ai);
     }
     Vout[j] = cmk<V>(ar, ai);
-    Vres[j] = make_double2(0.0, 0.0);
   }
 }
 
@@ -339,9 +352,8 @@ constexpr int kSpreadBatch = SMILI_SPREAD_BATCH;
 
 template <typename R>
 __device__ void spread_block(int bid, const SpreadItem* __restrict__ items, const int* __restrict__ pt_cell,
-                             const R* __restrict__ pt_ex, const R* __restrict__ pt_ey,
-                             const double2* __restrict__ Vres, typename Cx<R>::T* __restrict__ gridB,
-                             R beta) {
+                             const R* __restrict__ wts, long long npt, const ResidSrc& res,
+                             typename Cx<R>::T* __restrict__ gridB) {
   using V = typename Cx<R>::T;
   constexpr int T = kCellBlock + kW - 1;
   constexpr int NW = kThreads / 64;
@@ -357,14 +369,13 @@ __device__ void spread_block(int bid, const SpreadItem* __restrict__ items, cons
   if (lane < np) {
     const int j = base + lane;
     R wx[kW], wy[kW];
-    es_weights(pt_ex[j], beta, wx);
-    es_weights(pt_ey[j], beta, wy);
+    load_weights(wts, npt, j, wx, wy);
 #pragma unroll
     for (int q = 0; q < kW; ++q) {
       wxs[w][q][lane] = wx[q];
       wys[w][q][lane] = wy[q];
     }
-    const double2 r = Vres[j];
+    const double2 r = point_resid(res, j);
     cs[w][lane] = cmk<V>((R)r.x, (R)(-r.y));
     const int cell = pt_cell[j];
     cl[w][lane] = ((cell >> 16) - it.gy0) * T + ((cell & 0xffff) - it.gx0);
@@ -424,9 +435,8 @@ __device__ void spread_block(int bid, const SpreadItem* __restrict__ items, cons
 template <typename R>
 __global__ void __launch_bounds__(kThreads)
 k_spread(const SpreadItem* __restrict__ items, const int* __restrict__ pt_cell,
-         const R* __restrict__ pt_ex, const R* __restrict__ pt_ey,
-         const double2* __restrict__ Vres, typename Cx<R>::T* __restrict__ gridB, R beta) {
-  spread_block<R>(blockIdx.x, items, pt_cell, pt_ex, pt_ey, Vres, gridB, beta);
+         const R* __restrict__ wts, long long npt, ResidSrc res, typename Cx<R>::T* __restrict__ gridB) {
+  spread_block<R>(blockIdx.x, items, pt_cell, wts, npt, res, gridB);
 }
 
 // ---------------------------------------------------------------------------
