@@ -312,7 +312,7 @@ struct Plan {
   DevBuf<int> d_cell;
   DevBuf<double> d_wts;                    // float or double ES weights, [2 kW][npt] (sized for double)
   DevBuf<double2> d_sres;                  // chi-square chunks' slot shares of the residual vectors
-  DevBuf<int4> d_pref;                     // per point: first slot, number of slots, offset in refs
+  DevBuf<int4> d_pref;                     // per point: first two slots, number of slots, offset in refs
   DevBuf<int> d_refs;
   DevBuf<double2> d_V;                     // float2 or double2 visibilities (sized for double2)
   DevBuf<double2> d_Vres;
@@ -722,6 +722,7 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     Q = ProbDev{};
     Q.unit_begin = P.unit_begin;
     Q.unit_end = P.unit_end;
+    for (int ui = P.unit_begin; ui < P.unit_end; ++ui) Q.any_moments |= units[ui].need_moments;
     Q.n[DT_FCV] = D.nfcv; Q.n[DT_AMP] = D.namp; Q.n[DT_CP] = D.ncp; Q.n[DT_CA] = D.nca;
     Q.w[DT_FCV] = D.wfcv; Q.w[DT_AMP] = D.wamp; Q.w[DT_CP] = D.wcp; Q.w[DT_CA] = D.wca;
     Q.terms_off = (int)nterms_total;
@@ -786,11 +787,12 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
       for (int c = Q.chunk_begin[type]; c < Q.chunk_end[type]; ++c) h_chunks[c].start += (int)base;
     };
     build_type(DT_FCV, D.nfcv, 1, D.uvidxfcv, [&](long long i, const unsigned short* sl) {
-      fcv.push_back(FcvItem{D.vfcv_re[i], D.vfcv_im[i], D.wfcv / D.varfcv[i], sl[0], 0, 0});
+      fcv.push_back(FcvItem{D.vfcv_re[i], D.vfcv_im[i], D.wfcv / D.varfcv[i], sl[0], 0,
+                            np[std::abs(D.uvidxfcv[i]) - 1]});
     });
     shift_chunks(DT_FCV, fcv0);
     build_type(DT_AMP, D.namp, 1, D.uvidxamp, [&](long long i, const unsigned short* sl) {
-      amp.push_back(AmpItem{D.vamp[i], D.wamp / D.varamp[i], sl[0], 0, 0});
+      amp.push_back(AmpItem{D.vamp[i], D.wamp / D.varamp[i], sl[0], 0, np[std::abs(D.uvidxamp[i]) - 1]});
     });
     shift_chunks(DT_AMP, amp0);
     build_type(DT_CP, D.ncp, 3, D.uvidxcp, [&](long long i, const unsigned short* sl) {
@@ -812,8 +814,8 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
       for (int k = 0; k < c.nslots; ++k) of[h_slots[c.slot_off + k]].push_back(c.slot_off + k);
     for (size_t j = 0; j < npt; ++j) {
       const auto& r = of[j];
-      pref[j] = make_int4(r.empty() ? 0 : r[0], (int)r.size(), (int)refs.size(), 0);
-      for (size_t k = 1; k < r.size(); ++k) refs.push_back(r[k]);
+      pref[j] = make_int4(r.size() > 0 ? r[0] : 0, r.size() > 1 ? r[1] : 0, (int)r.size(), (int)refs.size());
+      for (size_t k = 2; k < r.size(); ++k) refs.push_back(r[k]);
     }
   }
 
@@ -1240,7 +1242,7 @@ void Plan::launch_eval() {
   const hipStream_t side_stream = fork ? side : stream;
   const RegArgs reg_args{d_regs.p, d_units.p, d_pix.p, d_x.p, d_I2d.p, d_mom.p, d_movies.p,
                          d_greg.p, d_grad.p, d_rpart.p};
-  const FinArgs fin_args{d_probs.p, d_units.p, d_regs.p, d_cpart.p, d_rpart.p, d_mom.p, d_terms.p};
+  const FinArgs fin_args{d_probs.p, d_cpart.p, d_rpart.p, d_mom.p, d_terms.p};
   const bool fused = !two_branches && npt > 0;
   if (any_window)
     LAUNCH(SMILI_S_SCATTER, k_scatter, h_regs.size(), kThreads, 0, d_regs.p, d_units.p, d_pix.p, d_x.p, d_I2d.p);

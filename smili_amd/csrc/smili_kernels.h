@@ -53,8 +53,10 @@ struct UnitDev {
 // sits at (-u, -v)) in bit 15.  w = fnorm / variance.
 constexpr int kMaxSlots = 1024;      // distinct uv points per chunk
 constexpr int kSlotNeg = 0x8000;
-struct FcvItem { double vr, vi, w; unsigned short slot, pad0; int pad1; };
-struct AmpItem { double amp, w; unsigned short slot, pad0; int pad1; };
+// pt: the datum's global point (the direct path reads V[pt] without the
+// slot table)
+struct FcvItem { double vr, vi, w; unsigned short slot, pad0; int pt; };
+struct AmpItem { double amp, w; unsigned short slot, pad0; int pt; };
 struct CpItem  { double cp, w; unsigned short slot[3], pad0; };
 struct CaItem  { double ca, w; unsigned short slot[4]; };
 
@@ -117,10 +119,14 @@ struct ProbDev {
   int terms_off;
   int nterms;                        // 15, or 18 for a movie
   int lc_mode;                       // movie in light-curve mode: tfd terms are the light curve
+  int any_moments;                   // some unit of the problem has tfd / cen / sm
 };
 
+// the unit's image moments (k_moments) and, from them, its tfd / cen / sm
+// costs (image.f90:111-135) scaled by reg_scale (k_moments_reduce)
 struct UnitMoments {
   double S, A, Sx, Sy, Sxx, Syy, Sxy, P, Px, Py;
+  double tfd_c, cen_c, sm_c;
 };
 
 }  // namespace smili

@@ -16,23 +16,28 @@ namespace smili {
 // block reduction of doubles (deterministic: fixed shuffle tree + fixed
 // wave order)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
-}
-
 // red must hold NV * NW doubles; blockDim.x is a multiple of 64 and at
 // most 64 NW.  The result is valid in thread 0.
 template <int NV, int NW = kThreads / 64>
 __device__ __forceinline__ void block_sum(double* v, double* red) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int nw = blockDim.x >> 6;
+  // the NV shuffle trees advance together: one LDS round trip per level
+  // for all values instead of one per value and level
+  double s[NV];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    double s = wave_sum(v[k]);
-    if (lane == 0) red[k * NW + wid] = s;
+  for (int k = 0; k < NV; ++k) s[k] = v[k];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    double t[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) t[k] = __shfl_down(s[k], off, 64);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) s[k] += t[k];
   }
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) red[k * NW + wid] = s[k];
   __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
@@ -81,6 +86,14 @@ __device__ __noinline__ double arg_nl(double y, double x) { return atan2(y, x); 
 __device__ __noinline__ double log_nl(double x) { return log(x); }
 __device__ __noinline__ void sincos_nl(double r, double& s, double& c) { sincos(r, &s, &c); }
 
+// Each thread loads kChisqBatch of its data before it processes them, so
+// a wave keeps that many item loads in flight (measured neutral at 1, 2, 4
+// and 8 on the 256^2 workload; kept at 4).
+#ifndef SMILI_CHISQ_BATCH
+#define SMILI_CHISQ_BATCH 4
+#endif
+constexpr int kChisqBatch = SMILI_CHISQ_BATCH;
+
 // LEAN: the rare or heavy math (FP64 transcendentals here, the tfd / cen /
 // sm gradients of the regularizers) is called out of line, so the kernel
 // needs about half the registers and a CU holds twice the work-groups.
@@ -88,12 +101,14 @@ __device__ __noinline__ void sincos_nl(double r, double& s, double& c) { sincos(
 // of large plans (movie 0.258 -> 0.252 ms); slower inside the fused
 // chi-square + regularizer launch of a single problem (C2 0.1045 ->
 // 0.1055 ms), which keeps the inline code.
+
 template <typename VT, bool LEAN>
 __device__ void chisq_block(int bid, const Chunk* __restrict__ chunks, const int* __restrict__ slots,
                             const FcvItem* __restrict__ fcv, const AmpItem* __restrict__ amp,
                             const CpItem* __restrict__ cp, const CaItem* __restrict__ ca,
                             const VT* __restrict__ V, double2* __restrict__ sres,
                             double* __restrict__ partial) {
+  constexpr int NB = kChisqBatch;
   __shared__ double2 Vl[kMaxSlots];
   __shared__ double2 Al[kMaxSlots];      // per-point quantities shared by the chunk's data
   __shared__ double Sl[2][kMaxSlots];    // per-point accumulators
@@ -101,31 +116,50 @@ __device__ void chisq_block(int bid, const Chunk* __restrict__ chunks, const int
   const Chunk ch = chunks[bid];
   const int* sl = slots + ch.slot_off;
   double2* out = sres + ch.slot_off;
+  const int nt = blockDim.x;
+  double acc = 0.0;
   if (ch.direct) {
     // one datum per uv point: no staging, the datum's share goes straight
     // to its slot (the same arithmetic as the staged path below)
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < ch.count; i += blockDim.x) {
+    for (int i0 = threadIdx.x; i0 < ch.count; i0 += nt * NB) {
       if (ch.type == DT_FCV) {
-        const FcvItem d = fcv[ch.start + i];
-        const int s = d.slot & 0x7fff;
-        const VT mv = V[sl[s]];
-        const double sg = (d.slot & kSlotNeg) ? -1.0 : 1.0;
-        const double rr = d.vr - (double)mv.x;
-        const double ri = d.vi - sg * (double)mv.y;
-        acc += (rr * rr + ri * ri) * d.w;
-        const double f = -2.0 * d.w;
-        out[s] = make_double2(f * rr, f * ri * sg);
+        FcvItem d[NB];
+        VT mv[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+          if (i0 + b * nt < ch.count) d[b] = fcv[ch.start + i0 + b * nt];
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+          if (i0 + b * nt < ch.count) mv[b] = V[d[b].pt];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          if (i0 + b * nt >= ch.count) break;
+          const double sg = (d[b].slot & kSlotNeg) ? -1.0 : 1.0;
+          const double rr = d[b].vr - (double)mv[b].x;
+          const double ri = d[b].vi - sg * (double)mv[b].y;
+          acc += (rr * rr + ri * ri) * d[b].w;
+          const double f = -2.0 * d[b].w;
+          out[d[b].slot & 0x7fff] = make_double2(f * rr, f * ri * sg);
+        }
       } else {
-        const AmpItem d = amp[ch.start + i];
-        const int s = d.slot & 0x7fff;
-        const VT mv = V[sl[s]];
-        const double mx = (double)mv.x, my = (double)mv.y;
-        const double mod = sqrt(mx * mx + my * my);
-        const double r = d.amp - mod;
-        acc += r * r * d.w;
-        const double f = -2.0 * r * d.w * (1.0 / mod);
-        out[s] = make_double2(f * mx, f * my);
+        AmpItem d[NB];
+        VT mv[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+          if (i0 + b * nt < ch.count) d[b] = amp[ch.start + i0 + b * nt];
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+          if (i0 + b * nt < ch.count) mv[b] = V[d[b].pt];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          if (i0 + b * nt >= ch.count) break;
+          const double mx = (double)mv[b].x, my = (double)mv[b].y;
+          const double mod = sqrt(mx * mx + my * my);
+          const double r = d[b].amp - mod;
+          acc += r * r * d[b].w;
+          const double f = -2.0 * r * d[b].w * (1.0 / mod);
+          out[d[b].slot & 0x7fff] = make_double2(f * mx, f * my);
+        }
       }
     }
     double v[1] = {acc};
@@ -136,7 +170,7 @@ __device__ void chisq_block(int bid, const Chunk* __restrict__ chunks, const int
   // stage the chunk's model visibilities and, once per point rather than
   // once per reference: amp 1/|V|; closure phase arg V and 1/|V|^2;
   // closure amplitude log|V|^2 and 1/|V|^2
-  for (int s = threadIdx.x; s < ch.nslots; s += blockDim.x) {
+  for (int s = threadIdx.x; s < ch.nslots; s += nt) {
     const VT mv = V[sl[s]];
     const double2 m = make_double2((double)mv.x, (double)mv.y);
     Vl[s] = m;
@@ -153,79 +187,112 @@ __device__ void chisq_block(int bid, const Chunk* __restrict__ chunks, const int
     }
   }
   __syncthreads();
-  double acc = 0.0;
   if (ch.type == DT_FCV) {
-    for (int i = threadIdx.x; i < ch.count; i += blockDim.x) {
-      const FcvItem d = fcv[ch.start + i];
-      const int s = d.slot & 0x7fff;
-      const double sg = (d.slot & kSlotNeg) ? -1.0 : 1.0;
-      const double2 m = Vl[s];
-      const double rr = d.vr - m.x;
-      const double ri = d.vi - sg * m.y;               // conj for negative index
-      acc += (rr * rr + ri * ri) * d.w;
-      const double f = -2.0 * d.w;
-      atomicAdd(&Sl[0][s], f * rr);
-      atomicAdd(&Sl[1][s], f * ri * sg);
+    for (int i0 = threadIdx.x; i0 < ch.count; i0 += nt * NB) {
+      FcvItem d[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+        if (i0 + b * nt < ch.count) d[b] = fcv[ch.start + i0 + b * nt];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        if (i0 + b * nt >= ch.count) break;
+        const int s = d[b].slot & 0x7fff;
+        const double sg = (d[b].slot & kSlotNeg) ? -1.0 : 1.0;
+        const double2 m = Vl[s];
+        const double rr = d[b].vr - m.x;
+        const double ri = d[b].vi - sg * m.y;           // conj for negative index
+        acc += (rr * rr + ri * ri) * d[b].w;
+        const double f = -2.0 * d[b].w;
+        atomicAdd(&Sl[0][s], f * rr);
+        atomicAdd(&Sl[1][s], f * ri * sg);
+      }
     }
   } else if (ch.type == DT_AMP) {
-    for (int i = threadIdx.x; i < ch.count; i += blockDim.x) {
-      const AmpItem d = amp[ch.start + i];
-      const int s = d.slot & 0x7fff;
-      const double2 al = Al[s];
-      const double r = d.amp - al.x;
-      acc += r * r * d.w;
-      atomicAdd(&Sl[0][s], -2.0 * r * d.w * al.y);
+    for (int i0 = threadIdx.x; i0 < ch.count; i0 += nt * NB) {
+      AmpItem d[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+        if (i0 + b * nt < ch.count) d[b] = amp[ch.start + i0 + b * nt];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        if (i0 + b * nt >= ch.count) break;
+        const int s = d[b].slot & 0x7fff;
+        const double2 al = Al[s];
+        const double r = d[b].amp - al.x;
+        acc += r * r * d[b].w;
+        atomicAdd(&Sl[0][s], -2.0 * r * d[b].w * al.y);
+      }
     }
   } else if (ch.type == DT_CP) {
     const int q = (ch.count + 63) >> 6;
-    for (int j = threadIdx.x; j < 64 * q; j += blockDim.x) {
-      const int i = closure_item(j, q);
-      if (i >= ch.count) continue;
-      const CpItem d = cp[ch.start + i];
-      double sg[3], model = 0.0;
-      int s[3];
+    for (int j0 = threadIdx.x; j0 < 64 * q; j0 += nt * NB) {
+      CpItem d[NB];
+      bool ok[NB];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        s[k] = d.slot[k] & 0x7fff;
-        sg[k] = (d.slot[k] & kSlotNeg) ? -1.0 : 1.0;
-        model += Al[s[k]].x * sg[k];
+      for (int b = 0; b < NB; ++b) {
+        const int j = j0 + b * nt;
+        const int i = closure_item(j, q);
+        ok[b] = j < 64 * q && i < ch.count;
+        if (ok[b]) d[b] = cp[ch.start + i];
       }
-      const double r = d.cp - model;
-      double sr, cr;
-      if (LEAN)
-        sincos_nl(r, sr, cr);
-      else
-        sincos(r, &sr, &cr);
-      acc += 2.0 * (1.0 - cr) * d.w;
-      const double f = -2.0 * sr * d.w;
 #pragma unroll
-      for (int k = 0; k < 3; ++k) atomicAdd(&Sl[0][s[k]], f * sg[k]);
+      for (int b = 0; b < NB; ++b) {
+        if (!ok[b]) continue;
+        double sg[3], model = 0.0;
+        int s[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          s[k] = d[b].slot[k] & 0x7fff;
+          sg[k] = (d[b].slot[k] & kSlotNeg) ? -1.0 : 1.0;
+          model += Al[s[k]].x * sg[k];
+        }
+        const double r = d[b].cp - model;
+        double sr, cr;
+        if (LEAN)
+          sincos_nl(r, sr, cr);
+        else
+          sincos(r, &sr, &cr);
+        acc += 2.0 * (1.0 - cr) * d[b].w;
+        const double f = -2.0 * sr * d[b].w;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) atomicAdd(&Sl[0][s[k]], f * sg[k]);
+      }
     }
   } else {
     const int q = (ch.count + 63) >> 6;
-    for (int j = threadIdx.x; j < 64 * q; j += blockDim.x) {
-      const int i = closure_item(j, q);
-      if (i >= ch.count) continue;
-      const CaItem d = ca[ch.start + i];
-      double l[4];
-      int s[4];
+    for (int j0 = threadIdx.x; j0 < 64 * q; j0 += nt * NB) {
+      CaItem d[NB];
+      bool ok[NB];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        s[k] = d.slot[k] & 0x7fff;       // signs are ignored (fftlib.f90:549-553)
-        l[k] = Al[s[k]].x;
+      for (int b = 0; b < NB; ++b) {
+        const int j = j0 + b * nt;
+        const int i = closure_item(j, q);
+        ok[b] = j < 64 * q && i < ch.count;
+        if (ok[b]) d[b] = ca[ch.start + i];
       }
-      // log|V| = 0.5 log|V|^2
-      const double model = 0.5 * (l[0] + l[1] - l[2] - l[3]);
-      const double r = d.ca - model;
-      acc += r * r * d.w;
-      const double f = -2.0 * r * d.w;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) atomicAdd(&Sl[0][s[k]], k < 2 ? f : -f);
+      for (int b = 0; b < NB; ++b) {
+        if (!ok[b]) continue;
+        double l[4];
+        int s[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          s[k] = d[b].slot[k] & 0x7fff;     // signs are ignored (fftlib.f90:549-553)
+          l[k] = Al[s[k]].x;
+        }
+        // log|V| = 0.5 log|V|^2
+        const double model = 0.5 * (l[0] + l[1] - l[2] - l[3]);
+        const double r = d[b].ca - model;
+        acc += r * r * d[b].w;
+        const double f = -2.0 * r * d[b].w;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) atomicAdd(&Sl[0][s[k]], k < 2 ? f : -f);
+      }
     }
   }
   __syncthreads();
   // each point's share: its accumulator(s) times its fixed vector
-  for (int s = threadIdx.x; s < ch.nslots; s += blockDim.x) {
+  for (int s = threadIdx.x; s < ch.nslots; s += nt) {
     const double2 m = Vl[s];
     const double a = Sl[0][s];
     double2 r;
@@ -309,14 +376,31 @@ k_moments(const RegItem* __restrict__ items, const UnitDev* __restrict__ units,
 }
 
 // one block per unit: sum its items' moments
+__device__ __noinline__ void unit_moment_costs(const UnitDev& U, const UnitMoments& M, double& tfd, double& cen,
+                                               double& smc);
+
+// one block per unit: sum its items' moments, then its tfd / cen / sm costs
 __global__ void k_moments_reduce(const UnitDev* __restrict__ units, const int* __restrict__ unit_reg_begin,
                                  const double* __restrict__ mpart, UnitMoments* __restrict__ mom) {
+  __shared__ double m[kMomTerms];
   const int u = blockIdx.x;
   const int b = unit_reg_begin[u], e = unit_reg_begin[u + 1];
   if (threadIdx.x < kMomTerms) {
     double s = 0.0;
     for (int i = b; i < e; ++i) s += mpart[(long long)i * kMomTerms + threadIdx.x];
+    m[threadIdx.x] = s;
     reinterpret_cast<double*>(&mom[u])[threadIdx.x] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    UnitMoments M{};
+    double* mp = reinterpret_cast<double*>(&M);
+    for (int k = 0; k < kMomTerms; ++k) mp[k] = m[k];
+    double tfd = 0.0, cen = 0.0, smc = 0.0;
+    if (units[u].need_moments) unit_moment_costs(units[u], M, tfd, cen, smc);
+    mom[u].tfd_c = tfd;
+    mom[u].cen_c = cen;
+    mom[u].sm_c = smc;
   }
 }
 
@@ -575,8 +659,10 @@ __device__ __forceinline__ void reg_block_unit(int bid, const RegArgs& A, const 
     else grad[U.x_off + k] = 0.0 + gr;
   }
   block_sum<kRegTerms, kChisqThreadsMax / 64>(v, red);
+  // l1 .. gs partials scaled here by reg_scale (the sums read them as is);
+  // Rt and Ri are the movie's terms, not scaled
   if (threadIdx.x == 0)
-    for (int k = 0; k < kRegTerms; ++k) rpart[(long long)bid * kRegTerms + k] = v[k];
+    for (int k = 0; k < kRegTerms; ++k) rpart[(long long)bid * kRegTerms + k] = k < 5 ? U.reg_scale * v[k] : v[k];
 }
 
 // The unit's descriptor is read in place (LEAN) or copied to registers
@@ -632,24 +718,35 @@ __device__ __noinline__ void unit_moment_costs(const UnitDev& U, const UnitMomen
   }
 }
 
-__device__ void finalize_problem(const ProbDev& P, const UnitDev* __restrict__ units,
-                                 const RegItem* __restrict__ reg_items, const double* cpart,
-                                 const double* rpart, const UnitMoments* __restrict__ mom,
-                                 double* __restrict__ terms, double* red) {
-  double v[16];
+// The problem's chunks are one contiguous range in type order (fcv, amp,
+// cp, ca) and its regularizer items another; each thread sums a strided
+// share of both, then one block reduction.  Every load depends only on the
+// problem descriptor, so the sums are two memory round trips deep.
+__device__ void finalize_problem(const ProbDev& P, const double* cpart, const double* rpart,
+                                 const UnitMoments* __restrict__ mom, double* __restrict__ terms, double* red) {
+  double v[14];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) v[k] = 0.0;
-  for (int t = 0; t < 4; ++t)
-    for (int i = P.chunk_begin[t] + threadIdx.x; i < P.chunk_end[t]; i += blockDim.x) v[t] += cpart[i];
-  for (int i = P.reg_begin + threadIdx.x; i < P.reg_end; i += blockDim.x) {
-    const double sc = units[reg_items[i].unit].reg_scale;
-    for (int k = 0; k < 5; ++k) v[4 + k] += sc * rpart[(long long)i * kRegTerms + k];
-    v[12] += rpart[(long long)i * kRegTerms + 5];
-    v[13] += rpart[(long long)i * kRegTerms + 6];
+  for (int k = 0; k < 14; ++k) v[k] = 0.0;
+  for (int i = P.chunk_begin[0] + threadIdx.x; i < P.chunk_end[3]; i += blockDim.x) {
+    const int t = i < P.chunk_end[0] ? 0 : i < P.chunk_end[1] ? 1 : i < P.chunk_end[2] ? 2 : 3;
+    const double c = cpart[i];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (k == t) v[k] += c;
   }
-  // tfd / cen / sm costs per unit
-  for (int u = P.unit_begin + threadIdx.x; u < P.unit_end; u += blockDim.x)
-    if (units[u].need_moments) unit_moment_costs(units[u], mom[u], v[9], v[10], v[11]);
+  for (int i = P.reg_begin + threadIdx.x; i < P.reg_end; i += blockDim.x) {
+    const double* r = rpart + (long long)i * kRegTerms;
+    for (int k = 0; k < 5; ++k) v[4 + k] += r[k];
+    v[12] += r[5];
+    v[13] += r[6];
+  }
+  // tfd / cen / sm costs per unit (k_moments_reduce)
+  if (P.any_moments)
+    for (int u = P.unit_begin + threadIdx.x; u < P.unit_end; u += blockDim.x) {
+      v[9] += mom[u].tfd_c;
+      v[10] += mom[u].cen_c;
+      v[11] += mom[u].sm_c;
+    }
   block_sum<14>(v, red);
   if (threadIdx.x == 0) {
     double* T = terms + P.terms_off;
@@ -683,8 +780,6 @@ __device__ void finalize_problem(const ProbDev& P, const UnitDev* __restrict__ u
 
 struct FinArgs {
   const ProbDev* probs;
-  const UnitDev* units;
-  const RegItem* reg_items;
   const double* cpart;
   const double* rpart;
   const UnitMoments* mom;
@@ -694,7 +789,8 @@ struct FinArgs {
 // one block per problem
 __global__ void __launch_bounds__(kThreads) k_finalize(FinArgs F) {
   __shared__ double red[16 * (kThreads / 64)];
-  finalize_problem(F.probs[blockIdx.x], F.units, F.reg_items, F.cpart, F.rpart, F.mom, F.terms, red);
+  const ProbDev P = F.probs[blockIdx.x];
+  finalize_problem(P, F.cpart, F.rpart, F.mom, F.terms, red);
 }
 
 // Spreading work items followed by one finalize work-group per problem: the
@@ -709,9 +805,10 @@ k_spread_fin(int nspread, const SpreadItem* __restrict__ items, const int* __res
   // the finalize work-groups first: dispatched with the launch, they
   // finish under the spreading work instead of trailing it
   const int nfin = (int)gridDim.x - nspread;
-  if ((int)blockIdx.x < nfin)
-    finalize_problem(F.probs[blockIdx.x], F.units, F.reg_items, F.cpart, F.rpart, F.mom, F.terms, red);
-  else
+  if ((int)blockIdx.x < nfin) {
+    const ProbDev P = F.probs[blockIdx.x];
+    finalize_problem(P, F.cpart, F.rpart, F.mom, F.terms, red);
+  } else
     spread_block<R>(blockIdx.x - nfin, items, pt_cell, wts, npt, res, gridB);
 }
 

@@ -168,9 +168,11 @@ __device__ __forceinline__ void load_weights(const R* __restrict__ wts, long lon
 }
 
 // Residual of each point for the type-1 transform: the sum of the
-// chi-square chunks' slot shares that reference the point (pref[j] = first
-// slot, number of slots, offset of the others in refs; summed in that fixed
-// order), or, for the bare adjoint, the caller's Vres[j].
+// chi-square chunks' slot shares that reference the point, in a fixed
+// order, or, for the bare adjoint, the caller's Vres[j].  pref[j] = (first
+// slot, second slot, number of slots, offset of the others in refs): the
+// common one- and two-slot points need one load round trip after pref, the
+// others load their further slots four at a time.
 struct ResidSrc {
   const double2* sres;
   const int4* pref;
@@ -181,12 +183,26 @@ struct ResidSrc {
 __device__ __forceinline__ double2 point_resid(const ResidSrc& S, int j) {
   if (S.Vres) return S.Vres[j];
   const int4 p = S.pref[j];
-  double2 r = make_double2(0.0, 0.0);
-  if (p.y > 0) r = S.sres[p.x];
-  for (int e = 1; e < p.y; ++e) {
-    const double2 t = S.sres[S.refs[p.z + e - 1]];
-    r.x += t.x;
-    r.y += t.y;
+  double2 r = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
+  if (p.z > 0) r = S.sres[p.x];
+  if (p.z > 1) b = S.sres[p.y];
+  r.x += b.x;
+  r.y += b.y;
+  for (int e = 0; e < p.z - 2; e += 4) {
+    int id[4];
+    double2 t[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (e + k < p.z - 2) id[k] = S.refs[p.w + e + k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (e + k < p.z - 2) t[k] = S.sres[id[k]];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (e + k < p.z - 2) {
+        r.x += t[k].x;
+        r.y += t[k].y;
+      }
   }
   return r;
 }
@@ -301,6 +317,14 @@ k_interp(const SubProb* __restrict__ sps, const typename Cx<R>::T* __restrict__ 
   const int T = bin + kW - 1;
   const V* g = gridA + sp.grid_off;
   const int gx0 = sp.bin_x * bin, gy0 = sp.bin_y * bin;
+  // the first point's cell and weights are loaded before the tile, so both
+  // loads are in flight together
+  int cell = 0;
+  R wx[kW], wy[kW];
+  if ((int)threadIdx.x < sp.count) {
+    cell = pt_cell[sp.start + threadIdx.x];
+    load_weights(wts, npt, sp.start + threadIdx.x, wx, wy);
+  }
   for (int c = threadIdx.x; c < T * T; c += blockDim.x) {
     const int ty = c / T, tx = c - ty * T;
     const int gy = (gy0 + ty) % sp.nf2;
@@ -310,11 +334,12 @@ k_interp(const SubProb* __restrict__ sps, const typename Cx<R>::T* __restrict__ 
   __syncthreads();
   for (int i = threadIdx.x; i < sp.count; i += blockDim.x) {
     const int j = sp.start + i;
-    const int cell = pt_cell[j];
+    if (i != (int)threadIdx.x) {
+      cell = pt_cell[j];
+      load_weights(wts, npt, j, wx, wy);
+    }
     const int lx = (cell & 0xffff) - gx0;
     const int ly = (cell >> 16) - gy0;
-    R wx[kW], wy[kW];
-    load_weights(wts, npt, j, wx, wy);
     R ar = 0, ai = 0;
 #pragma unroll
     for (int qy = 0; qy < kW; ++qy) {
