@@ -265,3 +265,16 @@ def test_fp64_mode_matches_fp64_es_oracle(gpu_lib, shape, window):
     assert rel_l2(V, Ve) <= 1e-11
     assert abs(c - c_es) <= 1e-12 * abs(c_es)
     assert rel_l2(gr, g_es) <= 1e-10
+
+
+def test_reported_chisq_terms_at_one_sigma(gpu_lib):
+    """Data one standard deviation off the ES-kernel model: the GPU reports
+    chisq_fcv = chisq_amp = chisq_cp = chisq_ca = 1 (the re-normalisation
+    of fftlib.f90:342-366) on the FP64 path."""
+    from tests.test_oracle_closed_forms import one_sigma_problem
+    rng = np.random.default_rng(18)
+    p, x = one_sigma_problem(rng, fwd=O.es_nufft_type2)
+    with _mk(p, tune=dict(precision=64)) as g:
+        c, gr, t = g.cost_grad(x)
+    for k in ("chisq_fcv", "chisq_amp", "chisq_cp", "chisq_ca"):
+        assert t[k] == pytest.approx(1.0, abs=1e-9), k
