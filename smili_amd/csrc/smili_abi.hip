@@ -517,7 +517,11 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     if (U.m > 0 && tune.transform != 1) {
       win_x[ui] = tap_window(us[ui].u, U.m, U.nf1);
       win_y[ui] = tap_window(us[ui].v, U.m, U.nf2);
-      dft = tune.transform == 2 ||
+      // the LDS FFT keeps a row or column and its scratch copy plus the
+      // twiddles in 64 KB: longer transforms take the products
+      const size_t vsz = fp64 ? sizeof(double2) : sizeof(float2);
+      const bool fft_fits = 3 * (size_t)std::max(U.nf1, U.nf2) * vsz <= 65536;
+      dft = tune.transform == 2 || !fft_fits ||
             dft_cheaper(d->nx, d->ny, U.nf1, U.nf2, win_x[ui].width, win_y[ui].width);
     }
     unit_dft[ui] = dft;

@@ -174,3 +174,18 @@ def test_movie_on_small_windows(gpu_lib):
     assert prof["dft2_y"] > 0 and prof["t2_rows"] == 0
     assert abs(cost - c_es) <= 1e-11 * abs(c_es)
     assert rel_l2(grad, g_es) <= 1e-10
+
+
+def test_grids_beyond_the_lds_fft_take_the_products(gpu_lib):
+    """A 768 x 640 image has a 1536 x 1280 fine grid, whose rows and columns
+    do not fit the LDS FFT's 64 KB in FP64: the default takes the products
+    even at nearly full coverage."""
+    rng = np.random.default_rng(15)
+    p, x = random_problem(rng, 768, 640, M=300, umax=np.pi * 0.9, nfcv=150, namp=100, ncp=60, nca=50)
+    with _mk(p) as g:
+        prof = g.profile(reps=1)
+        c, gr, _ = g.cost_grad(x)
+    assert prof["dft2_y"] > 0 and prof["t2_rows"] == 0
+    c_es, g_es, _ = O.calc_cost(x, p, **ES)
+    assert abs(c - c_es) <= 1e-12 * abs(c_es)
+    assert rel_l2(gr, g_es) <= 1e-10
