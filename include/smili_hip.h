@@ -192,6 +192,19 @@ int smili_cost_grad_device(smili_problem *p, const double *x_dev,
                            double *grad_dev, double *terms_dev);
 int smili_synchronize(smili_problem *p);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory while they run (code under hipGraph capture, tight serving loops):
+ * the same evaluation as smili_cost_grad_device (x_dev / grad_dev / terms_dev
+ * device pointers, terms_dev with nprob * nterms entries, asynchronous on the
+ * problem's stream), with every intermediate buffer in one static device
+ * array of 64 MiB per GPU instead of the handle's own; it calls no hipMalloc.
+ * The array holds the intermediates of problems up to about a 256^2 image
+ * with 1e6 uv points; larger ones are rejected (SMILI_EINVAL).  All calls on
+ * one GPU share the array: calls that overlap on one GPU (any handles, any
+ * streams) race and must be serialised by the caller. */
+int smili_cost_grad_device_noalloc(smili_problem *p, const double *x_dev,
+                                   double *grad_dev, double *terms_dev);
+
 /* Zero-copy use: the problem's own device buffers -- x (npix doubles, the
  * caller writes it), grad (npix) and terms (15) -- and one evaluation on
  * them, asynchronous on the problem's stream (no copies). */

@@ -80,6 +80,7 @@ SIGNATURES = {
     "smili_problem_destroy": (None, [C.c_void_p]),
     "smili_cost_grad": (C.c_int, [C.c_void_p, _pd, _pd, _pd, _pd]),
     "smili_cost_grad_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "smili_cost_grad_device_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "smili_synchronize": (C.c_int, [C.c_void_p]),
     "smili_problem_buffers": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                         C.POINTER(C.c_void_p)]),

@@ -149,6 +149,13 @@ class _Handle:
         check(self._lib.smili_cost_grad_device(self._h, C.c_void_p(x_ptr), C.c_void_p(grad_ptr),
                                                 C.c_void_p(terms_ptr) if terms_ptr else None))
 
+    def cost_grad_device_noalloc(self, x_ptr, grad_ptr, terms_ptr):
+        """smili_cost_grad_device_noalloc: the evaluation with its
+        intermediates in the library's static device scratch (no
+        allocation); device pointers, asynchronous on the handle's stream."""
+        check(self._lib.smili_cost_grad_device_noalloc(self._h, C.c_void_p(x_ptr), C.c_void_p(grad_ptr),
+                                                       C.c_void_p(terms_ptr)))
+
     def buffers(self):
         """Device addresses (ints) of the handle's own x, grad and terms."""
         x, g, t = C.c_void_p(), C.c_void_p(), C.c_void_p()

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <numeric>
 #include <stdexcept>
@@ -264,6 +265,16 @@ struct MovieSpec {
   std::vector<double> rt_w, ri_w, lc;
 };
 
+// Device buffers one evaluation writes (the plan's own, or a carve of the
+// no-allocation entry point's scratch) and the caller's x / grad / terms.
+struct EvalBufs {
+  double *x = nullptr, *grad = nullptr, *terms = nullptr;
+  double2 *V = nullptr, *sres = nullptr, *gridA = nullptr, *gridB = nullptr, *H = nullptr;
+  double *I2d = nullptr, *grad2d = nullptr, *greg = nullptr;
+  double *cpart = nullptr, *rpart = nullptr, *mpart = nullptr;
+  UnitMoments* mom = nullptr;
+};
+
 struct Plan {
   int device = 0;
   hipStream_t stream = nullptr;            // main stream: the transforms and chi-square
@@ -336,6 +347,7 @@ struct Plan {
   DevBuf<MovieDev> d_movies;
   DevBuf<double> d_movw, d_iavg;
 
+  EvalBufs B;                              // buffers the launches use (the plan's own after build)
   size_t lds_rows = 0, lds_cols = 0, lds_tile = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -1090,6 +1102,8 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     if (!f.empty()) HIP_CHECK(hipMemcpy(d_dftm.p, f.data(), f.size() * sizeof(float2), hipMemcpyHostToDevice));
   }
   d_H.alloc(h_total);
+  B = EvalBufs{d_x.p, d_grad.p, d_terms.p, d_V.p, d_sres.p, d_gridA.p, d_gridB.p, d_H.p,
+               d_I2d.p, d_grad2d.p, d_greg.p, d_cpart.p, d_rpart.p, d_mpart.p, d_mom.p};
   HIP_CHECK(hipDeviceSynchronize());
 }
 
@@ -1121,23 +1135,23 @@ static void launch_forward_t(Plan& P, bool with_scatter) {
   const V* tw = reinterpret_cast<const V*>(P.fp64 ? (const void*)P.d_twd.p : (const void*)P.d_tw.p);
   const R* corr = reinterpret_cast<const R*>(P.fp64 ? (const void*)P.d_corrd.p : (const void*)P.d_corr.p);
   const R* wts = reinterpret_cast<const R*>(P.d_wts.p);
-  V* gA = reinterpret_cast<V*>(P.d_gridA.p);
-  V* Vv = reinterpret_cast<V*>(P.d_V.p);
+  V* gA = reinterpret_cast<V*>(P.B.gridA);
+  V* Vv = reinterpret_cast<V*>(P.B.V);
   if (P.any_window && with_scatter)
-    LAUNCH(SMILI_S_SCATTER, k_scatter, P.h_regs.size(), kThreads, 0, P.d_regs.p, P.d_units.p, P.d_pix.p, P.d_x.p,
-           P.d_I2d.p);
+    LAUNCH(SMILI_S_SCATTER, k_scatter, P.h_regs.size(), kThreads, 0, P.d_regs.p, P.d_units.p, P.d_pix.p, P.B.x,
+           P.B.I2d);
   if (P.npt == 0) return;
   if (P.any_fft) {
     LAUNCH(SMILI_S_T2_ROWS, k_t2_rows<R>, P.h_t2rows.size(), P.threads_rows, P.lds_rows, P.d_t2rows.p, tw, corr,
-           P.d_x.p, P.d_I2d.p, gA);
+           P.B.x, P.B.I2d, gA);
     LAUNCH(SMILI_S_T2_COLS, k_t2_cols<R>, xcd_grid(P.h_cols.size()), P.threads_cols, P.lds_cols, P.d_cols.p,
            (int)P.h_cols.size(), tw, gA);
   }
   if (P.any_dft) {
     const V* M = reinterpret_cast<const V*>(P.d_dftm.p);
-    V* H = reinterpret_cast<V*>(P.d_H.p);
+    V* H = reinterpret_cast<V*>(P.B.H);
     LAUNCH(SMILI_S_DFT2_Y, (k_dft<R, ShapeT2Y::nw, ShapeT2Y::rb, ShapeT2Y::ks, true, DFT_STORE>), xcd_grid(P.h_dft[0].size()),
-           ShapeT2Y::threads, 0, P.d_dft[0].p, (int)P.h_dft[0].size(), M, (const V*)nullptr, P.d_x.p, P.d_I2d.p, H,
+           ShapeT2Y::threads, 0, P.d_dft[0].p, (int)P.h_dft[0].size(), M, (const V*)nullptr, P.B.x, P.B.I2d, H,
            (V*)nullptr, (double*)nullptr, (double*)nullptr, (const double*)nullptr, (const int*)nullptr, 0);
     LAUNCH(SMILI_S_DFT2_X, (k_dft<R, ShapeT2X::nw, ShapeT2X::rb, ShapeT2X::ks, false, DFT_STORE>),
            xcd_grid(P.h_dft[1].size()), ShapeT2X::threads, 0, P.d_dft[1].p, (int)P.h_dft[1].size(), (const V*)H, M,
@@ -1161,17 +1175,17 @@ static void launch_rows_t(Plan& P, bool assemble = true) {
   if (P.npt == 0) return;
   const V* tw = reinterpret_cast<const V*>(P.fp64 ? (const void*)P.d_twd.p : (const void*)P.d_tw.p);
   const R* corr = reinterpret_cast<const R*>(P.fp64 ? (const void*)P.d_corrd.p : (const void*)P.d_corr.p);
-  V* gA = reinterpret_cast<V*>(P.d_gridA.p);
-  V* gB = reinterpret_cast<V*>(P.d_gridB.p);
+  V* gA = reinterpret_cast<V*>(P.B.gridA);
+  V* gB = reinterpret_cast<V*>(P.B.gridB);
   if (P.any_fft)
     LAUNCH(SMILI_S_T1_ROWS, k_t1_rows<R>, P.h_t1rows.size(), P.threads_rows, P.lds_rows, P.d_t1rows.p, tw, corr,
-           gA, P.d_grad2d.p, P.d_grad.p, P.d_greg.p, P.d_pixinv.p, assemble ? 1 : 0);
+           gA, P.B.grad2d, P.B.grad, P.B.greg, P.d_pixinv.p, assemble ? 1 : 0);
   if (P.any_dft) {
     const V* M = reinterpret_cast<const V*>(P.d_dftm.p);
-    const V* H = reinterpret_cast<const V*>(P.d_H.p);
+    const V* H = reinterpret_cast<const V*>(P.B.H);
     LAUNCH(SMILI_S_DFT1_Y, (k_dft<R, ShapeT1Y::nw, ShapeT1Y::rb, ShapeT1Y::ks, false, DFT_GRAD>), xcd_grid(P.h_dft[3].size()),
            ShapeT1Y::threads, 0, P.d_dft[3].p, (int)P.h_dft[3].size(), M, H, (const double*)nullptr,
-           (const double*)nullptr, (V*)nullptr, gB, P.d_grad2d.p, P.d_grad.p, P.d_greg.p,
+           (const double*)nullptr, (V*)nullptr, gB, P.B.grad2d, P.B.grad, P.B.greg,
            P.d_pixinv.p, assemble ? 1 : 0);
   }
 }
@@ -1193,10 +1207,10 @@ static void launch_adjoint_t(Plan& P, int rows, const FinArgs* fin = nullptr, bo
   const V* tw = reinterpret_cast<const V*>(P.fp64 ? (const void*)P.d_twd.p : (const void*)P.d_tw.p);
   const R* corr = reinterpret_cast<const R*>(P.fp64 ? (const void*)P.d_corrd.p : (const void*)P.d_corr.p);
   const R* wts = reinterpret_cast<const R*>(P.d_wts.p);
-  const ResidSrc res{P.d_sres.p, P.d_pref.p, P.d_refs.p, from_chisq ? nullptr : P.d_Vres.p};
+  const ResidSrc res{P.B.sres, P.d_pref.p, P.d_refs.p, from_chisq ? nullptr : P.d_Vres.p};
   const long long npt = (long long)P.npt;
-  V* gA = reinterpret_cast<V*>(P.d_gridA.p);
-  V* gB = reinterpret_cast<V*>(P.d_gridB.p);
+  V* gA = reinterpret_cast<V*>(P.B.gridA);
+  V* gB = reinterpret_cast<V*>(P.B.gridB);
   if (fin)
     LAUNCH(SMILI_S_SPREAD, k_spread_fin<R>, P.h_spread.size() + P.probs.size(), kThreads, 0,
            (int)P.h_spread.size(), P.d_spread.p, P.d_cell.p, wts, npt, res, gB, *fin);
@@ -1208,7 +1222,7 @@ static void launch_adjoint_t(Plan& P, int rows, const FinArgs* fin = nullptr, bo
            (int)P.h_cols.size(), tw, gB, gA);
   if (P.any_dft) {
     const V* M = reinterpret_cast<const V*>(P.d_dftm.p);
-    V* H = reinterpret_cast<V*>(P.d_H.p);
+    V* H = reinterpret_cast<V*>(P.B.H);
     LAUNCH(SMILI_S_DFT1_X, (k_dft<R, ShapeT1X::nw, ShapeT1X::rb, ShapeT1X::ks, false, DFT_STORE>), xcd_grid(P.h_dft[2].size()),
            ShapeT1X::threads, 0, P.d_dft[2].p, (int)P.h_dft[2].size(), (const V*)gB, M, (const double*)nullptr,
            (const double*)nullptr, H, (V*)nullptr, (double*)nullptr, (double*)nullptr, (const double*)nullptr,
@@ -1244,12 +1258,12 @@ void Plan::launch_eval() {
   const bool fork = two_branches && !prof;
   const hipStream_t main_stream = stream;
   const hipStream_t side_stream = fork ? side : stream;
-  const RegArgs reg_args{d_regs.p, d_units.p, d_pix.p, d_x.p, d_I2d.p, d_mom.p, d_movies.p,
-                         d_greg.p, d_grad.p, d_rpart.p};
-  const FinArgs fin_args{d_probs.p, d_cpart.p, d_rpart.p, d_mom.p, d_terms.p};
+  const RegArgs reg_args{d_regs.p, d_units.p, d_pix.p, B.x, B.I2d, B.mom, d_movies.p,
+                         B.greg, B.grad, B.rpart};
+  const FinArgs fin_args{d_probs.p, B.cpart, B.rpart, B.mom, B.terms};
   const bool fused = !two_branches && npt > 0;
   if (any_window)
-    LAUNCH(SMILI_S_SCATTER, k_scatter, h_regs.size(), kThreads, 0, d_regs.p, d_units.p, d_pix.p, d_x.p, d_I2d.p);
+    LAUNCH(SMILI_S_SCATTER, k_scatter, h_regs.size(), kThreads, 0, d_regs.p, d_units.p, d_pix.p, B.x, B.I2d);
   if (fork) {
     HIP_CHECK(hipEventRecord(ev_fork, main_stream));
     HIP_CHECK(hipStreamWaitEvent(side_stream, ev_fork, 0));
@@ -1257,15 +1271,15 @@ void Plan::launch_eval() {
   {
     const hipStream_t stream = side_stream;
     if (any_moments) {
-      LAUNCH(SMILI_S_MOMENTS, k_moments, h_regs.size(), kThreads, 0, d_regs.p, d_units.p, d_pix.p, d_x.p,
-             d_mpart.p);
+      LAUNCH(SMILI_S_MOMENTS, k_moments, h_regs.size(), kThreads, 0, d_regs.p, d_units.p, d_pix.p, B.x,
+             B.mpart);
       LAUNCH(SMILI_S_MOMENTS_REDUCE, k_moments_reduce, units.size(), 64, 0, d_units.p, d_unit_reg_begin.p,
-             d_mpart.p, d_mom.p);
+             B.mpart, B.mom);
     }
     if (any_ri)
       for (size_t mi = 0; mi < movies.size(); ++mi)
         LAUNCH(SMILI_S_FRAME_MEAN, k_frame_mean, (movies[mi].npix + kThreads - 1) / kThreads, kThreads, 0,
-               d_movies.p + mi, d_x.p);
+               d_movies.p + mi, B.x);
     if (!fused) LAUNCH(SMILI_S_REG, k_reg, h_regs.size(), kThreads, 0, reg_args);
     if (fork) HIP_CHECK(hipEventRecord(ev_reg, side_stream));
   }
@@ -1277,18 +1291,18 @@ void Plan::launch_eval() {
     if (fp64) {
       if (fused)
         LAUNCH(SMILI_S_CHISQ, k_chisq_reg<double2>, grid, chisq_threads, 0, nch, d_chunks.p, d_slots.p, d_fcv.p,
-               d_amp.p, d_cp.p, d_ca.p, d_V.p, d_sres.p, d_cpart.p, reg_args);
+               d_amp.p, d_cp.p, d_ca.p, B.V, B.sres, B.cpart, reg_args);
       else
         LAUNCH(SMILI_S_CHISQ, k_chisq<double2>, grid, chisq_threads, 0, d_chunks.p, d_slots.p, d_fcv.p, d_amp.p,
-               d_cp.p, d_ca.p, d_V.p, d_sres.p, d_cpart.p);
+               d_cp.p, d_ca.p, B.V, B.sres, B.cpart);
     } else {
-      const float2* Vf = reinterpret_cast<const float2*>(d_V.p);
+      const float2* Vf = reinterpret_cast<const float2*>(B.V);
       if (fused)
         LAUNCH(SMILI_S_CHISQ, k_chisq_reg<float2>, grid, chisq_threads, 0, nch, d_chunks.p, d_slots.p, d_fcv.p, d_amp.p,
-               d_cp.p, d_ca.p, Vf, d_sres.p, d_cpart.p, reg_args);
+               d_cp.p, d_ca.p, Vf, B.sres, B.cpart, reg_args);
       else
         LAUNCH(SMILI_S_CHISQ, k_chisq<float2>, grid, chisq_threads, 0, d_chunks.p, d_slots.p, d_fcv.p, d_amp.p,
-               d_cp.p, d_ca.p, Vf, d_sres.p, d_cpart.p);
+               d_cp.p, d_ca.p, Vf, B.sres, B.cpart);
     }
   }
   if (fork) {
@@ -1330,6 +1344,91 @@ void Plan::eval() {
   } else {
     launch_eval();
   }
+}
+
+
+// ---------------------------------------------------------------------------
+// The no-allocation entry point's scratch: one array per device, for the
+// evaluation's intermediate buffers (model visibilities, slot shares, grid
+// windows, partial sums ...).  Its host-side name is not a device address:
+// the device address comes from hipGetSymbolAddress, once per device.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+__global__ void __launch_bounds__(kThreads) k_zero(double* __restrict__ p, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+static std::mutex g_scratch_mu;
+static std::map<int, unsigned char*> g_scratch;     // device -> scratch address
+
+// the scratch array's device address on the current device (looked up on
+// the first call on that device)
+static unsigned char* scratch_ptr(int dev) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch.find(dev);
+  if (it != g_scratch.end()) return it->second;
+  void* ptr = nullptr;
+  HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  require(ptr != nullptr, "scratch symbol has no device address");
+  g_scratch[dev] = static_cast<unsigned char*>(ptr);
+  return static_cast<unsigned char*>(ptr);
+}
+
+// One evaluation of the plan on x_dev -> grad_dev, terms_dev with every
+// intermediate buffer carved from the scratch array: no allocation.  The
+// spreading grids and (for windowed images) the 2-D image start at zero, so
+// they are cleared by a kernel first.  Launched eagerly on the plan's stream
+// (its captured graph holds the plan's own buffers).
+static void eval_in_scratch(Plan& P, const double* x_dev, double* grad_dev, double* terms_dev) {
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  require(dev == P.device, "the handle was created on another device");
+  unsigned char* base = scratch_ptr(dev);
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    unsigned char* q = base + off;
+    off += (bytes + 255) / 256 * 256;
+    return q;
+  };
+  EvalBufs S;
+  S.x = const_cast<double*>(x_dev);
+  S.grad = grad_dev;
+  S.terms = terms_dev;
+  S.V = reinterpret_cast<double2*>(carve(P.d_V.n * sizeof(double2)));
+  S.sres = reinterpret_cast<double2*>(carve(P.d_sres.n * sizeof(double2)));
+  S.gridA = reinterpret_cast<double2*>(carve(P.d_gridA.n * sizeof(double2)));
+  S.gridB = reinterpret_cast<double2*>(carve(P.d_gridB.n * sizeof(double2)));
+  S.H = reinterpret_cast<double2*>(carve(P.d_H.n * sizeof(double2)));
+  S.I2d = reinterpret_cast<double*>(carve(P.d_I2d.n * sizeof(double)));
+  S.grad2d = nullptr;                          // the evaluation assembles grad directly
+  S.greg = reinterpret_cast<double*>(carve(P.d_greg.n * sizeof(double)));
+  S.cpart = reinterpret_cast<double*>(carve(P.d_cpart.n * sizeof(double)));
+  S.rpart = reinterpret_cast<double*>(carve(P.d_rpart.n * sizeof(double)));
+  S.mpart = reinterpret_cast<double*>(carve(P.d_mpart.n * sizeof(double)));
+  S.mom = reinterpret_cast<UnitMoments*>(carve(P.d_mom.n * sizeof(UnitMoments)));
+  if (off > kScratchBytes)
+    throw Error(SMILI_EINVAL, "the problem needs " + std::to_string(off >> 20) + " MiB of scratch; "
+                "smili_cost_grad_device_noalloc holds at most " + std::to_string(kScratchBytes >> 20) + " MiB");
+  auto zero = [&](double* p, size_t n) {
+    if (!n) return;
+    const size_t blocks = std::min<size_t>((n + kThreads - 1) / kThreads, 4096);
+    hipLaunchKernelGGL(k_zero, dim3(blocks), dim3(kThreads), 0, P.stream, p, (long long)n);
+    HIP_CHECK(hipGetLastError());
+  };
+  zero(reinterpret_cast<double*>(S.gridB), 2 * P.d_gridB.n);
+  if (P.any_window) zero(S.I2d, P.d_I2d.n);
+  const EvalBufs own = P.B;
+  P.B = S;
+  try {
+    P.launch_eval();
+  } catch (...) {
+    P.B = own;
+    throw;
+  }
+  P.B = own;
 }
 
 }  // namespace smili
@@ -1582,6 +1681,13 @@ int smili_cost_grad_device(smili_problem* p, const double* x_dev, double* grad_d
     if (terms_dev && terms_dev != P.d_terms.p)
       HIP_CHECK(hipMemcpyAsync(terms_dev, P.d_terms.p, P.nterms_total * sizeof(double),
                                hipMemcpyDeviceToDevice, P.stream));
+  });
+}
+
+int smili_cost_grad_device_noalloc(smili_problem* p, const double* x_dev, double* grad_dev, double* terms_dev) {
+  return guarded([&] {
+    require(p && x_dev && grad_dev && terms_dev, "NULL argument");
+    eval_in_scratch(p->plan, x_dev, grad_dev, terms_dev);
   });
 }
 
