@@ -55,45 +55,87 @@ def fine_size(n):
         m += 1
 
 
-def stage_bytes(kws, fp64=True):
-    """Algorithmic HBM bytes per launch of each kernel stage, summed over
-    the problems (2-D cost keyword dicts; a movie's dict carries one entry
-    per frame in ``units``).  fp64: complex grid / visibilities of 16 bytes
-    and FP64 point offsets, else 8 bytes and FP32 offsets."""
+def stage_bytes(kws, units, fp64=True):
+    """Algorithmic bytes per kernel stage of one evaluation: every input the
+    stage's algorithm needs read once and every output written once, in the
+    precision it computes in (complex grid, model visibility and DFT-matrix
+    elements of 16 bytes in FP64, 8 in FP32; ES weights 14 x 8 or 14 x 4 bytes
+    per point), with the data records of the cost model (index + datum +
+    variance: 28 B per full visibility or closure phase, 20 per amplitude,
+    32 per log closure amplitude) and one FP64 complex residual per point.
+    ``kws``: the problems' cost keyword dicts (a movie's dict carries one
+    entry per frame in ``units``); ``units``: the handle's unit_info() of
+    each unit in the same order (transform, grid or window width / height)."""
     import numpy as np
     cb = 16 if fp64 else 8                 # complex grid element / model visibility
-    pb = 4 + (16 if fp64 else 8)           # packed cell + two offsets per point
+    wb = 2 * 7 * (8 if fp64 else 4)        # ES weights per point
     tot = {}
+
+    def add(k, v):
+        tot[k] = tot.get(k, 0) + int(v)
+
+    ui = 0
     for kw in kws:
+        N = np.asarray(kw["xidx"]).size
+        nreg = sum(1 for r in ("l1", "tv", "tsv", "kl", "gs") if kw.get(r + "_l", -1) > 0)
+        nreg += sum(1 for r in ("rt", "ri") if kw.get(r + "_l", -1) > 0)
+        M_all = 0
         for u in kw.get("units", [kw]):
-            Nx, Ny = u["Nx"], u["Ny"]
-            nf1, nf2 = fine_size(Nx), fine_size(Ny)
-            N = np.asarray(kw["xidx"]).size
-            M = u["M"] if "M" in u else np.asarray(kw["u"]).size
-            G = nf1 * nf2
-            half = cb * nf1 * Ny
-            nreg = sum(1 for r in ("l1", "tv", "tsv", "kl", "gs") if kw.get(r + "_l", -1) > 0)
-            nreg += sum(1 for r in ("rt", "ri") if kw.get(r + "_l", -1) > 0)
-            live = M > 0
-            part = {
-                "t2_rows": (8 * Nx * Ny + half) * live,          # x in (f64), half grid out
-                "t2_cols": (half + cb * G) * live,               # half grid in, grid out
-                "interp": cb * G * live + pb * M + cb * M + 16 * M,  # grid, points, V out, Vres cleared
-                "spread": pb * M + 16 * M + cb * G * live,       # points, Vres, grid out
-                "t1_cols": (2 * cb * G + half) * live,           # grid in, grid cleared, half out
-                "t1_rows": (half + 8 * Nx * Ny) * live,          # half in, grad2d out (f64)
-                "reg": 8 * N * (3 + nreg) + 8 * N,               # x, grad2d, weights in, grad out
-            }
-            for k, v in part.items():
-                tot[k] = tot.get(k, 0) + int(v)
-        M = np.asarray(kw["u"]).size
+            info = units[ui]
+            ui += 1
+            Nx, Ny, M = info["nx"], info["ny"], info["m"]
+            M_all += M
+            Npx = N                                    # pixels per image (frame)
+            if info["transform"] == 1:
+                G = info["nf1"] * info["nf2"]
+                half = cb * info["nf1"] * Ny
+                add("t2_rows", 8 * Nx * Ny + half)
+                add("t2_cols", half + cb * G)
+                add("t1_cols", 2 * cb * G + half)
+                add("t1_rows", half + 16 * Npx)
+            elif info["transform"] == 2:
+                gw, gh = info["gw"], info["gh"]
+                G = gw * gh
+                add("dft2_y", 8 * Nx * Ny + cb * gh * Ny + cb * gh * Nx)
+                add("dft2_x", cb * gh * Nx + cb * Nx * gw + cb * G)
+                add("dft1_x", cb * G + cb * gw * Nx + cb * gh * Nx)
+                add("dft1_y", cb * Ny * gh + cb * gh * Nx + 16 * Npx + cb * G)
+            else:
+                G = 0
+            if M:
+                add("interp", cb * G + M * (4 + wb + cb))
+                add("spread", M * (4 + wb + 16 + 16) + cb * G)   # cell, weights, slot refs, residual; grid
+        nun = len(kw.get("units", [kw]))
+        add("reg", 8 * N * nun * (1 + nreg) + 8 * N * nun)         # x, weights in, gradreg out
         nfcv = 0 if kw.get("uvidxfcv") is None else len(kw["uvidxfcv"])
         namp = 0 if kw.get("uvidxamp") is None else len(kw["uvidxamp"])
         ncp = 0 if kw.get("uvidxcp") is None else kw["uvidxcp"].shape[1]
         nca = 0 if kw.get("uvidxca") is None else kw["uvidxca"].shape[1]
-        tot["chisq"] = tot.get("chisq", 0) + int(32 * nfcv + 24 * namp + 32 * ncp + 32 * nca +
-                                                 cb * M + 32 * M)  # data, V, Vres r+w
+        add("chisq", 28 * nfcv + 20 * namp + 28 * ncp + 32 * nca + cb * M_all + 16 * M_all)
     return tot
+
+
+def survey_bytes(kws):
+    """The cost model's bytes per evaluation (complex64 grids, two passes
+    per transform, FP32 uv pairs read by both transforms, complex64 model
+    visibilities written and read, FP64 residuals written and read, the
+    data records read once): the figure the HBM ceiling of the headline
+    metric is quoted on."""
+    import numpy as np
+    b = 0
+    for kw in kws:
+        N = np.asarray(kw["xidx"]).size
+        for u in kw.get("units", [kw]):
+            M = u["M"] if "M" in u else np.asarray(kw["u"]).size
+            G = fine_size(u["Nx"]) * fine_size(u["Ny"])
+            b += 32 * G + 16 * M + 16 * M + 32 * M
+        b += 16 * N
+        nfcv = 0 if kw.get("uvidxfcv") is None else len(kw["uvidxfcv"])
+        namp = 0 if kw.get("uvidxamp") is None else len(kw["uvidxamp"])
+        ncp = 0 if kw.get("uvidxcp") is None else kw["uvidxcp"].shape[1]
+        nca = 0 if kw.get("uvidxca") is None else kw["uvidxca"].shape[1]
+        b += 28 * nfcv + 20 * namp + 28 * ncp + 32 * nca
+    return int(b)
 
 
 def _movie_units(kw):
@@ -277,10 +319,18 @@ def _traffic(workload, precision, kernel):
 
 
 def _roofline(g, kws, fp64, ms, steps, workload="single_large"):
-    prof = g.profile(reps=max(10, min(steps, 100)))
-    sb = stage_bytes(kws, fp64)
+    """The dominant kernel's algorithmic bytes per launch over its launch
+    duration: hipEvents on the library's stream around 100 back-to-back
+    launches of each stage (one pass), so the events' own overhead does not
+    count against the kernel."""
+    nunits = g.nprob if getattr(g, "nprob", 1) > 1 else getattr(g, "Nz", 1)
+    units = [g.unit_info(u) for u in range(nunits)]
+    prof = g.profile(reps=100, back_to_back=True)
+    sb = stage_bytes(kws, units, fp64)
     eval_bytes = int(sum(sb.values()))
     names = {k: "k_" + k for k in sb}
+    for k in ("dft2_y", "dft2_x", "dft1_x", "dft1_y"):
+        names[k] = "k_dft"
     if prof.get("reg", 0.0) == 0.0 and prof.get("chisq", 0.0) > 0.0:
         # one-stream schedule: the regularizers ride on the chi-square launch
         # and the final sums on the spreading launch
@@ -288,14 +338,18 @@ def _roofline(g, kws, fp64, ms, steps, workload="single_large"):
         names["chisq"], names["spread"] = "k_chisq_reg", "k_spread_fin"
     dom = max((k for k in prof if k in sb), key=lambda k: prof[k])
     achieved = sb[dom] / (prof[dom] * 1e-3) / 1e9
+    sv = survey_bytes(kws)
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": _traffic(workload, "fp64" if fp64 else "fp32", names[dom]),
-            "kernel": names[dom], "kernel_ms": round(prof[dom], 5),
+            "kernel": names[dom], "stage": dom, "kernel_ms": round(prof[dom], 5),
             "kernel_bytes": sb[dom],
             "eval_bytes": eval_bytes,
             "eval_achieved_GBs": round(eval_bytes / (ms * 1e-3) / 1e9, 2),
-            "stage_ms": {k: round(v, 5) for k, v in prof.items() if v > 0}}
+            "model_bytes": sv, "model_achieved_GBs": round(sv / (ms * 1e-3) / 1e9, 2),
+            "model_ceiling_evals_s": round(HBM_PEAK_GBS * 1e9 / sv, 1),
+            "stage_ms": {k: round(v, 5) for k, v in prof.items() if v > 0},
+            "stage_bytes": sb}
 
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0
@@ -604,11 +658,24 @@ def main():
     assert np.array_equal(np.asarray(first[0]), np.asarray(again[0]))
     info = g.info()
     # host-pointer API (x host->device, grad device->host each call): PCIe-inclusive
-    nh = max(5, min(args.steps // 10, 50))
+    # host-pointer API (x host->device, grad device->host each call): the
+    # imager's path works in the handle's page-locked x and grad
+    # (host_buffers), a caller with its own arrays pays two host copies more
+    nh = max(20, min(args.steps // 5, 200))
+    if per_step == 1:
+        hx, hg = g.host_buffers()
+        hx[:] = x0
+        g.cost_grad(hx, out=hg)
+        th = time.perf_counter()
+        for _ in range(nh):
+            g.cost_grad(hx, out=hg)
+        host_rate = nh / (time.perf_counter() - th)
+    else:
+        host_rate = None
     th = time.perf_counter()
     for _ in range(nh):
         g.cost_grad(x0)
-    host_rate = per_step * nh / (time.perf_counter() - th)
+    host_rate_copies = per_step * nh / (time.perf_counter() - th)
     roofline = _roofline(g, kws, fp64, ms, args.steps, args.workload)
 
     cpu = None
@@ -636,7 +703,8 @@ def main():
         "data": "synthetic",
         "config": dict({"workload": desc, "nuv": info["nuv"], "graph": bool(info["graph"]),
                         "setup_s": round(t_setup, 2), "precision": args.precision, "cost": cost,
-                        "host_api_" + unit.replace("/", "_"): round(host_rate, 1)}, **extra),
+                        "host_api_" + unit.replace("/", "_"): None if host_rate is None else round(host_rate, 1),
+                        "host_api_own_arrays_" + unit.replace("/", "_"): round(host_rate_copies, 1)}, **extra),
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

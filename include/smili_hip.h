@@ -213,6 +213,15 @@ int smili_eval_async(smili_problem *p);
 /* hipStream_t of the problem, as void* (plain pointer, no HIP types). */
 void *smili_problem_stream(smili_problem *p);
 
+/* Page-locked host arrays of the handle's image-vector length, valid until
+ * smili_problem_destroy: smili_cost_grad / smili_batch_cost_grad copy x and
+ * grad through them, and a caller that keeps its iterate in x_host and
+ * passes grad_host as grad (the host L-BFGS-B loop of
+ * src/fortlib/fftim2d.f90:286-351 works in place on x and g) saves both host
+ * copies.  No counterpart in the reference (its calc_cost reads the
+ * caller's arrays directly on the CPU). */
+int smili_problem_host_buffers(smili_problem *p, double **x_host, double **grad_host);
+
 /* Batched survey: nprob independent problems (any mix of sizes and data)
  * evaluated together, one launch per kernel stage for all of them (captured
  * in one hipGraph).  x / grad are the problems' image vectors concatenated
@@ -281,8 +290,20 @@ enum smili_stage {
 
 /* Average device time (ms) of each stage over `reps` evaluations launched
  * kernel by kernel with hipEvents on the problem's stream; ms has
- * SMILI_NSTAGES entries (0 for stages that do not run). */
+ * SMILI_NSTAGES entries (0 for stages that do not run).  reps < 0: one pass
+ * in which every stage is launched -reps times back to back between its two
+ * events, ms = elapsed / -reps per launch (the events' and the dispatch's
+ * own overheads spread over the launches).  The repeated spreading adds
+ * its residuals -reps times, so the grad and terms left by such a call are
+ * not an evaluation's; the next evaluation computes them afresh. */
 int smili_problem_profile(smili_problem *p, int32_t reps, double *ms);
+
+/* Grid of unit `unit` (a problem of a batch, a frame of a movie): [0] =
+ * transform (0: no uv points, 1: FFTs over the fine grid, 2: pruned DFT
+ * products), [1], [2] = width, height of the grid (1) or of the window of
+ * fine-grid cells the points touch (2), [3] = uv points, [4], [5] = nx, ny,
+ * [6], [7] = nf1, nf2. */
+int smili_problem_unit_info(smili_problem *p, int32_t unit, int64_t *info8);
 
 /* Sizes: [0]=nf1 [1]=nf2 [2]=nuv [3]=subproblems [4]=bin [5]=chi2 chunks
  * [6]=kernel launches per evaluation [7]=graph in use. */

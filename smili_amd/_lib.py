@@ -85,10 +85,12 @@ SIGNATURES = {
     "smili_problem_buffers": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                         C.POINTER(C.c_void_p)]),
     "smili_eval_async": (C.c_int, [C.c_void_p]),
+    "smili_problem_host_buffers": (C.c_int, [C.c_void_p, C.POINTER(_pd), C.POINTER(_pd)]),
     "smili_problem_stream": (C.c_void_p, [C.c_void_p]),
     "smili_model_vis": (C.c_int, [C.c_void_p, _pd, _pd, _pd]),
     "smili_adjoint_resid": (C.c_int, [C.c_void_p, _pd, _pd, _pd]),
     "smili_problem_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "smili_problem_unit_info": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64)]),
     "smili_problem_profile": (C.c_int, [C.c_void_p, C.c_int32, _pd]),
     "smili_batch_create": (C.c_int, [C.POINTER(SmiliDesc), C.c_int32, C.POINTER(SmiliTune),
                                      C.POINTER(C.c_void_p)]),

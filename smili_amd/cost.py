@@ -128,6 +128,7 @@ class _Handle:
     # ------------------------------------------------------------------
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
+            self._hostbufs = None
             self._lib.smili_problem_destroy(self._h)
             self._h = None
 
@@ -162,6 +163,18 @@ class _Handle:
         check(self._lib.smili_problem_buffers(self._h, C.byref(x), C.byref(g), C.byref(t)))
         return x.value, g.value, t.value
 
+    def host_buffers(self):
+        """The handle's page-locked host x and grad (numpy views, valid
+        until close()): evaluating with x in the first and grad into the
+        second (``cost_grad(x_view, out=grad_view)``) skips the host copies
+        of the host-pointer path."""
+        if getattr(self, "_hostbufs", None) is None:
+            x, g = _lib._pd(), _lib._pd()
+            check(self._lib.smili_problem_host_buffers(self._h, C.byref(x), C.byref(g)))
+            n = self.nx_total
+            self._hostbufs = (np.ctypeslib.as_array(x, shape=(n,)), np.ctypeslib.as_array(g, shape=(n,)))
+        return self._hostbufs
+
     def eval_async(self):
         """One evaluation on the handle's own buffers (no copies, async)."""
         check(self._lib.smili_eval_async(self._h))
@@ -191,11 +204,21 @@ class _Handle:
         check(self._lib.smili_adjoint_resid(self._h, dptr(_f64(vres_re)), dptr(_f64(vres_im)), dptr(out)))
         return out
 
-    def profile(self, reps=10):
-        """Average device ms per kernel stage (eager launches with hipEvents)."""
+    def profile(self, reps=10, back_to_back=False):
+        """Average device ms per kernel stage (eager launches with hipEvents);
+        back_to_back: one pass with each stage launched `reps` times between
+        its events (per-launch time with the event overhead amortised; the
+        handle's grad and terms are stale afterwards)."""
         ms = np.zeros(len(_lib.STAGE_NAMES))
-        check(self._lib.smili_problem_profile(self._h, int(reps), dptr(ms)))
+        check(self._lib.smili_problem_profile(self._h, -int(reps) if back_to_back else int(reps), dptr(ms)))
         return dict(zip(_lib.STAGE_NAMES, ms.tolist()))
+
+    def unit_info(self, unit=0):
+        """Transform and grid of one unit (problem of a batch / movie frame):
+        dict(transform=0 none | 1 fft | 2 dft, gw, gh, m, nx, ny, nf1, nf2)."""
+        a = (C.c_int64 * 8)()
+        check(self._lib.smili_problem_unit_info(self._h, int(unit), a))
+        return dict(zip(("transform", "gw", "gh", "m", "nx", "ny", "nf1", "nf2"), list(a)))
 
     def info(self):
         a = (C.c_int64 * 8)()
@@ -203,11 +226,14 @@ class _Handle:
         keys = ("nf1", "nf2", "nuv", "subproblems", "bin", "chunks", "launches", "graph")
         return dict(zip(keys, list(a)))
 
-    def _cost_grad(self, x, allow_nonfinite):
+    def _cost_grad(self, x, allow_nonfinite, out=None):
         x = _f64(x)
         if x.size != self.nx_total:
             raise ValueError(f"x has {x.size} entries, expected {self.nx_total}")
-        grad = np.empty(self.nx_total)
+        if out is not None and (out.dtype != np.float64 or out.size != self.nx_total or
+                                not out.flags.c_contiguous):
+            raise ValueError("out must be a contiguous float64 array of the image-vector length")
+        grad = np.empty(self.nx_total) if out is None else out
         terms = np.empty(self.nterms)
         cost = C.c_double()
         check(self._lib.smili_cost_grad(self._h, dptr(x), C.byref(cost), dptr(grad), dptr(terms)),
@@ -240,9 +266,11 @@ class CostProblem(_Handle):
         del keep                  # the library copied everything it needs
         self._adopt(lib, h)
 
-    def cost_grad(self, x, allow_nonfinite=False):
-        """(cost, grad[npix], terms dict) for image vector x."""
-        return self._cost_grad(x, allow_nonfinite)
+    def cost_grad(self, x, allow_nonfinite=False, out=None):
+        """(cost, grad[npix], terms dict) for image vector x; ``out``: a
+        float64 array of npix entries to write the gradient into (e.g. the
+        handle's page-locked grad, host_buffers())."""
+        return self._cost_grad(x, allow_nonfinite, out)
 
 
 class MovieProblem(_Handle):
@@ -294,9 +322,10 @@ class MovieProblem(_Handle):
         del keep
         self._adopt(lib, h)
 
-    def cost_grad(self, x, allow_nonfinite=False):
-        """(cost, grad[Nz * npix], terms dict incl. rt, ri, lc)."""
-        return self._cost_grad(x, allow_nonfinite)
+    def cost_grad(self, x, allow_nonfinite=False, out=None):
+        """(cost, grad[Nz * npix], terms dict incl. rt, ri, lc); ``out`` as
+        for CostProblem.cost_grad."""
+        return self._cost_grad(x, allow_nonfinite, out)
 
 
 class BatchProblem(_Handle):
@@ -327,7 +356,7 @@ class BatchProblem(_Handle):
         """Per-problem views of a concatenated image vector."""
         return [vec[a:b] for a, b in zip(self.offsets[:-1], self.offsets[1:])]
 
-    def cost_grad(self, x, allow_nonfinite=False):
+    def cost_grad(self, x, allow_nonfinite=False, out=None):
         """(costs[nprob], grad (concatenated), terms[nprob, 15]) for the
         concatenated image vectors x."""
         x = _f64(x)

@@ -301,6 +301,7 @@ struct Plan {
   // pruned-DFT transforms: per unit whether it uses them, the four product
   // work lists (S = Ey I, G = S Ex, P = Gs ExT, Re(EyT P)) and their matrices
   std::vector<char> unit_dft;
+  std::vector<std::pair<int, int>> unit_grid;   // grid (FFT) or window (DFT) width, height
   bool any_fft = false, any_dft = false;
   std::vector<DftItem> h_dft[4];
   DevBuf<DftItem> d_dft[4];
@@ -330,7 +331,8 @@ struct Plan {
   DevBuf<double2> d_gridA, d_gridB;        // float2 or double2 grids (sized for double2)
   DevBuf<double> d_I2d, d_grad2d, d_greg;
   DevBuf<int> d_pixinv;                    // 2-D pixel -> window index (-1 outside), per unit
-  DevBuf<double> d_x, d_grad, d_terms;
+  DevBuf<double> d_x, d_grad;                // grad: npix_total entries, then the terms
+  double* d_terms_p = nullptr;
   DevBuf<double> d_cpart, d_rpart, d_mpart;
   DevBuf<UnitMoments> d_mom;
   DevBuf<int> d_unit_reg_begin;
@@ -354,10 +356,17 @@ struct Plan {
   bool graph_ready = false;
   int launches = 0;
   bool prof = false;                       // record events around each kernel
+  int prof_repeat = 1;                     // ... around this many back-to-back launches of it
   hipEvent_t prof_ev[2 * SMILI_NSTAGES] = {};
   int prof_used[SMILI_NSTAGES] = {};
 
+  // page-locked host staging of x, grad and the terms (the host-pointer
+  // entry points copy through them; callers may work in them directly)
+  double *h_x = nullptr, *h_grad = nullptr, *h_terms = nullptr;
+
   ~Plan() {
+    for (double* h : {h_x, h_grad})
+      if (h) (void)hipHostFree(h);
     for (auto& e : prof_ev)
       if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : {ev_fork, ev_reg, ev_chisq, ev_fin})
@@ -897,6 +906,7 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     for (int s = 0; s < U.nf1; s += cc) h_cols.push_back(fft_item(s, std::min(cc, U.nf1 - s)));
   }
   unit_reg_begin[units.size()] = (int)h_regs.size();
+  unit_grid = gdim;
   // pruned-DFT products: matrices (shared by units of one geometry and
   // window) and work items, ordered by output column block so that the
   // items reading one block of B meet on one XCD (k_dft maps items to XCDs
@@ -1064,7 +1074,10 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   d_grad2d.zero();
   d_x.alloc(npix_total);
   d_x.zero();
-  d_grad.alloc(npix_total);
+  // grad and the terms in one allocation: the host path copies both back
+  // with one transfer
+  d_grad.alloc(npix_total + nterms_total);
+  d_terms_p = d_grad.p + npix_total;
   d_greg.alloc(npix_total);
   {
     std::vector<int> inv(std::max<long long>(img_total, 1), -1);
@@ -1072,7 +1085,6 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
       for (int k = 0; k < units[ui].npix; ++k) inv[units[ui].img_off + winpix[ui][k]] = k;
     d_pixinv.upload(inv);
   }
-  d_terms.alloc(nterms_total);
   d_cpart.alloc(std::max<size_t>(h_chunks.size(), 1));
   d_rpart.alloc(std::max<size_t>(h_regs.size(), 1) * kRegTerms);
   d_mpart.alloc(std::max<size_t>(h_regs.size(), 1) * kMomTerms);
@@ -1102,7 +1114,10 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     if (!f.empty()) HIP_CHECK(hipMemcpy(d_dftm.p, f.data(), f.size() * sizeof(float2), hipMemcpyHostToDevice));
   }
   d_H.alloc(h_total);
-  B = EvalBufs{d_x.p, d_grad.p, d_terms.p, d_V.p, d_sres.p, d_gridA.p, d_gridB.p, d_H.p,
+  HIP_CHECK(hipHostMalloc(&h_x, std::max<size_t>(npix_total, 1) * sizeof(double), hipHostMallocDefault));
+  HIP_CHECK(hipHostMalloc(&h_grad, (npix_total + nterms_total) * sizeof(double), hipHostMallocDefault));
+  h_terms = h_grad + npix_total;
+  B = EvalBufs{d_x.p, d_grad.p, d_terms_p, d_V.p, d_sres.p, d_gridA.p, d_gridB.p, d_H.p,
                d_I2d.p, d_grad2d.p, d_greg.p, d_cpart.p, d_rpart.p, d_mpart.p, d_mom.p};
   HIP_CHECK(hipDeviceSynchronize());
 }
@@ -1114,8 +1129,10 @@ static size_t xcd_grid(size_t nitems) { return (nitems + kXcds - 1) / kXcds * kX
   do {                                                                        \
     if ((grid) > 0) {                                                         \
       if (prof) HIP_CHECK(hipEventRecord(prof_ev[2 * (stage)], stream));      \
-      hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, stream, __VA_ARGS__); \
-      HIP_CHECK(hipGetLastError());                                           \
+      for (int rep_ = 0; rep_ < (prof ? prof_repeat : 1); ++rep_) {           \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, stream, __VA_ARGS__); \
+        HIP_CHECK(hipGetLastError());                                         \
+      }                                                                       \
       if (prof) {                                                             \
         HIP_CHECK(hipEventRecord(prof_ev[2 * (stage) + 1], stream));          \
         prof_used[stage] = 1;                                                 \
@@ -1129,6 +1146,7 @@ static void launch_forward_t(Plan& P, bool with_scatter) {
   using V = typename Cx<R>::T;
   hipStream_t stream = P.stream;
   const bool prof = P.prof;
+  const int prof_repeat = P.prof_repeat;
   auto& prof_ev = P.prof_ev;
   auto& prof_used = P.prof_used;
   int& launches = P.launches;
@@ -1169,6 +1187,7 @@ static void launch_rows_t(Plan& P, bool assemble = true) {
   using V = typename Cx<R>::T;
   hipStream_t stream = P.stream;
   const bool prof = P.prof;
+  const int prof_repeat = P.prof_repeat;
   auto& prof_ev = P.prof_ev;
   auto& prof_used = P.prof_used;
   int& launches = P.launches;
@@ -1200,6 +1219,7 @@ static void launch_adjoint_t(Plan& P, int rows, const FinArgs* fin = nullptr, bo
   using V = typename Cx<R>::T;
   hipStream_t stream = P.stream;
   const bool prof = P.prof;
+  const int prof_repeat = P.prof_repeat;
   auto& prof_ev = P.prof_ev;
   auto& prof_used = P.prof_used;
   int& launches = P.launches;
@@ -1623,17 +1643,28 @@ void smili_problem_destroy(smili_problem* p) { delete p; }
 
 // copy x in, evaluate, copy grad and all terms out; returns whether every
 // cost and gradient entry is finite
+// (through the handle's page-locked staging: a caller that works in those
+// buffers themselves -- smili_problem_host_buffers -- saves the two host
+// copies, and the device copies run at the DMA rate of pinned memory)
 static bool eval_host(Plan& P, const double* x, double* grad, std::vector<double>& t) {
   const size_t n = P.npix_total;
-  HIP_CHECK(hipMemcpyAsync(P.d_x.p, x, n * sizeof(double), hipMemcpyHostToDevice, P.stream));
+  if (x != P.h_x) std::memcpy(P.h_x, x, n * sizeof(double));
+  HIP_CHECK(hipMemcpyAsync(P.d_x.p, P.h_x, n * sizeof(double), hipMemcpyHostToDevice, P.stream));
   P.eval();
   t.resize(P.nterms_total);
-  HIP_CHECK(hipMemcpyAsync(grad, P.d_grad.p, n * sizeof(double), hipMemcpyDeviceToHost, P.stream));
-  HIP_CHECK(hipMemcpyAsync(t.data(), P.d_terms.p, t.size() * sizeof(double), hipMemcpyDeviceToHost, P.stream));
+  HIP_CHECK(hipMemcpyAsync(P.h_grad, P.d_grad.p, (n + t.size()) * sizeof(double), hipMemcpyDeviceToHost,
+                           P.stream));
   HIP_CHECK(hipStreamSynchronize(P.stream));
+  if (grad != P.h_grad) std::memcpy(grad, P.h_grad, n * sizeof(double));
+  std::memcpy(t.data(), P.h_terms, t.size() * sizeof(double));
   bool ok = true;
   for (const ProbDev& Q : P.probs) ok = ok && std::isfinite(t[Q.terms_off + SMILI_T_COST]);
-  for (size_t i = 0; ok && i < n; ++i) ok = std::isfinite(grad[i]);
+  // Inf and NaN have every exponent bit set: one branch-free integer pass
+  // (the compiler vectorises it)
+  const uint64_t* gb = reinterpret_cast<const uint64_t*>(grad);
+  uint64_t bad = 0;
+  for (size_t i = 0; i < n; ++i) bad |= (uint64_t)((~gb[i] & 0x7ff0000000000000ull) == 0);
+  ok = ok && bad == 0;
   if (!ok) g_last_error = "cost or gradient is not finite";
   return ok;
 }
@@ -1678,8 +1709,8 @@ int smili_cost_grad_device(smili_problem* p, const double* x_dev, double* grad_d
     P.eval();
     if (grad_dev != P.d_grad.p)
       HIP_CHECK(hipMemcpyAsync(grad_dev, P.d_grad.p, n * sizeof(double), hipMemcpyDeviceToDevice, P.stream));
-    if (terms_dev && terms_dev != P.d_terms.p)
-      HIP_CHECK(hipMemcpyAsync(terms_dev, P.d_terms.p, P.nterms_total * sizeof(double),
+    if (terms_dev && terms_dev != P.d_terms_p)
+      HIP_CHECK(hipMemcpyAsync(terms_dev, P.d_terms_p, P.nterms_total * sizeof(double),
                                hipMemcpyDeviceToDevice, P.stream));
   });
 }
@@ -1688,6 +1719,14 @@ int smili_cost_grad_device_noalloc(smili_problem* p, const double* x_dev, double
   return guarded([&] {
     require(p && x_dev && grad_dev && terms_dev, "NULL argument");
     eval_in_scratch(p->plan, x_dev, grad_dev, terms_dev);
+  });
+}
+
+int smili_problem_host_buffers(smili_problem* p, double** x_host, double** grad_host) {
+  return guarded([&] {
+    require(p != nullptr, "NULL problem");
+    if (x_host) *x_host = p->plan.h_x;
+    if (grad_host) *grad_host = p->plan.h_grad;
   });
 }
 
@@ -1706,7 +1745,7 @@ int smili_problem_buffers(smili_problem* p, double** x, double** grad, double** 
     require(p != nullptr, "NULL problem");
     if (x) *x = p->plan.d_x.p;
     if (grad) *grad = p->plan.d_grad.p;
-    if (terms) *terms = p->plan.d_terms.p;
+    if (terms) *terms = p->plan.d_terms_p;
   });
 }
 
@@ -1774,14 +1813,20 @@ int smili_adjoint_resid(smili_problem* p, const double* vres_re, const double* v
 
 int smili_problem_profile(smili_problem* p, int32_t reps, double* ms) {
   return guarded([&] {
-    require(p && ms && reps > 0, "invalid argument");
+    require(p && ms && reps != 0, "invalid argument");
     Plan& P = p->plan;
     for (auto& e : P.prof_ev)
       if (!e) HIP_CHECK(hipEventCreate(&e));
     double acc[SMILI_NSTAGES] = {};
+    // reps < 0: one pass, each stage launched -reps times back to back
+    // between its events (the event and dispatch overheads spread over
+    // the launches); per-launch time = elapsed / -reps
+    P.prof_repeat = reps < 0 ? -reps : 1;
+    const int passes = reps < 0 ? 1 : reps;
+    const double div = reps < 0 ? (double)-reps : (double)reps;
     P.prof = true;
     try {
-      for (int r = 0; r < reps; ++r) {
+      for (int r = 0; r < passes; ++r) {
         std::fill(P.prof_used, P.prof_used + SMILI_NSTAGES, 0);
         P.launch_eval();
         HIP_CHECK(hipStreamSynchronize(P.stream));
@@ -1794,10 +1839,29 @@ int smili_problem_profile(smili_problem* p, int32_t reps, double* ms) {
       }
     } catch (...) {
       P.prof = false;
+      P.prof_repeat = 1;
       throw;
     }
     P.prof = false;
-    for (int s = 0; s < SMILI_NSTAGES; ++s) ms[s] = acc[s] / reps;
+    P.prof_repeat = 1;
+    for (int s = 0; s < SMILI_NSTAGES; ++s) ms[s] = acc[s] / div;
+  });
+}
+
+int smili_problem_unit_info(smili_problem* p, int32_t unit, int64_t* info) {
+  return guarded([&] {
+    require(p && info, "NULL argument");
+    const Plan& P = p->plan;
+    require(unit >= 0 && unit < (int32_t)P.units.size(), "unit out of range");
+    const UnitDev& U = P.units[unit];
+    info[0] = U.m == 0 ? 0 : (P.unit_dft[unit] ? 2 : 1);
+    info[1] = P.unit_grid[unit].first;
+    info[2] = P.unit_grid[unit].second;
+    info[3] = U.m;
+    info[4] = U.nx;
+    info[5] = U.ny;
+    info[6] = U.nf1;
+    info[7] = U.nf2;
   });
 }
 

@@ -74,7 +74,7 @@ def _summary(terms, inormfactr, x, nz=1):
     return "\n".join(lines)
 
 
-def _setulb_loop(fg, x0, nonneg, m, factr, pgtol, maxiter, maxls, callback):
+def _setulb_loop(fg, x0, nonneg, m, factr, pgtol, maxiter, maxls, callback, xbuf=None):
     """L-BFGS-B through SciPy's own v3.0 routine ``setulb`` and its
     reverse-communication protocol, the loop fftim2d.f90:286-351 runs over
     lbfgsb.f's setulb: task FG -> evaluate f, g at x; task NEW_X -> one
@@ -82,6 +82,8 @@ def _setulb_loop(fg, x0, nonneg, m, factr, pgtol, maxiter, maxls, callback):
     ends the loop.  ``scipy.optimize.minimize`` drives the same routine the
     same way, but builds its bound arrays in a Python loop over every
     variable (seconds at a million pixels); here they are set whole.
+    ``xbuf``: an array of n float64 to keep the iterate in (the handle's
+    page-locked x, so an evaluation reads it without a host copy).
     Returns (x, iterations, message, success) or None when this SciPy's
     ``setulb`` does not take the call (then ``minimize`` is used)."""
     try:
@@ -90,7 +92,11 @@ def _setulb_loop(fg, x0, nonneg, m, factr, pgtol, maxiter, maxls, callback):
     except ImportError:
         return None
     n = x0.size
-    x = np.array(np.maximum(x0, 0.0) if nonneg else x0, dtype=np.float64)
+    if xbuf is not None:
+        x = xbuf
+        x[:] = np.maximum(x0, 0.0) if nonneg else x0
+    else:
+        x = np.array(np.maximum(x0, 0.0) if nonneg else x0, dtype=np.float64)
     nbd = np.full(n, 1 if nonneg else 0, np.int32)       # 1: lower bound only
     low, upp = np.zeros(n), np.zeros(n)
     f, g = 0.0, np.zeros(n)
@@ -123,6 +129,7 @@ def _setulb_loop(fg, x0, nonneg, m, factr, pgtol, maxiter, maxls, callback):
                 task[0], task[1] = 5, 504
         else:
             break
+    x = np.array(x, copy=True)               # the iterate outlives the handle's buffer
     if task[0] == 5 and task[1] == 505:
         return x, nit, "`callback` raised `StopIteration`.", False      # minimize's wording
     return x, nit, status_messages[task[0]] + ": " + task_messages[task[1]], bool(task[0] == 4)
@@ -145,9 +152,18 @@ def solve(problem: CostProblem, x0, niter=1000, nonneg=True, m=None, factr=None,
     hist, hist_t = [], []
     nz = getattr(problem, "Nz", 1)
 
+    # the handle's page-locked x and grad: setulb keeps its iterate in the
+    # first and reads the gradient from the second, so an evaluation copies
+    # nothing on the host (the reference's loop passes its x and g to
+    # calc_cost in place, fftim2d.f90:286-351)
+    bufs = problem.host_buffers() if hasattr(problem, "host_buffers") else None
+
     def fg(x):
         t = time.perf_counter()
-        c, g, terms = problem.cost_grad(x)
+        if bufs is not None:
+            c, g, terms = problem.cost_grad(x, out=bufs[1])
+        else:
+            c, g, terms = problem.cost_grad(x)
         state["eval_s"] += time.perf_counter() - t
         state["nfev"] += 1
         state["last"] = terms
@@ -173,11 +189,17 @@ def solve(problem: CostProblem, x0, niter=1000, nonneg=True, m=None, factr=None,
         print("Summary of the initial image")
         print(_summary(terms, inormfactr, x0, nz))
     out = None if driver == "minimize" else _setulb_loop(fg, x0, nonneg, int(m), float(factr), float(pgtol),
-                                                          int(niter) + 1, 20, cb)
+                                                          int(niter) + 1, 20, cb,
+                                                          None if bufs is None else bufs[0])
     if out is None:
         if driver == "setulb":
             raise RuntimeError("this SciPy's L-BFGS-B routine does not take the direct call")
-        res = minimize(fg, x0, jac=True, method="L-BFGS-B",
+        def fg_copy(x):
+            # minimize caches the gradient it is handed: give it its own array
+            c, g = fg(x)
+            return c, np.array(g, copy=True)
+
+        res = minimize(fg_copy, x0, jac=True, method="L-BFGS-B",
                        bounds=Bounds(np.zeros(x0.size), np.full(x0.size, np.inf)) if nonneg else None,
                        callback=cb,
                        options=dict(maxcor=int(m), ftol=float(factr) * np.finfo(float).eps, gtol=float(pgtol),
