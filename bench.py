@@ -470,7 +470,7 @@ def bench_warmstart(args):
         "config": {"workload": "warmstart: held-out seeds of the training distribution (rings, crescents, 1-5 "
                                "Gaussians, disks; 0.3-2 Jy), 128x128 at 1 uas, 6-10 stations at 230 GHz, "
                                "full-visibility chi-square + L1 + TSV, L-BFGS-B m=10",
-                   "seeds": [base, base + args.steps], "reached": sm["reached"],
+                   "seeds": [base, base + args.steps], "reached": sm["reached"], "n": sm["n"],
                    "median_iter_ratio": sm["median_iter_ratio"], "median_plain_s": sm["median_plain_s"],
                    "median_warm_s": sm["median_warm_s"], "max_nrmse": sm["max_nrmse"],
                    "proposals_used": sm["proposals_used"], "n_train": int(meta.get("n_train", 0))},
@@ -508,7 +508,8 @@ def bench_warmstart_movie(args):
         n_ev = 100
         ms = _timed_loop(g, n_ev, 5) / n_ev * 1e3
         roof = _roofline(g, [dict(kw, units=_movie_units(kw))], True, ms, n_ev, "movie")
-    ratios = [r["plain_s"] / r["warm_s"] for r in rows if r["warm_s"]]
+    from smili_amd.warmstart.solve import time_ratio
+    ratios = [time_ratio(r) for r in rows]          # unreached warm solves count as losses
     cpu = None
     if not args.no_cpu:
         c = cpu_baseline_movie(prep, args.cpu_budget)
@@ -540,6 +541,7 @@ def bench_warmstart_movie(args):
                    "movie_seeds": [5 + args.seed, 5 + args.seed + args.steps],
                    "rows": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in rows],
                    "median_plain_s": med("plain_s"), "median_warm_s": med("warm_s"),
+                   "reached": sum(bool(r["reached"]) for r in rows), "n": len(rows),
                    "n_train": int(meta.get("n_train", 0))},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -69,9 +69,11 @@ int smili_unet_train(smili_unet* net, const int64_t* index, int64_t nsteps, cons
                      const float* hyper, double* losses);
 int smili_unet_reset_optimizer(smili_unet* net);
 
-/* Times `reps` training steps on the first `batch` uploaded samples
- * (parameters are updated) with HIP events on the handle's stream:
- * ms[0] = mean step, ms[1] = mean forward pass. */
+/* Times `reps` training steps on the first `batch` uploaded samples with
+ * HIP events on the handle's stream: ms[0] = mean step, ms[1] = mean
+ * forward pass.  The steps run with learning rate 0, and the Adam moments
+ * and step count are restored afterwards: the handle's training state is
+ * what it was before the call. */
 int smili_unet_time_steps(smili_unet* net, int32_t reps, float* ms);
 
 /* Times `reps` launches of one 3x3 convolution kernel of layer `layer`

@@ -1596,20 +1596,46 @@ int smili_unet_time_steps(smili_unet* n, int32_t reps, float* ms) {
   std::vector<float> lr(reps, 0.f);
   int r = upload_steps(n, index.data(), reps, lr.data());
   if (r) return r;
+  // the timed steps run with learning rate 0 (and no weight decay), so the
+  // parameters stay; the Adam moments and step count they advance are
+  // saved here and restored below, so a training run can continue as if
+  // the timing had not happened
+  float* saved = nullptr;
+  HIPCHK(hipMalloc(&saved, 2 * n->np * sizeof(float)));
+  const int64_t step0 = n->step;
+  auto restore = [&]() {
+    (void)hipStreamSynchronize(n->s);
+    (void)hipMemcpy(n->m, saved, n->np * sizeof(float), hipMemcpyDeviceToDevice);
+    (void)hipMemcpy(n->v, saved + n->np, n->np * sizeof(float), hipMemcpyDeviceToDevice);
+    (void)hipFree(saved);
+    n->step = step0;
+  };
+  if (hipMemcpyAsync(saved, n->m, n->np * sizeof(float), hipMemcpyDeviceToDevice, n->s) != hipSuccess ||
+      hipMemcpyAsync(saved + n->np, n->v, n->np * sizeof(float), hipMemcpyDeviceToDevice, n->s) != hipSuccess) {
+    (void)hipFree(saved);
+    return fail(-2, "copy of the optimizer state failed");
+  }
   const float hyper[4] = {0.9f, 0.999f, 1e-8f, 0.f};
-  HIPCHK(hipStreamSynchronize(n->s));
-  HIPCHK(hipEventRecord(n->ev0, n->s));
-  r = train_steps(n, reps, hyper);
-  if (r) return r;
-  HIPCHK(hipEventRecord(n->ev1, n->s));
-  // forward passes alone
-  for (int i = 0; i < reps; ++i) forward(n, n->B);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(n->ev2, n->s));
-  HIPCHK(hipEventSynchronize(n->ev2));
+  if (hipStreamSynchronize(n->s) != hipSuccess) {
+    restore();
+    return fail(-2, "stream synchronisation failed");
+  }
   float t_train = 0.f, t_fwd = 0.f;
-  HIPCHK(hipEventElapsedTime(&t_train, n->ev0, n->ev1));
-  HIPCHK(hipEventElapsedTime(&t_fwd, n->ev1, n->ev2));
+  hipError_t e = hipEventRecord(n->ev0, n->s);
+  r = e == hipSuccess ? train_steps(n, reps, hyper) : -2;
+  if (r == 0) {
+    (void)hipEventRecord(n->ev1, n->s);
+    // forward passes alone
+    for (int i = 0; i < reps; ++i) forward(n, n->B);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(n->ev2, n->s);
+    if (e == hipSuccess) e = hipEventSynchronize(n->ev2);
+    if (e == hipSuccess) e = hipEventElapsedTime(&t_train, n->ev0, n->ev1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&t_fwd, n->ev1, n->ev2);
+  }
+  restore();
+  if (r) return r;
+  if (e != hipSuccess) return fail(-2, hipGetErrorString(e));
   ms[0] = t_train / reps;
   ms[1] = t_fwd / reps;
   return 0;

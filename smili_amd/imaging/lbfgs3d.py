@@ -59,6 +59,16 @@ def imaging3d(initmovie, imregion=None, vistable=None, amptable=None, bstable=No
     Nt = initmovie.Nt
     initimage = initmovie.images[0]
     hdr = initimage.header
+    if warmstart is not None:
+        # the proposals are made from each frame's visibilities and binned
+        # onto a grid centred on its reference pixel with RA to the left
+        if vistable is None:
+            raise ValueError("warmstart needs vistable: the proposals come from the frames' visibilities")
+        centred = (abs(hdr["nxref"] - (hdr["nx"] + 1) / 2.0) < 1e-9 and
+                   abs(hdr["nyref"] - (hdr["ny"] + 1) / 2.0) < 1e-9)
+        if not centred or not hdr["dx"] < 0:
+            raise ValueError("warmstart needs the reference pixel at the image centre ((nx + 1) / 2, "
+                             "(ny + 1) / 2) and dx < 0")
     window = _window_of(imregion, initimage, istokes, ifreq)
 
     def prior(p):

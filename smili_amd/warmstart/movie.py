@@ -92,6 +92,7 @@ def warm_solve_movie(net, vis, prep, pix_uas, niter=1000, m=10, stop_cost=None, 
     proposal)."""
     from ..cost import MovieProblem
     from ..imaging import lbfgs
+    t0 = time.perf_counter()
     with MovieProblem(**prep.kwargs, tune=tune) as prob:
         x0, tm = propose_movie(net, vis, prep, prob, pix_uas)
         f_prior = prob.cost_grad(prep.x0)[0]
@@ -99,8 +100,9 @@ def warm_solve_movie(net, vis, prep, pix_uas, niter=1000, m=10, stop_cost=None, 
         start = "proposal"
         if not f_prop < f_prior:
             x0, start = prep.x0, "prior"
+        t_setup = time.perf_counter() - t0
         x, info = lbfgs.solve(prob, x0, niter=niter, m=m, verbose=False, stop_cost=stop_cost)
-    return x, dict(info=info, start=start, f_prior=f_prior, f_proposal=f_prop, **tm)
+    return x, dict(info=info, start=start, f_prior=f_prior, f_proposal=f_prop, setup_s=t_setup, **tm)
 
 
 def time_to_image_movie(net, prep, vis, pix_uas, rel=1e-6, niter=1000, m=10, tune=None, truth=None,
@@ -141,7 +143,7 @@ def time_to_image_movie(net, prep, vis, pix_uas, rel=1e-6, niter=1000, m=10, tun
     xw, w = warm_solve_movie(net, vis, prep, pix_uas, niter=niter, m=m, stop_cost=target, tune=tune)
     wall_w = time.perf_counter() - t0
     iw = w["info"]
-    kw, _ = _reach(iw, target)
+    kw, tw = _reach(iw, target)
 
     def dist(a, b):
         return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
@@ -150,7 +152,8 @@ def time_to_image_movie(net, prep, vis, pix_uas, rel=1e-6, niter=1000, m=10, tun
                gap_proposal=w["f_proposal"] / ip.cost - 1.0, start=w["start"],
                plain_iters=kp, plain_s=t_setup + tp, plain_total_iters=ip.niter, plain_nfev=ip.nfev,
                plain_total_s=t_setup + ip.wall_s, plain_cost=ip.cost,
-               warm_iters=kw, warm_nfev=iw.nfev, warm_s=wall_w if kw is not None else None,
+               warm_iters=kw, warm_nfev=iw.nfev, warm_s=(w["setup_s"] + tw) if kw is not None else None,
+               warm_wall_s=wall_w,
                warm_cost=iw.cost, reached=kw is not None, nrmse=dist(xw, xp),
                nrmse_plain_at_target=dist(x_tgt, xp) if track else None,
                dirty_s=w["dirty_s"], net_s=w["net_s"], fit_s=w["fit_s"], frames_proposed=w["frames_proposed"],

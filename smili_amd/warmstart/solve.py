@@ -133,13 +133,24 @@ def gap_curve(net, seed, cfg=D.CFG, gaps=GAPS):
                 warm=[first(w["info"].history, g) for g in gaps])
 
 
+def time_ratio(r):
+    """plain / warm time-to-image of one row; a warm solve that never
+    reached the matched objective counts as a loss, plain time over its
+    whole wall time, capped at 1."""
+    if r["reached"]:
+        return r["plain_s"] / r["warm_s"]
+    return min(r["plain_s"] / r["warm_wall_s"], 1.0)
+
+
 def summary(rows):
+    """Medians over all rows (unreached warm solves count as losses, see
+    time_ratio); ``reached`` of ``n`` rows reached the matched objective."""
     ok = [r for r in rows if r["reached"]]
     it_ratio = [r["plain_iters"] / r["warm_iters"] for r in ok]
-    t_ratio = [r["plain_s"] / r["warm_s"] for r in ok]
+    t_ratio = [time_ratio(r) for r in rows]
     return dict(n=len(rows), reached=len(ok), median_iter_ratio=float(np.median(it_ratio)) if ok else None,
-                median_time_ratio=float(np.median(t_ratio)) if ok else None,
-                geomean_time_ratio=float(np.exp(np.mean(np.log(t_ratio)))) if ok else None,
+                median_time_ratio=float(np.median(t_ratio)) if rows else None,
+                geomean_time_ratio=float(np.exp(np.mean(np.log(t_ratio)))) if rows else None,
                 median_plain_s=float(np.median([r["plain_s"] for r in rows])),
                 median_warm_s=float(np.median([r["warm_s"] for r in ok])) if ok else None,
                 max_nrmse=float(max(r["nrmse"] for r in ok)) if ok else None,

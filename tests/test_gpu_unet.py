@@ -183,3 +183,30 @@ def test_symmetry_index_matches_host_transform(gpu_lib):
             Lr, _ = net.loss_grad(xr, tr)
             assert abs(L - Lr) <= 1e-9 * Lr, (k, L, Lr)
         assert np.array_equal(net.params, theta)
+
+
+def test_time_steps_leave_the_training_state(net_case):
+    """Timing steps in the middle of training leave the parameters, the Adam
+    moments and the step count as they were: training on afterwards gives
+    the same parameters as training without the timing (to the float
+    reductions' rounding)."""
+    net, theta, rng = net_case
+    from smili_amd.warmstart import unet
+    x = _inputs(rng, 4, 64, 128)
+    t = np.zeros((4, 64, 128), np.float32)
+    t[:, 24:40, 48:80] = 1.0
+    idx = 8 * np.array([[0, 1], [2, 3], [1, 2]])
+    out = []
+    for timed in (False, True):
+        with unet.UNet(batch=2, height=64, width=128, params=theta) as n2:
+            n2.set_data(x, t)
+            n2.train(idx[:2], 1e-3)
+            if timed:
+                p_before = n2.params
+                n2.time_steps(3)
+                assert np.array_equal(n2.params, p_before)
+            n2.train(idx[2:], 1e-3)
+            out.append(n2.params)
+    rel = np.linalg.norm(out[1] - out[0]) / np.linalg.norm(out[0])
+    print("params after timing vs without", rel)
+    assert rel <= 1e-6

@@ -127,3 +127,28 @@ def test_unsorted_frames_keep_data_with_their_points():
     np.testing.assert_array_equal(a["u"], b["u"])
     key = lambda kw: np.sort(kw["uvidxfcv"] * 1e6 + np.round(np.abs(kw["Vfcv"]), 6))
     np.testing.assert_allclose(key(a), key(b))
+
+
+def test_warmstart_rejects_off_centre_or_flipped_movies():
+    """imaging3d(warmstart=...) bins the network's proposals onto a grid
+    centred on the reference pixel with RA to the left: other geometries
+    and closure-only imaging (no visibilities to propose from) are rejected
+    before anything is evaluated."""
+    import pytest
+    from smili_amd.imaging.image import Image, Movie
+    from smili_amd.imaging.lbfgs3d import imaging3d
+    vis = {"u": np.zeros(3), "v": np.zeros(3), "amp": np.ones(3), "phase": np.zeros(3),
+           "sigma": np.ones(3), "frmidx": np.zeros(3, int)}
+    net = object()
+    ok = Movie([Image(16, 16, 2.0) for _ in range(2)])
+    assert ok.images[0].header["dx"] < 0
+    with pytest.raises(ValueError, match="vistable"):
+        imaging3d(ok, warmstart=net)
+    off = Movie([Image(16, 16, 2.0, nxref=5.0) for _ in range(2)])
+    with pytest.raises(ValueError, match="reference pixel"):
+        imaging3d(off, vistable=vis, warmstart=net)
+    flipped = Movie([Image(16, 16, 2.0) for _ in range(2)])
+    for im in flipped.images:
+        im.header["dx"] = abs(im.header["dx"])
+    with pytest.raises(ValueError, match="dx < 0"):
+        imaging3d(flipped, vistable=vis, warmstart=net)

@@ -98,9 +98,12 @@ def test_time_to_image_bookkeeping():
                  gap_proposal=1.0, warm_start="proposal"),
             dict(reached=True, plain_iters=90, warm_iters=90, plain_s=0.9, warm_s=0.9, nrmse=2e-3, gap_prior=20.0,
                  gap_proposal=3.0, warm_start="proposal"),
-            dict(reached=False, plain_iters=80, warm_iters=None, plain_s=0.8, warm_s=None, nrmse=1.0, gap_prior=5.0,
-                 gap_proposal=9.0, warm_start="prior")]
+            dict(reached=False, plain_iters=80, warm_iters=None, plain_s=0.8, warm_s=None, warm_wall_s=1.6,
+                 nrmse=1.0, gap_prior=5.0, gap_proposal=9.0, warm_start="prior")]
     s = solve.summary(rows)
     assert s["n"] == 3 and s["reached"] == 2 and s["proposals_used"] == 2
-    assert np.isclose(s["median_iter_ratio"], 1.5) and np.isclose(s["median_time_ratio"], 1.5)
-    assert np.isclose(s["geomean_time_ratio"], math.sqrt(2.0)) and s["max_nrmse"] == 2e-3
+    # the unreached row counts as a loss: 0.8 / 1.6; ratios 2, 1, 0.5
+    assert np.isclose(s["median_iter_ratio"], 1.5) and np.isclose(s["median_time_ratio"], 1.0)
+    assert np.isclose(s["geomean_time_ratio"], 1.0) and s["max_nrmse"] == 2e-3
+    # an unreached row never counts as a gain
+    assert solve.time_ratio(dict(reached=False, plain_s=5.0, warm_wall_s=1.0)) == 1.0
