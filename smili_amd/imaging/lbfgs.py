@@ -21,6 +21,7 @@ import numpy as np
 from scipy.optimize import Bounds, minimize
 
 from ..cost import CostProblem
+from . import deburr as D
 from .prep import prepare
 
 lbfgsbprms = {
@@ -248,8 +249,6 @@ def imaging(initimage, imregion=None, vistable=None, amptable=None, bstable=None
     used, lbfgs.py:279-283).  Returns a copy of initimage holding the
     solution (lbfgs.py:565-576); with ``return_info`` also the SolveInfo.
     """
-    if pulse_fwhm > 0:
-        raise NotImplementedError("pulse_fwhm (deburring, lbfgs.py:1114-1194) is not supported")
     if isinstance(tune, str) and tune != "auto":
         raise ValueError('tune must be a dict of smili_tune fields, None or "auto"')
     hdr = initimage.header
@@ -258,6 +257,18 @@ def imaging(initimage, imregion=None, vistable=None, amptable=None, bstable=None
     def prior(p):
         return None if p is None else np.asarray(p.data)[0, 0]
 
+    if pulse_fwhm > 0:
+        # the total flux comes from the tables as given (lbfgs.py:203-209),
+        # then every table is divided by the pulse's Fourier transform
+        # (lbfgs.py:436-478)
+        if totalflux is None:
+            amps = [np.max(np.asarray(t["amp"], np.float64)) for t in (vistable, amptable) if t is not None]
+            totalflux = float(np.max(amps)) if amps else None
+        fw = np.deg2rad(pulse_fwhm * _angconv(initimage))
+        vistable = None if vistable is None else D.deburr_vistable(vistable, fw)
+        amptable = None if amptable is None else D.deburr_vistable(amptable, fw)
+        bstable = None if bstable is None else D.deburr_bstable(bstable, fw)
+        catable = None if catable is None else D.deburr_catable(catable, fw)
     prep = prepare(np.asarray(initimage.data, np.float64)[istokes, ifreq],
                    np.deg2rad(hdr["dx"]), np.deg2rad(hdr["dy"]), hdr["nxref"], hdr["nyref"], window,
                    vistable, amptable, bstable, catable, w_vis, w_amp, w_bs, w_ca,
@@ -277,6 +288,12 @@ def imaging(initimage, imregion=None, vistable=None, amptable=None, bstable=None
     data = np.array(out.data, dtype=np.float64, copy=True)
     data[istokes, ifreq] = 0.0
     data[istokes, ifreq, prep.yidx - 1, prep.xidx - 1] = Iout
+    if pulse_fwhm > 0:
+        # the solution convolved with the pulse, total flux kept
+        # (lbfgs.py:574-575; IMFITS.convolve_gauss, imdata.py:1705-1774)
+        fpx = pulse_fwhm * _angconv(initimage) / abs(hdr["dx"])
+        fpy = pulse_fwhm * _angconv(initimage) / abs(hdr["dy"])
+        data[istokes, ifreq] = D.convolve_gauss(data[istokes, ifreq], fpx, fpy, save_totalflux=True)
     out.data = data
     if hasattr(out, "update_fits"):
         out.update_fits()

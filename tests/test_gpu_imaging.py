@@ -67,3 +67,36 @@ def test_imaging_reproduces_reference_solve(gpu_lib, precision, tol_img):
           "ref", float(gold["cost"]), "NRMSE", e, "wall", info.wall_s, "eval", info.eval_s)
     assert e <= tol_img
     assert abs(info.cost - float(gold["cost"])) <= 1e-6 * abs(float(gold["cost"]))
+
+
+def test_imaging_pulse_fwhm(gpu_lib):
+    """imaging(pulse_fwhm=theta) (lbfgs.py:203-209, 436-478, 574-575) equals
+    the same solve on tables divided by the pulse's Fourier transform, with
+    the total flux taken from the raw table, followed by the Gaussian
+    convolution of the solution; the convolution keeps the solution's total
+    flux.  200 iterations of the 64x64 ring workload, FP64."""
+    from smili_amd import synth, workloads
+    from smili_amd.imaging import deburr as D
+    from smili_amd.imaging import lbfgs
+    from smili_amd.imaging.image import Image
+    prep, extra = workloads.small_fcv()
+    init = Image(64, 64, 2.0, angunit="uas")
+    init.data[0, 0] = synth.sky_image([synth.Gaussian(0.6, 20 * synth.UAS)], 64, 64, -2 * synth.UAS, 2 * synth.UAS)
+    vis = extra["vis"]
+    theta_uas = 6.0
+    common = dict(l1_lambda=1.0, tsv_lambda=1e4, niter=200, nonneg=True, nprint=0, verbose=False,
+                  return_info=True, tune=dict(precision=64))
+    out, info = lbfgs.imaging(init, vistable=vis, pulse_fwhm=theta_uas, **common)
+    tf = float(np.max(np.asarray(vis["amp"], np.float64)))
+    vis_d = D.deburr_vistable(vis, np.deg2rad(theta_uas / 3.6e9))
+    plain, info_p = lbfgs.imaging(init, vistable=vis_d, totalflux=tf, **common)
+    conv = D.convolve_gauss(plain.data[0, 0], theta_uas / 2.0, theta_uas / 2.0, save_totalflux=True)
+    e = rel_l2(out.data[0, 0], conv)
+    print("pulse_fwhm: nit", info.niter, info_p.niter, "cost", info.cost, info_p.cost, "rel", e)
+    # the same problem solved twice: only the spreading atomics' rounding
+    # order differs between the two runs
+    assert info.cost == pytest.approx(info_p.cost, rel=1e-8)
+    assert e <= 1e-5
+    assert conv.sum() == pytest.approx(plain.data[0, 0].sum(), rel=1e-12)
+    assert out.data[0, 0].sum() == pytest.approx(plain.data[0, 0].sum(), rel=1e-5)
+    assert np.array_equal(np.asarray(vis["amp"]), np.asarray(extra["vis"]["amp"]))
