@@ -821,13 +821,39 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     });
     shift_chunks(DT_AMP, amp0);
     build_type(DT_CP, D.ncp, 3, D.uvidxcp, [&](long long i, const unsigned short* sl) {
-      cp.push_back(CpItem{D.cp[i], D.wcp / D.varcp[i], {sl[0], sl[1], sl[2]}, 0});
+      cp.push_back(CpItem{std::cos(D.cp[i]), std::sin(D.cp[i]), D.wcp / D.varcp[i], {sl[0], sl[1], sl[2]}, 0});
     });
     shift_chunks(DT_CP, cp0);
     build_type(DT_CA, D.nca, 4, D.uvidxca, [&](long long i, const unsigned short* sl) {
       ca.push_back(CaItem{D.ca[i], D.wca / D.varca[i], {sl[0], sl[1], sl[2], sl[3]}});
     });
     shift_chunks(DT_CA, ca0);
+  }
+  // Closure chunks in the order their threads read them: lane l of a wave
+  // takes the chunk's closures l Q, l Q + 1, ... (closure_item), and the
+  // closure at memory position j of the chunk is closure_item(j, Q), so a
+  // wave's loads are contiguous.  64 Q positions per chunk; the few past
+  // the chunk's count stay zero and are never read.
+  auto interleave = [&](auto& vec, int type) {
+    using Item = typename std::decay_t<decltype(vec)>::value_type;
+    std::vector<Item> out;
+    out.reserve(vec.size() + 64 * h_chunks.size());
+    for (Chunk& c : h_chunks) {
+      if (c.type != type) continue;
+      const int q = (c.count + 63) >> 6;
+      const size_t base = out.size();
+      out.resize(base + 64 * (size_t)q, Item{});
+      for (int j = 0; j < 64 * q; ++j) {
+        const int i = (j & 63) * q + (j >> 6);
+        if (i < c.count) out[base + j] = vec[(size_t)c.start + i];
+      }
+      c.start = (int)base;
+    }
+    vec.swap(out);
+  };
+  if (SMILI_CHISQ_INTERLEAVE) {
+    interleave(cp, DT_CP);
+    interleave(ca, DT_CA);
   }
 
   // each point's references into the chunks' slot shares, in chunk order
@@ -858,7 +884,7 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
       col_group /= 2;
   }
   // regularizer items, row/column items
-  const int reg_chunk = 512;
+  const int reg_chunk = kThreads;
   std::vector<int> unit_reg_begin(units.size() + 1, 0);
   // FFT work-groups: about 512 points per row group (one 64-thread wave of
   // radix-8 butterflies), 64 threads per column of a column group

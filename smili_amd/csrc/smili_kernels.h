@@ -51,13 +51,16 @@ struct UnitDev {
 // LDS; each datum refers to them by chunk-local slot numbers.  A slot field
 // holds the slot in bits 0..14 and the sign of the uv index (1 = the datum
 // sits at (-u, -v)) in bit 15.  w = fnorm / variance.
-constexpr int kMaxSlots = 1024;      // distinct uv points per chunk
+#ifndef SMILI_MAX_SLOTS
+#define SMILI_MAX_SLOTS 1024
+#endif
+constexpr int kMaxSlots = SMILI_MAX_SLOTS;      // distinct uv points per chunk
 constexpr int kSlotNeg = 0x8000;
 // pt: the datum's global point (the direct path reads V[pt] without the
 // slot table)
 struct FcvItem { double vr, vi, w; unsigned short slot, pad0; int pt; };
 struct AmpItem { double amp, w; unsigned short slot, pad0; int pt; };
-struct CpItem  { double cp, w; unsigned short slot[3], pad0; };
+struct CpItem  { double cc, sc, w; unsigned short slot[3], pad0; };   // cos, sin of the closure phase
 struct CaItem  { double ca, w; unsigned short slot[4]; };
 
 enum { DT_FCV = 0, DT_AMP = 1, DT_CP = 2, DT_CA = 3 };

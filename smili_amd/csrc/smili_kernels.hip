@@ -80,11 +80,9 @@ __device__ __forceinline__ int closure_item(int j, int q) {
   return SMILI_CHISQ_INTERLEAVE ? (j & 63) * q + (j >> 6) : j;
 }
 
-// The FP64 transcendental functions of the chi-square terms, out of line
-// for the LEAN kernels (below).
-__device__ __noinline__ double arg_nl(double y, double x) { return atan2(y, x); }
+// The FP64 logarithm of the closure-amplitude points, out of line for the
+// LEAN kernels (below).
 __device__ __noinline__ double log_nl(double x) { return log(x); }
-__device__ __noinline__ void sincos_nl(double r, double& s, double& c) { sincos(r, &s, &c); }
 
 // Each thread loads kChisqBatch of its data before it processes them, so
 // a wave keeps that many item loads in flight (measured neutral at 1, 2, 4
@@ -101,6 +99,174 @@ constexpr int kChisqBatch = SMILI_CHISQ_BATCH;
 // of large plans (movie 0.258 -> 0.252 ms); slower inside the fused
 // chi-square + regularizer launch of a single problem (C2 0.1045 ->
 // 0.1055 ms), which keeps the inline code.
+
+// A chunk's data, kChisqBatch per thread per round trip.  Closure chunks
+// are stored in the interleaved order (position j holds closure
+// closure_item(j, Q), arranged so on the host), so the loads of a wave are
+// contiguous; j >= lim or a position past the chunk's count is not loaded
+// (ok = false).
+template <typename Item>
+__device__ __forceinline__ void load_items(Item (&d)[kChisqBatch], bool (&ok)[kChisqBatch],
+                                           const Item* __restrict__ src, int j0, int lim, int count, int q,
+                                           bool closure) {
+#pragma unroll
+  for (int b = 0; b < kChisqBatch; ++b) {
+    const int j = j0 + b * (int)blockDim.x;
+    const int i = closure ? closure_item(j, q) : j;
+    ok[b] = j < lim && i < count;
+    if (ok[b]) d[b] = src[closure && SMILI_CHISQ_INTERLEAVE ? j : i];
+  }
+}
+
+// One datum of a staged chunk: its chi-square, and its scalar(s) added to
+// the accumulators of the points it references.
+template <bool LEAN>
+__device__ __forceinline__ double datum(const FcvItem& d, const double2* Vl, const double2*, double (*Sl)[kMaxSlots]) {
+  const int s = d.slot & 0x7fff;
+  const double sg = (d.slot & kSlotNeg) ? -1.0 : 1.0;
+  const double2 m = Vl[s];
+  const double rr = d.vr - m.x;
+  const double ri = d.vi - sg * m.y;           // conj for negative index
+  const double f = -2.0 * d.w;
+  atomicAdd(&Sl[0][s], f * rr);
+  atomicAdd(&Sl[1][s], f * ri * sg);
+  return (rr * rr + ri * ri) * d.w;
+}
+template <bool LEAN>
+__device__ __forceinline__ double datum(const AmpItem& d, const double2*, const double2* Al, double (*Sl)[kMaxSlots]) {
+  const int s = d.slot & 0x7fff;
+  const double2 al = Al[s];
+  const double r = d.amp - al.x;
+  atomicAdd(&Sl[0][s], -2.0 * r * d.w * al.y);
+  return r * r * d.w;
+}
+// Closure phase: with B = V1 V2 V3 (each conjugated for a negative
+// index), arg B is the model closure phase and |B| = |V1||V2||V3|, so the
+// residual r = cp - arg B has cos r = (cos cp Re B + sin cp Im B) / |B| and
+// sin r = (sin cp Re B - cos cp Im B) / |B|: the terms 2 (1 - cos r) and
+// -2 sin r of fftlib.f90:617-653 with no transcendental per datum (cos cp
+// and sin cp come with the datum, 1/|V| with the staged point).
+template <bool LEAN>
+__device__ __forceinline__ double datum(const CpItem& d, const double2* Vl, const double2* Al, double (*Sl)[kMaxSlots]) {
+  double sg[3], br = 1.0, bi = 0.0, inv = 1.0;
+  int s[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    s[k] = d.slot[k] & 0x7fff;
+    sg[k] = (d.slot[k] & kSlotNeg) ? -1.0 : 1.0;
+    const double2 m = Vl[s[k]];
+    const double mi = sg[k] * m.y;
+    const double tr = br * m.x - bi * mi;
+    bi = br * mi + bi * m.x;
+    br = tr;
+    inv *= Al[s[k]].x;
+  }
+  const double cr = (d.cc * br + d.sc * bi) * inv;
+  const double sr = (d.sc * br - d.cc * bi) * inv;
+  const double f = -2.0 * sr * d.w;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) atomicAdd(&Sl[0][s[k]], f * sg[k]);
+  return 2.0 * (1.0 - cr) * d.w;
+}
+template <bool LEAN>
+__device__ __forceinline__ double datum(const CaItem& d, const double2*, const double2* Al, double (*Sl)[kMaxSlots]) {
+  double l[4];
+  int s[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    s[k] = d.slot[k] & 0x7fff;     // signs are ignored (fftlib.f90:549-553)
+    l[k] = Al[s[k]].x;
+  }
+  // log|V| = 0.5 log|V|^2
+  const double model = 0.5 * (l[0] + l[1] - l[2] - l[3]);
+  const double r = d.ca - model;
+  const double f = -2.0 * r * d.w;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) atomicAdd(&Sl[0][s[k]], k < 2 ? f : -f);
+  return r * r * d.w;
+}
+
+// A chunk whose data share uv points.  Every step is one round trip to
+// memory at most, and the data loads overlap the rest: the first batch of
+// data is requested before the points are staged, and each later batch
+// while the one before it is processed.
+//   stage: the chunk's point indices, then their model visibilities
+//   (kMaxSlots / kThreads per thread, all requested at once), and once per
+//   point rather than once per reference: amp |V| and 1/|V|; closure phase
+//   1/|V| and 1/|V|^2; closure amplitude log|V|^2 and 1/|V|^2
+//   data: chi-square, and the datum's scalars into the points' accumulators
+//   out: each point's share, its accumulator(s) times its fixed vector
+template <int TYPE, bool LEAN, typename Item, typename VT>
+__device__ __forceinline__ double staged_chunk(const Chunk& ch, const int* __restrict__ sl,
+                                               const Item* __restrict__ src, const VT* __restrict__ V,
+                                               double2* __restrict__ out, double2* Vl, double2* Al,
+                                               double (*Sl)[kMaxSlots]) {
+  constexpr int NB = kChisqBatch;
+  constexpr int SB = kMaxSlots / kThreads;
+  constexpr bool closure = TYPE == DT_CP || TYPE == DT_CA;
+  const int nt = blockDim.x;
+  const int q = (ch.count + 63) >> 6;
+  const int lim = closure ? 64 * q : ch.count;
+  src += ch.start;
+  Item cur[NB];
+  bool ok[NB];
+  load_items(cur, ok, src, threadIdx.x, lim, ch.count, q, closure);
+  for (int s0 = threadIdx.x; s0 < ch.nslots; s0 += nt * SB) {
+    int g[SB];
+    VT mv[SB];
+#pragma unroll
+    for (int b = 0; b < SB; ++b)
+      if (s0 + b * nt < ch.nslots) g[b] = sl[s0 + b * nt];
+#pragma unroll
+    for (int b = 0; b < SB; ++b)
+      if (s0 + b * nt < ch.nslots) mv[b] = V[g[b]];
+#pragma unroll
+    for (int b = 0; b < SB; ++b) {
+      const int s = s0 + b * nt;
+      if (s >= ch.nslots) break;
+      const double2 m = make_double2((double)mv[b].x, (double)mv[b].y);
+      Vl[s] = m;
+      Sl[0][s] = 0.0;
+      Sl[1][s] = 0.0;
+      const double a2 = m.x * m.x + m.y * m.y;
+      if (TYPE == DT_AMP) {
+        const double mod = sqrt(a2);
+        Al[s] = make_double2(mod, 1.0 / mod);
+      } else if (TYPE == DT_CP) {
+        Al[s] = make_double2(1.0 / sqrt(a2), 1.0 / a2);
+      } else if (TYPE == DT_CA) {
+        Al[s] = make_double2(LEAN ? log_nl(a2) : log(a2), 1.0 / a2);
+      }
+    }
+  }
+  __syncthreads();
+  double acc = 0.0;
+  for (int j0 = threadIdx.x; j0 < lim; j0 += nt * NB) {
+    Item nxt[NB];
+    bool nok[NB];
+    load_items(nxt, nok, src, j0 + nt * NB, lim, ch.count, q, closure);
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+      if (ok[b]) acc += datum<LEAN>(cur[b], Vl, Al, Sl);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      cur[b] = nxt[b];
+      ok[b] = nok[b];
+    }
+  }
+  __syncthreads();
+  for (int s = threadIdx.x; s < ch.nslots; s += nt) {
+    const double2 m = Vl[s];
+    const double a = Sl[0][s];
+    double2 r;
+    if (TYPE == DT_FCV) r = make_double2(a, Sl[1][s]);
+    else if (TYPE == DT_AMP) r = make_double2(a * m.x, a * m.y);
+    else if (TYPE == DT_CP) r = make_double2(-a * Al[s].y * m.y, a * Al[s].y * m.x);
+    else r = make_double2(a * Al[s].y * m.x, a * Al[s].y * m.y);
+    out[s] = r;
+  }
+  return acc;
+}
 
 template <typename VT, bool LEAN>
 __device__ void chisq_block(int bid, const Chunk* __restrict__ chunks, const int* __restrict__ slots,
@@ -167,142 +333,11 @@ __device__ void chisq_block(int bid, const Chunk* __restrict__ chunks, const int
     if (threadIdx.x == 0) partial[bid] = v[0];
     return;
   }
-  // stage the chunk's model visibilities and, once per point rather than
-  // once per reference: amp 1/|V|; closure phase arg V and 1/|V|^2;
-  // closure amplitude log|V|^2 and 1/|V|^2
-  for (int s = threadIdx.x; s < ch.nslots; s += nt) {
-    const VT mv = V[sl[s]];
-    const double2 m = make_double2((double)mv.x, (double)mv.y);
-    Vl[s] = m;
-    Sl[0][s] = 0.0;
-    Sl[1][s] = 0.0;
-    const double a2 = m.x * m.x + m.y * m.y;
-    if (ch.type == DT_AMP) {
-      const double mod = sqrt(a2);
-      Al[s] = make_double2(mod, 1.0 / mod);
-    } else if (ch.type == DT_CP) {
-      Al[s] = make_double2(LEAN ? arg_nl(m.y, m.x) : atan2(m.y, m.x), 1.0 / a2);
-    } else if (ch.type == DT_CA) {
-      Al[s] = make_double2(LEAN ? log_nl(a2) : log(a2), 1.0 / a2);
-    }
-  }
-  __syncthreads();
-  if (ch.type == DT_FCV) {
-    for (int i0 = threadIdx.x; i0 < ch.count; i0 += nt * NB) {
-      FcvItem d[NB];
-#pragma unroll
-      for (int b = 0; b < NB; ++b)
-        if (i0 + b * nt < ch.count) d[b] = fcv[ch.start + i0 + b * nt];
-#pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        if (i0 + b * nt >= ch.count) break;
-        const int s = d[b].slot & 0x7fff;
-        const double sg = (d[b].slot & kSlotNeg) ? -1.0 : 1.0;
-        const double2 m = Vl[s];
-        const double rr = d[b].vr - m.x;
-        const double ri = d[b].vi - sg * m.y;           // conj for negative index
-        acc += (rr * rr + ri * ri) * d[b].w;
-        const double f = -2.0 * d[b].w;
-        atomicAdd(&Sl[0][s], f * rr);
-        atomicAdd(&Sl[1][s], f * ri * sg);
-      }
-    }
-  } else if (ch.type == DT_AMP) {
-    for (int i0 = threadIdx.x; i0 < ch.count; i0 += nt * NB) {
-      AmpItem d[NB];
-#pragma unroll
-      for (int b = 0; b < NB; ++b)
-        if (i0 + b * nt < ch.count) d[b] = amp[ch.start + i0 + b * nt];
-#pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        if (i0 + b * nt >= ch.count) break;
-        const int s = d[b].slot & 0x7fff;
-        const double2 al = Al[s];
-        const double r = d[b].amp - al.x;
-        acc += r * r * d[b].w;
-        atomicAdd(&Sl[0][s], -2.0 * r * d[b].w * al.y);
-      }
-    }
-  } else if (ch.type == DT_CP) {
-    const int q = (ch.count + 63) >> 6;
-    for (int j0 = threadIdx.x; j0 < 64 * q; j0 += nt * NB) {
-      CpItem d[NB];
-      bool ok[NB];
-#pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        const int j = j0 + b * nt;
-        const int i = closure_item(j, q);
-        ok[b] = j < 64 * q && i < ch.count;
-        if (ok[b]) d[b] = cp[ch.start + i];
-      }
-#pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        if (!ok[b]) continue;
-        double sg[3], model = 0.0;
-        int s[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          s[k] = d[b].slot[k] & 0x7fff;
-          sg[k] = (d[b].slot[k] & kSlotNeg) ? -1.0 : 1.0;
-          model += Al[s[k]].x * sg[k];
-        }
-        const double r = d[b].cp - model;
-        double sr, cr;
-        if (LEAN)
-          sincos_nl(r, sr, cr);
-        else
-          sincos(r, &sr, &cr);
-        acc += 2.0 * (1.0 - cr) * d[b].w;
-        const double f = -2.0 * sr * d[b].w;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) atomicAdd(&Sl[0][s[k]], f * sg[k]);
-      }
-    }
-  } else {
-    const int q = (ch.count + 63) >> 6;
-    for (int j0 = threadIdx.x; j0 < 64 * q; j0 += nt * NB) {
-      CaItem d[NB];
-      bool ok[NB];
-#pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        const int j = j0 + b * nt;
-        const int i = closure_item(j, q);
-        ok[b] = j < 64 * q && i < ch.count;
-        if (ok[b]) d[b] = ca[ch.start + i];
-      }
-#pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        if (!ok[b]) continue;
-        double l[4];
-        int s[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          s[k] = d[b].slot[k] & 0x7fff;     // signs are ignored (fftlib.f90:549-553)
-          l[k] = Al[s[k]].x;
-        }
-        // log|V| = 0.5 log|V|^2
-        const double model = 0.5 * (l[0] + l[1] - l[2] - l[3]);
-        const double r = d[b].ca - model;
-        acc += r * r * d[b].w;
-        const double f = -2.0 * r * d[b].w;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) atomicAdd(&Sl[0][s[k]], k < 2 ? f : -f);
-      }
-    }
-  }
-  __syncthreads();
-  // each point's share: its accumulator(s) times its fixed vector
-  for (int s = threadIdx.x; s < ch.nslots; s += nt) {
-    const double2 m = Vl[s];
-    const double a = Sl[0][s];
-    double2 r;
-    if (ch.type == DT_FCV) r = make_double2(a, Sl[1][s]);
-    else if (ch.type == DT_AMP) r = make_double2(a * m.x, a * m.y);
-    else if (ch.type == DT_CP) r = make_double2(-a * Al[s].y * m.y, a * Al[s].y * m.x);
-    else r = make_double2(a * Al[s].y * m.x, a * Al[s].y * m.y);
-    out[s] = r;
-  }
-  double v[1] = {acc};
+  double v[1] = {0.0};
+  if (ch.type == DT_FCV) v[0] = staged_chunk<DT_FCV, LEAN>(ch, sl, fcv, V, out, Vl, Al, Sl);
+  else if (ch.type == DT_AMP) v[0] = staged_chunk<DT_AMP, LEAN>(ch, sl, amp, V, out, Vl, Al, Sl);
+  else if (ch.type == DT_CP) v[0] = staged_chunk<DT_CP, LEAN>(ch, sl, cp, V, out, Vl, Al, Sl);
+  else v[0] = staged_chunk<DT_CA, LEAN>(ch, sl, ca, V, out, Vl, Al, Sl);
   block_sum<1, kChisqThreadsMax / 64>(v, red);
   if (threadIdx.x == 0) partial[bid] = v[0];
 }
@@ -426,46 +461,57 @@ __device__ __forceinline__ SmState sm_state(const UnitDev& U, const UnitMoments&
   return s;
 }
 
-__device__ __forceinline__ double img_at(const double* I2d, int nx, int ny, int ix, int iy) {
-  return (ix < 0 || iy < 0 || ix >= nx || iy >= ny) ? 0.0 : I2d[(long long)iy * nx + ix];
+// The pixel and the six neighbours the TV and TSV gradients read, loaded
+// together; a neighbour outside the image is 0 and never used.
+struct Nbr { double c, r, u, l, lu, d, rd; };   // (ix, iy) and +x, +y, -x, (-x, +y), -y, (+x, -y)
+
+__device__ __forceinline__ Nbr load_nbr(const double* I, int nx, int ny, int ix, int iy) {
+  Nbr n{};
+  const long long o = (long long)iy * nx + ix;
+  n.c = I[o];
+  if (ix + 1 < nx) n.r = I[o + 1];
+  if (iy + 1 < ny) n.u = I[o + nx];
+  if (ix > 0) n.l = I[o - 1];
+  if (ix > 0 && iy + 1 < ny) n.lu = I[o + nx - 1];
+  if (iy > 0) n.d = I[o - nx];
+  if (iy > 0 && ix + 1 < nx) n.rd = I[o - nx + 1];
+  return n;
 }
 
-// tv_e (image.f90:477-508) at (ix, iy), 0-based
-__device__ __forceinline__ void fwd_diff(const double* I, int nx, int ny, int ix, int iy, double& dx, double& dy) {
-  const double c = I[(long long)iy * nx + ix];
-  dx = (ix + 1 < nx) ? I[(long long)iy * nx + ix + 1] - c : 0.0;
-  dy = (iy + 1 < ny) ? I[(long long)(iy + 1) * nx + ix] - c : 0.0;
+// tv_e (image.f90:477-508), tv_grade with the reference's reset quirk
+// (image.f90:512-612) and tsv_grade (image.f90:713-754) at (ix, iy),
+// 0-based, on the loaded neighbourhood
+__device__ __forceinline__ void nbr_fwd_diff(const Nbr& n, int nx, int ny, int ix, int iy, double& dx, double& dy) {
+  dx = (ix + 1 < nx) ? n.r - n.c : 0.0;
+  dy = (iy + 1 < ny) ? n.u - n.c : 0.0;
 }
 
-// tv_grade with the reference's reset quirk (image.f90:512-612)
-__device__ __forceinline__ double tv_grade(const double* I, int nx, int ny, int ix, int iy) {
+__device__ __forceinline__ double nbr_tv_grade(const Nbr& n, int nx, int ny, int ix, int iy) {
   double g = 0.0, dx, dy;
-  fwd_diff(I, nx, ny, ix, iy, dx, dy);
+  nbr_fwd_diff(n, nx, ny, ix, iy, dx, dy);
   double e = sqrt(dx * dx + dy * dy);
   if (e > kZeroEps) g -= (dx + dy) / e;
   if (ix > 0) {
-    dx = I[(long long)iy * nx + ix] - I[(long long)iy * nx + ix - 1];
-    dy = (iy + 1 < ny) ? I[(long long)(iy + 1) * nx + ix - 1] - I[(long long)iy * nx + ix - 1] : 0.0;
+    dx = n.c - n.l;
+    dy = (iy + 1 < ny) ? n.lu - n.l : 0.0;
     e = sqrt(dx * dx + dy * dy);
     if (e > kZeroEps) g += dx / e; else g = 0.0;
   }
   if (iy > 0) {
-    dy = I[(long long)iy * nx + ix] - I[(long long)(iy - 1) * nx + ix];
-    dx = (ix + 1 < nx) ? I[(long long)(iy - 1) * nx + ix + 1] - I[(long long)(iy - 1) * nx + ix] : 0.0;
+    dy = n.c - n.d;
+    dx = (ix + 1 < nx) ? n.rd - n.d : 0.0;
     e = sqrt(dx * dx + dy * dy);
     if (e > kZeroEps) g += dy / e; else g = 0.0;
   }
   return g;
 }
 
-// tsv_grade (image.f90:713-754)
-__device__ __forceinline__ double tsv_grade(const double* I, int nx, int ny, int ix, int iy) {
-  const double c = I[(long long)iy * nx + ix];
+__device__ __forceinline__ double nbr_tsv_grade(const Nbr& n, int nx, int ny, int ix, int iy) {
   double g = 0.0;
-  if (ix + 1 < nx) g -= 2.0 * (I[(long long)iy * nx + ix + 1] - c);
-  if (iy + 1 < ny) g -= 2.0 * (I[(long long)(iy + 1) * nx + ix] - c);
-  if (ix > 0) g += 2.0 * (c - I[(long long)iy * nx + ix - 1]);
-  if (iy > 0) g += 2.0 * (c - I[(long long)(iy - 1) * nx + ix]);
+  if (ix + 1 < nx) g -= 2.0 * (n.r - n.c);
+  if (iy + 1 < ny) g -= 2.0 * (n.u - n.c);
+  if (ix > 0) g += 2.0 * (n.c - n.l);
+  if (iy > 0) g += 2.0 * (n.c - n.d);
   return g;
 }
 
@@ -576,7 +622,27 @@ __device__ __forceinline__ void reg_block_unit(int bid, const RegArgs& A, const 
     const int k = it.start + i;
     const int p = U.identity ? k : pix[U.pix_off + k];
     const int ix = p % U.nx, iy = p / U.nx;
+    // every load of the pixel first, all in flight together (issued inside
+    // each term's branch they would be one round trip to memory per term)
     const double I = x[U.x_off + k];
+    double wl1 = 0.0, Pkl = 1.0, Pgs = 1.0, wtv = 0.0, wtsv = 0.0;
+    if (U.l1_l > 0) wl1 = U.l1_w[k];
+    if (U.kl_l > 0) Pkl = U.kl_w[k];
+    if (U.gs_l > 0) Pgs = U.gs_w[k];
+    if (U.tv_l > 0) wtv = U.tv_w[k];
+    if (U.tsv_l > 0) wtsv = U.tsv_w[k];
+    Nbr nb{};
+    if (U.tv_l > 0 || U.tsv_l > 0) nb = load_nbr(I2d, U.nx, U.ny, ix, iy);
+    double Iu = 0.0, Il = 0.0, avg = 0.0, wrt = 0.0, wri = 0.0;
+    if (rt_on) {
+      wrt = Mv.rt_w[k];
+      Iu = x[U.x_off + Mv.npix + k];
+      if (U.frame > 0) Il = x[U.x_off - Mv.npix + k];
+    }
+    if (ri_on) {
+      wri = Mv.ri_w[k];
+      avg = Mv.iavg[k];
+    }
     double g = 0.0;
     if (LEAN && U.need_moments) g += moment_grad(U, mom[it.unit], ix, iy, I);
     if (moments && U.tfd_l > 0) g += tfd_g;
@@ -587,38 +653,36 @@ __device__ __forceinline__ void reg_block_unit(int bid, const RegArgs& A, const 
       g += U.cen_l * (cen_sx * (dix - cen_sx) + cen_sy * (diy - cen_sy)) / cen_sIp / cen_cost * gIp;
     }
     if (U.l1_l > 0) {
-      const double w = U.l1_l * U.l1_w[k];
+      const double w = U.l1_l * wl1;
       const double sa = smabs(I);
       v[0] += w * sa;
       g += w * (I / sa);
     }
     if (U.kl_l > 0) {
-      const double P = U.kl_w[k];
       const double sa = smabs(I);
-      const double lg = log(sa / P);
-      v[3] += U.kl_l * (sa * lg + P / 2.71828182845904523536);
+      const double lg = log(sa / Pkl);
+      v[3] += U.kl_l * (sa * lg + Pkl / 2.71828182845904523536);
       g += U.kl_l * (I / sa) * (lg + 1.0);
     }
     if (U.gs_l > 0) {
-      const double P = U.gs_w[k];
       const double sa = smabs(I);
-      const double lg = log(sa / P);
-      v[4] += U.gs_l * (sa * (lg - 1.0) + P);
+      const double lg = log(sa / Pgs);
+      v[4] += U.gs_l * (sa * (lg - 1.0) + Pgs);
       g += U.gs_l * (I / sa) * lg;
     }
     if (U.tv_l > 0) {
-      const double w = U.tv_l * U.tv_w[k];
+      const double w = U.tv_l * wtv;
       double dx, dy;
-      fwd_diff(I2d, U.nx, U.ny, ix, iy, dx, dy);
+      nbr_fwd_diff(nb, U.nx, U.ny, ix, iy, dx, dy);
       v[1] += w * sqrt(dx * dx + dy * dy);
-      g += w * tv_grade(I2d, U.nx, U.ny, ix, iy);
+      g += w * nbr_tv_grade(nb, U.nx, U.ny, ix, iy);
     }
     if (U.tsv_l > 0) {
-      const double w = U.tsv_l * U.tsv_w[k];
+      const double w = U.tsv_l * wtsv;
       double dx, dy;
-      fwd_diff(I2d, U.nx, U.ny, ix, iy, dx, dy);
+      nbr_fwd_diff(nb, U.nx, U.ny, ix, iy, dx, dy);
       v[2] += w * (dx * dx + dy * dy);
-      g += w * tsv_grade(I2d, U.nx, U.ny, ix, iy);
+      g += w * nbr_tsv_grade(nb, U.nx, U.ny, ix, iy);
     }
     if (moments && U.sm_l > 0) {
       const double xi = ix + 1, yi = iy + 1;
@@ -633,21 +697,21 @@ __device__ __forceinline__ void reg_block_unit(int bid, const RegArgs& A, const 
     if (rt_on || ri_on) {
       const double II = I / lz;
       if (rt_on) {
-        const double w = Mv.rt_l * Mv.rt_w[k];
-        const double Iu = x[U.x_off + Mv.npix + k] / lu;
-        v[5] += w * ((Iu - II) * (Iu - II));
+        const double w = Mv.rt_l * wrt;
+        const double Ju = Iu / lu;
+        v[5] += w * ((Ju - II) * (Ju - II));
         double gr;
         if (U.frame == 0) {
-          gr = 2.0 * (II - Iu);
+          gr = 2.0 * (II - Ju);
         } else {
-          const double Il = x[U.x_off - Mv.npix + k] / ll;
-          gr = 2.0 * (2.0 * II - Il - Iu);
+          const double Jl = Il / ll;
+          gr = 2.0 * (2.0 * II - Jl - Ju);
         }
         gt += w * gr / lz;
       }
       if (ri_on) {
-        const double w = Mv.ri_l * Mv.ri_w[k];
-        const double d = II - Mv.iavg[k];
+        const double w = Mv.ri_l * wri;
+        const double d = II - avg;
         v[6] += w * (d * d);
         gt += w * (2.0 * d) / lz;
       }
