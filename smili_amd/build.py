@@ -1,4 +1,6 @@
-"""Builds libsmili_hip.so in-tree for gfx950 (``python -m smili_amd.build``)."""
+"""Builds the package's native libraries in-tree (``python -m smili_amd.build``):
+libsmili_hip.so and libsmili_unet.so (HIP, gfx950) and libsmili_lbfgsb.so
+(the host optimizer, C++ / OpenMP)."""
 from __future__ import annotations
 
 import os
@@ -11,6 +13,9 @@ SRC = os.path.join(HERE, "csrc", "smili_abi.hip")
 OUT = os.path.join(HERE, "libsmili_hip.so")
 UNET_SRC = os.path.join(HERE, "csrc", "smili_unet.hip")
 UNET_OUT = os.path.join(HERE, "libsmili_unet.so")
+LBFGSB_SRC = os.path.join(HERE, "csrc", "smili_lbfgsb.cpp")
+LBFGSB_OUT = os.path.join(HERE, "libsmili_lbfgsb.so")
+CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp"]
 
 HIPCC_FLAGS = [
     "-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-shared",
@@ -36,9 +41,26 @@ def _hipcc(src, out, verbose):
     return out
 
 
+def build_lbfgsb(verbose=False):
+    """The host L-BFGS-B (g++, no GPU code)."""
+    cxx = os.environ.get("CXX", "g++")
+    tmp = LBFGSB_OUT + ".tmp"
+    cmd = [cxx, *CXX_FLAGS, "-o", tmp, LBFGSB_SRC]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed: " + " ".join(cmd))
+    if verbose and (r.stdout + r.stderr).strip():
+        print(r.stdout + r.stderr)
+    os.replace(tmp, LBFGSB_OUT)
+    return LBFGSB_OUT
+
+
 def build_hip(verbose=False):
-    """Both libraries: the imaging cost (libsmili_hip.so) and the
-    warm-start network (libsmili_unet.so)."""
+    """Every native library: the imaging cost (libsmili_hip.so), the
+    warm-start network (libsmili_unet.so) and the host optimizer
+    (libsmili_lbfgsb.so)."""
+    build_lbfgsb(verbose)
     _hipcc(UNET_SRC, UNET_OUT, verbose)
     return _hipcc(SRC, OUT, verbose)
 

@@ -4,8 +4,9 @@ l1_lambda=..., ...)`` (smili/imaging/lbfgs.py:44-576).
 The objective+gradient runs on the GPU (``CostProblem``, the C ABI of
 libsmili_hip.so) and the optimiser is L-BFGS-B (Byrd, Lu, Nocedal & Zhu,
 v3.0 of Zhu, Byrd, Lu & Nocedal / Morales & Nocedal 2011 -- the algorithm of
-the reference's src/lbfgsb/lbfgsb.f) in FP64 on the host, through SciPy's
-implementation of the same algorithm.  The reference's reverse-communication
+the reference's src/lbfgsb/lbfgsb.f) in FP64 on the host: the package's
+native, threaded implementation (``smili_amd.lbfgsb``, libsmili_lbfgsb.so)
+by default, or SciPy's implementation of the same algorithm.  The reference's reverse-communication
 loop (src/fortlib/fftim2d.f90:286-351) stops once the iteration counter
 exceeds ``niter`` at a NEW_X, i.e. after ``niter + 1`` iterations; ``m``,
 ``factr`` and ``pgtol`` come from the module dict ``lbfgsbprms``
@@ -20,6 +21,7 @@ from dataclasses import dataclass, field
 import numpy as np
 from scipy.optimize import Bounds, minimize
 
+from .. import lbfgsb as LB
 from ..cost import CostProblem
 from . import deburr as D
 from .prep import prepare
@@ -140,12 +142,13 @@ def solve(problem: CostProblem, x0, niter=1000, nonneg=True, m=None, factr=None,
           nprint=500, inormfactr=1.0, callback=None, verbose=True, stop_cost=None, driver="auto"):
     """L-BFGS-B over ``problem.cost_grad`` from x0 (scaled units); with
     ``stop_cost`` it also ends at the first iterate whose cost is at or
-    below it.  ``driver``: "setulb" (SciPy's routine driven directly),
-    "minimize" (through scipy.optimize.minimize; the same iterates), "auto"
-    (the first where this SciPy allows it).  ``callback(it, xk, terms)``
-    must not keep xk without copying it.  Returns (x, SolveInfo)."""
-    if driver not in ("auto", "setulb", "minimize"):
-        raise ValueError(f'driver must be "auto", "setulb" or "minimize", not {driver!r}')
+    below it.  ``driver``: "native" (libsmili_lbfgsb.so; also "auto"),
+    "setulb" (SciPy's routine driven directly), "minimize" (through
+    scipy.optimize.minimize; the same iterates as "setulb").
+    ``callback(it, xk, terms)`` must not keep xk without copying it.
+    Returns (x, SolveInfo)."""
+    if driver not in ("auto", "native", "setulb", "minimize"):
+        raise ValueError(f'driver must be "auto", "native", "setulb" or "minimize", not {driver!r}')
     m = lbfgsbprms["m"] if m is None else m
     factr = lbfgsbprms["factr"] if factr is None else factr
     pgtol = lbfgsbprms["pgtol"] if pgtol is None else pgtol
@@ -189,9 +192,17 @@ def solve(problem: CostProblem, x0, niter=1000, nonneg=True, m=None, factr=None,
         c, _, terms = problem.cost_grad(x0)
         print("Summary of the initial image")
         print(_summary(terms, inormfactr, x0, nz))
-    out = None if driver == "minimize" else _setulb_loop(fg, x0, nonneg, int(m), float(factr), float(pgtol),
-                                                          int(niter) + 1, 20, cb,
-                                                          None if bufs is None else bufs[0])
+    if driver in ("auto", "native"):
+        x, nit, message, success = LB.minimize(fg, x0, m=int(m), lower=0.0 if nonneg else None, factr=float(factr),
+                                               pgtol=float(pgtol), maxiter=int(niter) + 1, maxls=20, callback=cb,
+                                               xbuf=None if bufs is None else bufs[0])
+        if message == LB.STOPPED_BY_CALLBACK:
+            message = "`callback` raised `StopIteration`."      # the wording of the other drivers
+        out = x, nit, message, success
+    else:
+        out = None if driver == "minimize" else _setulb_loop(fg, x0, nonneg, int(m), float(factr), float(pgtol),
+                                                              int(niter) + 1, 20, cb,
+                                                              None if bufs is None else bufs[0])
     if out is None:
         if driver == "setulb":
             raise RuntimeError("this SciPy's L-BFGS-B routine does not take the direct call")
