@@ -15,7 +15,7 @@ UNET_SRC = os.path.join(HERE, "csrc", "smili_unet.hip")
 UNET_OUT = os.path.join(HERE, "libsmili_unet.so")
 LBFGSB_SRC = os.path.join(HERE, "csrc", "smili_lbfgsb.cpp")
 LBFGSB_OUT = os.path.join(HERE, "libsmili_lbfgsb.so")
-CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp"]
+CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-mavx2", "-mfma"]
 
 HIPCC_FLAGS = [
     "-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-shared",

@@ -21,10 +21,11 @@
 //     projected-gradient norm <= pgtol, a pair skipped when s'y <= eps
 //     (-g's), the memory dropped and the iteration restarted when the line
 //     search fails (ABNORMAL when no pair is left).
-// The compact matrices are handled explicitly (M as the inverse of its
-// 2c x 2c middle matrix, W'ZZ'W as Gram sums over the free or the fixed
-// variables, whichever set is smaller), with the pair products maintained
-// as pairs enter and leave.  Arithmetic is FP64 throughout.
+// The compact matrices are handled explicitly: M as the inverse of its
+// 2c x 2c middle matrix, and W'ZZ'W from the pair products over the free
+// set (S'ZZ'Y, S'ZZ'S, Y'ZZ'Y), kept as pairs enter and leave and as
+// variables enter or leave the free set, so an iteration costs O(n m) plus
+// O(m^2) per variable that changed side.  Arithmetic is FP64 throughout.
 #include "../../include/smili_lbfgsb.h"
 
 #include <omp.h>
@@ -250,8 +251,9 @@ struct smili_lbfgsb {
   std::vector<int> nbd;
   bool constrained = false, boxed = false;
 
-  // correction pairs, row-major n x m, circular over slots (logical pair j
-  // of the c held is slot (head + j) % m)
+  // correction pairs, one contiguous column of n per slot (S[q n + i]),
+  // circular over the m slots (logical pair j of the c held is slot
+  // (head + j) % m)
   std::vector<double> S, Y;
   int head = 0, col = 0;
   long long updates = 0;
@@ -259,6 +261,13 @@ struct smili_lbfgsb {
   // pair products in logical order, m x m: SY[i][j] = s_i'y_j (all i, j),
   // SS[i][j] = s_i's_j, YY[i][j] = y_i'y_j
   std::vector<double> SY, SS, YY;
+  // the same products over the free set of the last subspace step
+  // (FSY[i][j] = s_i'ZZ'y_j, FSS, FYY), kept as pairs enter and variables
+  // enter or leave the free set; fset: that free set
+  std::vector<double> FSY, FSS, FYY;
+  std::vector<char> fset;
+  std::vector<double> fmask;     // fset as 1.0 / 0.0
+  bool fvalid = false;
   // M (2c x 2c) for the current memory
   std::vector<double> M;
   bool m_ok = true;
@@ -283,9 +292,24 @@ struct smili_lbfgsb {
   // ---- threading helpers ----------------------------------------------
   int threads() const { return (n >= kParallelN && nthreads > 1) ? nthreads : 1; }
 
+  // f(i0, i1, t) over blocks of at most kBlock variables, each thread
+  // taking a contiguous range (one thread below kParallelN)
+  static constexpr long long kBlock = 1024;
+  template <class F>
+  void for_blocks(F&& f) const {
+    const int T = threads();
+#pragma omp parallel num_threads(T) if (T > 1)
+    {
+      const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+      const long long per = (n + nt - 1) / nt;
+      const long long a = std::min(n, (long long)t * per), b = std::min(n, a + per);
+      for (long long i0 = a; i0 < b; i0 += kBlock) f(i0, std::min(b, i0 + kBlock), t);
+    }
+  }
+
   double dot(const double* a, const double* b) const {
     double s = 0.0;
-#pragma omp parallel for reduction(+ : s) num_threads(threads()) if (threads() > 1) schedule(static)
+#pragma omp parallel for simd reduction(+ : s) num_threads(threads()) if (threads() > 1) schedule(static)
     for (long long i = 0; i < n; ++i) s += a[i] * b[i];
     return s;
   }
@@ -350,15 +374,14 @@ struct smili_lbfgsb {
 
   // row i of W = [Y, theta S] in logical order (2c)
   void w_row(long long i, double* w) const {
-    const double* yr = &Y[i * m];
-    const double* sr = &S[i * m];
     for (int j = 0; j < col; ++j) {
-      w[j] = yr[slots[j]];
-      w[col + j] = theta * sr[slots[j]];
+      w[j] = Y[(size_t)slots[j] * n + i];
+      w[col + j] = theta * S[(size_t)slots[j] * n + i];
     }
   }
 
   void drop_memory() {
+    fvalid = false;
     slots.clear();
     col = 0;
     head = 0;
@@ -380,62 +403,62 @@ struct smili_lbfgsb {
     // breakpoints and the steepest-descent direction, per thread
     const int T = threads();
     if ((int)tbp.size() < T) tbp.resize(T);
-    std::vector<double> tp((size_t)T * c2, 0.0), tf1(T, 0.0);
+    std::vector<double> tp((size_t)T * 64, 0.0), tf1(T, 0.0);
     std::vector<long long> tfree(T, 0);
     std::vector<char> tunb(T, 0);
-#pragma omp parallel num_threads(T) if (T > 1)
-    {
-      const int t = omp_get_thread_num();
+    for (int t = 0; t < T; ++t) tbp[t].clear();
+    for_blocks([&](long long i0, long long i1, int t) {
       auto& bpl = tbp[t];
-      bpl.clear();
-      double pl[64] = {0.0};
+      const size_t base = bpl.size();
+      bpl.resize(base + (size_t)(i1 - i0));
+      size_t nb = base;
       double f1l = 0.0;
       long long nfl = 0;
-      char unb = 0;
-#pragma omp for schedule(static)
-      for (long long i = 0; i < n; ++i) {
+      int unb = 0;
+      // status, direction and breakpoint of each variable without branches
+      // on the data (the same rules as the reference's scan, in selects)
+      for (long long i = i0; i < i1; ++i) {
         const double neg = -g[i];
-        double tl = 0.0, tu = 0.0;
-        if (where[i] != 3 && where[i] != -1) {
-          if (nbd[i] <= 2) tl = x[i] - lo[i];
-          if (nbd[i] >= 2) tu = up[i] - x[i];
-          const bool atl = nbd[i] <= 2 && tl <= 0.0;
-          const bool atu = nbd[i] >= 2 && tu <= 0.0;
-          where[i] = 0;
-          if (atl) {
-            if (neg <= 0.0) where[i] = 1;
-          } else if (atu) {
-            if (neg >= 0.0) where[i] = 2;
-          } else if (std::fabs(neg) <= 0.0) {
-            where[i] = -3;
-          }
-        }
-        if (where[i] != 0 && where[i] != -1) {
-          d[i] = 0.0;
-          continue;
-        }
-        d[i] = neg;
-        f1l -= neg * neg;
-        const double* yr = &Y[i * m];
-        const double* sr = &S[i * m];
-        for (int j = 0; j < c; ++j) {
-          pl[j] += yr[slots[j]] * neg;
-          pl[c + j] += sr[slots[j]] * neg;
-        }
-        if (nbd[i] <= 2 && nbd[i] != 0 && neg < 0.0) {
-          bpl.emplace_back(tl / (-neg), i);
-        } else if (nbd[i] >= 2 && neg > 0.0) {
-          bpl.emplace_back(tu / neg, i);
-        } else {
-          ++nfl;
-          if (neg != 0.0) unb = 1;
-        }
+        const int nb_i = nbd[i];
+        const int w0 = where[i];
+        const bool has_l = nb_i == 1 || nb_i == 2, has_u = nb_i >= 2;
+        const double tl = has_l ? x[i] - lo[i] : 0.0;
+        const double tu = has_u ? up[i] - x[i] : 0.0;
+        const bool atl = has_l && tl <= 0.0, atu = has_u && tu <= 0.0;
+        int w = atl ? (neg <= 0.0 ? 1 : 0) : atu ? (neg >= 0.0 ? 2 : 0) : (std::fabs(neg) <= 0.0 ? -3 : 0);
+        w = (w0 == 3 || w0 == -1) ? w0 : w;
+        where[i] = w;
+        const bool fr = w == 0 || w == -1;
+        const double di = fr ? neg : 0.0;
+        d[i] = di;
+        f1l -= di * di;
+        const bool bl = fr && has_l && neg < 0.0;
+        const bool bu = fr && !bl && has_u && neg > 0.0;
+        bpl[nb] = std::make_pair(bl ? tl / (-neg) : tu / neg, i);
+        nb += (bl || bu) ? 1 : 0;
+        const bool nobr = fr && !bl && !bu;
+        nfl += nobr ? 1 : 0;
+        unb |= (nobr && neg != 0.0) ? 1 : 0;
       }
-      for (int j = 0; j < c2; ++j) tp[(size_t)t * c2 + j] = pl[j];
-      tf1[t] = f1l;
-      tfree[t] = nfl;
-      tunb[t] = unb;
-    }
+      bpl.resize(nb);
+      // p = W'd over the block (d = 0 off the free set), slot order
+      double* pl = tp.data() + (size_t)t * 64;
+      for (int j = 0; j < c; ++j) {
+        const double* yq = &Y[(size_t)slots[j] * n];
+        const double* sq = &S[(size_t)slots[j] * n];
+        double ay = 0.0, as = 0.0;
+#pragma omp simd reduction(+ : ay, as)
+        for (long long i = i0; i < i1; ++i) {
+          ay += yq[i] * d[i];
+          as += sq[i] * d[i];
+        }
+        pl[j] += ay;
+        pl[32 + j] += as;
+      }
+      tf1[t] += f1l;
+      tfree[t] += nfl;
+      tunb[t] |= unb;
+    });
     std::vector<std::pair<double, long long>>& bp = bps;
     bp.clear();
     bool bounded = true;
@@ -443,7 +466,10 @@ struct smili_lbfgsb {
     double f1 = 0.0;
     for (int t = 0; t < T; ++t) {
       bp.insert(bp.end(), tbp[t].begin(), tbp[t].end());
-      for (int j = 0; j < c2; ++j) pvec[j] += tp[(size_t)t * c2 + j];
+      for (int j = 0; j < c; ++j) {
+        pvec[j] += tp[(size_t)t * 64 + j];
+        pvec[c + j] += tp[(size_t)t * 64 + 32 + j];
+      }
       f1 += tf1[t];
       nfree += tfree[t];
       if (tunb[t]) bounded = false;
@@ -543,98 +569,165 @@ struct smili_lbfgsb {
     const int c = col, c2 = 2 * col;
     if (c == 0) return true;
     const int T = threads();
-    // reduced gradient r = -(g + theta (xc - x) - W M c) on the free set,
-    // W'Z r (2c), and Gram sums of W's rows over the free set or over the
-    // fixed set (then W'ZZ'W = W'W - the fixed sums), whichever is smaller
     std::vector<double> mc(c2);
     apply_m(cvec.data(), mc.data());
-    long long nf = 0;
-#pragma omp parallel for reduction(+ : nf) num_threads(T) if (T > 1) schedule(static)
-    for (long long i = 0; i < n; ++i) nf += where[i] <= 0;
+    long long nf = 0, nchg = 0;
+#pragma omp parallel for reduction(+ : nf, nchg) num_threads(T) if (T > 1) schedule(static)
+    for (long long i = 0; i < n; ++i) {
+      const bool fr = where[i] <= 0;
+      nf += fr;
+      nchg += fvalid && (fset[i] != 0) != fr;
+    }
     if (nf == 0) return true;
-    const bool over_free = nf <= n - nf;
-    std::vector<double> gram((size_t)T * c2 * c2, 0.0), wzr((size_t)T * c2, 0.0);
-#pragma omp parallel num_threads(T) if (T > 1)
-    {
-      const int t = omp_get_thread_num();
-      std::vector<double> gml((size_t)c2 * c2, 0.0);     // this thread's sums
-      double* gm = gml.data();
-      double vr[64] = {0.0};
-      double w[64];
-#pragma omp for schedule(static)
-      for (long long i = 0; i < n; ++i) {
-        const bool fr = where[i] <= 0;
-        if (fr) {
-          w_row(i, w);
-          double s = 0.0;
-          for (int j = 0; j < c2; ++j) s += w[j] * mc[j];
-          const double ri = -theta * (xc[i] - x[i]) - g[i] + s;
-          rr[i] = ri;
-          for (int j = 0; j < c2; ++j) vr[j] += w[j] * ri;
-        } else {
-          rr[i] = 0.0;
+    // the free-set products: from scratch when first needed or when many
+    // variables changed side, else by the rows that changed
+    const bool scratch = !fvalid || nchg * 16 > nf;
+    // reduced gradient r = -(g + theta (xc - x) - W M c) on the free set (0
+    // elsewhere) and W'Z r (2c, theta on the S half)
+    const int nacc = 3 * m * m + 64;
+    std::vector<double> part((size_t)T * nacc, 0.0);
+    for_blocks([&](long long i0, long long i1, int t) {
+      double* acc = part.data() + (size_t)t * nacc;
+      double* aSY = acc;
+      double* aSS = aSY + m * m;
+      double* aYY = aSS + m * m;
+      double* vr = aYY + m * m;
+      double* tmp = &rr[i0];
+      const long long len = i1 - i0;
+      for (long long k = 0; k < len; ++k) tmp[k] = -theta * (xc[i0 + k] - x[i0 + k]) - g[i0 + k];
+      for (int j = 0; j < c; ++j) {
+        const double* yq = &Y[(size_t)slots[j] * n + i0];
+        const double* sq = &S[(size_t)slots[j] * n + i0];
+        const double my = mc[j], ms = theta * mc[c + j];
+        for (long long k = 0; k < len; ++k) tmp[k] += yq[k] * my + sq[k] * ms;
+      }
+      for (long long k = 0; k < len; ++k)
+        if (where[i0 + k] > 0) tmp[k] = 0.0;
+      for (int j = 0; j < c; ++j) {
+        const double* yq = &Y[(size_t)slots[j] * n + i0];
+        const double* sq = &S[(size_t)slots[j] * n + i0];
+        double ay = 0.0, as = 0.0;
+#pragma omp simd reduction(+ : ay, as)
+        for (long long k = 0; k < len; ++k) {
+          ay += yq[k] * tmp[k];
+          as += sq[k] * tmp[k];
         }
-        if (fr == over_free) {
-          if (!fr) w_row(i, w);
-          for (int a = 0; a < c2; ++a) {
-            const double wa = w[a];
-            double* row = gm + a * c2;
-            for (int b = 0; b <= a; ++b) row[b] += wa * w[b];
+        vr[j] += ay;
+        vr[c + j] += theta * as;
+      }
+      // free-set products
+      double* fm = &fmask[i0];
+      if (scratch) {
+        for (long long k = 0; k < len; ++k) fm[k] = where[i0 + k] <= 0 ? 1.0 : 0.0;
+        for (int a = 0; a < c; ++a) {
+          const double* sa = &S[(size_t)slots[a] * n + i0];
+          const double* ya = &Y[(size_t)slots[a] * n + i0];
+          for (int b = 0; b < c; ++b) {
+            const double* yb = &Y[(size_t)slots[b] * n + i0];
+            const double* sb = &S[(size_t)slots[b] * n + i0];
+            double psy = 0.0, pss = 0.0, pyy = 0.0;
+#pragma omp simd reduction(+ : psy, pss, pyy)
+            for (long long k = 0; k < len; ++k) {
+              const double f = fm[k];
+              psy += f * sa[k] * yb[k];
+              pss += f * sa[k] * sb[k];
+              pyy += f * ya[k] * yb[k];
+            }
+            aSY[a * m + b] += psy;
+            aSS[a * m + b] += pss;
+            aYY[a * m + b] += pyy;
           }
         }
-      }
-      std::copy(gml.begin(), gml.end(), gram.begin() + (size_t)t * c2 * c2);
-      std::copy(vr, vr + c2, wzr.begin() + (size_t)t * c2);
-    }
-    std::vector<double> WZZW((size_t)c2 * c2, 0.0), v(c2, 0.0);
-    for (int t = 0; t < T; ++t) {
-      for (int k = 0; k < c2 * c2; ++k) WZZW[k] += gram[(size_t)t * c2 * c2 + k];
-      for (int k = 0; k < c2; ++k) v[k] += wzr[(size_t)t * c2 + k];
-    }
-    for (int a = 0; a < c2; ++a)
-      for (int b = 0; b < a; ++b) WZZW[b * c2 + a] = WZZW[a * c2 + b];
-    if (!over_free) {
-      // W'W = [[Y'Y, theta Y'S], [theta S'Y, theta^2 S'S]] minus the fixed set
-      for (int i = 0; i < c; ++i)
-        for (int j = 0; j < c; ++j) {
-          WZZW[i * c2 + j] = YY[i * m + j] - WZZW[i * c2 + j];
-          WZZW[i * c2 + c + j] = theta * SY[j * m + i] - WZZW[i * c2 + c + j];
-          WZZW[(c + i) * c2 + j] = theta * SY[i * m + j] - WZZW[(c + i) * c2 + j];
-          WZZW[(c + i) * c2 + c + j] = theta * theta * SS[i * m + j] - WZZW[(c + i) * c2 + c + j];
+      } else {
+        for (long long i = i0; i < i1; ++i) {
+          const bool fr = where[i] <= 0;
+          if ((fset[i] != 0) == fr) continue;
+          const double sign = fr ? 1.0 : -1.0;
+          double sv[32], yv[32];
+          for (int j = 0; j < c; ++j) {
+            sv[j] = S[(size_t)slots[j] * n + i];
+            yv[j] = Y[(size_t)slots[j] * n + i];
+          }
+          for (int a = 0; a < c; ++a) {
+            const double sa = sign * sv[a], ya = sign * yv[a];
+            for (int b = 0; b < c; ++b) {
+              aSY[a * m + b] += sa * yv[b];
+              aSS[a * m + b] += sa * sv[b];
+              aYY[a * m + b] += ya * yv[b];
+            }
+          }
+          fmask[i] = fr ? 1.0 : 0.0;
         }
+      }
+      for (long long i = i0; i < i1; ++i) fset[i] = where[i] <= 0;
+    });
+    if (scratch) {
+      std::fill(FSY.begin(), FSY.end(), 0.0);
+      std::fill(FSS.begin(), FSS.end(), 0.0);
+      std::fill(FYY.begin(), FYY.end(), 0.0);
     }
+    std::vector<double> v(c2, 0.0);
+    for (int t = 0; t < T; ++t) {
+      const double* pp = part.data() + (size_t)t * nacc;
+      for (int k = 0; k < m * m; ++k) {
+        FSY[k] += pp[k];
+        FSS[k] += pp[m * m + k];
+        FYY[k] += pp[2 * m * m + k];
+      }
+      for (int k = 0; k < c2; ++k) v[k] += pp[3 * m * m + k];
+    }
+    fvalid = true;
+    // W'ZZ'W = [[Y'ZZ'Y, theta Y'ZZ'S], [theta S'ZZ'Y, theta^2 S'ZZ'S]]
+    std::vector<double> WZZW((size_t)c2 * c2);
+    for (int i = 0; i < c; ++i)
+      for (int j = 0; j < c; ++j) {
+        WZZW[i * c2 + j] = FYY[i * m + j];
+        WZZW[i * c2 + c + j] = theta * FSY[j * m + i];
+        WZZW[(c + i) * c2 + j] = theta * FSY[i * m + j];
+        WZZW[(c + i) * c2 + c + j] = theta * theta * FSS[i * m + j];
+      }
     // (I - M W'ZZ'W / theta) w = M v
     std::vector<double> N((size_t)c2 * c2), rhs(c2);
     for (int a = 0; a < c2; ++a)
       for (int b = 0; b < c2; ++b) {
-        double s = 0.0;
-        for (int k = 0; k < c2; ++k) s += M[a * c2 + k] * WZZW[k * c2 + b];
-        N[a * c2 + b] = (a == b ? 1.0 : 0.0) - s / theta;
+        double sm = 0.0;
+        for (int k = 0; k < c2; ++k) sm += M[a * c2 + k] * WZZW[k * c2 + b];
+        N[a * c2 + b] = (a == b ? 1.0 : 0.0) - sm / theta;
       }
     apply_m(v.data(), rhs.data());
     if (!lu_solve(N, c2, rhs.data(), 1)) return false;
     // d = r / theta + Z'W w / theta^2 on the free set; the projection of
     // xc + d is kept when it is a descent direction from x
-    double ddp = 0.0;
-#pragma omp parallel for reduction(+ : ddp) num_threads(T) if (T > 1) schedule(static)
-    for (long long i = 0; i < n; ++i) {
-      double zi = xc[i];
-      if (where[i] <= 0) {
-        double w[64];
-        w_row(i, w);
-        double s = 0.0;
-        for (int j = 0; j < c2; ++j) s += w[j] * rhs[j];
-        du[i] = rr[i] / theta + s / (theta * theta);
-        zi = xc[i] + du[i];
-        if (nbd[i] == 1) zi = std::max(lo[i], zi);
-        else if (nbd[i] == 2) zi = std::max(lo[i], std::min(up[i], zi));
-        else if (nbd[i] == 3) zi = std::min(up[i], zi);
-      } else {
-        du[i] = 0.0;
+    std::vector<double> tddp(T, 0.0);
+    for_blocks([&](long long i0, long long i1, int t) {
+      const long long len = i1 - i0;
+      double* dd = &du[i0];
+      for (long long k = 0; k < len; ++k) dd[k] = 0.0;
+      for (int j = 0; j < c; ++j) {
+        const double* yq = &Y[(size_t)slots[j] * n + i0];
+        const double* sq = &S[(size_t)slots[j] * n + i0];
+        const double wy = rhs[j], ws = theta * rhs[c + j];
+        for (long long k = 0; k < len; ++k) dd[k] += yq[k] * wy + sq[k] * ws;
       }
-      z[i] = zi;
-      ddp += (zi - x[i]) * g[i];
-    }
+      double acc = 0.0;
+      for (long long i = i0; i < i1; ++i) {
+        double zi = xc[i];
+        if (where[i] <= 0) {
+          du[i] = rr[i] / theta + du[i] / (theta * theta);
+          zi = xc[i] + du[i];
+          if (nbd[i] == 1) zi = std::max(lo[i], zi);
+          else if (nbd[i] == 2) zi = std::max(lo[i], std::min(up[i], zi));
+          else if (nbd[i] == 3) zi = std::min(up[i], zi);
+        } else {
+          du[i] = 0.0;
+        }
+        z[i] = zi;
+        acc += (zi - x[i]) * g[i];
+      }
+      tddp[t] += acc;
+    });
+    double ddp = 0.0;
+    for (int t = 0; t < T; ++t) ddp += tddp[t];
     if (ddp <= 0.0) return true;
     // else truncate the step at the first bound it meets (in variable order)
     std::copy(xc.begin(), xc.end(), z.begin());
@@ -684,46 +777,85 @@ struct smili_lbfgsb {
           SY[i * m + j] = SY[(i + 1) * m + (j + 1)];
           SS[i * m + j] = SS[(i + 1) * m + (j + 1)];
           YY[i * m + j] = YY[(i + 1) * m + (j + 1)];
+          FSY[i * m + j] = FSY[(i + 1) * m + (j + 1)];
+          FSS[i * m + j] = FSS[(i + 1) * m + (j + 1)];
+          FYY[i * m + j] = FYY[(i + 1) * m + (j + 1)];
         }
     }
     ++updates;
     const int c = col, k = c - 1;
     const int T = threads();
     // products of the new pair with the older ones (logical 0 .. k-1)
-    std::vector<double> acc((size_t)T * 4 * m, 0.0);
-    int old[32];
-    for (int j = 0; j < k; ++j) old[j] = slot(j);
-#pragma omp parallel num_threads(T) if (T > 1)
-    {
-      double a[4 * 32] = {0.0};
-#pragma omp for schedule(static)
-      for (long long i = 0; i < n; ++i) {
-        const double si = d[i], yi = r[i];
-        S[i * m + sl] = si;
-        Y[i * m + sl] = yi;
-        for (int j = 0; j < k; ++j) {
-          const int o = old[j];
-          const double so = S[i * m + o], yo = Y[i * m + o];
-          a[j] += si * yo;            // s_new' y_j
-          a[m + j] += so * yi;        // s_j' y_new
-          a[2 * m + j] += si * so;    // s_new' s_j
-          a[3 * m + j] += yi * yo;    // y_new' y_j
+    // (and, with fvalid, the same over the kept free set), per slot; the new
+    // pair's own slot gives its free-set products s's, y'y, s'y
+    const int na = 8 * m;
+    std::vector<double> acc((size_t)T * na, 0.0);
+    const bool fr_on = fvalid;
+    double* sn = &S[(size_t)sl * n];
+    double* yn = &Y[(size_t)sl * n];
+    for_blocks([&](long long i0, long long i1, int t) {
+      double* a = acc.data() + (size_t)t * na;
+      const long long len = i1 - i0;
+      for (long long kk = 0; kk < len; ++kk) {
+        sn[i0 + kk] = d[i0 + kk];
+        yn[i0 + kk] = r[i0 + kk];
+      }
+      const double* fm = &fmask[i0];
+      for (int q = 0; q < m; ++q) {
+        if (q != sl && (q - head + m) % m >= k) continue;      // an empty slot
+        const double* sq = &S[(size_t)q * n + i0];
+        const double* yq = &Y[(size_t)q * n + i0];
+        const double* si = sn + i0;
+        const double* yi = yn + i0;
+        double p0 = 0, p1 = 0, p2 = 0, p3 = 0;
+#pragma omp simd reduction(+ : p0, p1, p2, p3)
+        for (long long k2 = 0; k2 < len; ++k2) {
+          p0 += si[k2] * yq[k2];          // s_new' y_q
+          p1 += sq[k2] * yi[k2];          // s_q' y_new
+          p2 += si[k2] * sq[k2];          // s_new' s_q
+          p3 += yi[k2] * yq[k2];          // y_new' y_q
+        }
+        a[q] += p0;
+        a[m + q] += p1;
+        a[2 * m + q] += p2;
+        a[3 * m + q] += p3;
+        if (fr_on) {
+          double f0 = 0, f1 = 0, f2 = 0, f3 = 0;
+#pragma omp simd reduction(+ : f0, f1, f2, f3)
+          for (long long k2 = 0; k2 < len; ++k2) {
+            const double sf = si[k2] * fm[k2], yf = yi[k2] * fm[k2];
+            f0 += sf * yq[k2];
+            f1 += sq[k2] * yf;
+            f2 += sf * sq[k2];
+            f3 += yf * yq[k2];
+          }
+          a[4 * m + q] += f0;
+          a[5 * m + q] += f1;
+          a[6 * m + q] += f2;
+          a[7 * m + q] += f3;
         }
       }
-      std::copy(a, a + 4 * m, acc.begin() + (size_t)omp_get_thread_num() * 4 * m);
-    }
+    });
+    std::vector<double> tot(na, 0.0);
+    for (int t = 0; t < T; ++t)
+      for (int q = 0; q < na; ++q) tot[q] += acc[(size_t)t * na + q];
     for (int j = 0; j < k; ++j) {
-      double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-      for (int t = 0; t < T; ++t) {
-        a0 += acc[(size_t)t * 4 * m + j];
-        a1 += acc[(size_t)t * 4 * m + m + j];
-        a2 += acc[(size_t)t * 4 * m + 2 * m + j];
-        a3 += acc[(size_t)t * 4 * m + 3 * m + j];
+      const int o = slot(j);
+      SY[k * m + j] = tot[o];
+      SY[j * m + k] = tot[m + o];
+      SS[k * m + j] = SS[j * m + k] = tot[2 * m + o];
+      YY[k * m + j] = YY[j * m + k] = tot[3 * m + o];
+      if (fr_on) {
+        FSY[k * m + j] = tot[4 * m + o];
+        FSY[j * m + k] = tot[5 * m + o];
+        FSS[k * m + j] = FSS[j * m + k] = tot[6 * m + o];
+        FYY[k * m + j] = FYY[j * m + k] = tot[7 * m + o];
       }
-      SY[k * m + j] = a0;
-      SY[j * m + k] = a1;
-      SS[k * m + j] = SS[j * m + k] = a2;
-      YY[k * m + j] = YY[j * m + k] = a3;
+    }
+    if (fr_on) {
+      FSY[k * m + k] = tot[4 * m + sl];
+      FSS[k * m + k] = tot[6 * m + sl];
+      FYY[k * m + k] = tot[7 * m + sl];
     }
     SY[k * m + k] = sy;
     SS[k * m + k] = ss;
@@ -958,6 +1090,11 @@ smili_lbfgsb* smili_lbfgsb_create(int64_t n, int32_t m, const double* l, const d
   h->SY.assign((size_t)m * m, 0.0);
   h->SS.assign((size_t)m * m, 0.0);
   h->YY.assign((size_t)m * m, 0.0);
+  h->FSY.assign((size_t)m * m, 0.0);
+  h->FSS.assign((size_t)m * m, 0.0);
+  h->FYY.assign((size_t)m * m, 0.0);
+  h->fset.assign(n, 0);
+  h->fmask.assign(n, 0.0);
   for (auto* v : {&h->g, &h->xc, &h->d, &h->z, &h->xold, &h->gold, &h->r, &h->rr, &h->du}) v->assign(n, 0.0);
   h->where.assign(n, 0);
   return h;
