@@ -432,15 +432,44 @@ def bench_warmstart(args):
     from smili_amd.warmstart.unet import DEFAULT_WEIGHTS, UNet
     net, meta = UNet.from_file(DEFAULT_WEIGHTS, batch=1)
     base = 2_000_000 + 1000 * args.seed          # held out: training seeds are below 1e6
-    for s in range(base - args.warmup, base):
-        S.time_to_image(net, s)
+    seeds = list(range(base, base + args.steps))
+    if args.ws_child >= 0:
+        # one of --workers processes: its share of the seeds, one JSON row each
+        for s in range(base - args.warmup, base):
+            S.time_to_image(net, s)
+        for s in seeds[args.ws_child::args.workers]:
+            r = S.time_to_image(net, s)
+            print("ROW " + json.dumps({k: (float(v) if isinstance(v, (np.floating, np.integer)) else v)
+                                       for k, v in r.items() if np.isscalar(v) or v is None}), flush=True)
+        net.close()
+        return
     rows = []
-    for s in range(base, base + args.steps):
-        rows.append(S.time_to_image(net, s))
-        r = rows[-1]
-        # progress on stderr (stdout carries the one JSON line)
-        print(f"[warmstart] seed {s}: plain {r['plain_s']:.3f} s, warm "
-              f"{r['warm_s'] if r['warm_s'] is None else round(r['warm_s'], 3)} s", file=sys.stderr, flush=True)
+    if args.workers > 1:
+        # K processes share the GPU and the host's cores; each clocks its own
+        # plain and warm solves, so both clocks see the same contention
+        import subprocess
+        procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--workload", "warmstart",
+                                   "--steps", str(args.steps), "--warmup", str(args.warmup), "--seed", str(args.seed),
+                                   "--workers", str(args.workers), "--ws-child", str(i), "--no-cpu"],
+                                  stdout=subprocess.PIPE, text=True) for i in range(args.workers)]
+        for p in procs:
+            for ln in p.stdout:
+                if ln.startswith("ROW "):
+                    rows.append(json.loads(ln[4:]))
+                    if len(rows) % 50 == 0:
+                        print(f"[warmstart] {len(rows)}/{args.steps} seeds", file=sys.stderr, flush=True)
+        if any(p.wait() != 0 for p in procs):
+            raise SystemExit("a warm-start worker failed")
+        rows.sort(key=lambda r: r["seed"])
+    else:
+        for s in range(base - args.warmup, base):
+            S.time_to_image(net, s)
+        for s in seeds:
+            rows.append(S.time_to_image(net, s))
+            r = rows[-1]
+            # progress on stderr (stdout carries the one JSON line)
+            print(f"[warmstart] seed {s}: plain {r['plain_s']:.3f} s, warm "
+                  f"{r['warm_s'] if r['warm_s'] is None else round(r['warm_s'], 3)} s", file=sys.stderr, flush=True)
     sm = S.summary(rows)
     roof = _unet_roofline(net.params)
     cpu = None
@@ -473,7 +502,8 @@ def bench_warmstart(args):
                    "seeds": [base, base + args.steps], "reached": sm["reached"], "n": sm["n"],
                    "median_iter_ratio": sm["median_iter_ratio"], "median_plain_s": sm["median_plain_s"],
                    "median_warm_s": sm["median_warm_s"], "max_nrmse": sm["max_nrmse"],
-                   "proposals_used": sm["proposals_used"], "n_train": int(meta.get("n_train", 0))},
+                   "proposals_used": sm["proposals_used"], "n_train": int(meta.get("n_train", 0)),
+                   "workers": args.workers},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
@@ -562,6 +592,9 @@ def main():
                     help="NUFFT arithmetic (chi-square and regularizers are FP64 in both)")
     ap.add_argument("--workload", choices=("single_large", "movie", "survey", "warmstart", "warmstart_movie"),
                     default="single_large")
+    ap.add_argument("--workers", type=int, default=1,
+                    help="warmstart: processes sharing the GPU, each timing its share of the seeds")
+    ap.add_argument("--ws-child", type=int, default=-1, help=argparse.SUPPRESS)
     ap.add_argument("--pmc-run", action="store_true",
                     help="only evaluations plus the calibration copy (run under rocprofv3 --pmc)")
     ap.add_argument("--pmc-ingest", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
