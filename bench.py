@@ -39,6 +39,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6        # MI355X dense FP64 (matrix and vector)
 
 
 def fine_size(n):
@@ -112,6 +113,22 @@ def stage_bytes(kws, units, fp64=True):
         ncp = 0 if kw.get("uvidxcp") is None else kw["uvidxcp"].shape[1]
         nca = 0 if kw.get("uvidxca") is None else kw["uvidxca"].shape[1]
         add("chisq", 28 * nfcv + 20 * namp + 28 * ncp + 32 * nca + cb * M_all + 16 * M_all)
+    return tot
+
+
+def stage_flops(units):
+    """FP64 FLOP of the pruned-DFT products per evaluation (complex
+    multiply-add = 8 FLOP, complex x real = 4): S = Ey I (I real), G = S Ex,
+    P = Gs ExT, Re(EyT P) (two real products).  Units on the FFT path add
+    nothing here."""
+    tot = {}
+    for info in units:
+        if info["transform"] != 2:
+            continue
+        nx, ny, gw, gh = info["nx"], info["ny"], info["gw"], info["gh"]
+        for k, v in (("dft2_y", 4 * gh * ny * nx), ("dft2_x", 8 * gh * nx * gw),
+                     ("dft1_x", 8 * gh * gw * nx), ("dft1_y", 4 * ny * gh * nx)):
+            tot[k] = tot.get(k, 0) + v
     return tot
 
 
@@ -339,8 +356,18 @@ def _roofline(g, kws, fp64, ms, steps, workload="single_large"):
     dom = max((k for k in prof if k in sb), key=lambda k: prof[k])
     achieved = sb[dom] / (prof[dom] * 1e-3) / 1e9
     sv = survey_bytes(kws)
-    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5),
+    sf = stage_flops(units)
+    if dom in sf and fp64:
+        # the DFT products are FP64 FMA chains: their roofline is the FP64
+        # rate (their algorithmic bytes over the same time stay beside it)
+        tf = sf[dom] / (prof[dom] * 1e-3) / 1e12
+        head = {"bound": "mfma", "achieved": round(tf, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(tf / FP64_PEAK_TFLOPS, 5), "kernel_flop": sf[dom],
+                "hbm_achieved_GBs": round(achieved, 2), "hbm_frac": round(achieved / HBM_PEAK_GBS, 5)}
+    else:
+        head = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5)}
+    return {**head,
             "traffic": _traffic(workload, "fp64" if fp64 else "fp32", names[dom]),
             "kernel": names[dom], "stage": dom, "kernel_ms": round(prof[dom], 5),
             "kernel_bytes": sb[dom],
@@ -349,7 +376,7 @@ def _roofline(g, kws, fp64, ms, steps, workload="single_large"):
             "model_bytes": sv, "model_achieved_GBs": round(sv / (ms * 1e-3) / 1e9, 2),
             "model_ceiling_evals_s": round(HBM_PEAK_GBS * 1e9 / sv, 1),
             "stage_ms": {k: round(v, 5) for k, v in prof.items() if v > 0},
-            "stage_bytes": sb}
+            "stage_bytes": sb, "stage_flop": sf}
 
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0
