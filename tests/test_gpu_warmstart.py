@@ -7,7 +7,15 @@ For every case: the network's proposal (flux scale fitted along the
 objective) has a lower objective than the flat prior the plain imager
 starts from; the warm-started L-BFGS-B reaches the plain solve's final
 objective within 1e-6 relative; its image then matches the plain image to
-NRMSE <= 5e-3; and over the cases it needs fewer iterations in total."""
+NRMSE <= 5e-3; and over the cases it needs fewer iterations in total, with
+a median iteration ratio >= 1 and a median time-to-image ratio >= 0.85.
+
+Why 5e-3 and not the 1e-3 image target: a 1e-6 match of an objective that
+is quadratic near its minimum fixes the image only to about sqrt(1e-6) =
+1e-3 along its flattest directions, so two solves that both stop at that
+objective differ by a few 1e-3 (measured up to 5.5e-3 over 1,000 held-out
+seeds).  Why 0.85: over 1,000 held-out seeds the median time ratio is
+1.05 and over 20 it is 1.1; six cases on a shared box scatter by ~10 %."""
 import numpy as np
 import pytest
 
@@ -50,6 +58,9 @@ def test_warm_start_reaches_plain_objective_in_fewer_iterations(net):
     assert all(r["warm_start"] == "proposal" for r in rows)
     assert max(r["nrmse"] for r in rows) <= 5e-3
     assert sum(r["warm_iters"] for r in rows) < sum(r["plain_iters"] for r in rows)
+    sm = summary(rows)
+    assert sm["median_iter_ratio"] >= 1.0
+    assert sm["median_time_ratio"] >= 0.85
 
 
 def test_gap_curves(net):
