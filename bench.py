@@ -614,7 +614,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay a captured hipGraph instead of launching the kernels one by one")
     ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp64",
                     help="NUFFT arithmetic (chi-square and regularizers are FP64 in both)")
     ap.add_argument("--workload", choices=("single_large", "movie", "survey", "warmstart", "warmstart_movie"),
@@ -658,7 +659,7 @@ def main():
     from smili_amd.cost import BatchProblem, CostProblem, MovieProblem
 
     fp64 = args.precision == "fp64"
-    tune = dict(use_graph=-1 if args.no_graph else 0, precision=64 if fp64 else 32)
+    tune = dict(use_graph=1 if args.graph else 0, precision=64 if fp64 else 32)
     t_setup = time.perf_counter()
     if args.workload == "single_large":
         prep, _ = workloads.single_large(seed=args.seed)
@@ -711,7 +712,8 @@ def main():
               flush=True)
         return
     # x goes to the handle's own device buffer once; every timed step is one
-    # evaluation on resident data (the captured graph of all kernels)
+    # evaluation on resident data (the kernels launched one by one, or the
+    # captured graph of all of them with --graph)
     first = g.cost_grad(x0)
     dt = _timed_loop(g, args.steps, args.warmup)
     ms = dt / args.steps * 1e3

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -360,9 +361,17 @@ struct Plan {
   hipEvent_t prof_ev[2 * SMILI_NSTAGES] = {};
   int prof_used[SMILI_NSTAGES] = {};
 
-  // page-locked host staging of x, grad and the terms (the host-pointer
-  // entry points copy through them; callers may work in them directly)
+  // page-locked, device-mapped host staging of x, grad and the terms (the
+  // host-pointer entry points evaluate through them; callers may work in
+  // them directly).  hd_* are the same buffers' device addresses: the host
+  // evaluation's graph reads x from there into d_x with a copy kernel, and
+  // its last kernels write grad and the terms straight into them, so no DMA
+  // transfer is queued per call.
   double *h_x = nullptr, *h_grad = nullptr, *h_terms = nullptr;
+  double *hd_x = nullptr, *hd_grad = nullptr;
+  hipGraph_t hgraph = nullptr;
+  hipGraphExec_t hgexec = nullptr;
+  bool hgraph_ready = false;
 
   ~Plan() {
     for (double* h : {h_x, h_grad})
@@ -373,6 +382,8 @@ struct Plan {
       if (e) (void)hipEventDestroy(e);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (hgexec) (void)hipGraphExecDestroy(hgexec);
+    if (hgraph) (void)hipGraphDestroy(hgraph);
     if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -383,6 +394,8 @@ struct Plan {
   void launch_adjoint();
   void launch_eval();
   void eval();
+  void launch_eval_mapped();
+  void eval_mapped();
 };
 
 static void check_desc_geometry(const smili_desc* d) {
@@ -450,7 +463,7 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   require(tune.chisq_threads == 0 || tune.chisq_threads == 256 || tune.chisq_threads == kChisqThreadsMax,
           "chisq_threads must be 0, 256 or 512");
   chisq_threads = tune.chisq_threads > 0 ? tune.chisq_threads : kThreads;
-  tune.use_graph = (tune.use_graph < 0) ? 0 : 1;
+  tune.use_graph = (tune.use_graph > 0) ? 1 : 0;
   require(tune.branches >= 0 && tune.branches <= 2, "branches must be 0, 1 or 2");
 
   HIP_CHECK(hipGetDevice(&device));
@@ -716,11 +729,11 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   }
   require(npt < (1u << 30), "too many uv points");
   // two branches pay off once the stages are long enough to hide the
-  // cross-stream dependencies of the graph; below that, riding the
-  // regularizers and sums on other launches is faster (measured: a single
-  // 1.3e5-point problem 0.107 ms fused vs 0.137 ms forked; a 64-frame movie
-  // of 3e5 points 0.268 vs 0.260 ms; a survey of 7.6e6 points 5.4 vs 4.6 ms)
-  two_branches = tune.branches == 2 || (tune.branches == 0 && (npt >= 200000 || us.size() >= 16));
+  // cross-stream dependencies; below that, riding the regularizers and sums
+  // on other launches is faster (measured with eager launches: a single
+  // 1.3e5-point problem 53.1 us fused vs 70.2 us forked; a 64-frame movie of
+  // 3e5 points 189 vs 200 us; a survey of 7.6e6 points 2.80 vs 2.68 ms)
+  two_branches = tune.branches == 2 || (tune.branches == 0 && npt >= 2000000);
 
   // problems: remapped data items and chunks
   std::vector<FcvItem> fcv;
@@ -1140,9 +1153,15 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     if (!f.empty()) HIP_CHECK(hipMemcpy(d_dftm.p, f.data(), f.size() * sizeof(float2), hipMemcpyHostToDevice));
   }
   d_H.alloc(h_total);
-  HIP_CHECK(hipHostMalloc(&h_x, std::max<size_t>(npix_total, 1) * sizeof(double), hipHostMallocDefault));
-  HIP_CHECK(hipHostMalloc(&h_grad, (npix_total + nterms_total) * sizeof(double), hipHostMallocDefault));
+  // coherent (fine-grained) so that kernels read and write them over the
+  // bus uncached: what the host wrote before a launch is what the copy
+  // kernel reads, and the gradient is in host memory when the stream is done
+  const unsigned hflags = hipHostMallocMapped | hipHostMallocCoherent;
+  HIP_CHECK(hipHostMalloc(&h_x, std::max<size_t>(npix_total, 1) * sizeof(double), hflags));
+  HIP_CHECK(hipHostMalloc(&h_grad, (npix_total + nterms_total) * sizeof(double), hflags));
   h_terms = h_grad + npix_total;
+  HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&hd_x), h_x, 0));
+  HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&hd_grad), h_grad, 0));
   B = EvalBufs{d_x.p, d_grad.p, d_terms_p, d_V.p, d_sres.p, d_gridA.p, d_gridB.p, d_H.p,
                d_I2d.p, d_grad2d.p, d_greg.p, d_cpart.p, d_rpart.p, d_mpart.p, d_mom.p};
   HIP_CHECK(hipDeviceSynchronize());
@@ -1202,8 +1221,21 @@ static void launch_forward_t(Plan& P, bool with_scatter) {
            (const double*)nullptr, (const double*)nullptr, gA, (V*)nullptr, (double*)nullptr, (double*)nullptr,
            (const double*)nullptr, (const int*)nullptr, 0);
   }
-  LAUNCH(SMILI_S_INTERP, k_interp<R>, P.h_subs.size(), kThreads, P.lds_tile, P.d_subs.p, gA, P.d_cell.p, wts,
-         (long long)P.npt, Vv, P.bin);
+  // static-LDS instantiation per bin edge (8 .. 64; the LDS budget check
+  // at build rejects an FP64 tile of 64)
+#define SMILI_INTERP_BIN(B_)                                                                        \
+  case B_:                                                                                          \
+    LAUNCH(SMILI_S_INTERP, (k_interp<R, (sizeof(R) == 8 && B_ == 64) ? 56 : B_>), P.h_subs.size(), \
+           kThreads, 0, P.d_subs.p, gA, P.d_cell.p, wts, (long long)P.npt, Vv);                   \
+    break;
+  if (sizeof(R) == 8 && P.bin == 64) throw Error(SMILI_EINVAL, "an FP64 bin of 64 exceeds the LDS budget");
+  switch (P.bin) {
+    SMILI_INTERP_BIN(8) SMILI_INTERP_BIN(16) SMILI_INTERP_BIN(24) SMILI_INTERP_BIN(32)
+    SMILI_INTERP_BIN(40) SMILI_INTERP_BIN(48) SMILI_INTERP_BIN(56) SMILI_INTERP_BIN(64)
+    default:
+      throw Error(SMILI_EINVAL, "bin must be a multiple of 8 in [8, 64]");
+  }
+#undef SMILI_INTERP_BIN
 }
 
 // the last pass of the type-1 transform (after the regularizer branch has
@@ -1368,6 +1400,63 @@ void Plan::launch_eval() {
   if (fp64) launch_rows_t<double>(*this);
   else launch_rows_t<float>(*this);
   if (fork) HIP_CHECK(hipStreamWaitEvent(main_stream, ev_fin, 0));
+}
+
+// x from the handle's mapped host buffer into d_x (16 bytes per lane: host
+// reads over the bus want many wide requests in flight)
+__global__ void __launch_bounds__(kThreads) k_fetch_x(const double* __restrict__ src, double* __restrict__ dst,
+                                                     long long n) {
+  const long long n2 = n >> 1;
+  const double2* s2 = reinterpret_cast<const double2*>(src);
+  double2* d2 = reinterpret_cast<double2*>(dst);
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (long long)gridDim.x * blockDim.x)
+    d2[i] = s2[i];
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) dst[n - 1] = src[n - 1];
+}
+
+// the host-pointer evaluation: x fetched from the mapped host buffer, then
+// the evaluation with grad and the terms written into the mapped host
+// buffers by the kernels that produce them (they only ever store to them)
+void Plan::launch_eval_mapped() {
+  const long long n = (long long)npix_total;
+  if (n > 0) {
+    const long long blocks = std::min<long long>(((n >> 1) + kThreads - 1) / kThreads + 1, 1024);
+    hipLaunchKernelGGL(k_fetch_x, dim3((unsigned)blocks), dim3(kThreads), 0, stream, (const double*)hd_x, d_x.p, n);
+    HIP_CHECK(hipGetLastError());
+  }
+  const EvalBufs own = B;
+  B.x = d_x.p;
+  B.grad = hd_grad;
+  B.terms = hd_grad + npix_total;
+  try {
+    launch_eval();
+  } catch (...) {
+    B = own;
+    throw;
+  }
+  B = own;
+}
+
+void Plan::eval_mapped() {
+  if (tune.use_graph) {
+    if (!hgraph_ready) {
+      HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      try {
+        launch_eval_mapped();
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      HIP_CHECK(hipStreamEndCapture(stream, &hgraph));
+      HIP_CHECK(hipGraphInstantiate(&hgexec, hgraph, nullptr, nullptr, 0));
+      hgraph_ready = true;
+    }
+    HIP_CHECK(hipGraphLaunch(hgexec, stream));
+  } else {
+    launch_eval_mapped();
+  }
 }
 
 void Plan::eval() {
@@ -1667,20 +1756,49 @@ int smili_movie_create(const smili_movie_desc* md, const smili_tune* tune, smili
 
 void smili_problem_destroy(smili_problem* p) { delete p; }
 
+// The caller of a host-pointer evaluation waits for it anyway: poll the
+// stream for up to 50 ms (a blocking synchronisation wakes the thread tens
+// of microseconds after a ~60 us evaluation ends), then block.
+static void wait_stream(hipStream_t s) {
+  const auto t0 = std::chrono::steady_clock::now();
+  for (;;) {
+    const hipError_t e = hipStreamQuery(s);
+    if (e == hipSuccess) return;
+    if (e != hipErrorNotReady) HIP_CHECK(e);
+    if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50)) break;
+  }
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// Image vectors up to this many pixels go through the mapped buffers in
+// the graph; longer ones by DMA (MI355X, PCIe: the kernels read host memory
+// at ~35 GB/s, the copy engines move the 8 MB of a 64-frame 128^2 movie
+// faster: 1,340 vs 1,206 evaluations/s; a 256^2 image, 0.5 MB each way:
+// 8,851 mapped vs 7,708 by DMA)
+constexpr size_t kMappedMaxPix = 1u << 19;
+
 // copy x in, evaluate, copy grad and all terms out; returns whether every
 // cost and gradient entry is finite
-// (through the handle's page-locked staging: a caller that works in those
-// buffers themselves -- smili_problem_host_buffers -- saves the two host
-// copies, and the device copies run at the DMA rate of pinned memory)
+// (through the handle's page-locked, device-mapped staging: for image
+// vectors up to kMappedMaxPix the evaluation's graph fetches x from it and
+// writes grad and the terms into it, so a call queues one graph and no DMA
+// transfer, and the device-side grad and terms buffers are left as they
+// were; longer vectors are copied by DMA.  A caller that works in those
+// buffers themselves -- smili_problem_host_buffers -- also saves the two
+// host copies.)
 static bool eval_host(Plan& P, const double* x, double* grad, std::vector<double>& t) {
   const size_t n = P.npix_total;
   if (x != P.h_x) std::memcpy(P.h_x, x, n * sizeof(double));
-  HIP_CHECK(hipMemcpyAsync(P.d_x.p, P.h_x, n * sizeof(double), hipMemcpyHostToDevice, P.stream));
-  P.eval();
   t.resize(P.nterms_total);
-  HIP_CHECK(hipMemcpyAsync(P.h_grad, P.d_grad.p, (n + t.size()) * sizeof(double), hipMemcpyDeviceToHost,
-                           P.stream));
-  HIP_CHECK(hipStreamSynchronize(P.stream));
+  if (n <= kMappedMaxPix) {
+    P.eval_mapped();
+  } else {
+    HIP_CHECK(hipMemcpyAsync(P.d_x.p, P.h_x, n * sizeof(double), hipMemcpyHostToDevice, P.stream));
+    P.eval();
+    HIP_CHECK(hipMemcpyAsync(P.h_grad, P.d_grad.p, (n + t.size()) * sizeof(double), hipMemcpyDeviceToHost,
+                             P.stream));
+  }
+  wait_stream(P.stream);
   if (grad != P.h_grad) std::memcpy(grad, P.h_grad, n * sizeof(double));
   std::memcpy(t.data(), P.h_terms, t.size() * sizeof(double));
   bool ok = true;
@@ -1902,7 +2020,7 @@ int smili_problem_info(smili_problem* p, int64_t* info) {
     info[4] = P.bin;
     info[5] = (int64_t)P.h_chunks.size();
     info[6] = P.launches;
-    info[7] = P.graph_ready ? 1 : 0;
+    info[7] = (P.graph_ready || P.hgraph_ready) ? 1 : 0;
   });
 }
 

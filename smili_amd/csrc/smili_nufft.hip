@@ -303,18 +303,21 @@ k_t2_cols(const FftItem* __restrict__ items, int nitems, const typename Cx<R>::T
 
 // ---------------------------------------------------------------------------
 // Type-2 step 3: interpolation from an LDS-staged tile of the grid.  One work
-// item = up to sub_points points of one bin.
+// item = up to sub_points points of one bin.  The bin edge is a template
+// argument, so the tile is a static LDS array (a captured graph replays a
+// kernel with dynamic LDS after a gap of ~18 us on this stack; static LDS
+// kernels follow their predecessor directly).
 // ---------------------------------------------------------------------------
-template <typename R>
+template <typename R, int BIN>
 __global__ void __launch_bounds__(kThreads)
 k_interp(const SubProb* __restrict__ sps, const typename Cx<R>::T* __restrict__ gridA,
          const int* __restrict__ pt_cell, const R* __restrict__ wts, long long npt,
-         typename Cx<R>::T* __restrict__ Vout, int bin) {
+         typename Cx<R>::T* __restrict__ Vout) {
   using V = typename Cx<R>::T;
-  extern __shared__ __align__(16) unsigned char smraw[];
-  V* tile = reinterpret_cast<V*>(smraw);
+  constexpr int T = BIN + kW - 1;
+  constexpr int bin = BIN;
+  __shared__ __align__(16) V tile[T * T];
   const SubProb sp = sps[blockIdx.x];
-  const int T = bin + kW - 1;
   const V* g = gridA + sp.grid_off;
   const int gx0 = sp.bin_x * bin, gy0 = sp.bin_y * bin;
   // the first point's cell and weights are loaded before the tile, so both

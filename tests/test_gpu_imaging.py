@@ -51,14 +51,19 @@ def _solve_small_fcv(precision):
         lbfgs.lbfgsbprms.update(old)
 
 
-@pytest.mark.parametrize("precision,tol_img", [(64, 1e-4), (32, 1e-3)], ids=["fp64", "fp32"])
-def test_imaging_reproduces_reference_solve(gpu_lib, precision, tol_img):
+@pytest.mark.parametrize("precision,tol_img,tol_cost", [(64, 1e-4, 1e-6), (32, 1e-3, 1e-5)],
+                         ids=["fp64", "fp32"])
+def test_imaging_reproduces_reference_solve(gpu_lib, precision, tol_img, tol_cost):
     """64x64 ring, ~1,900 complex visibilities, fcv chi-square + L1 (1) + TSV
     (1e4), nonneg, L-BFGS-B m=10, 1000 iterations from a 20 uas Gaussian,
-    against the reference solve (exact-DFT oracle cost + L-BFGS-B): final
-    cost within 1e-6 relative; final image within NRMSE 1e-4 with the FP64
-    NUFFT, 1e-3 with the FP32 NUFFT (whose rounding noise in the cost ends
-    L-BFGS-B's factr test earlier)."""
+    against the reference solve (exact-DFT oracle cost + L-BFGS-B): with the
+    FP64 NUFFT (the default) final cost within 1e-6 relative and final image
+    within NRMSE 1e-4.  The FP32 NUFFT's rounding noise in the cost (~1e-7
+    relative, ~3e-7 absolute here) is far above the decreases L-BFGS-B still
+    tests for (factr = 10: factr * eps * |f| ~ 8e-15), so its line search
+    stops where the noise hides the decrease: after 80-100 iterations, ~1e-6
+    from the reference cost, at a point that moves from run to run with the
+    spreading's float atomics; it is held to 1e-5 and NRMSE 1e-3."""
     from smili_amd.imaging.image import nrmse
     gold = np.load(os.path.join(HERE, "solve_small_fcv.npz"))
     out, info = _solve_small_fcv(precision)
@@ -66,7 +71,7 @@ def test_imaging_reproduces_reference_solve(gpu_lib, precision, tol_img):
     print("imaging precision", precision, ": nit", info.niter, "nfev", info.nfev, "cost", info.cost,
           "ref", float(gold["cost"]), "NRMSE", e, "wall", info.wall_s, "eval", info.eval_s)
     assert e <= tol_img
-    assert abs(info.cost - float(gold["cost"])) <= 1e-6 * abs(float(gold["cost"]))
+    assert abs(info.cost - float(gold["cost"])) <= tol_cost * abs(float(gold["cost"]))
 
 
 def test_imaging_pulse_fwhm(gpu_lib):

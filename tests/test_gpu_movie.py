@@ -113,7 +113,7 @@ def test_movie_transforms_per_frame(gpu_lib):
 def test_movie_graph_equals_eager(gpu_lib):
     rng = np.random.default_rng(10)
     mv, x = random_movie(rng, 5, 24, 24, [100] * 5, lc=True)
-    with _movie(mv) as g, _movie(mv, tune={"use_graph": -1}) as e:
+    with _movie(mv, tune={"use_graph": 1}) as g, _movie(mv) as e:
         for k in range(3):
             xx = x * (1 + 0.05 * k)
             cg, gg, _ = g.cost_grad(xx)
@@ -166,7 +166,7 @@ def test_batch_survey_of_many_small_problems(gpu_lib):
                             regs=("l1", "tsv")) for _ in range(64)]
     X = np.concatenate([x for _, x in cases])
     kws = [problem_kwargs(p) for p, _ in cases]
-    with BatchProblem(kws) as b, BatchProblem(kws, tune={"use_graph": -1}) as be:
+    with BatchProblem(kws, tune={"use_graph": 1}) as b, BatchProblem(kws) as be:
         costs, grad, _ = b.cost_grad(X)
         costs_e, grad_e, _ = be.cost_grad(X)
         assert np.array_equal(costs, costs_e)

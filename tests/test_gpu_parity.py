@@ -139,7 +139,7 @@ def test_tv_reset_quirk_on_flat_patch(gpu_lib):
 def test_graph_and_eager_agree_and_repeat(gpu_lib):
     rng = np.random.default_rng(9)
     p, x = random_problem(rng, 32, 32, M=600)
-    with _mk(p) as g1, _mk(p, tune=dict(use_graph=-1)) as g2:
+    with _mk(p, tune=dict(use_graph=1)) as g1, _mk(p) as g2:
         c1, gr1, _ = g1.cost_grad(x)
         c1b, gr1b, _ = g1.cost_grad(x)
         c2, gr2, _ = g2.cost_grad(x)
