@@ -63,7 +63,9 @@ def stage_bytes(kws, units, fp64=True):
     elements of 16 bytes in FP64, 8 in FP32; ES weights 14 x 8 or 14 x 4 bytes
     per point), with the data records of the cost model (index + datum +
     variance: 28 B per full visibility or closure phase, 20 per amplitude,
-    32 per log closure amplitude) and one FP64 complex residual per point.
+    32 per log closure amplitude -- or the stored record where it is smaller:
+    24 B per log closure amplitude, whose four point indices are 16-bit
+    chunk-local slots) and one FP64 complex residual per point.
     ``kws``: the problems' cost keyword dicts (a movie's dict carries one
     entry per frame in ``units``); ``units``: the handle's unit_info() of
     each unit in the same order (transform, grid or window width / height)."""
@@ -112,7 +114,7 @@ def stage_bytes(kws, units, fp64=True):
         namp = 0 if kw.get("uvidxamp") is None else len(kw["uvidxamp"])
         ncp = 0 if kw.get("uvidxcp") is None else kw["uvidxcp"].shape[1]
         nca = 0 if kw.get("uvidxca") is None else kw["uvidxca"].shape[1]
-        add("chisq", 28 * nfcv + 20 * namp + 28 * ncp + 32 * nca + cb * M_all + 16 * M_all)
+        add("chisq", 28 * nfcv + 20 * namp + 28 * ncp + 24 * nca + cb * M_all + 16 * M_all)
     return tot
 
 

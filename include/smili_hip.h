@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define SMILI_ABI_VERSION 2
+#define SMILI_ABI_VERSION 3
 #define SMILI_NTERMS 15
 
 /* status codes */
@@ -165,6 +165,14 @@ typedef struct smili_tune {
                            the window of fine-grid cells its points touch;
                            0 (default) = per image, whichever the cost
                            model finds cheaper */
+  int32_t deterministic; /* > 0: cost and gradient the same bit for bit from
+                           run to run (no float atomics across waves: each
+                           chi-square chunk is one wave of at most 1024
+                           data, the spreading sums per-wave footprints and
+                           per-item grid slots in fixed order); slower
+                           (256^2 cost: ~1.2x).  0 (default): float atomics
+                           in the spreading and the chunks' point sums, so
+                           the gradient reproduces to rounding */
 } smili_tune;
 
 int smili_abi_version(void);
@@ -293,6 +301,7 @@ enum smili_stage {
   SMILI_S_DFT2_X,          /* G = S Ex on the window */
   SMILI_S_DFT1_X,          /* P = Gs Ex^T from the spread window */
   SMILI_S_DFT1_Y,          /* Re(Ey^T P) + crop (+ gradient assembly) */
+  SMILI_S_GATHER,          /* spreading slots added per grid cell, in fixed order */
   SMILI_NSTAGES
 };
 

@@ -69,7 +69,8 @@ class SmiliTune(C.Structure):
     _fields_ = [("bin", C.c_int32), ("sub_points", C.c_int32),
                 ("col_group", C.c_int32), ("chunk", C.c_int32),
                 ("use_graph", C.c_int32), ("precision", C.c_int32), ("branches", C.c_int32),
-                ("fft_radix", C.c_int32), ("chisq_threads", C.c_int32), ("transform", C.c_int32)]
+                ("fft_radix", C.c_int32), ("chisq_threads", C.c_int32), ("transform", C.c_int32),
+                ("deterministic", C.c_int32)]
 
 
 # every symbol include/smili_hip.h declares, with (restype, argtypes)
@@ -104,11 +105,11 @@ SIGNATURES = {
 
 STAGE_NAMES = ("scatter", "t2_rows", "t2_cols", "interp", "moments", "moments_reduce",
                "chisq", "spread", "t1_cols", "t1_rows", "reg", "finalize", "frame_mean",
-               "dft2_y", "dft2_x", "dft1_x", "dft1_y")
+               "dft2_y", "dft2_x", "dft1_x", "dft1_y", "gather")
 KERNEL_OF_STAGE = dict(zip(STAGE_NAMES, ("k_scatter", "k_t2_rows", "k_t2_cols", "k_interp", "k_moments",
                                          "k_moments_reduce", "k_chisq", "k_spread", "k_t1_cols",
                                          "k_t1_rows", "k_reg", "k_finalize", "k_frame_mean",
-                                         "k_dft", "k_dft", "k_dft", "k_dft")))
+                                         "k_dft", "k_dft", "k_dft", "k_dft", "k_spread_gather")))
 
 _lib = None
 
@@ -132,7 +133,7 @@ def load(path: str | None = None):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.smili_abi_version() != 2:
+    if lib.smili_abi_version() != 3:
         raise OSError("libsmili_hip ABI version mismatch")
     if path is None:
         _lib = lib

@@ -274,6 +274,7 @@ struct EvalBufs {
   double *I2d = nullptr, *grad2d = nullptr, *greg = nullptr;
   double *cpart = nullptr, *rpart = nullptr, *mpart = nullptr;
   UnitMoments* mom = nullptr;
+  double2* spread_slots = nullptr;         // spreading slots (float2 or double2)
 };
 
 struct Plan {
@@ -284,6 +285,7 @@ struct Plan {
   smili_tune tune{};
   int bin = 16, sub_points = 256, col_group = 2, chunk = 4096, rows_per_block = 1;
   bool fp64 = false;                       // NUFFT arithmetic in FP64 instead of FP32
+  bool det = false;                        // no float atomics across waves (smili_tune.deterministic)
   int threads_rows = 64, threads_cols = 128, fft_radix = 4;
   int chisq_threads = kThreads;            // work-group size of the chi-square (+ regularizer) launch
 
@@ -310,6 +312,13 @@ struct Plan {
   DevBuf<double2> d_H;                     // float2 or double2 S / P per DFT unit
   std::vector<SubProb> h_subs;
   std::vector<SpreadItem> h_spread;
+  // spreading slots: one per (item, reachable footprint cell), grouped by
+  // grid cell (cell g's slots are [goff[k], goff[k + 1]) for g = gcell[k]),
+  // in item order within a cell; slot_idx maps an item's reachable cells,
+  // in bit order from slot_base, to their slots
+  std::vector<int> h_slot_idx, h_goff;
+  std::vector<long long> h_gcell;
+  size_t nslots = 0;
   std::vector<Chunk> h_chunks;
   std::vector<int> h_slots;
   std::vector<RegItem> h_regs;
@@ -345,6 +354,9 @@ struct Plan {
   DevBuf<CaItem> d_ca;
   DevBuf<SubProb> d_subs;
   DevBuf<SpreadItem> d_spread;
+  DevBuf<int> d_slot_idx, d_goff;
+  DevBuf<long long> d_gcell;
+  DevBuf<double2> d_spread_slots;          // float2 or double2 slots (sized for double2)
   DevBuf<FftItem> d_t2rows, d_cols, d_t1rows;
   DevBuf<RegItem> d_regs;
   DevBuf<MovieDev> d_movies;
@@ -463,6 +475,14 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   require(tune.chisq_threads == 0 || tune.chisq_threads == 256 || tune.chisq_threads == kChisqThreadsMax,
           "chisq_threads must be 0, 256 or 512");
   chisq_threads = tune.chisq_threads > 0 ? tune.chisq_threads : kThreads;
+  // deterministic: every chi-square chunk is one wave (its point sums are
+  // then LDS atomics of one wave, applied in issue order), of at most 1024
+  // data so the chunks still fill the GPU
+  det = tune.deterministic > 0;
+  if (det) {
+    chisq_threads = 64;
+    chunk = std::min(chunk, 1024);
+  }
   tune.use_graph = (tune.use_graph > 0) ? 1 : 0;
   require(tune.branches >= 0 && tune.branches <= 2, "branches must be 0, 1 or 2");
 
@@ -708,6 +728,16 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
         si.nf1 = gw;
         si.nf2 = gh;
         si.grid_off = U.grid_off;
+        // footprint cells the item's points reach: taps (lx + b, ly + a)
+        constexpr int TF = kCellBlock + kW - 1;
+        for (int q = s; q < e; ++q) {
+          const int lx = l0x[order[q]] - si.gx0, ly = l0y[order[q]] - si.gy0;
+          for (int a = 0; a < kW; ++a)
+            for (int b = 0; b < kW; ++b) {
+              const int c = (ly + a) * TF + lx + b;
+              si.mask[c >> 5] |= 1u << (c & 31);
+            }
+        }
         h_spread.push_back(si);
         s = e;
       }
@@ -1138,7 +1168,46 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   d_cp.upload(cp);
   d_ca.upload(ca);
   d_subs.upload(h_subs);
+  if (det) {
+    // deterministic spreading: the slot tables
+    constexpr int TF = kCellBlock + kW - 1;
+    std::vector<std::pair<long long, int>> rec;   // (grid cell, item-major entry)
+    long long entry = 0;
+    for (SpreadItem& si : h_spread) {
+      si.slot_base = (int)entry;
+      for (int c = 0; c < TF * TF; ++c) {
+        if (!(si.mask[c >> 5] & (1u << (c & 31)))) continue;
+        const int ty = c / TF, tx = c - ty * TF;
+        const long long g = si.grid_off + (long long)((si.gy0 + ty) % si.nf2) * si.nf1 + (si.gx0 + tx) % si.nf1;
+        rec.emplace_back(g, (int)entry);
+        ++entry;
+        require(entry < (1LL << 31), "too many spreading slots");
+      }
+    }
+    // entries were made in item order: a stable sort by cell keeps it
+    std::stable_sort(rec.begin(), rec.end(),
+                     [](const std::pair<long long, int>& a, const std::pair<long long, int>& b) {
+                       return a.first < b.first;
+                     });
+    h_slot_idx.assign(rec.size(), 0);
+    h_gcell.clear();
+    h_goff.clear();
+    for (size_t k = 0; k < rec.size(); ++k) {
+      h_slot_idx[rec[k].second] = (int)k;
+      if (k == 0 || rec[k].first != rec[k - 1].first) {
+        h_gcell.push_back(rec[k].first);
+        h_goff.push_back((int)k);
+      }
+    }
+    h_goff.push_back((int)rec.size());
+    nslots = rec.size();
+  }
+  if (h_goff.empty()) h_goff.push_back(0);
   d_spread.upload(h_spread);
+  d_slot_idx.upload(h_slot_idx.empty() ? std::vector<int>{0} : h_slot_idx);
+  d_goff.upload(h_goff);
+  d_gcell.upload(h_gcell.empty() ? std::vector<long long>{0} : h_gcell);
+  d_spread_slots.alloc(std::max<size_t>(nslots, 1));
   d_t2rows.upload(h_t2rows);
   d_cols.upload(h_cols);
   d_t1rows.upload(h_t1rows);
@@ -1163,7 +1232,7 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&hd_x), h_x, 0));
   HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&hd_grad), h_grad, 0));
   B = EvalBufs{d_x.p, d_grad.p, d_terms_p, d_V.p, d_sres.p, d_gridA.p, d_gridB.p, d_H.p,
-               d_I2d.p, d_grad2d.p, d_greg.p, d_cpart.p, d_rpart.p, d_mpart.p, d_mom.p};
+               d_I2d.p, d_grad2d.p, d_greg.p, d_cpart.p, d_rpart.p, d_mpart.p, d_mom.p, d_spread_slots.p};
   HIP_CHECK(hipDeviceSynchronize());
 }
 
@@ -1289,12 +1358,26 @@ static void launch_adjoint_t(Plan& P, int rows, const FinArgs* fin = nullptr, bo
   const long long npt = (long long)P.npt;
   V* gA = reinterpret_cast<V*>(P.B.gridA);
   V* gB = reinterpret_cast<V*>(P.B.gridB);
-  if (fin)
-    LAUNCH(SMILI_S_SPREAD, k_spread_fin<R>, P.h_spread.size() + P.probs.size(), kThreads, 0,
-           (int)P.h_spread.size(), P.d_spread.p, P.d_cell.p, wts, npt, res, gB, *fin);
-  else
-    LAUNCH(SMILI_S_SPREAD, k_spread<R>, P.h_spread.size(), kThreads, 0, P.d_spread.p, P.d_cell.p, wts, npt, res,
-           gB);
+  V* sl = reinterpret_cast<V*>(P.B.spread_slots);
+  if (P.det) {
+    if (fin)
+      LAUNCH(SMILI_S_SPREAD, (k_spread_fin<R, true>), P.h_spread.size() + P.probs.size(), kThreads, 0,
+             (int)P.h_spread.size(), P.d_spread.p, P.d_cell.p, wts, npt, res, gB, P.d_slot_idx.p, sl, *fin);
+    else
+      LAUNCH(SMILI_S_SPREAD, (k_spread<R, true>), P.h_spread.size(), kThreads, 0, P.d_spread.p, P.d_cell.p, wts,
+             npt, res, gB, P.d_slot_idx.p, sl);
+    const int ncells = (int)P.h_gcell.size();
+    const size_t blocks = std::min<size_t>((ncells + kThreads - 1) / kThreads, 8192);
+    LAUNCH(SMILI_S_GATHER, k_spread_gather<V>, blocks, kThreads, 0, P.d_gcell.p, P.d_goff.p, ncells,
+           (const V*)sl, gB);
+  } else if (fin) {
+    LAUNCH(SMILI_S_SPREAD, (k_spread_fin<R, false>), P.h_spread.size() + P.probs.size(), kThreads, 0,
+           (int)P.h_spread.size(), P.d_spread.p, P.d_cell.p, wts, npt, res, gB, (const int*)nullptr, (V*)nullptr,
+           *fin);
+  } else {
+    LAUNCH(SMILI_S_SPREAD, (k_spread<R, false>), P.h_spread.size(), kThreads, 0, P.d_spread.p, P.d_cell.p, wts,
+           npt, res, gB, (const int*)nullptr, (V*)nullptr);
+  }
   if (P.any_fft)
     LAUNCH(SMILI_S_T1_COLS, k_t1_cols<R>, xcd_grid(P.h_cols.size()), P.threads_cols, P.lds_cols, P.d_cols.p,
            (int)P.h_cols.size(), tw, gB, gA);
@@ -1339,7 +1422,9 @@ void Plan::launch_eval() {
   const RegArgs reg_args{d_regs.p, d_units.p, d_pix.p, B.x, B.I2d, B.mom, d_movies.p,
                          B.greg, B.grad, B.rpart};
   const FinArgs fin_args{d_probs.p, B.cpart, B.rpart, B.mom, B.terms};
-  const bool fused = !two_branches && npt > 0;
+  // (deterministic plans launch the regularizers on their own: their
+  // work-groups need 256 threads, the chi-square chunks then have 64)
+  const bool fused = !two_branches && npt > 0 && !det;
   if (any_window)
     LAUNCH(SMILI_S_SCATTER, k_scatter, h_regs.size(), kThreads, 0, d_regs.p, d_units.p, d_pix.p, B.x, B.I2d);
   if (fork) {
@@ -1544,6 +1629,7 @@ static void eval_in_scratch(Plan& P, const double* x_dev, double* grad_dev, doub
   S.rpart = reinterpret_cast<double*>(carve(P.d_rpart.n * sizeof(double)));
   S.mpart = reinterpret_cast<double*>(carve(P.d_mpart.n * sizeof(double)));
   S.mom = reinterpret_cast<UnitMoments*>(carve(P.d_mom.n * sizeof(UnitMoments)));
+  S.spread_slots = reinterpret_cast<double2*>(carve(P.d_spread_slots.n * sizeof(double2)));
   if (off > kScratchBytes)
     throw Error(SMILI_EINVAL, "the problem needs " + std::to_string(off >> 20) + " MiB of scratch; "
                 "smili_cost_grad_device_noalloc holds at most " + std::to_string(kScratchBytes >> 20) + " MiB");

@@ -99,7 +99,16 @@ struct RegItem { int unit, start, count, pad; };
 // accumulated in LDS and added to the grid once per work item
 constexpr int kCellBlock = 8;
 constexpr int kSpreadPoints = 256;   // = work-group size: one staging round per wave
-struct SpreadItem { int gx0, gy0, pt_begin, pt_end, nf1, nf2; long long grid_off; };
+// mask: the item's footprint cells its points can reach (bit ty * 14 + tx of
+// the 14 x 14 footprint); slot_base: where its cells' slot indices start in
+// the plan's slot table (one per reachable cell, in bit order)
+struct SpreadItem {
+  int gx0, gy0, pt_begin, pt_end, nf1, nf2;
+  long long grid_off;
+  int slot_base, pad0;
+  unsigned mask[7];
+  int pad1;
+};
 
 // temporal regularizers of one movie (frames = consecutive units with
 // consecutive x ranges of npix pixels each)

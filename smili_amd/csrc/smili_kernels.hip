@@ -860,11 +860,11 @@ __global__ void __launch_bounds__(kThreads) k_finalize(FinArgs F) {
 // Spreading work items followed by one finalize work-group per problem: the
 // sums need only the chi-square and regularizer partials, complete before
 // this launch, so they overlap the spreading instead of ending the chain.
-template <typename R>
+template <typename R, bool DET>
 __global__ void __launch_bounds__(kThreads)
 k_spread_fin(int nspread, const SpreadItem* __restrict__ items, const int* __restrict__ pt_cell,
-             const R* __restrict__ wts, long long npt, ResidSrc res,
-             typename Cx<R>::T* __restrict__ gridB, FinArgs F) {
+             const R* __restrict__ wts, long long npt, ResidSrc res, typename Cx<R>::T* __restrict__ gridB,
+             const int* __restrict__ slot_idx, typename Cx<R>::T* __restrict__ slots, FinArgs F) {
   __shared__ double red[16 * (kThreads / 64)];
   // the finalize work-groups first: dispatched with the launch, they
   // finish under the spreading work instead of trailing it
@@ -873,7 +873,7 @@ k_spread_fin(int nspread, const SpreadItem* __restrict__ items, const int* __res
     const ProbDev P = F.probs[blockIdx.x];
     finalize_problem(P, F.cpart, F.rpart, F.mom, F.terms, red);
   } else
-    spread_block<R>(blockIdx.x - nfin, items, pt_cell, wts, npt, res, gridB);
+    spread_block<R, DET>(blockIdx.x - nfin, items, pt_cell, wts, npt, res, gridB, slot_idx, slots);
 }
 
 }  // namespace smili

@@ -368,29 +368,38 @@ k_interp(const SubProb* __restrict__ sps, const typename Cx<R>::T* __restrict__ 
 // (lane = point: 7 + 7 kernel values and the residual), then switches roles
 // (lane = one of the 49 taps) and walks the points in order, accumulating
 // each run of points of one cell in registers and adding the run to the
-// block's 14x14 LDS footprint when the cell changes.  The footprint is
-// added to the grid with one atomic per touched cell (work items of one
-// block and of neighbouring blocks overlap).  The grid is zero on entry
-// (k_t1_cols clears what it consumes).
+// 14x14 LDS footprint when the cell changes.  Two ways to finish:
+//  * default: one footprint for the work-group (LDS atomics of its four
+//    waves), added to the grid with one float atomic per touched cell
+//    (work items of one block and of neighbouring blocks overlap); the grid
+//    is zero on entry (its consumer clears what it reads);
+//  * DET (deterministic): one footprint per wave, summed in wave order, and
+//    every cell the item's points can reach stored with a plain store into
+//    that cell's slot of the slot array (a cell's slots are contiguous, in
+//    item order); k_spread_gather then adds each cell's slots in order into
+//    the grid.  No float atomics across waves: the spread grid is the same
+//    bit for bit from run to run.
 // ---------------------------------------------------------------------------
 #ifndef SMILI_SPREAD_BATCH
 #define SMILI_SPREAD_BATCH 4
 #endif
 constexpr int kSpreadBatch = SMILI_SPREAD_BATCH;
 
-template <typename R>
+template <typename R, bool DET>
 __device__ void spread_block(int bid, const SpreadItem* __restrict__ items, const int* __restrict__ pt_cell,
                              const R* __restrict__ wts, long long npt, const ResidSrc& res,
-                             typename Cx<R>::T* __restrict__ gridB) {
+                             typename Cx<R>::T* __restrict__ gridB, const int* __restrict__ slot_idx,
+                             typename Cx<R>::T* __restrict__ slots) {
   using V = typename Cx<R>::T;
   constexpr int T = kCellBlock + kW - 1;
   constexpr int NW = kThreads / 64;
-  __shared__ V tile[T * T];
+  constexpr int NT = DET ? NW : 1;                 // footprints: one per wave, or one shared
+  __shared__ V tile[NT][T * T];
   __shared__ R wxs[NW][kW][65], wys[NW][kW][65];   // padded: the 7 rows sit in distinct banks
   __shared__ V cs[NW][64];
   __shared__ int cl[NW][64];
   const SpreadItem it = items[bid];
-  for (int c = threadIdx.x; c < T * T; c += blockDim.x) tile[c] = cmk<V>(0, 0);
+  for (int c = threadIdx.x; c < NT * T * T; c += blockDim.x) (&tile[0][0])[c] = cmk<V>(0, 0);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int base = it.pt_begin + w * 64;
   const int np = max(0, min(64, it.pt_end - base));
@@ -410,14 +419,18 @@ __device__ void spread_block(int bid, const SpreadItem* __restrict__ items, cons
   }
   __syncthreads();
   if (lane < kW * kW && np > 0) {
+    // lanes hold distinct taps, so a wave's adds of one run go to distinct
+    // cells, and its runs are added in point order (a wave's LDS operations
+    // complete in issue order)
+    V* tw = tile[DET ? w : 0];
     const int a = lane / kW, b = lane - a * kW;
     const int off = a * T + b;
     R ar = 0, ai = 0;
     int cur = cl[w][0];
     auto step = [&](int c, R wt, V v) {
       if (c != cur) {
-        atomicAdd(&tile[cur + off].x, ar);
-        atomicAdd(&tile[cur + off].y, ai);
+        atomicAdd(&tw[cur + off].x, ar);
+        atomicAdd(&tw[cur + off].y, ai);
         ar = 0;
         ai = 0;
         cur = c;
@@ -442,29 +455,80 @@ __device__ void spread_block(int bid, const SpreadItem* __restrict__ items, cons
       for (int k = 0; k < kSpreadBatch; ++k) step(c[k], wt[k], v[k]);
     }
     for (; p < np; ++p) step(cl[w][p], wxs[w][b][p] * wys[w][a][p], cs[w][p]);
-    atomicAdd(&tile[cur + off].x, ar);
-    atomicAdd(&tile[cur + off].y, ai);
+    atomicAdd(&tw[cur + off].x, ar);
+    atomicAdd(&tw[cur + off].y, ai);
   }
   __syncthreads();
-  V* g = gridB + it.grid_off;
-  for (int c = threadIdx.x; c < T * T; c += blockDim.x) {
-    const V val = tile[c];
-    if (val.x != 0 || val.y != 0) {
-      const int ty = c / T, tx = c - ty * T;
-      const int gy = (it.gy0 + ty) % it.nf2;
-      const int gx = (it.gx0 + tx) % it.nf1;
-      V* dst = g + (long long)gy * it.nf1 + gx;
-      unsafeAtomicAdd(&dst->x, val.x);
-      unsafeAtomicAdd(&dst->y, val.y);
+  if constexpr (DET) {
+    // reachable cells in bit order: the slot index of cell c is entry
+    // slot_base + (number of reachable cells before c)
+    for (int c = threadIdx.x; c < T * T; c += blockDim.x) {
+      const int wd = c >> 5;
+      const unsigned bit = 1u << (c & 31);
+      if (!(it.mask[wd] & bit)) continue;
+      int rank = __popc(it.mask[wd] & (bit - 1));
+      for (int k = 0; k < wd; ++k) rank += __popc(it.mask[k]);
+      V val = tile[0][c];
+#pragma unroll
+      for (int k = 1; k < NW; ++k) {
+        val.x += tile[k][c].x;
+        val.y += tile[k][c].y;
+      }
+      slots[slot_idx[it.slot_base + rank]] = val;
+    }
+  } else {
+    V* g = gridB + it.grid_off;
+    for (int c = threadIdx.x; c < T * T; c += blockDim.x) {
+      const V val = tile[0][c];
+      if (val.x != 0 || val.y != 0) {
+        const int ty = c / T, tx = c - ty * T;
+        const int gy = (it.gy0 + ty) % it.nf2;
+        const int gx = (it.gx0 + tx) % it.nf1;
+        V* dst = g + (long long)gy * it.nf1 + gx;
+        unsafeAtomicAdd(&dst->x, val.x);
+        unsafeAtomicAdd(&dst->y, val.y);
+      }
     }
   }
 }
 
-template <typename R>
+template <typename R, bool DET>
 __global__ void __launch_bounds__(kThreads)
 k_spread(const SpreadItem* __restrict__ items, const int* __restrict__ pt_cell,
-         const R* __restrict__ wts, long long npt, ResidSrc res, typename Cx<R>::T* __restrict__ gridB) {
-  spread_block<R>(blockIdx.x, items, pt_cell, wts, npt, res, gridB);
+         const R* __restrict__ wts, long long npt, ResidSrc res, typename Cx<R>::T* __restrict__ gridB,
+         const int* __restrict__ slot_idx, typename Cx<R>::T* __restrict__ slots) {
+  spread_block<R, DET>(blockIdx.x, items, pt_cell, wts, npt, res, gridB, slot_idx, slots);
+}
+
+// Type-1 step 1b: every grid cell some spreading item can reach is the sum
+// of its slots, added in item order (cell c's slots are [off[c], off[c+1])).
+// Cells no item reaches are never written and stay zero.
+template <typename V>
+__global__ void __launch_bounds__(kThreads)
+k_spread_gather(const long long* __restrict__ cell, const int* __restrict__ off, int ncells,
+                const V* __restrict__ slots, V* __restrict__ grid) {
+  constexpr int U = 8;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += gridDim.x * blockDim.x) {
+    const int b = off[c], e = off[c + 1];
+    V s = slots[b];
+    int k = b + 1;
+    for (; k + U <= e; k += U) {
+      V v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = slots[k + u];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        s.x += v[u].x;
+        s.y += v[u].y;
+      }
+    }
+    for (; k < e; ++k) {
+      const V v = slots[k];
+      s.x += v.x;
+      s.y += v.y;
+    }
+    grid[cell[c]] = s;
+  }
 }
 
 // ---------------------------------------------------------------------------

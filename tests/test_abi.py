@@ -28,13 +28,13 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
         assert n in _lib.SIGNATURES, n
-    assert lib.smili_abi_version() == 2
+    assert lib.smili_abi_version() == 3
 
 
 def test_struct_layouts_match_header_sizes():
     # pointer-size fields in declared order: 10 doubles + 5 pointers (reg)
     assert C.sizeof(_lib.SmiliReg) == 18 * 8
-    assert C.sizeof(_lib.SmiliTune) == 10 * 4
+    assert C.sizeof(_lib.SmiliTune) == 11 * 4
 
 
 def test_struct_layouts_match_the_c_compiler(tmp_path):

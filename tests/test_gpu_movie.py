@@ -122,6 +122,22 @@ def test_movie_graph_equals_eager(gpu_lib):
             assert rel_l2(gg, ge) <= 1e-13
 
 
+def test_movie_deterministic_mode(gpu_lib):
+    """smili_tune.deterministic on a movie: repeated evaluations give the
+    same cost and gradient bit for bit, equal to the default path's to
+    rounding."""
+    rng = np.random.default_rng(12)
+    mv, x = random_movie(rng, 5, 24, 24, [400] * 5, lc=True)
+    with _movie(mv) as f, _movie(mv, tune={"deterministic": 1}) as d:
+        cf, gf, _ = f.cost_grad(x)
+        c0, g0, _ = d.cost_grad(x)
+        assert abs(c0 - cf) <= 1e-12 * abs(cf)
+        assert rel_l2(g0, gf) <= 1e-10
+        for _ in range(3):
+            c1, g1, _ = d.cost_grad(x)
+            assert c1 == c0 and np.array_equal(g1, g0)
+
+
 # ---------------------------------------------------------------------------
 # batched 2-D problems
 # ---------------------------------------------------------------------------

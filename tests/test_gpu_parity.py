@@ -150,6 +150,17 @@ def test_graph_and_eager_agree_and_repeat(gpu_lib):
     assert rel_l2(gr1b, gr1) < 1e-6 and rel_l2(gr2, gr1) < 1e-6
 
 
+@pytest.mark.parametrize("shape,window,prec", [((24, 20), False, 32), ((32, 32), True, 64), ((64, 48), True, 32),
+                                               ((17, 13), False, 64)])
+def test_deterministic_mode_matches_oracle(gpu_lib, shape, window, prec):
+    """smili_tune.deterministic: the same parity as the default path
+    (all four data types, five regularizers, windows), in FP32 and FP64."""
+    rng = np.random.default_rng(70 + shape[0] + window)
+    p, x = random_problem(rng, shape[0], shape[1], M=500, window=window)
+    with _mk(p, tune=dict(deterministic=1, precision=prec)) as g:
+        _check_cost(p, x, g)
+
+
 @pytest.mark.parametrize("precision", [32, 64])
 def test_term_errors_against_dft(gpu_lib, precision):
     """Every output term against the exact-DFT oracle: the chi-square terms
@@ -299,3 +310,37 @@ def test_host_buffers_give_the_same_evaluation(gpu_lib):
     assert c1 == c2 == c4 and c3 != c1
     assert t1 == t2
     assert rel_l2(g2, g1) <= 1e-13 and rel_l2(g4, g1) <= 1e-13
+
+
+def test_gradient_reproducible_at_full_size(gpu_lib):
+    """smili_tune.deterministic at the 256^2 image with 1e5 amplitudes and
+    the closure data of 12 stations x 500 timestamps (FP64 and FP32 NUFFT):
+    repeated evaluations, through the host path and on device buffers, and
+    graph replays give the same cost and gradient bit for bit (fixed-order
+    sums throughout, no float atomics across waves); they equal the default
+    path's to rounding."""
+    import torch
+    from smili_amd import workloads
+    from smili_amd.cost import CostProblem
+    prep, _ = workloads.single_large()
+    for prec in (64, 32):
+        with CostProblem(**prep.kwargs, tune=dict(precision=prec)) as f:
+            cf, gf, _ = f.cost_grad(prep.x0)
+        with CostProblem(**prep.kwargs, tune=dict(precision=prec, deterministic=1)) as g, \
+                CostProblem(**prep.kwargs, tune=dict(precision=prec, deterministic=1, use_graph=1)) as h:
+            c0, g0, t0 = g.cost_grad(prep.x0)
+            assert abs(c0 - cf) <= (1e-13 if prec == 64 else 1e-6) * abs(cf)
+            assert rel_l2(g0, gf) <= (1e-12 if prec == 64 else 1e-5)
+            for _ in range(2):
+                ch, gh, th = h.cost_grad(prep.x0)
+                assert ch == c0 and th == t0 and np.array_equal(gh, g0)
+            for _ in range(3):
+                c1, g1, t1 = g.cost_grad(prep.x0)
+                assert c1 == c0 and t1 == t0
+                assert np.array_equal(g1, g0)
+            xd = torch.from_numpy(prep.x0).to("cuda")
+            gd = torch.empty_like(xd)
+            for _ in range(3):
+                g.cost_grad_device(xd.data_ptr(), gd.data_ptr())
+                g.synchronize()
+                assert np.array_equal(gd.cpu().numpy(), g0)
