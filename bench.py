@@ -531,8 +531,9 @@ def bench_warmstart(args):
                    "seeds": [base, base + args.steps], "reached": sm["reached"], "n": sm["n"],
                    "median_iter_ratio": sm["median_iter_ratio"], "median_plain_s": sm["median_plain_s"],
                    "median_warm_s": sm["median_warm_s"], "max_nrmse": sm["max_nrmse"],
-                   "proposals_used": sm["proposals_used"], "n_train": int(meta.get("n_train", 0)),
-                   "workers": args.workers},
+                   "proposals_used": sm["proposals_used"],
+                   "n_train": int(meta.get("n_seen", meta.get("n_train", 0))),
+                   "training_parts": int(meta.get("parts", 1)), "workers": args.workers},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
@@ -601,7 +602,7 @@ def bench_warmstart_movie(args):
                    "rows": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in rows],
                    "median_plain_s": med("plain_s"), "median_warm_s": med("warm_s"),
                    "reached": sum(bool(r["reached"]) for r in rows), "n": len(rows),
-                   "n_train": int(meta.get("n_train", 0))},
+                   "n_train": int(meta.get("n_seen", meta.get("n_train", 0)))},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

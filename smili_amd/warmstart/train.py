@@ -6,6 +6,11 @@ device memory; every step draws a batch of (sample, symmetry of the square)
 pairs, so the eight symmetries multiply the set without copies.  Adam with
 a linear warm-up and a cosine decay; the last ``--holdout`` fraction of the
 seeds is never trained on and gives the validation error.
+
+``--init FILE.npz`` continues from a saved network (full float32
+parameters; Adam's moments start afresh), so a set too large to generate
+in one session can be trained on in consecutive parts of fresh seeds, each
+part generated and then trained on from the previous part's network.
 """
 from __future__ import annotations
 
@@ -94,16 +99,35 @@ def main(argv=None):
     ap.add_argument("--holdout", type=float, default=0.05)
     ap.add_argument("--weight-decay", type=float, default=0.0)
     ap.add_argument("--packed", action="store_true", help="write the packed (bfloat16 3x3 weights) form")
+    ap.add_argument("--init", default=None, help="continue from this saved network (full float32 form)")
     a = ap.parse_args(argv)
     x, t, meta = gen.load(a.data, log=lambda s: print(s, flush=True))
     print(f"[train] loaded {x.shape[0]} samples from {a.data}", flush=True)
+    params, prior = None, {}
+    if a.init:
+        z = np.load(a.init)
+        if "params" not in z.files:
+            raise SystemExit(f"{a.init} holds the packed form; continuing needs the full float32 parameters")
+        params = z["params"].astype(np.float32)
+        prior = {k: z[k] for k in ("seeds", "n_seen", "parts") if k in z.files}
+        print(f"[train] continuing from {a.init} ({int(prior.get('n_seen', 0))} samples seen before)", flush=True)
     net, rep = train(x, t, a.epochs, a.batch, a.lr, a.warmup, a.holdout, weight_decay=a.weight_decay,
-                     log=lambda s: print(s, flush=True))
+                     log=lambda s: print(s, flush=True), params=params)
     from .data import CFG
-    net.save(a.out, packed=a.packed, cfg=json.dumps(CFG), seeds=meta["seed"], n_train=rep["n_train"],
-             epochs=rep["epochs"], train_loss=rep["train_loss"][::max(1, rep["steps"] // 2000)], val=rep["val"],
-             seconds=rep["seconds"])
-    print(f"[train] saved {a.out}", flush=True)
+    # seeds of every part trained on so far, and how many were trained on
+    # (each part's held-out tail excluded)
+    seeds = np.concatenate([prior["seeds"], meta["seed"]]) if "seeds" in prior else meta["seed"]
+    n_seen = int(prior.get("n_seen", 0)) + rep["n_train"]
+    parts = int(prior.get("parts", 0)) + 1
+    # the packed form ships; the full float32 form (beside it) continues
+    outs = [(a.out, a.packed)]
+    if a.packed:
+        outs.append((a.out[:-4] + "_full.npz", False))
+    for out, packed in outs:
+        net.save(out, packed=packed, cfg=json.dumps(CFG), seeds=seeds, n_train=rep["n_train"], n_seen=n_seen,
+                 parts=parts, epochs=rep["epochs"], train_loss=rep["train_loss"][::max(1, rep["steps"] // 2000)],
+                 val=rep["val"], seconds=rep["seconds"])
+        print(f"[train] saved {out}", flush=True)
     return 0
 
 

@@ -1,6 +1,6 @@
 """What the trained warm-start network gains over the plain imager, on
 held-out seeds of its training distribution (seeds from 1,000,000 up; the
-network was trained on seeds below 10,000), with the committed weights
+network was trained on seeds below 62,500), with the committed weights
 (smili_amd/warmstart/weights/unet128_l1tsv.npz).
 
 For every case: the network's proposal (flux scale fitted along the

@@ -83,6 +83,23 @@ def test_shards_round_trip(tmp_path):
         gen.load(str(tmp_path), inputs=False)
 
 
+def test_shards_with_stored_inputs(tmp_path):
+    """--inputs shards carry the network inputs (float16), and load returns
+    them in seed order without recomputing them."""
+    metas, ts, xs = [], [], []
+    for s in (8, 2, 6):
+        ts.append(np.full((4, 4), s, np.float32))
+        xs.append(np.full((4, 4, 2), 0.5 * s, np.float32))
+        metas.append(dict(seed=s, kind="ring", nvis=400, niter=100, nfev=110, cost=2.0, solve_s=0.1, total_s=0.2,
+                          flux=1.0, dirty_peak=2.0, target_peak=3.0))
+    gen._save(str(tmp_path / "shard_00.npz"), ts[:2], metas[:2], xs[:2])
+    gen._save(str(tmp_path / "shard_01.npz"), ts[2:], metas[2:], xs[2:])
+    x, t, m = gen.load(str(tmp_path))
+    assert list(m["seed"]) == [2, 6, 8]
+    assert x.dtype == np.float32 and np.array_equal(x[:, 0, 0, 1], [1.0, 3.0, 4.0])
+    assert np.array_equal(t[:, 0, 0], [2, 6, 8])
+
+
 def test_lr_schedule():
     lr = train.lr_schedule(1000, 1e-3, 100)
     assert lr.shape == (1000,) and lr.dtype == np.float32
