@@ -47,6 +47,17 @@ struct Error : std::runtime_error {
 #define SMILI_CHISQ_DIRECT 1
 #endif
 
+// pixels per regularizer work item (the threads loop over them; measured
+// per evaluation, 256 / 512 / 1024 / 2048 / 4096: 256^2 single 52.8 / 53.2 /
+// 53.1 / 59.7 / 72.7 us (its regularizer items ride in the chi-square
+// launch and from 2048 on outlast the chunks), 64-frame movie 188.5 /
+// 173.2 / 165.7 / 162.0 / 173.5 us, 512-problem survey 2.72 / 2.60 / 2.53
+// / 2.52 / 2.51 ms: fewer, longer items spread each item's descriptor
+// loads and partial-sum reduction over more pixels)
+#ifndef SMILI_REG_CHUNK
+#define SMILI_REG_CHUNK 1024
+#endif
+
 constexpr long long kColGroupsMin = 256;   // column work-groups wanted (one per CU)
 
 static void require(bool ok, const std::string& msg) {
@@ -927,7 +938,7 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
       col_group /= 2;
   }
   // regularizer items, row/column items
-  const int reg_chunk = kThreads;
+  const int reg_chunk = SMILI_REG_CHUNK;
   std::vector<int> unit_reg_begin(units.size() + 1, 0);
   // FFT work-groups: about 512 points per row group (one 64-thread wave of
   // radix-8 butterflies), 64 threads per column of a column group
