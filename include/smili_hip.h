@@ -173,6 +173,12 @@ typedef struct smili_tune {
                            (256^2 cost: ~1.2x).  0 (default): float atomics
                            in the spreading and the chunks' point sums, so
                            the gradient reproduces to rounding */
+  int32_t dft_shapes;    /* work-group shapes of the pruned-DFT products:
+                           1 = for few products (split each product's sum
+                           many ways: one large image), 2 = for many (more
+                           rows per work-group: movies, surveys), 0
+                           (default) = 2 when the first product has at
+                           least 1024 work-groups with shape 1, else 1 */
 } smili_tune;
 
 int smili_abi_version(void);

@@ -70,7 +70,7 @@ class SmiliTune(C.Structure):
                 ("col_group", C.c_int32), ("chunk", C.c_int32),
                 ("use_graph", C.c_int32), ("precision", C.c_int32), ("branches", C.c_int32),
                 ("fft_radix", C.c_int32), ("chisq_threads", C.c_int32), ("transform", C.c_int32),
-                ("deterministic", C.c_int32)]
+                ("deterministic", C.c_int32), ("dft_shapes", C.c_int32)]
 
 
 # every symbol include/smili_hip.h declares, with (restype, argtypes)

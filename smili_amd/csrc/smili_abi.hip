@@ -317,6 +317,7 @@ struct Plan {
   std::vector<char> unit_dft;
   std::vector<std::pair<int, int>> unit_grid;   // grid (FFT) or window (DFT) width, height
   bool any_fft = false, any_dft = false;
+  bool dft_batch = false;                  // the many-product shapes of the DFT kernels
   std::vector<DftItem> h_dft[4];
   DevBuf<DftItem> d_dft[4];
   DevBuf<double2> d_dftm;                  // float2 or double2 DFT matrices (sized for double2)
@@ -496,6 +497,7 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   }
   tune.use_graph = (tune.use_graph > 0) ? 1 : 0;
   require(tune.branches >= 0 && tune.branches <= 2, "branches must be 0, 1 or 2");
+  require(tune.dft_shapes >= 0 && tune.dft_shapes <= 2, "dft_shapes must be 0, 1 or 2");
 
   HIP_CHECK(hipGetDevice(&device));
   HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -1012,6 +1014,19 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
           h_dft[stage].push_back(d);
         }
     };
+    // many products: the batch shapes (SMILI_DFT_BATCH_ITEMS)
+    {
+      long long first = 0;
+      for (size_t ui = 0; ui < units.size(); ++ui)
+        if (unit_dft[ui])
+          first += (long long)((units[ui].nx + 63) / 64) *
+                   ((gdim[ui].second + ShapeT2Y::rows - 1) / ShapeT2Y::rows);
+      dft_batch = tune.dft_shapes == 2 || (tune.dft_shapes == 0 && first >= SMILI_DFT_BATCH_ITEMS);
+    }
+    const int rows_t2y = dft_batch ? ShapeBT2Y::rows : ShapeT2Y::rows;
+    const int rows_t2x = dft_batch ? ShapeBT2X::rows : ShapeT2X::rows;
+    const int rows_t1x = dft_batch ? ShapeBT1X::rows : ShapeT1X::rows;
+    const int rows_t1y = dft_batch ? ShapeBT1Y::rows : ShapeT1Y::rows;
     for (size_t ui = 0; ui < units.size(); ++ui) {
       if (!unit_dft[ui]) continue;
       const UnitDev& U = units[ui];
@@ -1032,23 +1047,23 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
       t.bsel = U.identity ? 0 : 1;
       t.b_off = U.identity ? U.x_off : U.img_off;
       t.c_off = h_off;
-      tiles(0, ShapeT2Y::rows, t);
+      tiles(0, rows_t2y, t);
       // G = S Ex (and the spreading window cleared)
       t = b;
       t.M = gh; t.N = gw; t.K = U.nx; t.lda = U.nx; t.ldb = gw; t.ldc = gw;
       t.a_off = h_off; t.b_off = (long long)ox; t.c_off = U.grid_off;
-      tiles(1, ShapeT2X::rows, t);
+      tiles(1, rows_t2x, t);
       // P = Gs ExT
       t = b;
       t.M = gh; t.N = U.nx; t.K = gw; t.lda = gw; t.ldb = U.nx; t.ldc = U.nx;
       t.a_off = U.grid_off; t.b_off = (long long)(ox + (size_t)U.nx * gw); t.c_off = h_off;
-      tiles(2, ShapeT1X::rows, t);
+      tiles(2, rows_t1x, t);
       // grad = Re(EyT P) (and the spreading window cleared)
       t = b;
       t.M = U.ny; t.N = U.nx; t.K = gh; t.lda = gh; t.ldb = U.nx;
       t.a_off = (long long)oy; t.b_off = h_off;
       t.z_off = U.grid_off; t.zn = gw * gh;
-      tiles(3, ShapeT1Y::rows, t);
+      tiles(3, rows_t1y, t);
     }
   }
   for (size_t pi = 0; pi < ps.size(); ++pi) {
@@ -1266,6 +1281,17 @@ static size_t xcd_grid(size_t nitems) { return (nitems + kXcds - 1) / kXcds * kX
     }                                                                         \
   } while (0)
 
+// one DFT product with the plan's shape set (few or many products)
+#define DFT_LAUNCH(stage, SH, SHB, BREAL, EPI, nitems, ...)                                        \
+  do {                                                                                            \
+    if (P.dft_batch)                                                                              \
+      LAUNCH(stage, (k_dft<R, SHB::nw, SHB::rb, SHB::ks, BREAL, EPI>), xcd_grid(nitems), SHB::threads, 0, \
+             __VA_ARGS__);                                                                        \
+    else                                                                                          \
+      LAUNCH(stage, (k_dft<R, SH::nw, SH::rb, SH::ks, BREAL, EPI>), xcd_grid(nitems), SH::threads, 0, \
+             __VA_ARGS__);                                                                        \
+  } while (0)
+
 template <typename R>
 static void launch_forward_t(Plan& P, bool with_scatter) {
   using V = typename Cx<R>::T;
@@ -1293,13 +1319,12 @@ static void launch_forward_t(Plan& P, bool with_scatter) {
   if (P.any_dft) {
     const V* M = reinterpret_cast<const V*>(P.d_dftm.p);
     V* H = reinterpret_cast<V*>(P.B.H);
-    LAUNCH(SMILI_S_DFT2_Y, (k_dft<R, ShapeT2Y::nw, ShapeT2Y::rb, ShapeT2Y::ks, true, DFT_STORE>), xcd_grid(P.h_dft[0].size()),
-           ShapeT2Y::threads, 0, P.d_dft[0].p, (int)P.h_dft[0].size(), M, (const V*)nullptr, P.B.x, P.B.I2d, H,
-           (V*)nullptr, (double*)nullptr, (double*)nullptr, (const double*)nullptr, (const int*)nullptr, 0);
-    LAUNCH(SMILI_S_DFT2_X, (k_dft<R, ShapeT2X::nw, ShapeT2X::rb, ShapeT2X::ks, false, DFT_STORE>),
-           xcd_grid(P.h_dft[1].size()), ShapeT2X::threads, 0, P.d_dft[1].p, (int)P.h_dft[1].size(), (const V*)H, M,
-           (const double*)nullptr, (const double*)nullptr, gA, (V*)nullptr, (double*)nullptr, (double*)nullptr,
-           (const double*)nullptr, (const int*)nullptr, 0);
+    DFT_LAUNCH(SMILI_S_DFT2_Y, ShapeT2Y, ShapeBT2Y, true, DFT_STORE, P.h_dft[0].size(), P.d_dft[0].p,
+               (int)P.h_dft[0].size(), M, (const V*)nullptr, P.B.x, P.B.I2d, H, (V*)nullptr, (double*)nullptr,
+               (double*)nullptr, (const double*)nullptr, (const int*)nullptr, 0);
+    DFT_LAUNCH(SMILI_S_DFT2_X, ShapeT2X, ShapeBT2X, false, DFT_STORE, P.h_dft[1].size(), P.d_dft[1].p,
+               (int)P.h_dft[1].size(), (const V*)H, M, (const double*)nullptr, (const double*)nullptr, gA,
+               (V*)nullptr, (double*)nullptr, (double*)nullptr, (const double*)nullptr, (const int*)nullptr, 0);
   }
   // static-LDS instantiation per bin edge (8 .. 64; the LDS budget check
   // at build rejects an FP64 tile of 64)
@@ -1340,10 +1365,9 @@ static void launch_rows_t(Plan& P, bool assemble = true) {
   if (P.any_dft) {
     const V* M = reinterpret_cast<const V*>(P.d_dftm.p);
     const V* H = reinterpret_cast<const V*>(P.B.H);
-    LAUNCH(SMILI_S_DFT1_Y, (k_dft<R, ShapeT1Y::nw, ShapeT1Y::rb, ShapeT1Y::ks, false, DFT_GRAD>), xcd_grid(P.h_dft[3].size()),
-           ShapeT1Y::threads, 0, P.d_dft[3].p, (int)P.h_dft[3].size(), M, H, (const double*)nullptr,
-           (const double*)nullptr, (V*)nullptr, gB, P.B.grad2d, P.B.grad, P.B.greg,
-           P.d_pixinv.p, assemble ? 1 : 0);
+    DFT_LAUNCH(SMILI_S_DFT1_Y, ShapeT1Y, ShapeBT1Y, false, DFT_GRAD, P.h_dft[3].size(), P.d_dft[3].p,
+               (int)P.h_dft[3].size(), M, H, (const double*)nullptr, (const double*)nullptr, (V*)nullptr, gB,
+               P.B.grad2d, P.B.grad, P.B.greg, P.d_pixinv.p, assemble ? 1 : 0);
   }
 }
 
@@ -1395,10 +1419,9 @@ static void launch_adjoint_t(Plan& P, int rows, const FinArgs* fin = nullptr, bo
   if (P.any_dft) {
     const V* M = reinterpret_cast<const V*>(P.d_dftm.p);
     V* H = reinterpret_cast<V*>(P.B.H);
-    LAUNCH(SMILI_S_DFT1_X, (k_dft<R, ShapeT1X::nw, ShapeT1X::rb, ShapeT1X::ks, false, DFT_STORE>), xcd_grid(P.h_dft[2].size()),
-           ShapeT1X::threads, 0, P.d_dft[2].p, (int)P.h_dft[2].size(), (const V*)gB, M, (const double*)nullptr,
-           (const double*)nullptr, H, (V*)nullptr, (double*)nullptr, (double*)nullptr, (const double*)nullptr,
-           (const int*)nullptr, 0);
+    DFT_LAUNCH(SMILI_S_DFT1_X, ShapeT1X, ShapeBT1X, false, DFT_STORE, P.h_dft[2].size(), P.d_dft[2].p,
+               (int)P.h_dft[2].size(), (const V*)gB, M, (const double*)nullptr, (const double*)nullptr, H,
+               (V*)nullptr, (double*)nullptr, (double*)nullptr, (const double*)nullptr, (const int*)nullptr, 0);
   }
   if (rows) launch_rows_t<R>(P, rows == 2);
 }

@@ -178,6 +178,30 @@ k_dft(const DftItem* __restrict__ items, int nitems, const typename Cx<R>::T* __
 #define SMILI_DFT_T1Y 16, 1, 4       // grad = Re(EyT P)
 #endif
 
+// The shapes above suit a plan with few products (one 256^2 image: a few
+// hundred work-groups, so each splits K many ways).  A plan with many
+// (movies, surveys: thousands of work-groups) takes these, with more rows
+// and fewer K slices per work-group (measured per evaluation: 64-frame
+// movie 166 -> ~100 us, 512-problem survey 2.53 -> ~1.65 ms; the 256^2
+// image is 53 us with the shapes above, 73 us with these).
+#ifndef SMILI_DFT_B_T2Y
+#define SMILI_DFT_B_T2Y 16, 4, 4
+#endif
+#ifndef SMILI_DFT_B_T2X
+#define SMILI_DFT_B_T2X 16, 2, 4
+#endif
+#ifndef SMILI_DFT_B_T1X
+#define SMILI_DFT_B_T1X 16, 4, 4
+#endif
+#ifndef SMILI_DFT_B_T1Y
+#define SMILI_DFT_B_T1Y 16, 2, 1
+#endif
+// plans with at least this many work-groups in their first product (with
+// the few-product shape) take the batch shapes
+#ifndef SMILI_DFT_BATCH_ITEMS
+#define SMILI_DFT_BATCH_ITEMS 1024
+#endif
+
 template <int NW, int RB, int KS> struct DftShape {
   static constexpr int nw = NW, rb = RB, ks = KS, rows = RB * (NW / KS), threads = 64 * NW;
 };
@@ -185,5 +209,9 @@ using ShapeT2Y = DftShape<SMILI_DFT_T2Y>;
 using ShapeT2X = DftShape<SMILI_DFT_T2X>;
 using ShapeT1X = DftShape<SMILI_DFT_T1X>;
 using ShapeT1Y = DftShape<SMILI_DFT_T1Y>;
+using ShapeBT2Y = DftShape<SMILI_DFT_B_T2Y>;
+using ShapeBT2X = DftShape<SMILI_DFT_B_T2X>;
+using ShapeBT1X = DftShape<SMILI_DFT_B_T1X>;
+using ShapeBT1Y = DftShape<SMILI_DFT_B_T1Y>;
 
 }  // namespace smili

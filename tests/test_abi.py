@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_header_sizes():
     # pointer-size fields in declared order: 10 doubles + 5 pointers (reg)
     assert C.sizeof(_lib.SmiliReg) == 18 * 8
-    assert C.sizeof(_lib.SmiliTune) == 11 * 4
+    assert C.sizeof(_lib.SmiliTune) == 12 * 4
 
 
 def test_struct_layouts_match_the_c_compiler(tmp_path):

@@ -33,14 +33,16 @@ CASES = [((24, 20), 0.25, False), ((17, 13), 0.3, True), ((64, 48), 0.2, False),
          ((40, 30), 0.95, False), ((33, 17), 0.95, True), ((128, 96), 0.15, False)]
 
 
-@pytest.mark.parametrize("transform", [1, 2])
+@pytest.mark.parametrize("transform,dft_shapes", [(1, 0), (2, 1), (2, 2)])
 @pytest.mark.parametrize("shape,ufrac,window", CASES)
-def test_fp64_matches_es_oracle(gpu_lib, shape, ufrac, window, transform):
+def test_fp64_matches_es_oracle(gpu_lib, shape, ufrac, window, transform, dft_shapes):
+    """Both transform paths; the DFT path with both work-group shape sets of
+    its products (1: few products, 2: many)."""
     rng = np.random.default_rng(500 + shape[0] + 3 * shape[1] + window)
     p, x = random_problem(rng, shape[0], shape[1], M=600, window=window, umax=np.pi * ufrac)
     a = rng.standard_normal(p.u.size)
     b = rng.standard_normal(p.u.size)
-    with _mk(p, precision=64, transform=transform) as g:
+    with _mk(p, precision=64, transform=transform, dft_shapes=dft_shapes) as g:
         V = g.model_vis(x)
         c, gr, t = g.cost_grad(x)
         # the adjoint right after an evaluation, then the evaluation again:
@@ -67,14 +69,15 @@ def test_fp64_matches_es_oracle(gpu_lib, shape, ufrac, window, transform):
         assert t[k] == pytest.approx(v, rel=1e-10, abs=1e-12 * abs(c_es)), k
 
 
+@pytest.mark.parametrize("dft_shapes", [1, 2])
 @pytest.mark.parametrize("shape,ufrac,window", CASES[:4])
-def test_fp32_matches_es_oracle(gpu_lib, shape, ufrac, window):
+def test_fp32_matches_es_oracle(gpu_lib, shape, ufrac, window, dft_shapes):
     """FP32 products (FP32 DFT matrices, FP32 accumulation over <= 2 nf terms)
     within the FP32 path's tolerances: visibilities rel-l2 <= 1e-6, cost rel
     <= 1e-6, gradient rel-l2 <= 5e-6 + 0.5 e_es."""
     rng = np.random.default_rng(600 + shape[0])
     p, x = random_problem(rng, shape[0], shape[1], M=500, window=window, umax=np.pi * ufrac)
-    with _mk(p, precision=32, transform=2) as g:
+    with _mk(p, precision=32, transform=2, dft_shapes=dft_shapes) as g:
         V = g.model_vis(x)
         c, gr, _ = g.cost_grad(x)
     I2d = O.image_1d_to_2d(x, p.xidx, p.yidx, p.Nx, p.Ny)
@@ -163,11 +166,12 @@ def test_batch_mixing_both_paths(gpu_lib):
         assert rel_l2(grads[i], g_es) <= 1e-10
 
 
-def test_movie_on_small_windows(gpu_lib):
+@pytest.mark.parametrize("dft_shapes", [1, 2])
+def test_movie_on_small_windows(gpu_lib, dft_shapes):
     from smili_amd.cost import MovieProblem
     rng = np.random.default_rng(14)
     mv, x = random_movie(rng, 5, 32, 24, [150, 0, 120, 200, 90], umax=np.pi * 0.3, window=True)
-    with MovieProblem(**movie_kwargs(mv), tune={"transform": 2}) as g:
+    with MovieProblem(**movie_kwargs(mv), tune={"transform": 2, "dft_shapes": dft_shapes}) as g:
         cost, grad, terms = g.cost_grad(x)
         prof = g.profile(reps=1)
     c_es, g_es, t_es = O.calc_cost3d(x, mv, **ES)
