@@ -344,3 +344,32 @@ def test_gradient_reproducible_at_full_size(gpu_lib):
                 g.cost_grad_device(xd.data_ptr(), gd.data_ptr())
                 g.synchronize()
                 assert np.array_equal(gd.cpu().numpy(), g0)
+
+
+@pytest.mark.parametrize("nimg", [32, 33])
+def test_host_path_on_both_sides_of_the_mapped_limit(gpu_lib, nimg):
+    """The host-pointer entry point moves x and the gradient through the
+    handle's mapped buffers up to 2^19 pixels (32 images of 128^2 exactly)
+    and by DMA beyond (33): both give the device entry point's cost and
+    terms bit for bit and its gradient to rounding."""
+    import torch
+    from smili_amd.cost import BatchProblem
+    rng = np.random.default_rng(90 + nimg)
+    cases = [random_problem(rng, 128, 128, M=120, nfcv=80, namp=20, ncp=10, nca=10, regs=("l1", "tsv"))
+             for _ in range(nimg)]
+    kws = [problem_kwargs(p) for p, _ in cases]
+    X = np.concatenate([x for _, x in cases])
+    assert (X.size <= 1 << 19) == (nimg == 32)
+    with BatchProblem(kws) as b:
+        costs, grad, terms = b.cost_grad(X)
+        xd = torch.from_numpy(X).to("cuda")
+        gd = torch.empty_like(xd)
+        td = torch.empty(b.nprob * b.nterms, dtype=torch.float64, device="cuda")
+        b.cost_grad_device(xd.data_ptr(), gd.data_ptr(), td.data_ptr())
+        b.synchronize()
+        # and the host path again, after the device one
+        costs2, grad2, _ = b.cost_grad(X)
+    tdev = td.cpu().numpy().reshape(b.nprob, b.nterms)
+    assert np.array_equal(np.asarray(terms).reshape(b.nprob, b.nterms), tdev)
+    assert np.array_equal(costs, costs2)
+    assert rel_l2(gd.cpu().numpy(), grad) <= 1e-13 and rel_l2(grad2, grad) <= 1e-13

@@ -45,6 +45,21 @@ def test_noalloc_matches_the_device_entry_point(gpu_lib):
         assert rel_l2(gb, ga) <= (1e-6 if tune and tune.get("precision") == 32 else 1e-13)
 
 
+def test_noalloc_deterministic_is_bitwise(gpu_lib):
+    """With smili_tune.deterministic the two entry points agree bit for bit,
+    gradient included (the scratch carries the spreading's slot array), on
+    two problems in a row."""
+    from smili_amd import workloads
+    from smili_amd.cost import CostProblem
+    rng = np.random.default_rng(63)
+    big, _ = workloads.single_large(seed=1)
+    small, xs = random_problem(rng, 32, 28, M=400, window=True)
+    for kw, x0 in ((big.kwargs, big.x0), (problem_kwargs(small), xs)):
+        with CostProblem(**kw, tune={"deterministic": 1}) as g:
+            ga, ta, gb, tb = _both(g, x0)
+        assert np.array_equal(ta, tb) and np.array_equal(ga, gb)
+
+
 def test_noalloc_rejects_problems_beyond_the_scratch(gpu_lib):
     from smili_amd._lib import SmiliError
     from smili_amd.cost import CostProblem
