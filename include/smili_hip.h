@@ -146,9 +146,12 @@ typedef struct smili_tune {
                            the largest of 4, 2, 1 whose LDS fits 64 KB) */
   int32_t chunk;        /* data items per chi-square work-group (default 4096) */
   int32_t use_graph;    /* > 0: capture one evaluation in a hipGraph and replay
-                           it; <= 0 (default): launch the kernels one by one
-                           (on ROCm 7 / MI355X measured faster: 256^2 cost
-                           53.3 vs 58.3 us per evaluation) */
+                           it; < 0: launch the kernels one by one; 0
+                           (default): the graph for batches of several
+                           problems (as fast: 512-problem survey 1.628 vs
+                           1.623 ms), one by one otherwise (faster on ROCm 7
+                           / MI355X: 256^2 cost 53.3 vs 58.3 us, 64-frame
+                           movie 97.8 vs 106.6 us) */
   int32_t precision;    /* bits of the NUFFT arithmetic (spreading,
                            interpolation, FFT): 64 (default, also 0) or 32;
                            chi-square and regularizers are FP64 in both */

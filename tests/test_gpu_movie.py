@@ -182,7 +182,7 @@ def test_batch_survey_of_many_small_problems(gpu_lib):
                             regs=("l1", "tsv")) for _ in range(64)]
     X = np.concatenate([x for _, x in cases])
     kws = [problem_kwargs(p) for p, _ in cases]
-    with BatchProblem(kws, tune={"use_graph": 1}) as b, BatchProblem(kws) as be:
+    with BatchProblem(kws, tune={"use_graph": 1}) as b, BatchProblem(kws, tune={"use_graph": -1}) as be:
         costs, grad, _ = b.cost_grad(X)
         costs_e, grad_e, _ = be.cost_grad(X)
         assert np.array_equal(costs, costs_e)
