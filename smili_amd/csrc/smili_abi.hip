@@ -553,7 +553,7 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   // units: geometry, pixels, weights
   std::vector<int> pix;
   std::vector<double> wgt;
-  std::vector<size_t> wgt_off_l1(us.size()), wgt_off_tv(us.size()), wgt_off_tsv(us.size()),
+  std::vector<long long> wgt_off_l1(us.size()), wgt_off_tv(us.size()), wgt_off_tsv(us.size()),
       wgt_off_kl(us.size()), wgt_off_gs(us.size());
   long long grid_total = 0, img_total = 0;
   int max_nf1 = 0, max_nf2 = 0;
@@ -630,9 +630,20 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     if (us[ui].no_cen) U.cen_l = -1.0;
     U.need_moments = (U.tfd_l > 0 || U.cen_l > 0 || U.sm_l > 0) ? 1 : 0;
     any_moments = any_moments || U.need_moments;
-    auto put = [&](double lam, const double* w, double s, size_t& off) {
-      off = wgt.size();
+    // a regularizer's weights go to the packed array, unless every pixel
+    // has the same weight: then only that value (off = -1, *u = it)
+    auto put = [&](double lam, const double* w, double s, long long& off, double& u) {
+      off = -1;
+      u = 0.0;
       if (!(lam > 0)) return;
+      const double w0 = w[0] * s;
+      bool uniform = true;
+      for (int i = 1; i < d->npix && uniform; ++i) uniform = (w[i] * s == w0);
+      if (uniform) {
+        u = w0;
+        return;
+      }
+      off = (long long)wgt.size();
       if (s == 1.0) {
         wgt.insert(wgt.end(), w, w + d->npix);
       } else {
@@ -640,17 +651,20 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
       }
     };
     const double* ws = us[ui].wscale;
-    put(r.l1_l, r.l1_wgt, ws[0], wgt_off_l1[ui]);
-    put(r.tv_l, r.tv_wgt, ws[1], wgt_off_tv[ui]);
-    put(r.tsv_l, r.tsv_wgt, ws[2], wgt_off_tsv[ui]);
-    put(r.kl_l, r.kl_wgt, ws[3], wgt_off_kl[ui]);
-    put(r.gs_l, r.gs_wgt, ws[4], wgt_off_gs[ui]);
-    if (r.kl_l > 0)
-      for (int i = 0; i < d->npix; ++i)
-        require(wgt[wgt_off_kl[ui] + i] > 0, "kl weights (prior) must be positive");
-    if (r.gs_l > 0)
-      for (int i = 0; i < d->npix; ++i)
-        require(wgt[wgt_off_gs[ui] + i] > 0, "gs weights (prior) must be positive");
+    put(r.l1_l, r.l1_wgt, ws[0], wgt_off_l1[ui], U.l1_u);
+    put(r.tv_l, r.tv_wgt, ws[1], wgt_off_tv[ui], U.tv_u);
+    put(r.tsv_l, r.tsv_wgt, ws[2], wgt_off_tsv[ui], U.tsv_u);
+    put(r.kl_l, r.kl_wgt, ws[3], wgt_off_kl[ui], U.kl_u);
+    put(r.gs_l, r.gs_wgt, ws[4], wgt_off_gs[ui], U.gs_u);
+    auto positive = [&](long long off, double u, const char* msg) {
+      if (off < 0) {
+        require(u > 0, msg);
+        return;
+      }
+      for (int i = 0; i < d->npix; ++i) require(wgt[off + i] > 0, msg);
+    };
+    if (r.kl_l > 0) positive(wgt_off_kl[ui], U.kl_u, "kl weights (prior) must be positive");
+    if (r.gs_l > 0) positive(wgt_off_gs[ui], U.gs_u, "gs weights (prior) must be positive");
     units[ui] = U;
   }
 
@@ -1089,22 +1103,28 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   d_wgt.upload(wgt);
   for (size_t ui = 0; ui < units.size(); ++ui) {
     UnitDev& U = units[ui];
-    U.l1_w = U.l1_l > 0 ? d_wgt.p + wgt_off_l1[ui] : nullptr;
-    U.tv_w = U.tv_l > 0 ? d_wgt.p + wgt_off_tv[ui] : nullptr;
-    U.tsv_w = U.tsv_l > 0 ? d_wgt.p + wgt_off_tsv[ui] : nullptr;
-    U.kl_w = U.kl_l > 0 ? d_wgt.p + wgt_off_kl[ui] : nullptr;
-    U.gs_w = U.gs_l > 0 ? d_wgt.p + wgt_off_gs[ui] : nullptr;
+    U.l1_w = wgt_off_l1[ui] >= 0 ? d_wgt.p + wgt_off_l1[ui] : nullptr;
+    U.tv_w = wgt_off_tv[ui] >= 0 ? d_wgt.p + wgt_off_tv[ui] : nullptr;
+    U.tsv_w = wgt_off_tsv[ui] >= 0 ? d_wgt.p + wgt_off_tsv[ui] : nullptr;
+    U.kl_w = wgt_off_kl[ui] >= 0 ? d_wgt.p + wgt_off_kl[ui] : nullptr;
+    U.gs_w = wgt_off_gs[ui] >= 0 ? d_wgt.p + wgt_off_gs[ui] : nullptr;
   }
   // movies: temporal weights and light curves, mean-frame buffers
   {
     std::vector<double> mw;
     std::vector<size_t> rt_off(ms.size()), ri_off(ms.size()), lc_off(ms.size()), avg_off(ms.size());
     size_t navg = 0;
+    // uniform temporal weights are passed as one value, not per pixel
+    auto uniform = [](const std::vector<double>& w) {
+      for (double v : w)
+        if (v != w[0]) return false;
+      return !w.empty();
+    };
     for (size_t mi = 0; mi < ms.size(); ++mi) {
       rt_off[mi] = mw.size();
-      mw.insert(mw.end(), ms[mi].rt_w.begin(), ms[mi].rt_w.end());
+      if (!uniform(ms[mi].rt_w)) mw.insert(mw.end(), ms[mi].rt_w.begin(), ms[mi].rt_w.end());
       ri_off[mi] = mw.size();
-      mw.insert(mw.end(), ms[mi].ri_w.begin(), ms[mi].ri_w.end());
+      if (!uniform(ms[mi].ri_w)) mw.insert(mw.end(), ms[mi].ri_w.begin(), ms[mi].ri_w.end());
       lc_off[mi] = mw.size();
       mw.insert(mw.end(), ms[mi].lc.begin(), ms[mi].lc.end());
       avg_off[mi] = navg;
@@ -1121,8 +1141,10 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
       M.x_off = units[ms[mi].unit_begin].x_off;
       M.rt_l = ms[mi].rt_l;
       M.ri_l = ms[mi].ri_l;
-      M.rt_w = ms[mi].rt_w.empty() ? nullptr : d_movw.p + rt_off[mi];
-      M.ri_w = ms[mi].ri_w.empty() ? nullptr : d_movw.p + ri_off[mi];
+      M.rt_w = (ms[mi].rt_w.empty() || uniform(ms[mi].rt_w)) ? nullptr : d_movw.p + rt_off[mi];
+      M.ri_w = (ms[mi].ri_w.empty() || uniform(ms[mi].ri_w)) ? nullptr : d_movw.p + ri_off[mi];
+      M.rt_u = uniform(ms[mi].rt_w) ? ms[mi].rt_w[0] : 0.0;
+      M.ri_u = uniform(ms[mi].ri_w) ? ms[mi].ri_w[0] : 0.0;
       M.lc = d_movw.p + lc_off[mi];
       M.iavg = d_iavg.p + avg_off[mi];
       any_ri = any_ri || M.ri_l > 0;

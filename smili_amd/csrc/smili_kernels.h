@@ -41,7 +41,11 @@ struct UnitDev {
   double reg_scale;                  // multiplies this unit's reg cost and gradient
   // regularizers (lambda <= 0 disables)
   double l1_l, tv_l, tsv_l, kl_l, gs_l;
+  // per-pixel weights; a null pointer means every pixel's weight is the
+  // matching *_u (uniform weights: the flat-prior TV / TSV of the
+  // reference's set-up), which then is not read per pixel
   const double *l1_w, *tv_w, *tsv_w, *kl_w, *gs_w;
+  double l1_u, tv_u, tsv_u, kl_u, gs_u;
   double tfd_l, tfd_tgtfd, cen_l, cen_alpha, sm_l, sm_maj, sm_min, sm_phi;
   int need_moments;
 };
@@ -117,9 +121,10 @@ struct MovieDev {
   int x_off;                         // offset of frame 0 in x / grad
   int pad;
   double rt_l, ri_l;
-  const double *rt_w, *ri_w;         // per window pixel
+  const double *rt_w, *ri_w;         // per window pixel; null: uniform, rt_u / ri_u
   const double *lc;                  // light-curve normalisation per frame
   double *iavg;                      // mean normalised frame (npix)
+  double rt_u, ri_u;
 };
 
 struct ProbDev {

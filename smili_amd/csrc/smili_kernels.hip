@@ -626,21 +626,21 @@ __device__ __forceinline__ void reg_block_unit(int bid, const RegArgs& A, const 
     // each term's branch they would be one round trip to memory per term)
     const double I = x[U.x_off + k];
     double wl1 = 0.0, Pkl = 1.0, Pgs = 1.0, wtv = 0.0, wtsv = 0.0;
-    if (U.l1_l > 0) wl1 = U.l1_w[k];
-    if (U.kl_l > 0) Pkl = U.kl_w[k];
-    if (U.gs_l > 0) Pgs = U.gs_w[k];
-    if (U.tv_l > 0) wtv = U.tv_w[k];
-    if (U.tsv_l > 0) wtsv = U.tsv_w[k];
+    if (U.l1_l > 0) wl1 = U.l1_w ? U.l1_w[k] : U.l1_u;
+    if (U.kl_l > 0) Pkl = U.kl_w ? U.kl_w[k] : U.kl_u;
+    if (U.gs_l > 0) Pgs = U.gs_w ? U.gs_w[k] : U.gs_u;
+    if (U.tv_l > 0) wtv = U.tv_w ? U.tv_w[k] : U.tv_u;
+    if (U.tsv_l > 0) wtsv = U.tsv_w ? U.tsv_w[k] : U.tsv_u;
     Nbr nb{};
     if (U.tv_l > 0 || U.tsv_l > 0) nb = load_nbr(I2d, U.nx, U.ny, ix, iy);
     double Iu = 0.0, Il = 0.0, avg = 0.0, wrt = 0.0, wri = 0.0;
     if (rt_on) {
-      wrt = Mv.rt_w[k];
+      wrt = Mv.rt_w ? Mv.rt_w[k] : Mv.rt_u;
       Iu = x[U.x_off + Mv.npix + k];
       if (U.frame > 0) Il = x[U.x_off - Mv.npix + k];
     }
     if (ri_on) {
-      wri = Mv.ri_w[k];
+      wri = Mv.ri_w ? Mv.ri_w[k] : Mv.ri_u;
       avg = Mv.iavg[k];
     }
     double g = 0.0;
