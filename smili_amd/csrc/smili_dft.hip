@@ -181,20 +181,21 @@ k_dft(const DftItem* __restrict__ items, int nitems, const typename Cx<R>::T* __
 // The shapes above suit a plan with few products (one 256^2 image: a few
 // hundred work-groups, so each splits K many ways).  A plan with many
 // (movies, surveys: thousands of work-groups) takes these, with more rows
-// and fewer K slices per work-group (measured per evaluation: 64-frame
-// movie 166 -> ~100 us, 512-problem survey 2.53 -> ~1.65 ms; the 256^2
-// image is 53 us with the shapes above, 73 us with these).
+// and fewer K slices per work-group and (for three of them) 8 waves
+// (swept per product over 30 variants; per evaluation: 64-frame movie 166
+// -> ~97 us, 512-problem survey 2.53 -> ~1.42 ms; the 256^2 image is 53 us
+// with the shapes above, 73 us with the first of these sets).
 #ifndef SMILI_DFT_B_T2Y
-#define SMILI_DFT_B_T2Y 16, 4, 4
+#define SMILI_DFT_B_T2Y 8, 4, 4
 #endif
 #ifndef SMILI_DFT_B_T2X
-#define SMILI_DFT_B_T2X 16, 2, 4
+#define SMILI_DFT_B_T2X 8, 2, 4
 #endif
 #ifndef SMILI_DFT_B_T1X
-#define SMILI_DFT_B_T1X 16, 4, 4
+#define SMILI_DFT_B_T1X 8, 4, 4
 #endif
 #ifndef SMILI_DFT_B_T1Y
-#define SMILI_DFT_B_T1Y 16, 2, 1
+#define SMILI_DFT_B_T1Y 8, 4, 1
 #endif
 // plans with at least this many work-groups in their first product (with
 // the few-product shape) take the batch shapes
