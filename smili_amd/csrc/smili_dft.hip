@@ -172,7 +172,7 @@ k_dft(const DftItem* __restrict__ items, int nitems, const typename Cx<R>::T* __
 #define SMILI_DFT_T2X 16, 1, 16      // G = S Ex
 #endif
 #ifndef SMILI_DFT_T1X
-#define SMILI_DFT_T1X 16, 1, 8       // P = Gs ExT
+#define SMILI_DFT_T1X 8, 1, 8        // P = Gs ExT (256^2 image: 3.9 us vs 4.5 with 16, 1, 8)
 #endif
 #ifndef SMILI_DFT_T1Y
 #define SMILI_DFT_T1Y 16, 1, 4       // grad = Re(EyT P)
