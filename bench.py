@@ -65,7 +65,10 @@ def stage_bytes(kws, units, fp64=True):
     variance: 28 B per full visibility or closure phase, 20 per amplitude,
     32 per log closure amplitude -- or the stored record where it is smaller:
     24 B per log closure amplitude, whose four point indices are 16-bit
-    chunk-local slots) and one FP64 complex residual per point.
+    chunk-local slots) and one FP64 complex residual per point; a
+    regularizer's per-pixel weights count only where they vary over the
+    image (uniform weights -- the flat-prior TV / TSV -- are one value the
+    kernels do not read per pixel).
     ``kws``: the problems' cost keyword dicts (a movie's dict carries one
     entry per frame in ``units``); ``units``: the handle's unit_info() of
     each unit in the same order (transform, grid or window width / height)."""
@@ -80,8 +83,13 @@ def stage_bytes(kws, units, fp64=True):
     ui = 0
     for kw in kws:
         N = np.asarray(kw["xidx"]).size
-        nreg = sum(1 for r in ("l1", "tv", "tsv", "kl", "gs") if kw.get(r + "_l", -1) > 0)
-        nreg += sum(1 for r in ("rt", "ri") if kw.get(r + "_l", -1) > 0)
+
+        def varies(w):
+            w = np.asarray(w)
+            return w.size > 1 and not np.all(w == w.flat[0])
+        # per-pixel weight arrays the regularizer launch reads
+        nreg = sum(1 for r in ("l1", "tv", "tsv", "kl", "gs", "rt", "ri")
+                   if kw.get(r + "_l", -1) > 0 and varies(kw.get(r + "_wgt")))
         M_all = 0
         for u in kw.get("units", [kw]):
             info = units[ui]
