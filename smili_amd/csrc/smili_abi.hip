@@ -47,6 +47,12 @@ struct Error : std::runtime_error {
 #define SMILI_CHISQ_DIRECT 1
 #endif
 
+// single-point data chunked in the order of their points (1) or in the
+// caller's order (0)
+#ifndef SMILI_POINT_ORDER
+#define SMILI_POINT_ORDER 1
+#endif
+
 // pixels per regularizer work item (the threads loop over them; measured
 // per evaluation, 256 / 512 / 1024 / 2048 / 4096: 256^2 single 52.8 / 53.2 /
 // 53.1 / 59.7 / 72.7 us (its regularizer items ride in the chi-square
@@ -865,14 +871,30 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
       }
       return n;
     };
+    // Single-point data (full visibilities, amplitudes) are emitted in the
+    // order of their points (stable: data of one point keep the caller's
+    // order), so a chunk's model-visibility reads and its residual shares,
+    // which the spreading gathers point by point, run along memory instead
+    // of jumping with the caller's time order.  Only the summation order of
+    // the cost changes.
+    std::vector<long long> order;
     auto build_type = [&](int type, long long n, int k, const int32_t* idx, auto&& emit) {
       Q.chunk_begin[type] = (int)h_chunks.size();
+      const bool by_point = SMILI_POINT_ORDER && k == 1;
+      if (by_point) {
+        order.resize(n);
+        for (long long i = 0; i < n; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](long long a, long long b) {
+          return np[std::abs(idx[a]) - 1] < np[std::abs(idx[b]) - 1];
+        });
+      }
       long long start = 0;
-      for (long long i = 0; i < n; ++i) {
+      for (long long e = 0; e < n; ++e) {
+        const long long i = by_point ? order[e] : e;
         const int32_t* ix = idx + i * k;
-        if (i - start >= chunk || (int)cur_slots.size() + new_slots(ix, k) > kMaxSlots) {
-          close_chunk(type, start, i - start);
-          start = i;
+        if (e - start >= chunk || (int)cur_slots.size() + new_slots(ix, k) > kMaxSlots) {
+          close_chunk(type, start, e - start);
+          start = e;
         }
         unsigned short sl[4] = {0, 0, 0, 0};
         for (int a = 0; a < k; ++a) sl[a] = slot(ix[a]);
