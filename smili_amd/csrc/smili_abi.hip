@@ -353,6 +353,10 @@ struct Plan {
   DevBuf<double> d_wts;                    // float or double ES weights, [2 kW][npt] (sized for double)
   DevBuf<double2> d_sres;                  // chi-square chunks' slot shares of the residual vectors
   DevBuf<int4> d_pref;                     // per point: first two slots, number of slots, offset in refs
+  // every point's residual is the one slot share at the point's own index
+  // (one datum per point, data chunked in point order): the spreading reads
+  // sres[j] and skips pref
+  bool resid_identity = false;
   DevBuf<int> d_refs;
   DevBuf<double2> d_V;                     // float2 or double2 visibilities (sized for double2)
   DevBuf<double2> d_Vres;
@@ -965,6 +969,8 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
       for (size_t k = 2; k < r.size(); ++k) refs.push_back(r[k]);
     }
   }
+  resid_identity = npt > 0;
+  for (size_t j = 0; j < npt && resid_identity; ++j) resid_identity = pref[j].z == 1 && pref[j].x == (int)j;
 
   // column work-groups: the largest of 4, 2, 1 columns whose LDS fits and
   // that still gives a work-group per CU (a single 512-column grid in FP32
@@ -1437,7 +1443,8 @@ static void launch_adjoint_t(Plan& P, int rows, const FinArgs* fin = nullptr, bo
   const V* tw = reinterpret_cast<const V*>(P.fp64 ? (const void*)P.d_twd.p : (const void*)P.d_tw.p);
   const R* corr = reinterpret_cast<const R*>(P.fp64 ? (const void*)P.d_corrd.p : (const void*)P.d_corr.p);
   const R* wts = reinterpret_cast<const R*>(P.d_wts.p);
-  const ResidSrc res{P.B.sres, P.d_pref.p, P.d_refs.p, from_chisq ? nullptr : P.d_Vres.p};
+  const ResidSrc res{P.B.sres, P.resid_identity ? nullptr : P.d_pref.p, P.d_refs.p,
+                     from_chisq ? nullptr : P.d_Vres.p};
   const long long npt = (long long)P.npt;
   V* gA = reinterpret_cast<V*>(P.B.gridA);
   V* gB = reinterpret_cast<V*>(P.B.gridB);

@@ -172,7 +172,8 @@ __device__ __forceinline__ void load_weights(const R* __restrict__ wts, long lon
 // order, or, for the bare adjoint, the caller's Vres[j].  pref[j] = (first
 // slot, second slot, number of slots, offset of the others in refs): the
 // common one- and two-slot points need one load round trip after pref, the
-// others load their further slots four at a time.
+// others load their further slots four at a time.  A null pref: point j's
+// residual is the single share sres[j].
 struct ResidSrc {
   const double2* sres;
   const int4* pref;
@@ -182,6 +183,7 @@ struct ResidSrc {
 
 __device__ __forceinline__ double2 point_resid(const ResidSrc& S, int j) {
   if (S.Vres) return S.Vres[j];
+  if (!S.pref) return S.sres[j];
   const int4 p = S.pref[j];
   double2 r = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
   if (p.z > 0) r = S.sres[p.x];
