@@ -148,10 +148,11 @@ typedef struct smili_tune {
   int32_t use_graph;    /* > 0: capture one evaluation in a hipGraph and replay
                            it; < 0: launch the kernels one by one; 0
                            (default): the graph for batches of several
-                           problems (as fast: 512-problem survey 1.628 vs
-                           1.623 ms), one by one otherwise (faster on ROCm 7
-                           / MI355X: 256^2 cost 53.3 vs 58.3 us, 64-frame
-                           movie 97.8 vs 106.6 us) */
+                           problems evaluated in one branch (under 2e6 uv
+                           points, see branches), one by one otherwise
+                           (faster on ROCm 7 / MI355X: 256^2 cost 53.3 vs
+                           58.3 us, 64-frame movie 97.8 vs 106.6 us,
+                           512-problem survey 1.258 vs 1.282 ms) */
   int32_t precision;    /* bits of the NUFFT arithmetic (spreading,
                            interpolation, FFT): 64 (default, also 0) or 32;
                            chi-square and regularizers are FP64 in both */

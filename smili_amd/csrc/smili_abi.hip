@@ -505,11 +505,7 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
     chisq_threads = 64;
     chunk = std::min(chunk, 1024);
   }
-  // default: a captured graph for batches of several problems (as fast as
-  // launching one by one: 512-problem survey 1.628 vs 1.623 ms), kernels
-  // launched one by one otherwise (faster: 256^2 image 53.3 vs 58.3 us,
-  // 64-frame movie 97.8 vs 106.6 us)
-  tune.use_graph = tune.use_graph > 0 ? 1 : tune.use_graph < 0 ? 0 : (ps.size() > 1 ? 1 : 0);
+  // use_graph 0 (default) is resolved once the launch layout is known
   require(tune.branches >= 0 && tune.branches <= 2, "branches must be 0, 1 or 2");
   require(tune.dft_shapes >= 0 && tune.dft_shapes <= 2, "dft_shapes must be 0, 1 or 2");
 
@@ -805,6 +801,14 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   // 1.3e5-point problem 53.1 us fused vs 70.2 us forked; a 64-frame movie of
   // 3e5 points 189 vs 200 us; a survey of 7.6e6 points 2.80 vs 2.68 ms)
   two_branches = tune.branches == 2 || (tune.branches == 0 && npt >= 2000000);
+  // default: a captured graph for batches of several problems evaluated in
+  // one branch (their many short launches), kernels launched one by one
+  // otherwise -- faster for a single problem (256^2 image 53.3 vs 58.3 us,
+  // 64-frame movie 97.8 vs 106.6 us) and for a two-branch batch (512-problem
+  // survey 1.258 vs 1.282 ms)
+  tune.use_graph = tune.use_graph > 0    ? 1
+                   : tune.use_graph < 0  ? 0
+                                         : (ps.size() > 1 && !two_branches ? 1 : 0);
 
   // problems: remapped data items and chunks
   std::vector<FcvItem> fcv;

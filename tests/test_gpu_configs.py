@@ -77,11 +77,11 @@ def test_survey_one_problem_per_size_matches_es_oracle(gpu_lib, survey):
 def test_survey_graph_equals_eager(gpu_lib, survey):
     from smili_amd.cost import BatchProblem
     preps, meta, kws, X, b = survey
-    costs, grad, _ = b.cost_grad(X)
-    # the batch handle replays a captured graph by default
-    with BatchProblem(kws, tune={"use_graph": -1}) as eag:
-        costs_e, grad_e, _ = eag.cost_grad(X)
-        assert b.info()["graph"] == 1 and eag.info()["graph"] == 0
+    costs_e, grad_e, _ = b.cost_grad(X)
+    # a batch of this size (two branches) launches one by one by default
+    with BatchProblem(kws, tune={"use_graph": 1}) as gr:
+        costs, grad, _ = gr.cost_grad(X)
+        assert b.info()["graph"] == 0 and gr.info()["graph"] == 1
     assert np.array_equal(costs, costs_e)
     assert rel_l2(grad_e, grad) <= 1e-13
 
