@@ -618,79 +618,87 @@ __device__ __forceinline__ void reg_block_unit(int bid, const RegArgs& A, const 
     if (rt_on) lu = Mv.lc[U.frame + 1];
     if (rt_on && U.frame > 0) ll = Mv.lc[U.frame - 1];
   }
+  // the unit's fields the pixel loop reads, in registers: read in place
+  // (LEAN) they would be loaded again after every gradient store, which
+  // may alias the descriptor -- a round trip to memory per field and pixel
+  const struct {
+    int nx, ny, x_off, pix_off, identity, m, frame, need_moments;
+    const double *l1_w, *tv_w, *tsv_w, *kl_w, *gs_w;
+    double cen_alpha, cen_l, gs_l, gs_u, kl_l, kl_u, l1_l, l1_u, nxref, nyref, reg_scale, sm_l, tfd_l, tsv_l, tsv_u, tv_l, tv_u;
+  } L = {U.nx, U.ny, U.x_off, U.pix_off, U.identity, U.m, U.frame, U.need_moments, U.l1_w, U.tv_w, U.tsv_w, U.kl_w, U.gs_w, U.cen_alpha, U.cen_l, U.gs_l, U.gs_u, U.kl_l, U.kl_u, U.l1_l, U.l1_u, U.nxref, U.nyref, U.reg_scale, U.sm_l, U.tfd_l, U.tsv_l, U.tsv_u, U.tv_l, U.tv_u};
   for (int i = threadIdx.x; i < it.count; i += blockDim.x) {
     const int k = it.start + i;
-    const int p = U.identity ? k : pix[U.pix_off + k];
-    const int ix = p % U.nx, iy = p / U.nx;
+    const int p = L.identity ? k : pix[L.pix_off + k];
+    const int ix = p % L.nx, iy = p / L.nx;
     // every load of the pixel first, all in flight together (issued inside
     // each term's branch they would be one round trip to memory per term)
-    const double I = x[U.x_off + k];
+    const double I = x[L.x_off + k];
     double wl1 = 0.0, Pkl = 1.0, Pgs = 1.0, wtv = 0.0, wtsv = 0.0;
-    if (U.l1_l > 0) wl1 = U.l1_w ? U.l1_w[k] : U.l1_u;
-    if (U.kl_l > 0) Pkl = U.kl_w ? U.kl_w[k] : U.kl_u;
-    if (U.gs_l > 0) Pgs = U.gs_w ? U.gs_w[k] : U.gs_u;
-    if (U.tv_l > 0) wtv = U.tv_w ? U.tv_w[k] : U.tv_u;
-    if (U.tsv_l > 0) wtsv = U.tsv_w ? U.tsv_w[k] : U.tsv_u;
+    if (L.l1_l > 0) wl1 = L.l1_w ? L.l1_w[k] : L.l1_u;
+    if (L.kl_l > 0) Pkl = L.kl_w ? L.kl_w[k] : L.kl_u;
+    if (L.gs_l > 0) Pgs = L.gs_w ? L.gs_w[k] : L.gs_u;
+    if (L.tv_l > 0) wtv = L.tv_w ? L.tv_w[k] : L.tv_u;
+    if (L.tsv_l > 0) wtsv = L.tsv_w ? L.tsv_w[k] : L.tsv_u;
     Nbr nb{};
-    if (U.tv_l > 0 || U.tsv_l > 0) nb = load_nbr(I2d, U.nx, U.ny, ix, iy);
+    if (L.tv_l > 0 || L.tsv_l > 0) nb = load_nbr(I2d, L.nx, L.ny, ix, iy);
     double Iu = 0.0, Il = 0.0, avg = 0.0, wrt = 0.0, wri = 0.0;
     if (rt_on) {
       wrt = Mv.rt_w ? Mv.rt_w[k] : Mv.rt_u;
-      Iu = x[U.x_off + Mv.npix + k];
-      if (U.frame > 0) Il = x[U.x_off - Mv.npix + k];
+      Iu = x[L.x_off + Mv.npix + k];
+      if (L.frame > 0) Il = x[L.x_off - Mv.npix + k];
     }
     if (ri_on) {
       wri = Mv.ri_w ? Mv.ri_w[k] : Mv.ri_u;
       avg = Mv.iavg[k];
     }
     double g = 0.0;
-    if (LEAN && U.need_moments) g += moment_grad(U, mom[it.unit], ix, iy, I);
-    if (moments && U.tfd_l > 0) g += tfd_g;
-    if (moments && U.cen_l > 0) {
-      const double dix = (ix + 1) - U.nxref, diy = (iy + 1) - U.nyref;
+    if (LEAN && L.need_moments) g += moment_grad(U, mom[it.unit], ix, iy, I);
+    if (moments && L.tfd_l > 0) g += tfd_g;
+    if (moments && L.cen_l > 0) {
+      const double dix = (ix + 1) - L.nxref, diy = (iy + 1) - L.nyref;
       const double sa = smabs(I);
-      const double gIp = alpha1 ? I / sa : I / sa * U.cen_alpha * pow(sa, U.cen_alpha - 1.0);
-      g += U.cen_l * (cen_sx * (dix - cen_sx) + cen_sy * (diy - cen_sy)) / cen_sIp / cen_cost * gIp;
+      const double gIp = alpha1 ? I / sa : I / sa * L.cen_alpha * pow(sa, L.cen_alpha - 1.0);
+      g += L.cen_l * (cen_sx * (dix - cen_sx) + cen_sy * (diy - cen_sy)) / cen_sIp / cen_cost * gIp;
     }
-    if (U.l1_l > 0) {
-      const double w = U.l1_l * wl1;
+    if (L.l1_l > 0) {
+      const double w = L.l1_l * wl1;
       const double sa = smabs(I);
       v[0] += w * sa;
       g += w * (I / sa);
     }
-    if (U.kl_l > 0) {
+    if (L.kl_l > 0) {
       const double sa = smabs(I);
       const double lg = log(sa / Pkl);
-      v[3] += U.kl_l * (sa * lg + Pkl / 2.71828182845904523536);
-      g += U.kl_l * (I / sa) * (lg + 1.0);
+      v[3] += L.kl_l * (sa * lg + Pkl / 2.71828182845904523536);
+      g += L.kl_l * (I / sa) * (lg + 1.0);
     }
-    if (U.gs_l > 0) {
+    if (L.gs_l > 0) {
       const double sa = smabs(I);
       const double lg = log(sa / Pgs);
-      v[4] += U.gs_l * (sa * (lg - 1.0) + Pgs);
-      g += U.gs_l * (I / sa) * lg;
+      v[4] += L.gs_l * (sa * (lg - 1.0) + Pgs);
+      g += L.gs_l * (I / sa) * lg;
     }
-    if (U.tv_l > 0) {
-      const double w = U.tv_l * wtv;
+    if (L.tv_l > 0) {
+      const double w = L.tv_l * wtv;
       double dx, dy;
-      nbr_fwd_diff(nb, U.nx, U.ny, ix, iy, dx, dy);
+      nbr_fwd_diff(nb, L.nx, L.ny, ix, iy, dx, dy);
       v[1] += w * sqrt(dx * dx + dy * dy);
-      g += w * nbr_tv_grade(nb, U.nx, U.ny, ix, iy);
+      g += w * nbr_tv_grade(nb, L.nx, L.ny, ix, iy);
     }
-    if (U.tsv_l > 0) {
-      const double w = U.tsv_l * wtsv;
+    if (L.tsv_l > 0) {
+      const double w = L.tsv_l * wtsv;
       double dx, dy;
-      nbr_fwd_diff(nb, U.nx, U.ny, ix, iy, dx, dy);
+      nbr_fwd_diff(nb, L.nx, L.ny, ix, iy, dx, dy);
       v[2] += w * (dx * dx + dy * dy);
-      g += w * nbr_tsv_grade(nb, U.nx, U.ny, ix, iy);
+      g += w * nbr_tsv_grade(nb, L.nx, L.ny, ix, iy);
     }
-    if (moments && U.sm_l > 0) {
+    if (moments && L.sm_l > 0) {
       const double xi = ix + 1, yi = iy + 1;
       const double xg = (xi - S.xc) / S.Isum, yg = (yi - S.yc) / S.Isum;
       const double g1 = (((xi - S.xc) * (xi - S.xc) - 2.0 * xg * S.mom[0]) - S.Sg[0]) / S.Isum;
       const double g2 = (((yi - S.yc) * (yi - S.yc) - 2.0 * yg * S.mom[1]) - S.Sg[1]) / S.Isum;
       const double g3 = ((xi - S.xc) * (yi - S.yc) - xg * S.mom[1] - yg * S.mom[0] - S.Sg[2]) / S.Isum;
-      g += U.sm_l * (2.0 * (S.Sg[0] - S.Sp[0]) * g1 + 2.0 * (S.Sg[1] - S.Sp[1]) * g2 +
+      g += L.sm_l * (2.0 * (S.Sg[0] - S.Sp[0]) * g1 + 2.0 * (S.Sg[1] - S.Sp[1]) * g2 +
                      4.0 * (S.Sg[2] - S.Sp[2]) * g3);
     }
     double gt = 0.0;
@@ -701,7 +709,7 @@ __device__ __forceinline__ void reg_block_unit(int bid, const RegArgs& A, const 
         const double Ju = Iu / lu;
         v[5] += w * ((Ju - II) * (Ju - II));
         double gr;
-        if (U.frame == 0) {
+        if (L.frame == 0) {
           gr = 2.0 * (II - Ju);
         } else {
           const double Jl = Il / ll;
@@ -718,9 +726,9 @@ __device__ __forceinline__ void reg_block_unit(int bid, const RegArgs& A, const 
     }
     // units with uv points: k_t1_rows adds the chi-square gradient; units
     // without (movie frames with no data) have none
-    const double gr = U.reg_scale * g + gt;
-    if (U.m > 0) greg[U.x_off + k] = gr;
-    else grad[U.x_off + k] = 0.0 + gr;
+    const double gr = L.reg_scale * g + gt;
+    if (L.m > 0) greg[L.x_off + k] = gr;
+    else grad[L.x_off + k] = 0.0 + gr;
   }
   block_sum<kRegTerms, kChisqThreadsMax / 64>(v, red);
   // l1 .. gs partials scaled here by reg_scale (the sums read them as is);
