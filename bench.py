@@ -65,8 +65,10 @@ def stage_bytes(kws, units, fp64=True):
     variance: 28 B per full visibility or closure phase, 20 per amplitude,
     32 per log closure amplitude -- or the stored record where it is smaller:
     24 B per log closure amplitude, whose four point indices are 16-bit
-    chunk-local slots) and one FP64 complex residual per point; a
-    regularizer's per-pixel weights count only where they vary over the
+    chunk-local slots) and one FP64 complex residual per point (plus the
+    spreading's 16-byte per-point reference record where a point's
+    residual shares must be gathered, i.e. unless every point has one
+    datum); a regularizer's per-pixel weights count only where they vary over the
     image (uniform weights -- the flat-prior TV / TSV -- are one value the
     kernels do not read per pixel).
     ``kws``: the problems' cost keyword dicts (a movie's dict carries one
@@ -81,6 +83,20 @@ def stage_bytes(kws, units, fp64=True):
         tot[k] = tot.get(k, 0) + int(v)
 
     ui = 0
+    def n_of(kw, key, k=1):
+        a = kw.get(key)
+        return 0 if a is None else np.asarray(a).size // k
+
+    def one_datum_per_point(kw):
+        idx = kw.get("uvidxfcv") if n_of(kw, "uvidxfcv") else kw.get("uvidxamp")
+        if idx is None or n_of(kw, "uvidxcp", 3) or n_of(kw, "uvidxca", 4) or \
+                (n_of(kw, "uvidxfcv") and n_of(kw, "uvidxamp")):
+            return False
+        a = np.abs(np.asarray(idx)).ravel()
+        return a.size == np.asarray(kw["u"]).size and np.unique(a).size == a.size
+    # the plan reads each point's residual share at the point's own index,
+    # with no per-point reference record, when every point has one datum
+    pref_bytes = 0 if all(one_datum_per_point(kw) for kw in kws) else 16
     for kw in kws:
         N = np.asarray(kw["xidx"]).size
 
@@ -115,7 +131,7 @@ def stage_bytes(kws, units, fp64=True):
                 G = 0
             if M:
                 add("interp", cb * G + M * (4 + wb + cb))
-                add("spread", M * (4 + wb + 16 + 16) + cb * G)   # cell, weights, slot refs, residual; grid
+                add("spread", M * (4 + wb + pref_bytes + 16) + cb * G)   # cell, weights, slot refs, residual; grid
         nun = len(kw.get("units", [kw]))
         add("reg", 8 * N * nun * (1 + nreg) + 8 * N * nun)         # x, weights in, gradreg out
         nfcv = 0 if kw.get("uvidxfcv") is None else len(kw["uvidxfcv"])
