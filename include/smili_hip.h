@@ -180,9 +180,11 @@ typedef struct smili_tune {
   int32_t dft_shapes;    /* work-group shapes of the pruned-DFT products:
                            1 = for few products (split each product's sum
                            many ways: one large image), 2 = for many (more
-                           rows per work-group: movies, surveys), 0
-                           (default) = 2 when the first product has at
-                           least 1024 work-groups with shape 1, else 1 */
+                           rows per work-group: movies), 3 = for very many
+                           (4-wave work-groups: surveys of hundreds of
+                           images), 0 (default) = 3 when the first product
+                           has at least 16384 work-groups with shape 1, 2
+                           from 1024, else 1 */
 } smili_tune;
 
 int smili_abi_version(void);

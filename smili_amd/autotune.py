@@ -30,7 +30,7 @@ SPACE = (
     ("col_group", (0, 1, 2, 4)),
     ("fft_radix", (4, 8)),
     ("chisq_threads", (256, 512)),
-    ("dft_shapes", (0, 1, 2)),
+    ("dft_shapes", (0, 1, 2, 3)),
 )
 
 

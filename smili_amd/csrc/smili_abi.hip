@@ -323,7 +323,7 @@ struct Plan {
   std::vector<char> unit_dft;
   std::vector<std::pair<int, int>> unit_grid;   // grid (FFT) or window (DFT) width, height
   bool any_fft = false, any_dft = false;
-  bool dft_batch = false;                  // the many-product shapes of the DFT kernels
+  int dft_set = 0;                         // DFT kernel shapes: 0 few products, 1 many, 2 very many
   std::vector<DftItem> h_dft[4];
   DevBuf<DftItem> d_dft[4];
   DevBuf<double2> d_dftm;                  // float2 or double2 DFT matrices (sized for double2)
@@ -507,7 +507,7 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
   }
   // use_graph 0 (default) is resolved once the launch layout is known
   require(tune.branches >= 0 && tune.branches <= 2, "branches must be 0, 1 or 2");
-  require(tune.dft_shapes >= 0 && tune.dft_shapes <= 2, "dft_shapes must be 0, 1 or 2");
+  require(tune.dft_shapes >= 0 && tune.dft_shapes <= 3, "dft_shapes must be 0, 1, 2 or 3");
 
   HIP_CHECK(hipGetDevice(&device));
   HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -1071,12 +1071,16 @@ void Plan::build(const std::vector<UnitSpec>& us, const std::vector<ProbSpec>& p
         if (unit_dft[ui])
           first += (long long)((units[ui].nx + 63) / 64) *
                    ((gdim[ui].second + ShapeT2Y::rows - 1) / ShapeT2Y::rows);
-      dft_batch = tune.dft_shapes == 2 || (tune.dft_shapes == 0 && first >= SMILI_DFT_BATCH_ITEMS);
+      dft_set = tune.dft_shapes > 0 ? tune.dft_shapes - 1
+                : first >= SMILI_DFT_LARGE_ITEMS ? 2
+                : first >= SMILI_DFT_BATCH_ITEMS ? 1
+                                                 : 0;
     }
-    const int rows_t2y = dft_batch ? ShapeBT2Y::rows : ShapeT2Y::rows;
-    const int rows_t2x = dft_batch ? ShapeBT2X::rows : ShapeT2X::rows;
-    const int rows_t1x = dft_batch ? ShapeBT1X::rows : ShapeT1X::rows;
-    const int rows_t1y = dft_batch ? ShapeBT1Y::rows : ShapeT1Y::rows;
+    auto rows_of = [&](int few, int many, int large) { return dft_set == 2 ? large : dft_set == 1 ? many : few; };
+    const int rows_t2y = rows_of(ShapeT2Y::rows, ShapeBT2Y::rows, ShapeLT2Y::rows);
+    const int rows_t2x = rows_of(ShapeT2X::rows, ShapeBT2X::rows, ShapeLT2X::rows);
+    const int rows_t1x = rows_of(ShapeT1X::rows, ShapeBT1X::rows, ShapeLT1X::rows);
+    const int rows_t1y = rows_of(ShapeT1Y::rows, ShapeBT1Y::rows, ShapeLT1Y::rows);
     for (size_t ui = 0; ui < units.size(); ++ui) {
       if (!unit_dft[ui]) continue;
       const UnitDev& U = units[ui];
@@ -1340,9 +1344,12 @@ static size_t xcd_grid(size_t nitems) { return (nitems + kXcds - 1) / kXcds * kX
   } while (0)
 
 // one DFT product with the plan's shape set (few or many products)
-#define DFT_LAUNCH(stage, SH, SHB, BREAL, EPI, nitems, ...)                                        \
+#define DFT_LAUNCH(stage, SH, SHB, SHL, BREAL, EPI, nitems, ...)                                   \
   do {                                                                                            \
-    if (P.dft_batch)                                                                              \
+    if (P.dft_set == 2)                                                                           \
+      LAUNCH(stage, (k_dft<R, SHL::nw, SHL::rb, SHL::ks, BREAL, EPI>), xcd_grid(nitems), SHL::threads, 0, \
+             __VA_ARGS__);                                                                        \
+    else if (P.dft_set == 1)                                                                      \
       LAUNCH(stage, (k_dft<R, SHB::nw, SHB::rb, SHB::ks, BREAL, EPI>), xcd_grid(nitems), SHB::threads, 0, \
              __VA_ARGS__);                                                                        \
     else                                                                                          \
@@ -1377,10 +1384,10 @@ static void launch_forward_t(Plan& P, bool with_scatter) {
   if (P.any_dft) {
     const V* M = reinterpret_cast<const V*>(P.d_dftm.p);
     V* H = reinterpret_cast<V*>(P.B.H);
-    DFT_LAUNCH(SMILI_S_DFT2_Y, ShapeT2Y, ShapeBT2Y, true, DFT_STORE, P.h_dft[0].size(), P.d_dft[0].p,
+    DFT_LAUNCH(SMILI_S_DFT2_Y, ShapeT2Y, ShapeBT2Y, ShapeLT2Y, true, DFT_STORE, P.h_dft[0].size(), P.d_dft[0].p,
                (int)P.h_dft[0].size(), M, (const V*)nullptr, P.B.x, P.B.I2d, H, (V*)nullptr, (double*)nullptr,
                (double*)nullptr, (const double*)nullptr, (const int*)nullptr, 0);
-    DFT_LAUNCH(SMILI_S_DFT2_X, ShapeT2X, ShapeBT2X, false, DFT_STORE, P.h_dft[1].size(), P.d_dft[1].p,
+    DFT_LAUNCH(SMILI_S_DFT2_X, ShapeT2X, ShapeBT2X, ShapeLT2X, false, DFT_STORE, P.h_dft[1].size(), P.d_dft[1].p,
                (int)P.h_dft[1].size(), (const V*)H, M, (const double*)nullptr, (const double*)nullptr, gA,
                (V*)nullptr, (double*)nullptr, (double*)nullptr, (const double*)nullptr, (const int*)nullptr, 0);
   }
@@ -1423,7 +1430,7 @@ static void launch_rows_t(Plan& P, bool assemble = true) {
   if (P.any_dft) {
     const V* M = reinterpret_cast<const V*>(P.d_dftm.p);
     const V* H = reinterpret_cast<const V*>(P.B.H);
-    DFT_LAUNCH(SMILI_S_DFT1_Y, ShapeT1Y, ShapeBT1Y, false, DFT_GRAD, P.h_dft[3].size(), P.d_dft[3].p,
+    DFT_LAUNCH(SMILI_S_DFT1_Y, ShapeT1Y, ShapeBT1Y, ShapeLT1Y, false, DFT_GRAD, P.h_dft[3].size(), P.d_dft[3].p,
                (int)P.h_dft[3].size(), M, H, (const double*)nullptr, (const double*)nullptr, (V*)nullptr, gB,
                P.B.grad2d, P.B.grad, P.B.greg, P.d_pixinv.p, assemble ? 1 : 0);
   }
@@ -1478,7 +1485,7 @@ static void launch_adjoint_t(Plan& P, int rows, const FinArgs* fin = nullptr, bo
   if (P.any_dft) {
     const V* M = reinterpret_cast<const V*>(P.d_dftm.p);
     V* H = reinterpret_cast<V*>(P.B.H);
-    DFT_LAUNCH(SMILI_S_DFT1_X, ShapeT1X, ShapeBT1X, false, DFT_STORE, P.h_dft[2].size(), P.d_dft[2].p,
+    DFT_LAUNCH(SMILI_S_DFT1_X, ShapeT1X, ShapeBT1X, ShapeLT1X, false, DFT_STORE, P.h_dft[2].size(), P.d_dft[2].p,
                (int)P.h_dft[2].size(), (const V*)gB, M, (const double*)nullptr, (const double*)nullptr, H,
                (V*)nullptr, (double*)nullptr, (double*)nullptr, (const double*)nullptr, (const int*)nullptr, 0);
   }

@@ -203,6 +203,26 @@ k_dft(const DftItem* __restrict__ items, int nitems, const typename Cx<R>::T* __
 #define SMILI_DFT_BATCH_ITEMS 1024
 #endif
 
+// Plans far larger still (hundreds of images: the 512-problem survey has
+// ~43,000 first-product work-groups, the 64-frame movie ~3,500) take 4-wave
+// work-groups with fewer K slices: survey 1.188 -> 1.136 ms against the
+// batch shapes, which stay the movie's (97.0 vs 91.8 us with these).
+#ifndef SMILI_DFT_LARGE_ITEMS
+#define SMILI_DFT_LARGE_ITEMS 16384
+#endif
+#ifndef SMILI_DFT_L_T2Y
+#define SMILI_DFT_L_T2Y 4, 4, 4
+#endif
+#ifndef SMILI_DFT_L_T2X
+#define SMILI_DFT_L_T2X 4, 4, 2
+#endif
+#ifndef SMILI_DFT_L_T1X
+#define SMILI_DFT_L_T1X 4, 4, 2
+#endif
+#ifndef SMILI_DFT_L_T1Y
+#define SMILI_DFT_L_T1Y 4, 4, 1
+#endif
+
 template <int NW, int RB, int KS> struct DftShape {
   static constexpr int nw = NW, rb = RB, ks = KS, rows = RB * (NW / KS), threads = 64 * NW;
 };
@@ -214,5 +234,9 @@ using ShapeBT2Y = DftShape<SMILI_DFT_B_T2Y>;
 using ShapeBT2X = DftShape<SMILI_DFT_B_T2X>;
 using ShapeBT1X = DftShape<SMILI_DFT_B_T1X>;
 using ShapeBT1Y = DftShape<SMILI_DFT_B_T1Y>;
+using ShapeLT2Y = DftShape<SMILI_DFT_L_T2Y>;
+using ShapeLT2X = DftShape<SMILI_DFT_L_T2X>;
+using ShapeLT1X = DftShape<SMILI_DFT_L_T1X>;
+using ShapeLT1Y = DftShape<SMILI_DFT_L_T1Y>;
 
 }  // namespace smili

@@ -33,7 +33,7 @@ CASES = [((24, 20), 0.25, False), ((17, 13), 0.3, True), ((64, 48), 0.2, False),
          ((40, 30), 0.95, False), ((33, 17), 0.95, True), ((128, 96), 0.15, False)]
 
 
-@pytest.mark.parametrize("transform,dft_shapes", [(1, 0), (2, 1), (2, 2)])
+@pytest.mark.parametrize("transform,dft_shapes", [(1, 0), (2, 1), (2, 2), (2, 3)])
 @pytest.mark.parametrize("shape,ufrac,window", CASES)
 def test_fp64_matches_es_oracle(gpu_lib, shape, ufrac, window, transform, dft_shapes):
     """Both transform paths; the DFT path with both work-group shape sets of
@@ -69,7 +69,7 @@ def test_fp64_matches_es_oracle(gpu_lib, shape, ufrac, window, transform, dft_sh
         assert t[k] == pytest.approx(v, rel=1e-10, abs=1e-12 * abs(c_es)), k
 
 
-@pytest.mark.parametrize("dft_shapes", [1, 2])
+@pytest.mark.parametrize("dft_shapes", [1, 2, 3])
 @pytest.mark.parametrize("shape,ufrac,window", CASES[:4])
 def test_fp32_matches_es_oracle(gpu_lib, shape, ufrac, window, dft_shapes):
     """FP32 products (FP32 DFT matrices, FP32 accumulation over <= 2 nf terms)
@@ -166,7 +166,7 @@ def test_batch_mixing_both_paths(gpu_lib):
         assert rel_l2(grads[i], g_es) <= 1e-10
 
 
-@pytest.mark.parametrize("dft_shapes", [1, 2])
+@pytest.mark.parametrize("dft_shapes", [1, 2, 3])
 def test_movie_on_small_windows(gpu_lib, dft_shapes):
     from smili_amd.cost import MovieProblem
     rng = np.random.default_rng(14)

@@ -232,7 +232,7 @@ def test_tunables_do_not_change_results(gpu_lib, tune):
 
 
 @pytest.mark.parametrize("tune", [dict(chisq_threads=128), dict(chisq_threads=1024), dict(fft_radix=3),
-                                  dict(chunk=8), dict(bin=12)])
+                                  dict(chunk=8), dict(bin=12), dict(dft_shapes=4)])
 def test_invalid_tunables_are_rejected(gpu_lib, tune):
     from smili_amd._lib import SmiliError
     rng = np.random.default_rng(22)
