@@ -205,7 +205,7 @@ k_dft(const DftItem* __restrict__ items, int nitems, const typename Cx<R>::T* __
 
 // Plans far larger still (hundreds of images: the 512-problem survey has
 // ~43,000 first-product work-groups, the 64-frame movie ~3,500) take 4-wave
-// work-groups with fewer K slices: survey 1.188 -> 1.136 ms against the
+// work-groups with fewer K slices: survey 1.188 -> 1.124 ms against the
 // batch shapes, which stay the movie's (97.0 vs 91.8 us with these).
 #ifndef SMILI_DFT_LARGE_ITEMS
 #define SMILI_DFT_LARGE_ITEMS 16384
@@ -220,7 +220,7 @@ k_dft(const DftItem* __restrict__ items, int nitems, const typename Cx<R>::T* __
 #define SMILI_DFT_L_T1X 4, 4, 2
 #endif
 #ifndef SMILI_DFT_L_T1Y
-#define SMILI_DFT_L_T1Y 4, 4, 1
+#define SMILI_DFT_L_T1Y 4, 8, 1      // survey 1.134 -> 1.124 ms vs 4, 4, 1
 #endif
 
 template <int NW, int RB, int KS> struct DftShape {
