@@ -195,7 +195,7 @@ k_dft(const DftItem* __restrict__ items, int nitems, const typename Cx<R>::T* __
 #define SMILI_DFT_B_T1X 8, 4, 4
 #endif
 #ifndef SMILI_DFT_B_T1Y
-#define SMILI_DFT_B_T1Y 4, 4, 1      // survey 1.189 -> 1.169 ms vs 8, 4, 1 (movie 92.1 vs 92.8 us; 4, 8, 1: 1.159 ms but movie 97.3 us)
+#define SMILI_DFT_B_T1Y 8, 4, 1      // 64-frame movie 92.2 us (92.8 with 4, 4, 1)
 #endif
 // plans with at least this many work-groups in their first product (with
 // the few-product shape) take the batch shapes
